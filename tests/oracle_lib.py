@@ -1,0 +1,324 @@
+"""ctypes bindings for the oracle (oracle/liboracle.so) and, where it was
+built, oracle/_ref/libref.so (the reference's own nanoflann and Eigen).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+STAGE_COVFEAT, STAGE_NORMAL, STAGE_EIGENVALUES = 0, 1, 2
+MODE_RAW, MODE_SQRT, MODE_NORMALIZED = 0, 1, 2
+F_LINEARITY = 1 << 0
+F_PLANARITY = 1 << 1
+F_SCATTERING = 1 << 2
+F_VERTICALITY = 1 << 3
+F_OMNIVARIANCE = 1 << 4
+F_ANISOTROPY = 1 << 5
+F_EIGENENTROPY = 1 << 6
+F_EIGENVALUESUM = 1 << 7
+F_SURFACEVARIATION = 1 << 8
+F_DEMANTKEVERTICALITY = 1 << 9
+F_DIMENSIONALITY = F_LINEARITY | F_PLANARITY | F_SCATTERING | F_VERTICALITY
+F_ALL10 = (1 << 10) - 1
+FEATURE_NAMES = ["Linearity", "Planarity", "Scattering", "Verticality",
+                 "Omnivariance", "Anisotropy", "Eigenentropy", "EigenvalueSum",
+                 "SurfaceVariation", "DemantkeVerticality"]
+
+ST_TIE, ST_INCOMPLETE, ST_ZERO_COV, ST_EIGEN_FAIL = 1, 2, 4, 8
+ST_FEATURE_INVALID, ST_FEW_NEIGHBORS = 16, 32
+
+
+class Stage(ctypes.Structure):
+    _fields_ = [("stage", ctypes.c_int32), ("knn", ctypes.c_int32),
+                ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
+                ("features", ctypes.c_uint32), ("always_up", ctypes.c_int32),
+                ("normalize", ctypes.c_int32)]
+
+
+def stage(kind, knn, stride=1, mode=MODE_SQRT, features=F_DIMENSIONALITY,
+          always_up=1, normalize=0):
+    return Stage(kind, knn, stride, mode, features, always_up, normalize)
+
+
+def stage_columns(st):
+    if st.stage == STAGE_NORMAL:
+        return 4
+    if st.stage == STAGE_EIGENVALUES:
+        return 3
+    return bin(st.features & F_ALL10).count("1")
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                f"{path} is missing: run `make -C oracle` (or __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        L.orc_kd3_build.restype = ctypes.c_void_p
+        L.orc_kd3_build.argtypes = [_dp, _dp, _dp, ctypes.c_uint64]
+        L.orc_kd3_free.argtypes = [ctypes.c_void_p]
+        L.orc_kd3_neighbors.restype = ctypes.c_uint64
+        L.orc_kd3_neighbors.argtypes = [ctypes.c_void_p, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_uint64, ctypes.c_uint64,
+                                        _u64p, _dp]
+        L.orc_kd3_neighbors_all.argtypes = [ctypes.c_void_p, ctypes.c_uint64,
+                                            ctypes.c_uint64, ctypes.c_int, _u32p]
+        L.orc_kd3_node_count.restype = ctypes.c_uint64
+        L.orc_kd3_node_count.argtypes = [ctypes.c_void_p]
+        L.orc_kd3_export.argtypes = [ctypes.c_void_p, _u64p, _i32p, _i32p,
+                                     _u64p, _u64p, _i32p, _dp, _dp]
+        L.orc_brute_knn.argtypes = [_dp, _dp, _dp, ctypes.c_uint64, _u64p,
+                                    ctypes.c_uint64, ctypes.c_uint64,
+                                    ctypes.c_int, _u32p, _dp]
+        L.orc_pdal_covariance.argtypes = [_dp, _dp, _dp, _u64p,
+                                          ctypes.c_uint64, _dp]
+        L.orc_is_zero.restype = ctypes.c_int
+        L.orc_is_zero.argtypes = [_dp]
+        L.orc_eigen_sym3.restype = ctypes.c_int
+        L.orc_eigen_sym3.argtypes = [_dp, _dp, _dp]
+        L.orc_jacobi_sym3.restype = ctypes.c_int
+        L.orc_jacobi_sym3.argtypes = [_dp, _dp, _dp]
+        sp = ctypes.POINTER(Stage)
+        L.orc_stage_run.argtypes = [ctypes.c_void_p, _dp, _dp, _dp,
+                                    ctypes.c_uint64, sp, ctypes.c_int,
+                                    ctypes.POINTER(_dp), _u8p]
+        L.orc_stage_from_lists.argtypes = [_dp, _dp, _dp, ctypes.c_uint64,
+                                           _u32p, ctypes.c_uint64, sp,
+                                           ctypes.c_int, ctypes.POINTER(_dp),
+                                           _u8p]
+        L.orc_truth_from_lists.argtypes = [_dp, _dp, _dp, ctypes.c_uint64,
+                                           _u32p, ctypes.c_uint64, sp,
+                                           ctypes.c_int, ctypes.POINTER(_dp),
+                                           _u8p, _dp]
+        self.L = L
+
+    # --------------------------------------------------------------- KD3
+    class Tree:
+        def __init__(self, orc, x, y, z):
+            self.orc = orc
+            self.x, self.y, self.z = (_c(v, np.float64) for v in (x, y, z))
+            self.n = len(self.x)
+            self.h = orc.L.orc_kd3_build(_ptr(self.x, _dp), _ptr(self.y, _dp),
+                                         _ptr(self.z, _dp), self.n)
+
+        def __del__(self):
+            if getattr(self, "h", None):
+                self.orc.L.orc_kd3_free(self.h)
+                self.h = None
+
+        def neighbors(self, qx, qy, qz, k, stride=1):
+            cap = max(1, stride * min(k, self.n))
+            ids = np.zeros(cap, np.uint64)
+            d2 = np.zeros(cap, np.float64)
+            c = self.orc.L.orc_kd3_neighbors(self.h, qx, qy, qz, k, stride,
+                                             _ptr(ids, _u64p), _ptr(d2, _dp))
+            return ids[:c].copy()
+
+        def neighbors_all(self, k, stride=1, threads=1):
+            kk = min(k, self.n)
+            out = np.zeros((self.n, kk), np.uint32)
+            self.orc.L.orc_kd3_neighbors_all(self.h, k, stride, threads,
+                                             _ptr(out, _u32p))
+            return out
+
+        def export(self):
+            m = self.orc.L.orc_kd3_node_count(self.h)
+            vind = np.zeros(self.n, np.uint64)
+            c1 = np.zeros(m, np.int32)
+            c2 = np.zeros(m, np.int32)
+            lo = np.zeros(m, np.uint64)
+            hi = np.zeros(m, np.uint64)
+            feat = np.zeros(m, np.int32)
+            dl = np.zeros(m, np.float64)
+            dh = np.zeros(m, np.float64)
+            self.orc.L.orc_kd3_export(self.h, _ptr(vind, _u64p), _ptr(c1, _i32p),
+                                      _ptr(c2, _i32p), _ptr(lo, _u64p),
+                                      _ptr(hi, _u64p), _ptr(feat, _i32p),
+                                      _ptr(dl, _dp), _ptr(dh, _dp))
+            return dict(vind=vind, child1=c1, child2=c2, left=lo, right=hi,
+                        divfeat=feat, divlow=dl, divhigh=dh)
+
+    def kd3(self, x, y, z):
+        return Oracle.Tree(self, x, y, z)
+
+    def brute_knn(self, x, y, z, kk, queries=None, threads=8):
+        x, y, z = (_c(v, np.float64) for v in (x, y, z))
+        n = len(x)
+        q = np.arange(n, dtype=np.uint64) if queries is None else _c(queries, np.uint64)
+        ids = np.zeros((len(q), kk), np.uint32)
+        d2 = np.zeros((len(q), kk), np.float64)
+        self.L.orc_brute_knn(_ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp), n,
+                             _ptr(q, _u64p), len(q), kk, threads,
+                             _ptr(ids, _u32p), _ptr(d2, _dp))
+        return ids, d2
+
+    # ------------------------------------------------------ arithmetic
+    def pdal_covariance(self, x, y, z, ids):
+        x, y, z = (_c(v, np.float64) for v in (x, y, z))
+        ids = _c(ids, np.uint64)
+        cov = np.zeros(9)
+        self.L.orc_pdal_covariance(_ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp),
+                                   _ptr(ids, _u64p), len(ids), _ptr(cov, _dp))
+        return cov.reshape(3, 3)
+
+    def is_zero(self, cov):
+        c = _c(np.asarray(cov).reshape(9), np.float64)
+        return bool(self.L.orc_is_zero(_ptr(c, _dp)))
+
+    def eigen_sym3(self, cov):
+        c = _c(np.asarray(cov).reshape(9), np.float64)
+        ev = np.zeros(3)
+        vec = np.zeros(9)
+        info = self.L.orc_eigen_sym3(_ptr(c, _dp), _ptr(ev, _dp), _ptr(vec, _dp))
+        return info, ev, vec.reshape(3, 3).T  # columns = eigenvectors
+
+    def jacobi_sym3(self, cov):
+        c = _c(np.asarray(cov).reshape(9), np.float64)
+        ev = np.zeros(3)
+        vec = np.zeros(9)
+        self.L.orc_jacobi_sym3(_ptr(c, _dp), _ptr(ev, _dp), _ptr(vec, _dp))
+        return ev, vec.reshape(3, 3).T
+
+    # ---------------------------------------------------------- stages
+    def _cols(self, st, n, init=0.0):
+        cols = [np.full(n, init, np.float64) for _ in range(stage_columns(st))]
+        arr = (_dp * max(1, len(cols)))(*[_ptr(c, _dp) for c in cols])
+        return cols, arr
+
+    def stage_run(self, tree, st, threads=1, init=0.0):
+        n = tree.n
+        cols, arr = self._cols(st, n, init)
+        status = np.zeros(n, np.uint8)
+        self.L.orc_stage_run(tree.h, _ptr(tree.x, _dp), _ptr(tree.y, _dp),
+                             _ptr(tree.z, _dp), n, ctypes.byref(st), threads,
+                             arr, _ptr(status, _u8p))
+        return cols, status
+
+    def stage_from_lists(self, x, y, z, nbr, st, threads=8, init=0.0):
+        x, y, z = (_c(v, np.float64) for v in (x, y, z))
+        nbr = _c(nbr, np.uint32)
+        n = len(x)
+        cols, arr = self._cols(st, n, init)
+        status = np.zeros(n, np.uint8)
+        self.L.orc_stage_from_lists(_ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp), n,
+                                    _ptr(nbr, _u32p), nbr.shape[1],
+                                    ctypes.byref(st), threads, arr,
+                                    _ptr(status, _u8p))
+        return cols, status
+
+    def truth_from_lists(self, x, y, z, nbr, st, threads=8, init=np.nan):
+        x, y, z = (_c(v, np.float64) for v in (x, y, z))
+        nbr = _c(nbr, np.uint32)
+        n = len(x)
+        cols, arr = self._cols(st, n, init)
+        status = np.zeros(n, np.uint8)
+        gap = np.zeros(n, np.float64)
+        self.L.orc_truth_from_lists(_ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp), n,
+                                    _ptr(nbr, _u32p), nbr.shape[1],
+                                    ctypes.byref(st), threads, arr,
+                                    _ptr(status, _u8p), _ptr(gap, _dp))
+        return cols, status, gap
+
+
+class Ref:
+    """The reference's own nanoflann + Eigen (oracle/_ref).  Present only
+    where /root/reference was available when build() ran."""
+
+    def __init__(self, path=REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = ctypes.CDLL(path)
+        L.ref_kd3_build.restype = ctypes.c_void_p
+        L.ref_kd3_build.argtypes = [_dp, _dp, _dp, ctypes.c_uint64, ctypes.c_int]
+        L.ref_kd3_free.argtypes = [ctypes.c_void_p]
+        L.ref_kd3_neighbors.restype = ctypes.c_uint64
+        L.ref_kd3_neighbors.argtypes = [ctypes.c_void_p, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_uint64, ctypes.c_uint64, _u64p]
+        L.ref_kd3_node_count.restype = ctypes.c_uint64
+        L.ref_kd3_node_count.argtypes = [ctypes.c_void_p]
+        L.ref_kd3_export.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p, _i32p,
+                                     _dp, _dp, _i32p]
+        L.ref_eigen_sym3.restype = ctypes.c_int
+        L.ref_eigen_sym3.argtypes = [_dp, _dp, _dp]
+        L.ref_pdal_covariance.argtypes = [_dp, _dp, _dp, _u64p, ctypes.c_uint64, _dp]
+        self.L = L
+
+    class Tree:
+        def __init__(self, ref, x, y, z, threads=1):
+            self.ref = ref
+            self.x, self.y, self.z = (_c(v, np.float64) for v in (x, y, z))
+            self.n = len(self.x)
+            self.h = ref.L.ref_kd3_build(_ptr(self.x, _dp), _ptr(self.y, _dp),
+                                         _ptr(self.z, _dp), self.n, threads)
+
+        def __del__(self):
+            if getattr(self, "h", None):
+                self.ref.L.ref_kd3_free(self.h)
+                self.h = None
+
+        def neighbors(self, qx, qy, qz, k, stride=1):
+            ids = np.zeros(max(1, min(k, self.n)), np.uint64)
+            c = self.ref.L.ref_kd3_neighbors(self.h, qx, qy, qz, k, stride,
+                                             _ptr(ids, _u64p))
+            return ids[:c].copy()
+
+        def export(self):
+            m = self.ref.L.ref_kd3_node_count(self.h)
+            vind = np.zeros(self.n, np.uint64)
+            lo = np.zeros(m, np.uint64)
+            hi = np.zeros(m, np.uint64)
+            feat = np.zeros(m, np.int32)
+            dl = np.zeros(m)
+            dh = np.zeros(m)
+            leaf = np.zeros(m, np.int32)
+            self.ref.L.ref_kd3_export(self.h, _ptr(vind, _u64p), _ptr(lo, _u64p),
+                                      _ptr(hi, _u64p), _ptr(feat, _i32p),
+                                      _ptr(dl, _dp), _ptr(dh, _dp),
+                                      _ptr(leaf, _i32p))
+            return dict(vind=vind, left=lo, right=hi, divfeat=feat, divlow=dl,
+                        divhigh=dh, leaf=leaf)
+
+    def kd3(self, x, y, z, threads=1):
+        return Ref.Tree(self, x, y, z, threads)
+
+    def eigen_sym3(self, cov):
+        c = _c(np.asarray(cov).reshape(9), np.float64)
+        ev = np.zeros(3)
+        vec = np.zeros(9)
+        info = self.L.ref_eigen_sym3(_ptr(c, _dp), _ptr(ev, _dp), _ptr(vec, _dp))
+        return info, ev, vec.reshape(3, 3).T
+
+    def pdal_covariance(self, x, y, z, ids):
+        x, y, z = (_c(v, np.float64) for v in (x, y, z))
+        ids = _c(ids, np.uint64)
+        cov = np.zeros(9)
+        self.L.ref_pdal_covariance(_ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp),
+                                   _ptr(ids, _u64p), len(ids), _ptr(cov, _dp))
+        return cov.reshape(3, 3)
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
