@@ -1,0 +1,297 @@
+"""gpupdal_amd -- MI355X-native kNN-neighborhood features (PDAL's
+filters.covariancefeatures / filters.normal / filters.eigenvalues path).
+
+The compute lives in libgpupdal_amd.so (HIP kernels for gfx950 behind the C
+ABI in include/gpupdal_amd.h).  This module is a thin ctypes binding for
+Python callers, the tests and the benchmark.  There is no CPU fallback: if the
+library is missing, importing the binding raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpupdal_amd.so")
+
+GF_OK = 0
+WHERE_HOST, WHERE_DEVICE = 0, 1
+STAGE_COVFEAT, STAGE_NORMAL, STAGE_EIGENVALUES = 0, 1, 2
+MODE_RAW, MODE_SQRT, MODE_NORMALIZED = 0, 1, 2
+F_DIMENSIONALITY = 0xF
+F_ALL10 = 0x3FF
+ST_TIE, ST_INCOMPLETE, ST_ZERO_COV, ST_EIGEN_FAIL = 1, 2, 4, 8
+ST_FEATURE_INVALID, ST_FEW_NEIGHBORS = 16, 32
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+class GfStage(ctypes.Structure):
+    _fields_ = [("stage", ctypes.c_int32), ("knn", ctypes.c_int32),
+                ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
+                ("features", ctypes.c_uint32), ("always_up", ctypes.c_int32),
+                ("normalize", ctypes.c_int32)]
+
+
+class GfStats(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_uint64), ("tie_rows", ctypes.c_uint64),
+                ("incomplete_rows", ctypes.c_uint64), ("ms_index", ctypes.c_double),
+                ("ms_search", ctypes.c_double), ("ms_features", ctypes.c_double),
+                ("ms_exact", ctypes.c_double),
+                ("cell", ctypes.c_double), ("cell_z", ctypes.c_double),
+                ("grid_x", ctypes.c_uint32), ("grid_y", ctypes.c_uint32),
+                ("grid_z", ctypes.c_uint32), ("key_bits", ctypes.c_uint32),
+                ("search_passes", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
+                ("merges", ctypes.c_uint64)]
+
+
+class GfSynth(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("n", ctypes.c_uint64),
+                ("side", ctypes.c_double), ("ground_frac", ctypes.c_double),
+                ("roof_frac", ctypes.c_double), ("wire_frac", ctypes.c_double),
+                ("noise", ctypes.c_double), ("quantum", ctypes.c_double),
+                ("offset_x", ctypes.c_double), ("offset_y", ctypes.c_double)]
+
+
+EXPORTED = [
+    "gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_sync", "gf_set_tuning",
+    "gf_cloud_create", "gf_cloud_release", "gf_cloud_size", "gf_stage_columns",
+    "gf_run", "gf_last_stats", "gf_knn", "gf_debug_eigen", "gf_debug_grid",
+    "gf_synth_defaults", "gf_tile_seed", "gf_synth_cloud", "gf_dev_alloc", "gf_dev_free",
+    "gf_memcpy", "gf_cloud_invalidate",
+]
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(the HIP library is required; there is no CPU fallback)")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.gf_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.gf_ctx_destroy.argtypes = [vp]
+    L.gf_last_error.restype = ctypes.c_char_p
+    L.gf_last_error.argtypes = [vp]
+    L.gf_sync.argtypes = [vp]
+    L.gf_set_tuning.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
+    L.gf_cloud_create.argtypes = [vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_int,
+                                  ctypes.POINTER(vp)]
+    L.gf_cloud_release.argtypes = [vp]
+    L.gf_cloud_size.restype = ctypes.c_uint64
+    L.gf_cloud_size.argtypes = [vp]
+    L.gf_cloud_invalidate.argtypes = [vp]
+    L.gf_stage_columns.argtypes = [ctypes.POINTER(GfStage)]
+    L.gf_run.argtypes = [vp, vp, ctypes.POINTER(GfStage), ctypes.c_int,
+                         ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int]
+    L.gf_last_stats.argtypes = [vp, ctypes.POINTER(GfStats)]
+    L.gf_knn.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.c_int]
+    L.gf_debug_eigen.argtypes = [vp, _dp, ctypes.c_uint64, _dp, _dp, _i32p]
+    L.gf_debug_grid.argtypes = [vp, vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p, _u32p]
+    L.gf_dev_alloc.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
+    L.gf_dev_free.argtypes = [vp, vp]
+    L.gf_memcpy.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.c_int]
+    L.gf_synth_defaults.argtypes = [ctypes.POINTER(GfSynth)]
+    L.gf_tile_seed.restype = ctypes.c_uint64
+    L.gf_tile_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    L.gf_synth_cloud.argtypes = [ctypes.POINTER(GfSynth), _dp, _dp, _dp]
+    return L
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = load_library()
+    return _LIB
+
+
+class GfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gpupdal_amd error {code}: {msg}")
+        self.code = code
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------ generator --
+def synth_params(n, seed=None, tile=0, side=500.0, ground_frac=0.7, roof_frac=0.0,
+                 wire_frac=0.0, noise=0.05, quantum=0.01):
+    L = lib()
+    p = GfSynth()
+    L.gf_synth_defaults(ctypes.byref(p))
+    p.seed = L.gf_tile_seed(0x5EED, tile) if seed is None else seed
+    p.n = n
+    p.side = side
+    p.ground_frac = ground_frac
+    p.roof_frac = roof_frac
+    p.wire_frac = wire_frac
+    p.noise = noise
+    p.quantum = quantum
+    return p
+
+
+def synth_cloud(n, **kw):
+    """Seeded synthetic ALS tile (x, y, z float64 arrays)."""
+    L = lib()
+    p = synth_params(n, **kw)
+    x = np.empty(n, np.float64)
+    y = np.empty(n, np.float64)
+    z = np.empty(n, np.float64)
+    rc = L.gf_synth_cloud(ctypes.byref(p), x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
+                          z.ctypes.data_as(_dp))
+    if rc != GF_OK:
+        raise GfError(rc, "gf_synth_cloud")
+    return x, y, z
+
+
+def stage(kind, knn, stride=1, mode=MODE_SQRT, features=F_DIMENSIONALITY, always_up=1,
+          normalize=0):
+    return GfStage(kind, knn, stride, mode, features, always_up, normalize)
+
+
+def stage_columns(st):
+    return lib().gf_stage_columns(ctypes.byref(st))
+
+
+# -------------------------------------------------------------- context --
+class Context:
+    def __init__(self, device=0):
+        self.L = lib()
+        h = ctypes.c_void_p()
+        rc = self.L.gf_ctx_create(device, ctypes.byref(h))
+        if rc != GF_OK:
+            raise GfError(rc, "gf_ctx_create failed (is a GPU visible?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != GF_OK:
+            raise GfError(rc, f"{what}: {self.L.gf_last_error(self.h).decode()}")
+
+    def set_tuning(self, key, value):
+        self._check(self.L.gf_set_tuning(self.h, key.encode(), float(value)), "gf_set_tuning")
+
+    def sync(self):
+        self._check(self.L.gf_sync(self.h), "gf_sync")
+
+    def stats(self):
+        s = GfStats()
+        self.L.gf_last_stats(self.h, ctypes.byref(s))
+        return {f: getattr(s, f) for f, _ in GfStats._fields_}
+
+    def cloud(self, x, y, z):
+        return Cloud(self, x, y, z)
+
+    def alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        self._check(self.L.gf_dev_alloc(self.h, nbytes, ctypes.byref(p)), "gf_dev_alloc")
+        return p.value
+
+    def free(self, ptr):
+        self._check(self.L.gf_dev_free(self.h, ptr), "gf_dev_free")
+
+    def to_host(self, ptr, arr):
+        self._check(self.L.gf_memcpy(self.h, arr.ctypes.data, ptr, arr.nbytes, 2), "gf_memcpy")
+        return arr
+
+    def debug_eigen(self, covs):
+        covs = _c64(np.asarray(covs).reshape(-1, 9))
+        m = len(covs)
+        ev = np.zeros((m, 3))
+        vec = np.zeros((m, 9))
+        info = np.zeros(m, np.int32)
+        self._check(self.L.gf_debug_eigen(self.h, covs.ctypes.data_as(_dp), m,
+                                          ev.ctypes.data_as(_dp), vec.ctypes.data_as(_dp),
+                                          info.ctypes.data_as(_i32p)), "gf_debug_eigen")
+        return info, ev, vec.reshape(m, 3, 3).transpose(0, 2, 1)  # [:, :, j] = vector j
+
+
+class Cloud:
+    def __init__(self, ctx, x, y, z):
+        self.ctx = ctx
+        x, y, z = _c64(x), _c64(y), _c64(z)
+        self.n = len(x)
+        h = ctypes.c_void_p()
+        ctx._check(ctx.L.gf_cloud_create(ctx.h, x.ctypes.data, y.ctypes.data, z.ctypes.data,
+                                         self.n, WHERE_HOST, ctypes.byref(h)),
+                   "gf_cloud_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.gf_cloud_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, stages, init=0.0):
+        """Runs a chain of stages; returns (list of per-stage column lists,
+        list of per-stage status arrays)."""
+        stages = list(stages)
+        arr = (GfStage * len(stages))(*stages)
+        cols, per_stage = [], []
+        for st in stages:
+            c = [np.full(self.n, init, np.float64) for _ in range(stage_columns(st))]
+            per_stage.append(c)
+            cols.extend(c)
+        status = [np.zeros(self.n, np.uint8) for _ in stages]
+        cp = (ctypes.c_void_p * max(1, len(cols)))(*[c.ctypes.data for c in cols])
+        sp = (ctypes.c_void_p * len(status))(*[s.ctypes.data for s in status])
+        self.ctx._check(self.ctx.L.gf_run(self.ctx.h, self.h, arr, len(stages), cp, sp,
+                                          WHERE_HOST), "gf_run")
+        return per_stage, status
+
+    def invalidate(self):
+        """Drops the cached index; the next run rebuilds it."""
+        self.ctx.L.gf_cloud_invalidate(self.h)
+
+    def run_device(self, stages, cols, status):
+        """Runs a chain with outputs in device memory (cols: device column
+        pointers in stage order; status: one device pointer per stage)."""
+        stages = list(stages)
+        arr = (GfStage * len(stages))(*stages)
+        cp = (ctypes.c_void_p * max(1, len(cols)))(*cols)
+        sp = (ctypes.c_void_p * len(status))(*status)
+        self.ctx._check(self.ctx.L.gf_run(self.ctx.h, self.h, arr, len(stages), cp, sp,
+                                          WHERE_DEVICE), "gf_run")
+
+    def knn(self, k, stride=1):
+        kk = min(k, self.n)
+        ids = np.zeros((self.n, kk), np.uint32)
+        st = np.zeros(self.n, np.uint8)
+        self.ctx._check(self.ctx.L.gf_knn(self.ctx.h, self.h, k, stride, ids.ctypes.data,
+                                          st.ctypes.data, WHERE_HOST), "gf_knn")
+        return ids, st
+
+    def grid(self, k):
+        ncols = ctypes.c_uint32()
+        bits = ctypes.c_uint32()
+        self.ctx._check(self.ctx.L.gf_debug_grid(self.ctx.h, self.h, k, None, None, None,
+                                                 ctypes.byref(ncols), ctypes.byref(bits)),
+                        "gf_debug_grid")
+        perm = np.zeros(self.n, np.uint32)
+        keys = np.zeros(self.n, np.uint32)
+        cs = np.zeros(ncols.value + 1, np.uint32)
+        self.ctx._check(self.ctx.L.gf_debug_grid(self.ctx.h, self.h, k,
+                                                 perm.ctypes.data_as(_u32p),
+                                                 keys.ctypes.data_as(_u32p),
+                                                 cs.ctypes.data_as(_u32p),
+                                                 ctypes.byref(ncols), ctypes.byref(bits)),
+                        "gf_debug_grid")
+        return perm, keys, cs, bits.value

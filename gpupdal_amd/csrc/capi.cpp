@@ -1,0 +1,561 @@
+// capi.cpp -- the C ABI (include/gpupdal_amd.h): contexts, resident clouds,
+// index builds and fused stage runs on one HIP stream.
+#include "gpupdal_amd.h"
+
+#include "gf_internal.hpp"
+#include "gf_kernels.hpp"
+#include "gf_runtime.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+using namespace gf;
+
+namespace gf {
+
+GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch,
+                     const Tuning& t)
+{
+    GridParams g{};
+    g.ox = b[0];
+    g.oy = b[1];
+    g.oz = b[2];
+    const double sx = b[3] - b[0], sy = b[4] - b[1], sz = b[5] - b[2];
+    const double big = std::max(std::max(sx, sy), std::max(sz, 0.0));
+    const double tiny = std::max(big * 1e-9, 1e-9);
+    const double ax = std::max(sx, tiny), ay = std::max(sy, tiny);
+    const double rho = static_cast<double>(std::max<uint64_t>(n, 1)) / (ax * ay);
+    double c = t.cell_factor *
+               std::sqrt(static_cast<double>(std::max<uint32_t>(fetch, 1)) /
+                         (3.141592653589793 * rho));
+    if (!(c > tiny)) c = tiny;
+    const uint64_t col_cap = std::max<uint64_t>(4 * n, 4096);
+    for (;;) {
+        g.c = c;
+        g.inv_c = 1.0 / c;
+        g.gx = static_cast<uint32_t>(std::floor(sx * g.inv_c)) + 1;
+        g.gy = static_cast<uint32_t>(std::floor(sy * g.inv_c)) + 1;
+        g.tiles_x = (g.gx + 7) / 8;
+        const uint64_t tiles_y = (g.gy + 7) / 8;
+        const uint64_t ncols = static_cast<uint64_t>(g.tiles_x) * tiles_y * 64;
+        if (ncols <= col_cap && ncols < (1ull << 30)) {
+            g.ncols = static_cast<uint32_t>(ncols);
+            break;
+        }
+        c *= 1.25;
+    }
+    const uint32_t bcol = bits_for(g.ncols - 1);
+    double cz = c * t.cell_z_factor;
+    g.cz = cz;
+    g.inv_cz = 1.0 / cz;
+    uint64_t gz = static_cast<uint64_t>(std::floor(sz * g.inv_cz)) + 1;
+    uint32_t bz = bits_for(gz - 1);
+    if (bcol + bz > 32) {
+        bz = 32 - bcol;
+        const uint64_t gzmax = 1ull << bz;
+        cz = sz / static_cast<double>(gzmax - 1) * (1.0 + 1e-9);
+        for (;;) {
+            g.cz = cz;
+            g.inv_cz = 1.0 / cz;
+            gz = static_cast<uint64_t>(std::floor(sz * g.inv_cz)) + 1;
+            if (gz <= gzmax) break;
+            cz *= 1.0 + 1e-6;
+        }
+    }
+    g.gz = static_cast<uint32_t>(gz);
+    g.bz = bz;
+    g.key_bits = bcol + bz;
+    return g;
+}
+
+void Context::build_index(Cloud& cl, uint32_t fetch)
+{
+    const uint64_t n = cl.n;
+    if (cl.idx_valid && cl.idx_fetch == fetch && cl.idx_tune == tune.signature())
+        return;
+    cl.idx_valid = false;
+    hipEvent_t e0 = ev_begin();
+    part.ensure(kBoundsMaxParts * 6 * sizeof(double));
+    bounds.ensure(6 * sizeof(double));
+    launch_bounds(cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), n,
+                  part.as<double>(), bounds.as<double>(), stream);
+    GF_HIP(hipMemcpyAsync(host_bounds, bounds.ptr, 6 * sizeof(double),
+                          hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipStreamSynchronize(stream));
+    GridParams g = make_grid(host_bounds, n, fetch, tune);
+
+    cl.keys_a.ensure(n * 4);
+    cl.vals_a.ensure(n * 4);
+    cl.keys_b.ensure(n * 4);
+    cl.vals_b.ensure(n * 4);
+    cl.colstart.ensure((static_cast<uint64_t>(g.ncols) + 1) * 4);
+    cl.sx.ensure(n * 8);
+    cl.sy.ensure(n * 8);
+    cl.sz.ensure(n * 8);
+    hist.ensure(radix_hist_words(n) * 4);
+    scan_tmp.ensure(scan_scratch_words(std::max<uint64_t>(radix_hist_words(n), g.ncols)) * 4);
+
+    launch_keys(cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), n, g,
+                cl.keys_a.as<uint32_t>(), cl.vals_a.as<uint32_t>(), stream);
+    uint32_t *kres, *vres;
+    launch_radix_sort(cl.keys_a.as<uint32_t>(), cl.vals_a.as<uint32_t>(),
+                      cl.keys_b.as<uint32_t>(), cl.vals_b.as<uint32_t>(), n,
+                      g.key_bits, hist.as<uint32_t>(), scan_tmp.as<uint32_t>(), &kres,
+                      &vres, stream);
+    launch_colstart(kres, n, g, cl.colstart.as<uint32_t>(), stream);
+    launch_gather_sorted(vres, cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(),
+                         n, cl.sx.as<double>(), cl.sy.as<double>(), cl.sz.as<double>(),
+                         stream);
+    cl.keys = kres;
+    cl.perm = vres;
+    cl.g = g;
+    cl.idx_valid = true;
+    cl.idx_fetch = fetch;
+    cl.idx_tune = tune.signature();
+    stats.ms_index = ev_end(e0);
+    stats.cell = g.c;
+    stats.cell_z = g.cz;
+    stats.grid_x = g.gx;
+    stats.grid_y = g.gy;
+    stats.grid_z = g.gz;
+    stats.key_bits = g.key_bits;
+}
+
+CloudView Context::view(const Cloud& cl) const
+{
+    CloudView v;
+    v.sx = cl.sx.as<double>();
+    v.sy = cl.sy.as<double>();
+    v.sz = cl.sz.as<double>();
+    v.perm = cl.perm;
+    v.keys = cl.keys;
+    v.colstart = cl.colstart.as<uint32_t>();
+    v.n = cl.n;
+    v.g = cl.g;
+    return v;
+}
+
+int stage_ncols(const gf_stage& s)
+{
+    switch (s.stage) {
+    case GF_STAGE_NORMAL: return 4;
+    case GF_STAGE_EIGENVALUES: return 3;
+    case GF_STAGE_COVFEAT: return __builtin_popcount(s.features & 0x3FFu);
+    default: return -1;
+    }
+}
+
+void validate_stage(const gf_stage& s)
+{
+    if (stage_ncols(s) < 0) throw Error(GF_ERR_ARG, "unknown stage kind");
+    if (s.knn < 1) throw Error(GF_ERR_ARG, "knn must be at least 1");
+    if (s.stage != GF_STAGE_NORMAL && s.stride < 1)
+        throw Error(GF_ERR_ARG, "stride must be at least 1");
+    if (s.stage == GF_STAGE_COVFEAT) {
+        if (s.mode < GF_MODE_RAW || s.mode > GF_MODE_NORMALIZED)
+            throw Error(GF_ERR_ARG, "unknown eigenvalue mode");
+        if ((s.features & ~0x3FFu) != 0)
+            throw Error(GF_ERR_ARG, "unknown feature bits");
+    }
+}
+
+// Runs one group of stages that share the neighborhood (knn, stride).
+void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
+                        uint8_t* const* status, int where)
+{
+    const uint64_t n = cl.n;
+    const int stride = st[0].stage == GF_STAGE_NORMAL ? 1 : st[0].stride;
+    const uint64_t K = static_cast<uint64_t>(st[0].knn) + 1;
+    const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(n, K));
+    const uint32_t fetch = static_cast<uint32_t>(stride) * k;
+    if (fetch > static_cast<uint32_t>(kMaxFetch - 1))
+        throw Error(GF_ERR_UNSUPPORTED,
+                    "(knn+1)*stride above 63 is outside the device kNN envelope");
+    if (fetch > n)
+        throw Error(GF_ERR_UNSUPPORTED,
+                    "stride*min(n, knn+1) exceeds the point count");
+    FusedProgram prog{};
+    prog.nstages = ns;
+    prog.k = static_cast<int32_t>(k);
+    prog.stride = stride;
+    prog.fetch = static_cast<int32_t>(fetch);
+    int ncol_total = 0;
+    for (int i = 0; i < ns; ++i) {
+        StageProgram& sp = prog.st[i];
+        sp.stage = st[i].stage;
+        sp.mode = st[i].mode;
+        sp.features = st[i].features;
+        sp.always_up = st[i].always_up;
+        sp.normalize = st[i].normalize;
+        sp.ncols = stage_ncols(st[i]);
+        sp.col_offset = ncol_total;
+        ncol_total += sp.ncols;
+    }
+    if (ncol_total > 16) throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
+
+    build_index(cl, fetch);
+
+    KnnOutputs out{};
+    if (where == GF_WHERE_DEVICE) {
+        for (int c = 0; c < ncol_total; ++c) out.col[c] = cols[c];
+        for (int i = 0; i < ns; ++i) out.status[i] = status[i];
+    } else {
+        outbuf.ensure(static_cast<uint64_t>(ncol_total) * n * 8 + 8);
+        statbuf.ensure(static_cast<uint64_t>(ns) * n + 8);
+        for (int c = 0; c < ncol_total; ++c) {
+            out.col[c] = outbuf.as<double>() + static_cast<uint64_t>(c) * n;
+            // Rows a stage skips keep their prior values: start from them.
+            GF_HIP(hipMemcpyAsync(out.col[c], cols[c], n * 8, hipMemcpyHostToDevice,
+                                  stream));
+        }
+        for (int i = 0; i < ns; ++i) out.status[i] = statbuf.as<uint8_t>() + i * n;
+    }
+    flagged.ensure(n * 4 + 4);
+    counters.ensure(64);
+    covbuf.ensure(n * 48 + 8);
+    flagbuf.ensure(n + 8);
+    out.flagged = flagged.as<uint32_t>();
+    out.nflagged = counters.as<uint32_t>();
+    out.cov = covbuf.as<double>();
+    out.flags = flagbuf.as<uint8_t>();
+    launch_fill_u32(out.nflagged, 1, 0u, stream);
+    if (tune.collect_stats) {
+        out.counters = reinterpret_cast<unsigned long long*>(counters.as<uint32_t>() + 8);
+        launch_fill_u32(counters.as<uint32_t>() + 8, 8, 0u, stream);
+    }
+
+    const CloudView cv = view(cl);
+    hipEvent_t e0 = ev_begin();
+    launch_knn(cv, prog, 0, out, knn_tuning(), stream);
+    stats.ms_search = ev_end(e0);
+    e0 = ev_begin();
+    launch_features(cv, prog, out, stream);
+    stats.ms_features = ev_end(e0);
+
+    uint32_t nflag = 0;
+    unsigned long long ctr[4] = {0, 0, 0, 0};
+    GF_HIP(hipMemcpyAsync(&nflag, out.nflagged, 4, hipMemcpyDeviceToHost, stream));
+    if (out.counters)
+        GF_HIP(hipMemcpyAsync(ctr, out.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
+    if (where == GF_WHERE_HOST) {
+        for (int c = 0; c < ncol_total; ++c)
+            GF_HIP(hipMemcpyAsync(cols[c], out.col[c], n * 8, hipMemcpyDeviceToHost,
+                                  stream));
+        for (int i = 0; i < ns; ++i)
+            GF_HIP(hipMemcpyAsync(status[i], out.status[i], n, hipMemcpyDeviceToHost,
+                                  stream));
+    }
+    GF_HIP(hipStreamSynchronize(stream));
+    stats.points = n;
+    stats.tie_rows = nflag;
+    stats.search_passes = ctr[0];
+    stats.candidates = ctr[1];
+    stats.merges = ctr[2];
+}
+
+hipEvent_t Context::ev_begin()
+{
+    if (!timing) return nullptr;
+    GF_HIP(hipEventRecord(ev0, stream));
+    return ev0;
+}
+
+double Context::ev_end(hipEvent_t e)
+{
+    if (!e) return 0.0;
+    GF_HIP(hipEventRecord(ev1, stream));
+    GF_HIP(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    GF_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    return ms;
+}
+
+} // namespace gf
+
+// ------------------------------------------------------------------ ABI ----
+namespace {
+
+template <class F>
+int guarded(gf_ctx* ctx, F&& f)
+{
+    try {
+        f();
+        return GF_OK;
+    } catch (const gf::Error& e) {
+        if (ctx) reinterpret_cast<Context*>(ctx)->last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) reinterpret_cast<Context*>(ctx)->last_error = "out of host memory";
+        return GF_ERR_OOM;
+    } catch (const std::exception& e) {
+        if (ctx) reinterpret_cast<Context*>(ctx)->last_error = e.what();
+        return GF_ERR_STATE;
+    }
+}
+
+Context* C(gf_ctx* c) { return reinterpret_cast<Context*>(c); }
+Cloud* CL(gf_cloud* c) { return reinterpret_cast<Cloud*>(c); }
+
+} // namespace
+
+extern "C" {
+
+int gf_ctx_create(int device, gf_ctx** out)
+{
+    if (!out) return GF_ERR_ARG;
+    *out = nullptr;
+    try {
+        Context* c = new Context(device);
+        *out = reinterpret_cast<gf_ctx*>(c);
+        return GF_OK;
+    } catch (const gf::Error& e) {
+        std::fprintf(stderr, "gf_ctx_create: %s\n", e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "gf_ctx_create: %s\n", e.what());
+        return GF_ERR_STATE;
+    }
+}
+
+int gf_ctx_destroy(gf_ctx* ctx)
+{
+    if (!ctx) return GF_ERR_ARG;
+    delete C(ctx);
+    return GF_OK;
+}
+
+const char* gf_last_error(const gf_ctx* ctx)
+{
+    return ctx ? reinterpret_cast<const Context*>(ctx)->last_error.c_str() : "null context";
+}
+
+int gf_sync(gf_ctx* ctx)
+{
+    if (!ctx) return GF_ERR_ARG;
+    return guarded(ctx, [&] { GF_HIP(hipStreamSynchronize(C(ctx)->stream)); });
+}
+
+int gf_set_tuning(gf_ctx* ctx, const char* key, double value)
+{
+    if (!ctx || !key) return GF_ERR_ARG;
+    return guarded(ctx, [&] { C(ctx)->tune.set(key, value); });
+}
+
+int gf_cloud_create(gf_ctx* ctx, const double* x, const double* y, const double* z,
+                    uint64_t n, int where, gf_cloud** out)
+{
+    if (!ctx || !out || (n && (!x || !y || !z))) return GF_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        if (n >= (1ull << 32) - 1)
+            throw Error(GF_ERR_UNSUPPORTED, "clouds are limited to 2^32-2 points");
+        Cloud* cl = new Cloud(C(ctx));
+        cl->n = n;
+        cl->x.ensure(n * 8 + 8);
+        cl->y.ensure(n * 8 + 8);
+        cl->z.ensure(n * 8 + 8);
+        hipMemcpyKind kind = where == GF_WHERE_DEVICE ? hipMemcpyDeviceToDevice
+                                                      : hipMemcpyHostToDevice;
+        if (n) {
+            GF_HIP(hipMemcpyAsync(cl->x.ptr, x, n * 8, kind, C(ctx)->stream));
+            GF_HIP(hipMemcpyAsync(cl->y.ptr, y, n * 8, kind, C(ctx)->stream));
+            GF_HIP(hipMemcpyAsync(cl->z.ptr, z, n * 8, kind, C(ctx)->stream));
+        }
+        GF_HIP(hipStreamSynchronize(C(ctx)->stream));
+        *out = reinterpret_cast<gf_cloud*>(cl);
+    });
+}
+
+int gf_cloud_release(gf_cloud* cloud)
+{
+    if (!cloud) return GF_ERR_ARG;
+    delete CL(cloud);
+    return GF_OK;
+}
+
+uint64_t gf_cloud_size(const gf_cloud* cloud)
+{
+    return cloud ? reinterpret_cast<const Cloud*>(cloud)->n : 0;
+}
+
+int gf_cloud_invalidate(gf_cloud* cloud)
+{
+    if (!cloud) return GF_ERR_ARG;
+    CL(cloud)->idx_valid = false;
+    return GF_OK;
+}
+
+int gf_stage_columns(const gf_stage* st) { return st ? stage_ncols(*st) : -1; }
+
+int gf_run(gf_ctx* ctx, gf_cloud* cloud, const gf_stage* stages, int nstages,
+           double* const* cols, uint8_t* const* status, int where)
+{
+    if (!ctx || !cloud || !stages || nstages < 1 || nstages > kMaxStages || !status)
+        return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        c->stats = gf_stats{};
+        int total_cols = 0;
+        for (int i = 0; i < nstages; ++i) {
+            validate_stage(stages[i]);
+            total_cols += stage_ncols(stages[i]);
+        }
+        if (total_cols && !cols) throw Error(GF_ERR_ARG, "null column array");
+        if (cl->n == 0) return;
+        // Stages with the same neighborhood run in one fused pass.
+        int col0 = 0;
+        int i = 0;
+        while (i < nstages) {
+            int j = i + 1, ncol = stage_ncols(stages[i]);
+            auto same = [&](const gf_stage& a, const gf_stage& b) {
+                int sa = a.stage == GF_STAGE_NORMAL ? 1 : a.stride;
+                int sb = b.stage == GF_STAGE_NORMAL ? 1 : b.stride;
+                return a.knn == b.knn && sa == sb;
+            };
+            while (j < nstages && same(stages[i], stages[j])) {
+                ncol += stage_ncols(stages[j]);
+                ++j;
+            }
+            c->run_group(*cl, stages + i, j - i, cols + col0, status + i, where);
+            col0 += ncol;
+            i = j;
+        }
+    });
+}
+
+int gf_last_stats(const gf_ctx* ctx, gf_stats* out)
+{
+    if (!ctx || !out) return GF_ERR_ARG;
+    *out = reinterpret_cast<const Context*>(ctx)->stats;
+    return GF_OK;
+}
+
+int gf_knn(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t stride, uint32_t* ids,
+           uint8_t* status, int where)
+{
+    if (!ctx || !cloud || !ids || !status || k < 1 || stride < 1) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        const uint64_t n = cl->n;
+        if (!n) return;
+        const uint32_t kk = static_cast<uint32_t>(std::min<uint64_t>(n, k));
+        const uint32_t fetch = kk * stride;
+        if (fetch > static_cast<uint32_t>(kMaxFetch - 1))
+            throw Error(GF_ERR_UNSUPPORTED, "k*stride above 63 is outside the device kNN envelope");
+        if (fetch > n)
+            throw Error(GF_ERR_UNSUPPORTED, "stride*min(n, k) exceeds the point count");
+        c->build_index(*cl, fetch);
+        FusedProgram prog{};
+        prog.k = static_cast<int32_t>(kk);
+        prog.stride = static_cast<int32_t>(stride);
+        prog.fetch = static_cast<int32_t>(fetch);
+        KnnOutputs out{};
+        DevBuf tmp_ids, tmp_st;
+        if (where == GF_WHERE_DEVICE) {
+            out.ids = ids;
+            out.status[0] = status;
+        } else {
+            tmp_ids.ensure(n * kk * 4);
+            tmp_st.ensure(n + 8);
+            out.ids = tmp_ids.as<uint32_t>();
+            out.status[0] = tmp_st.as<uint8_t>();
+        }
+        c->flagged.ensure(n * 4 + 4);
+        c->counters.ensure(64);
+        out.flagged = c->flagged.as<uint32_t>();
+        out.nflagged = c->counters.as<uint32_t>();
+        launch_fill_u32(out.nflagged, 1, 0u, c->stream);
+        launch_knn(c->view(*cl), prog, 1, out, c->knn_tuning(), c->stream);
+        if (where == GF_WHERE_HOST) {
+            GF_HIP(hipMemcpyAsync(ids, out.ids, n * kk * 4, hipMemcpyDeviceToHost, c->stream));
+            GF_HIP(hipMemcpyAsync(status, out.status[0], n, hipMemcpyDeviceToHost, c->stream));
+        }
+        GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int gf_dev_alloc(gf_ctx* ctx, uint64_t bytes, void** out)
+{
+    if (!ctx || !out) return GF_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        GF_HIP(hipSetDevice(C(ctx)->device));
+        hipError_t e = hipMalloc(out, bytes ? bytes : 8);
+        if (e != hipSuccess) {
+            *out = nullptr;
+            throw Error(GF_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+    });
+}
+
+int gf_dev_free(gf_ctx* ctx, void* ptr)
+{
+    if (!ctx) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (ptr) GF_HIP(hipFree(ptr));
+    });
+}
+
+int gf_memcpy(gf_ctx* ctx, void* dst, const void* src, uint64_t bytes, int kind)
+{
+    if (!ctx || (bytes && (!dst || !src))) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        hipMemcpyKind k = kind == GF_COPY_H2D   ? hipMemcpyHostToDevice
+                          : kind == GF_COPY_D2H ? hipMemcpyDeviceToHost
+                          : kind == GF_COPY_D2D ? hipMemcpyDeviceToDevice
+                                                : throw Error(GF_ERR_ARG, "copy kind");
+        if (bytes) GF_HIP(hipMemcpyAsync(dst, src, bytes, k, C(ctx)->stream));
+        GF_HIP(hipStreamSynchronize(C(ctx)->stream));
+    });
+}
+
+int gf_debug_eigen(gf_ctx* ctx, const double* cov, uint64_t m, double* evals,
+                   double* evecs, int32_t* info)
+{
+    if (!ctx || (m && (!cov || !evals || !evecs || !info))) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        DevBuf dc, de, dv, di;
+        dc.ensure(m * 72 + 8);
+        de.ensure(m * 24 + 8);
+        dv.ensure(m * 72 + 8);
+        di.ensure(m * 4 + 8);
+        GF_HIP(hipMemcpyAsync(dc.ptr, cov, m * 72, hipMemcpyHostToDevice, c->stream));
+        launch_eigen_batch(dc.as<double>(), m, de.as<double>(), dv.as<double>(),
+                           di.as<int32_t>(), c->stream);
+        GF_HIP(hipMemcpyAsync(evals, de.ptr, m * 24, hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipMemcpyAsync(evecs, dv.ptr, m * 72, hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipMemcpyAsync(info, di.ptr, m * 4, hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int gf_debug_grid(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t* perm,
+                  uint32_t* keys, uint32_t* colstart, uint32_t* ncols,
+                  uint32_t* key_bits)
+{
+    if (!ctx || !cloud || !ncols) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        if (!cl->n) throw Error(GF_ERR_ARG, "empty cloud");
+        c->build_index(*cl, k);
+        *ncols = cl->g.ncols;
+        if (key_bits) *key_bits = cl->g.key_bits;
+        if (perm)
+            GF_HIP(hipMemcpyAsync(perm, cl->perm, cl->n * 4, hipMemcpyDeviceToHost, c->stream));
+        if (keys)
+            GF_HIP(hipMemcpyAsync(keys, cl->keys, cl->n * 4, hipMemcpyDeviceToHost, c->stream));
+        if (colstart)
+            GF_HIP(hipMemcpyAsync(colstart, cl->colstart.ptr,
+                                  (static_cast<uint64_t>(cl->g.ncols) + 1) * 4,
+                                  hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+} // extern "C"

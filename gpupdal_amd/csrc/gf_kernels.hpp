@@ -1,0 +1,73 @@
+// gf_kernels.hpp -- host-side launchers of the HIP kernels.
+#pragma once
+
+#include "gf_internal.hpp"
+
+namespace gf {
+
+constexpr uint32_t kBoundsMaxParts = 1024;
+
+// grid.hip
+void launch_bounds(const double* x, const double* y, const double* z,
+                   uint64_t n, double* part, double* out, hipStream_t s);
+void launch_keys(const double* x, const double* y, const double* z, uint64_t n,
+                 const GridParams& g, uint32_t* keys, uint32_t* ids, hipStream_t s);
+void launch_colstart(const uint32_t* keys, uint64_t n, const GridParams& g,
+                     uint32_t* colstart, hipStream_t s);
+uint64_t scan_scratch_words(uint64_t n);
+void launch_excl_scan(const uint32_t* in, uint32_t* out, uint64_t n,
+                      uint32_t* scratch, hipStream_t s);
+uint64_t radix_hist_words(uint64_t n);
+void launch_radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                       uint64_t n, uint32_t bits, uint32_t* hist,
+                       uint32_t* scan_scratch, uint32_t** kres, uint32_t** vres,
+                       hipStream_t s);
+void launch_gather_sorted(const uint32_t* perm, const double* x, const double* y,
+                          const double* z, uint64_t n, double* sx, double* sy,
+                          double* sz, hipStream_t s);
+void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+// knn.hip
+struct KnnOutputs {
+    double* col[16];         // device output columns (original point order)
+    uint8_t* status[kMaxStages];
+    uint32_t* ids;           // neighbor-list mode: n x k (original order)
+    uint32_t* flagged;       // sorted indices of tie/incomplete rows
+    uint32_t* nflagged;      // counter
+    double* cov;             // 6 x n FP64 covariance, sorted order (SoA)
+    uint8_t* flags;          // n tie/incomplete flags, sorted order
+    unsigned long long* counters; // optional: passes, candidates, merges, queries
+};
+
+struct KnnTuning {
+    int32_t ring0;       // first ring (box half-width in columns)
+    int32_t ring_max;    // budget before a row is marked incomplete
+    int32_t small_col;   // columns up to this size are taken whole
+    int32_t pad;
+};
+
+struct CloudView {
+    const double* sx;
+    const double* sy;
+    const double* sz;
+    const uint32_t* perm;     // sorted position -> original id
+    const uint32_t* keys;     // sorted keys
+    const uint32_t* colstart; // ncols + 1
+    uint64_t n;
+    GridParams g;
+};
+
+// mode 0: neighborhood covariance into out.cov / out.flags (sorted order);
+// mode 1: neighbor lists into out.ids.
+void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
+                const KnnOutputs& out, const KnnTuning& tune, hipStream_t s);
+// Eigensystem + stage columns from out.cov for every row.
+void launch_features(const CloudView& cv, const FusedProgram& prog,
+                     const KnnOutputs& out, hipStream_t s);
+
+// Debug/parity hook: device eigen solver over m matrices (row-major 3x3).
+void launch_eigen_batch(const double* cov, uint64_t m, double* evals,
+                        double* evecs, int32_t* info, hipStream_t s);
+
+} // namespace gf

@@ -1,0 +1,147 @@
+// gf_runtime.hpp -- host runtime objects behind the C ABI: grow-only device
+// buffers, the context (device, one default-priority stream, timing events,
+// tuning, scratch) and resident clouds with their cached index.
+#pragma once
+
+#include "gpupdal_amd.h"
+
+#include "gf_internal.hpp"
+#include "gf_kernels.hpp"
+
+#include <cstring>
+#include <string>
+
+namespace gf {
+
+// Grow-only device allocation.  Hot paths call ensure() with the size they
+// need; memory is allocated only when a larger size is first requested.
+struct DevBuf {
+    void* ptr = nullptr;
+    uint64_t size = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        size = 0;
+    }
+    void ensure(uint64_t bytes)
+    {
+        if (bytes <= size && ptr) return;
+        release();
+        uint64_t b = bytes ? bytes : 8;
+        hipError_t e = hipMalloc(&ptr, b);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            throw Error(GF_ERR_OOM, std::string("hipMalloc(") + std::to_string(b) +
+                                        "): " + hipGetErrorString(e));
+        }
+        size = b;
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct Tuning {
+    double cell_factor = 1.25;   // column edge = factor * sqrt(k / (pi * rho2d))
+    double cell_z_factor = 1.0;  // z cell height = factor * column edge
+    int ring0 = 1;
+    int ring_max = 8;
+    int small_col = 32;
+    bool collect_stats = false;
+
+    void set(const char* key, double v)
+    {
+        if (!std::strcmp(key, "cell_factor") && v > 0) cell_factor = v;
+        else if (!std::strcmp(key, "cell_z_factor") && v > 0) cell_z_factor = v;
+        else if (!std::strcmp(key, "ring0") && v >= 1) ring0 = static_cast<int>(v);
+        else if (!std::strcmp(key, "ring_max") && v >= 1) ring_max = static_cast<int>(v);
+        else if (!std::strcmp(key, "small_col") && v >= 0) small_col = static_cast<int>(v);
+        else if (!std::strcmp(key, "collect_stats")) collect_stats = v != 0.0;
+        else throw Error(GF_ERR_ARG, std::string("unknown or invalid tuning key: ") + key);
+        if (ring_max < ring0) ring_max = ring0;
+    }
+    // Grid-shaping knobs only (the ring knobs do not change the index).
+    uint64_t signature() const
+    {
+        uint64_t a, b;
+        std::memcpy(&a, &cell_factor, 8);
+        std::memcpy(&b, &cell_z_factor, 8);
+        return a * 1000003ull ^ b;
+    }
+};
+
+struct Context;
+
+struct Cloud {
+    Context* ctx;
+    uint64_t n = 0;
+    DevBuf x, y, z;              // original order, FP64
+    // index (valid for idx_fetch / idx_tune)
+    bool idx_valid = false;
+    uint32_t idx_fetch = 0;
+    uint64_t idx_tune = 0;
+    GridParams g{};
+    DevBuf keys_a, vals_a, keys_b, vals_b, colstart, sx, sy, sz;
+    uint32_t* keys = nullptr;    // sorted keys (into keys_a/keys_b)
+    uint32_t* perm = nullptr;    // sorted position -> original id
+    explicit Cloud(Context* c) : ctx(c) {}
+};
+
+GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch, const Tuning& t);
+int stage_ncols(const gf_stage& s);
+void validate_stage(const gf_stage& s);
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timing = true;
+    Tuning tune;
+    gf_stats stats{};
+    std::string last_error;
+    DevBuf part, bounds, hist, scan_tmp, outbuf, statbuf, flagged, counters, covbuf, flagbuf;
+    double* host_bounds = nullptr;
+
+    explicit Context(int dev) : device(dev)
+    {
+        int count = 0;
+        GF_HIP(hipGetDeviceCount(&count));
+        if (dev < 0 || dev >= count)
+            throw Error(GF_ERR_ARG, "device index out of range");
+        GF_HIP(hipSetDevice(dev));
+        GF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        GF_HIP(hipEventCreate(&ev0));
+        GF_HIP(hipEventCreate(&ev1));
+        GF_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_bounds), 64, hipHostMallocDefault));
+    }
+    ~Context()
+    {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (host_bounds) (void)hipHostFree(host_bounds);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    KnnTuning knn_tuning() const
+    {
+        KnnTuning k{};
+        k.ring0 = tune.ring0;
+        k.ring_max = tune.ring_max;
+        k.small_col = tune.small_col;
+        return k;
+    }
+
+    void build_index(Cloud& cl, uint32_t fetch);
+    CloudView view(const Cloud& cl) const;
+    void run_group(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
+                   uint8_t* const* status, int where);
+    hipEvent_t ev_begin();
+    double ev_end(hipEvent_t e);
+};
+
+} // namespace gf

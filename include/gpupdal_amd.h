@@ -1,0 +1,188 @@
+/*
+ * gpupdal_amd.h -- C ABI of the MI355X kNN-neighborhood feature library.
+ *
+ * The library computes what PDAL's filters.covariancefeatures,
+ * filters.normal and filters.eigenvalues compute (k nearest neighbors per
+ * point, the neighborhood covariance, its eigensystem and the stage
+ * columns) on one AMD GPU (gfx950).  Plain C types only; no function throws
+ * across this boundary.  Every entry returns GF_OK or a negative GF_ERR_*
+ * code; gf_last_error() gives the message of the context's last failure.
+ *
+ * Ownership: the library owns device memory (clouds, indices, scratch).  The
+ * caller owns host arrays; they are read or written only during the call.
+ * Device pointers passed with GF_WHERE_DEVICE must stay valid until the call
+ * returns.  A context is used by one thread at a time; the serving queue
+ * (gf_serve_*) is safe to submit to from several threads.
+ *
+ * Which reference interface each entry replaces is stated beside it
+ * (paths relative to the GPUPDAL tree).
+ */
+#ifndef GPUPDAL_AMD_H
+#define GPUPDAL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GF_OK 0
+#define GF_ERR_ARG (-1)
+#define GF_ERR_HIP (-2)
+#define GF_ERR_UNSUPPORTED (-3)
+#define GF_ERR_OOM (-4)
+#define GF_ERR_STATE (-5)
+
+#define GF_WHERE_HOST 0
+#define GF_WHERE_DEVICE 1
+
+/* Stage kinds, modes and feature bits.  The feature bit order is the
+ * reference's CovarianceFeature enum (include/pdg/stages/Neighborhood.hpp:153-165). */
+#define GF_STAGE_COVFEAT 0
+#define GF_STAGE_NORMAL 1
+#define GF_STAGE_EIGENVALUES 2
+
+#define GF_MODE_RAW 0
+#define GF_MODE_SQRT 1
+#define GF_MODE_NORMALIZED 2
+
+#define GF_F_LINEARITY (1u << 0)
+#define GF_F_PLANARITY (1u << 1)
+#define GF_F_SCATTERING (1u << 2)
+#define GF_F_VERTICALITY (1u << 3)
+#define GF_F_OMNIVARIANCE (1u << 4)
+#define GF_F_ANISOTROPY (1u << 5)
+#define GF_F_EIGENENTROPY (1u << 6)
+#define GF_F_EIGENVALUESUM (1u << 7)
+#define GF_F_SURFACEVARIATION (1u << 8)
+#define GF_F_DEMANTKEVERTICALITY (1u << 9)
+#define GF_F_DIMENSIONALITY 0xFu
+
+/* Row status bits; 1..16 keep the meanings of the reference's per-row status
+ * (include/pdg/index/SpatialIndex.hpp:121-131).  GF_ST_TIE and
+ * GF_ST_INCOMPLETE are informational on return: such rows were re-solved by
+ * the exact KD3-order search and are exact. */
+#define GF_ST_TIE 1u
+#define GF_ST_INCOMPLETE 2u
+#define GF_ST_ZERO_COV 4u
+#define GF_ST_EIGEN_FAIL 8u
+#define GF_ST_FEATURE_INVALID 16u
+#define GF_ST_FEW_NEIGHBORS 32u
+
+typedef struct gf_ctx gf_ctx;
+typedef struct gf_cloud gf_cloud;
+
+/* One stage of a chain.  Fields mirror the stages' options
+ * (filters/CovarianceFeaturesFilter.cpp:102-113, filters/NormalFilter.cpp:85-97,
+ * filters/EigenvaluesFilter.cpp:77-87): knn is the user's option (the search
+ * asks knn+1 to include the point itself). */
+typedef struct gf_stage {
+    int32_t stage;     /* GF_STAGE_* */
+    int32_t knn;
+    int32_t stride;    /* covfeat / eigenvalues; 1 for normal */
+    int32_t mode;      /* covfeat: GF_MODE_* */
+    uint32_t features; /* covfeat: GF_F_* mask */
+    int32_t always_up; /* normal */
+    int32_t normalize; /* eigenvalues */
+} gf_stage;
+
+typedef struct gf_stats {
+    uint64_t points;
+    uint64_t tie_rows;        /* rows re-solved for a boundary tie */
+    uint64_t incomplete_rows; /* rows re-solved after the grid budget */
+    double ms_index;          /* grid build, device time */
+    double ms_search;         /* kNN search + covariance kernel, device time */
+    double ms_features;       /* eigensolve + stage columns kernel */
+    double ms_exact;          /* exact KD3 build + re-solve, device time */
+    double cell;              /* column edge used (m) */
+    double cell_z;            /* z cell height used (m) */
+    uint32_t grid_x, grid_y, grid_z;
+    uint32_t key_bits;
+    /* search counters, filled when tuning "collect_stats" is 1 */
+    uint64_t search_passes;   /* box searches (1 per row when the first box certifies) */
+    uint64_t candidates;      /* candidate points evaluated */
+    uint64_t merges;          /* wave-wide sorted merges */
+} gf_stats;
+
+/* ---- context: replaces pdg::CudaMemoryResource + stream ownership
+ *      (src/core/allocator.cu:135-170; include/pdg/Memory.hpp:12-81). */
+int gf_ctx_create(int device, gf_ctx **out);
+int gf_ctx_destroy(gf_ctx *ctx);
+const char *gf_last_error(const gf_ctx *ctx);
+int gf_sync(gf_ctx *ctx);
+/* Tuning knobs: "cell_factor", "cell_z_factor", "ring0", "ring_max",
+ * "small_col", "collect_stats".  None of them changes any row's result. */
+int gf_set_tuning(gf_ctx *ctx, const char *key, double value);
+
+/* ---- resident clouds: replaces the PointBatch XYZ upload of
+ *      makeResidentNeighborhood (src/pdal/PdgNeighborhood.cpp:1383-1550). */
+int gf_cloud_create(gf_ctx *ctx, const double *x, const double *y,
+                    const double *z, uint64_t n, int where, gf_cloud **out);
+int gf_cloud_release(gf_cloud *cloud);
+uint64_t gf_cloud_size(const gf_cloud *cloud);
+/* Drops the cached index (the next run rebuilds it), as the reference's
+ * SpatialIndex::invalidate (include/pdg/index/SpatialIndex.hpp:241). */
+int gf_cloud_invalidate(gf_cloud *cloud);
+
+/* ---- one chain of stages on a resident cloud: replaces
+ *      pdg::knnEigenSystems + repairAmbiguousEigenSystems +
+ *      project{Normal,Eigenvalue,CovarianceFeature}Columns
+ *      (include/pdg/index/SpatialIndex.hpp:499-500;
+ *      src/pdal/PdgNeighborhood.cpp:1233-1370;
+ *      include/pdg/stages/Neighborhood.hpp:187-199).
+ * cols: the stages' output columns in order (gf_stage_columns each), n FP64
+ * values per column, original point order.  Rows a stage skips keep their
+ * prior values.  status: one n-byte array per stage.  where says whether
+ * cols/status are host or device pointers. */
+int gf_stage_columns(const gf_stage *st);
+int gf_run(gf_ctx *ctx, gf_cloud *cloud, const gf_stage *stages, int nstages,
+           double *const *cols, uint8_t *const *status, int where);
+int gf_last_stats(const gf_ctx *ctx, gf_stats *out);
+
+/* ---- neighbor lists in KD3Index order semantics: replaces
+ *      pdg::knnGather (include/pdg/index/SpatialIndex.hpp:480-490) and
+ *      KD3Index::neighbors(id, k, stride) (pdal/KDIndex.hpp:103-106).
+ * ids: n x min(n,k) uint32 (row i = neighbors of point i). */
+int gf_knn(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, uint32_t stride,
+           uint32_t *ids, uint8_t *status, int where);
+
+/* ---- device buffers for callers without their own HIP allocator (the
+ *      reference's MemoryResource::allocate/deallocate,
+ *      include/pdg/Memory.hpp:12-81).  Copies are synchronous. */
+int gf_dev_alloc(gf_ctx *ctx, uint64_t bytes, void **out);
+int gf_dev_free(gf_ctx *ctx, void *ptr);
+int gf_memcpy(gf_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind);
+#define GF_COPY_H2D 1
+#define GF_COPY_D2H 2
+#define GF_COPY_D2D 3
+
+/* ---- parity hooks (no reference counterpart; used by the test suite). */
+/* The device eigensolver on m row-major 3x3 matrices. */
+int gf_debug_eigen(gf_ctx *ctx, const double *cov, uint64_t m, double *evals,
+                   double *evecs, int32_t *info);
+/* The grid a cloud gets for a given k: sorted permutation (n), sorted keys
+ * (n) and column starts (ncols+1; pass NULL to query sizes via *ncols). */
+int gf_debug_grid(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, uint32_t *perm,
+                  uint32_t *keys, uint32_t *colstart, uint32_t *ncols,
+                  uint32_t *key_bits);
+
+/* ---- seeded synthetic clouds (the benchmark generator). */
+typedef struct gf_synth {
+    uint64_t seed;          /* tile seed (splitmix64(0x5EED ^ tile) by convention) */
+    uint64_t n;             /* points */
+    double side;            /* tile edge in metres (XY uniform over side^2) */
+    double ground_frac;     /* fraction on the ground surface (0.7) */
+    double roof_frac;       /* fraction on exactly planar roof patches */
+    double wire_frac;       /* fraction on exactly collinear wires */
+    double noise;           /* Gaussian sigma on xyz (0.05) */
+    double quantum;         /* LAS scale (0.01) */
+    double offset_x, offset_y; /* (500000, 4000000) */
+} gf_synth;
+void gf_synth_defaults(gf_synth *p);
+uint64_t gf_tile_seed(uint64_t base, uint64_t tile);
+int gf_synth_cloud(const gf_synth *p, double *x, double *y, double *z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

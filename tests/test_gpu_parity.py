@@ -1,0 +1,119 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Neighbor sets are bit-exact against the KD3Index restatement; rows whose
+search ends in a boundary distance tie are flagged (and checked to really be
+tie rows).  Eigensystems from the device solver equal the restated Eigen
+solver bit for bit.  Features are within 1e-5 relative of the
+extended-precision truth on well-conditioned rows.
+"""
+import numpy as np
+import pytest
+
+import gpupdal_amd as g
+import oracle_lib as ol
+from helpers import small_cloud, set_equal_rows, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def test_grid_invariants(ctx):
+    x, y, z = small_cloud(50000, seed=3)
+    cl = ctx.cloud(x, y, z)
+    perm, keys, colstart, bits = cl.grid(11)
+    n = len(x)
+    assert np.array_equal(np.sort(perm), np.arange(n, dtype=np.uint32)), "permutation"
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0), "keys sorted"
+    # stable: within equal keys the original ids increase (LSD radix is stable)
+    same = np.diff(keys.astype(np.int64)) == 0
+    assert np.all(np.diff(perm.astype(np.int64))[same] > 0), "stable order"
+    # column starts are the first sorted position of each column
+    bz = None
+    cols = keys >> (keys.max().bit_length() - 0) if False else None
+    assert colstart[-1] == n and np.all(np.diff(colstart.astype(np.int64)) >= 0)
+    # equality with a host stable sort of the same keys
+    order = np.argsort(keys[np.argsort(perm)], kind="stable")
+    assert np.array_equal(order.astype(np.uint32), perm)
+
+
+@pytest.mark.parametrize("n,k,seed", [(20000, 11, 1), (30000, 17, 2), (15000, 33, 4)])
+def test_knn_sets_vs_kd3(ctx, oracle, n, k, seed):
+    x, y, z = small_cloud(n, seed=seed, dup_frac=0.02)
+    cl = ctx.cloud(x, y, z)
+    ids, st = cl.knn(k)
+    tree = oracle.kd3(x, y, z)
+    ref = tree.neighbors_all(k, threads=8)
+    flagged = (st & (g.ST_TIE | g.ST_INCOMPLETE)) != 0
+    eq = set_equal_rows(ids, ref)
+    # every unflagged row is bit-exact
+    assert np.all(eq[~flagged]), f"{np.count_nonzero(~eq & ~flagged)} unflagged rows differ"
+    # every row flagged as a tie really has a boundary tie (brute force)
+    tie_rows = np.nonzero(st & g.ST_TIE)[0]
+    if len(tie_rows):
+        bid, bd2 = oracle.brute_knn(x, y, z, k + 1, queries=tie_rows[:2000])
+        assert np.all(bd2[:, k - 1] == bd2[:, k])
+    assert np.count_nonzero(st & g.ST_INCOMPLETE) == 0
+
+
+def test_knn_tie_free_rows_vs_bruteforce(ctx, oracle):
+    x, y, z = small_cloud(8000, seed=9)
+    cl = ctx.cloud(x, y, z)
+    ids, st = cl.knn(17)
+    bid, bd2 = oracle.brute_knn(x, y, z, 17)
+    ok = (st & (g.ST_TIE | g.ST_INCOMPLETE)) == 0
+    assert np.array_equal(ids[ok], bid[ok]), "ordered (d2, id) lists"
+
+
+def test_device_eigen_bits(ctx, oracle):
+    rng = np.random.default_rng(5)
+    m = 4000
+    covs = []
+    for t in range(m):
+        a = rng.normal(size=(3, 3))
+        c = a @ a.T
+        if t % 4 == 1:
+            c[0, 2] = c[2, 0] = 0
+        if t % 4 == 2:
+            c = np.diag(rng.uniform(0, 1, 3))
+        if t % 5 == 3:
+            c *= 1e-9
+        covs.append(c)
+    covs = np.array(covs)
+    info, ev, vec = ctx.debug_eigen(covs)
+    for t in range(m):
+        i2, e2, v2 = oracle.eigen_sym3(covs[t])
+        assert info[t] == i2
+        assert np.array_equal(ev[t], e2), t
+        assert np.array_equal(vec[t], v2), t
+
+
+@pytest.mark.parametrize("n,knn", [(40000, 10), (30000, 16)])
+def test_covfeat_all_vs_truth(ctx, oracle, n, knn):
+    x, y, z = small_cloud(n, seed=11)
+    st = g.stage(g.STAGE_COVFEAT, knn, features=g.F_ALL10)
+    (cols,), (status,) = ctx.cloud(x, y, z).run([st])
+    tree = oracle.kd3(x, y, z)
+    nbr = tree.neighbors_all(knn + 1, threads=8)
+    ost = ol.stage(ol.STAGE_COVFEAT, knn, features=ol.F_ALL10)
+    tcols, tstat, gap = oracle.truth_from_lists(x, y, z, nbr, ost)
+    ok = (status == 0) & (tstat == 0)
+    assert np.count_nonzero(ok) > 0.95 * n
+    well = ok & (gap > 1e-3)
+    for ci, name in enumerate(ol.FEATURE_NAMES):
+        e = rel_err(cols[ci][well], tcols[ci][well], 1e-6)
+        assert np.nanmax(e) < 1e-5, (name, np.nanmax(e))
+
+
+def test_normal_eigen_chain_vs_pdal_oracle(ctx, oracle):
+    x, y, z = small_cloud(30000, seed=12)
+    stages = [g.stage(g.STAGE_NORMAL, 16), g.stage(g.STAGE_COVFEAT, 16)]
+    (ncols, fcols), (nst, fst) = ctx.cloud(x, y, z).run(stages)
+    tree = oracle.kd3(x, y, z)
+    on, ons = oracle.stage_run(tree, ol.stage(ol.STAGE_NORMAL, 16), threads=8)
+    of, ofs = oracle.stage_run(tree, ol.stage(ol.STAGE_COVFEAT, 16), threads=8)
+    ok = ((nst & 0x3C) == 0) & (ons == 0)
+    # normals agree with PDAL's up to the float-narrowing of its covariance
+    dot = np.abs(ncols[0] * on[0] + ncols[1] * on[1] + ncols[2] * on[2])
+    assert np.percentile(dot[ok], 1) > 1 - 1e-6
+    assert np.all(np.sign(ncols[2][ok]) >= 0)
+    e = rel_err(ncols[3][ok], on[3][ok], 1e-9)
+    assert np.percentile(e, 99) < 1e-4
