@@ -44,7 +44,7 @@ struct KnnTuning {
     int32_t ring0;       // first ring (box half-width in columns)
     int32_t ring_max;    // budget before a row is marked incomplete
     int32_t small_col;   // columns up to this size are taken whole
-    int32_t pad;
+    int32_t group_z;     // z cells beyond the first a query group may span
 };
 
 struct CloudView {

@@ -49,7 +49,8 @@ struct Tuning {
     double cell_z_factor = 1.0;  // z cell height = factor * column edge
     int ring0 = 1;
     int ring_max = 8;
-    int small_col = 32;
+    int small_col = 0;
+    int group_z = 1;
     bool collect_stats = false;
 
     void set(const char* key, double v)
@@ -59,6 +60,7 @@ struct Tuning {
         else if (!std::strcmp(key, "ring0") && v >= 1) ring0 = static_cast<int>(v);
         else if (!std::strcmp(key, "ring_max") && v >= 1) ring_max = static_cast<int>(v);
         else if (!std::strcmp(key, "small_col") && v >= 0) small_col = static_cast<int>(v);
+        else if (!std::strcmp(key, "group_z") && v >= 0) group_z = static_cast<int>(v);
         else if (!std::strcmp(key, "collect_stats")) collect_stats = v != 0.0;
         else throw Error(GF_ERR_ARG, std::string("unknown or invalid tuning key: ") + key);
         if (ring_max < ring0) ring_max = ring0;
@@ -133,6 +135,7 @@ struct Context {
         k.ring0 = tune.ring0;
         k.ring_max = tune.ring_max;
         k.small_col = tune.small_col;
+        k.group_z = tune.group_z;
         return k;
     }
 

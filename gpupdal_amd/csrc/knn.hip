@@ -1,29 +1,37 @@
 // knn.hip -- wavefront-per-query k-nearest-neighbor search over the column
-// grid, fused with the neighborhood covariance, the eigensolve and the stage
-// outputs.
+// grid, fused with the neighborhood covariance; plus the per-row eigensolve
+// and stage-output kernel.
 //
 // Replaces the reference's thread-per-query knnGatherKernel with its fused
 // covariance, the device eigen kernel and the projection kernels
 // (src/index/SpatialIndexKernels.cu:716-1086, 1716-1769;
 // src/stages/NeighborhoodKernels.cu:67-119, 422-485).
 //
-// Per query, the 64 lanes of one wave evaluate the candidates of a box of
-// (2r+1)^2 columns x (2r+1) z cells around the query's cell in parallel,
-// with FP64 squared distances accumulated exactly as KD3Index does
-// (d0*d0 + d1*d1 + d2*d2, left to right, no FMA).  Survivors are merged into
-// a wave-wide list of the 64 smallest (d2, point id) pairs by a bitonic
-// network over lanes.  The search is certified when the fetch-th distance is
-// below the distance from the query to the nearest face of the box that
-// borders unexamined space (minus a rounding margin); otherwise the box grows
-// to the radius the current fetch-th distance demands, up to a budget.
-// Rows whose fetch-th and (fetch+1)-th distances are equal have a boundary
-// tie, which KD3Index breaks by tree visit order rather than by index; they
-// and uncertified rows are flagged for the exact KD3 search.
+// Work split.  A wave owns Q consecutive sorted points (queries).  Sorted
+// order is (column, z cell), so consecutive queries form groups that share a
+// column and a short z range; every query of a group has the same candidate
+// box (the 3x3 columns around it, z cells [first-1, last+1]).  The wave loads
+// the box once -- column starts, z-window bounds, the candidates' FP64
+// coordinates and ids -- into registers (up to 256 candidates, 4 per lane),
+// then for each query of the group computes the FP64 squared distances
+// exactly as KD3Index does (d0*d0 + d1*d1 + d2*d2, left to right, no FMA)
+// and selects the fetch+1 smallest (d2, point id) pairs:  a bisection on d2
+// counted with ballots finds a threshold that keeps between fetch+1 and 64
+// candidates, which are compacted through LDS and sorted once by a bitonic
+// network across the lanes.
 //
-// Each wave runs Q queries back to back and parks each neighbor list in LDS;
-// then every lane takes one of those queries and computes its FP64
-// covariance (two-pass mean), the eigensystem and all requested stage
-// columns, so the per-row solve runs on all lanes.
+// Certification.  A query's list is exact when its fetch-th distance is below
+// the distance from the query to the nearest face of the examined box that
+// borders unexamined grid cells (minus a rounding margin).  Otherwise the
+// query alone is searched again with a growing box (streaming candidates
+// through sort-and-merge) up to a budget; a query still uncertified is
+// flagged incomplete.  Rows whose fetch-th and (fetch+1)-th distances are
+// equal have a boundary tie, which KD3Index breaks by tree visit order rather
+// than by index; they are flagged for the exact KD3 search.
+//
+// Covariance.  Each query's kept list is parked in LDS; after the wave's
+// queries every lane takes one of them and forms its FP64 covariance
+// (two-pass mean), written in sorted order for the eigen/feature kernel.
 #include "gf_eigen.hpp"
 #include "gf_internal.hpp"
 #include "gf_kernels.hpp"
@@ -36,6 +44,11 @@ constexpr uint64_t kInfD = ~0ull;
 constexpr uint32_t kInfU = ~0u;
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
+constexpr int kRegBatches = 4;
+constexpr uint32_t kRegCand = kRegBatches * 64;
+// Per-wave LDS scratch: 64 compacted entries (16 B), 64 ranked entries, and a
+// 64-entry list of queries left for the growing search.
+constexpr int kScratchWords = 256 + 256 + 64;
 
 struct Ent {
     uint64_t d;   // bit pattern of the non-negative FP64 squared distance
@@ -48,6 +61,11 @@ __device__ __forceinline__ Ent ent_inf() { return Ent{kInfD, kInfU, kInfU}; }
 __device__ __forceinline__ bool ent_less(const Ent& a, const Ent& b)
 {
     return a.d < b.d || (a.d == b.d && a.id < b.id);
+}
+
+__device__ __forceinline__ double ent_d2(const Ent& e)
+{
+    return __longlong_as_double(static_cast<long long>(e.d));
 }
 
 __device__ __forceinline__ Ent ent_shfl_xor(const Ent& e, int m)
@@ -143,77 +161,276 @@ struct KnnArgs {
     double margin;
 };
 
-// The k-NN search of one query by the whole wave.  On return lane l holds the
-// l-th smallest (d2, id) pair (64 kept); *incomplete tells whether the
-// budget ran out before certification.
-__device__ void wave_knn(const KnnArgs& a, double qx, double qy, double qz,
-                         uint32_t qcx, uint32_t qcy, uint32_t qcz, int lane,
-                         Ent& best, bool& incomplete, uint64_t* cnt)
+// The examined region: columns [x0, x1] x [y0, y1], z cells [z0, z1].
+struct Box {
+    int x0, x1, y0, y1, z0, z1;
+};
+
+// Distance from the query to the nearest face of the box that borders grid
+// cells outside it (faces on the grid's edge bound nothing: +inf).
+__device__ __forceinline__ double box_clearance(const GridParams& g, const Box& b,
+                                                double qx, double qy, double qz)
+{
+    double R = INFINITY;
+    if (b.x0 > 0) R = fmin(R, qx - (g.ox + static_cast<double>(b.x0) * g.c));
+    if (b.x1 < static_cast<int>(g.gx) - 1)
+        R = fmin(R, (g.ox + static_cast<double>(b.x1 + 1) * g.c) - qx);
+    if (b.y0 > 0) R = fmin(R, qy - (g.oy + static_cast<double>(b.y0) * g.c));
+    if (b.y1 < static_cast<int>(g.gy) - 1)
+        R = fmin(R, (g.oy + static_cast<double>(b.y1 + 1) * g.c) - qy);
+    if (b.z0 > 0) R = fmin(R, qz - (g.oz + static_cast<double>(b.z0) * g.cz));
+    if (b.z1 < static_cast<int>(g.gz) - 1)
+        R = fmin(R, (g.oz + static_cast<double>(b.z1 + 1) * g.cz) - qz);
+    return R;
+}
+
+__device__ __forceinline__ bool certified(const KnnArgs& a, const Box& b, const Ent& kth,
+                                          double qx, double qy, double qz)
+{
+    const double R = box_clearance(a.cv.g, b, qx, qy, qz);
+    if (R == INFINITY) return true;
+    if (kth.d == kInfD) return false;
+    const double Rm = R - a.margin;
+    return Rm > 0.0 && ent_d2(kth) < Rm * Rm;
+}
+
+// Column ranges of a box, one column per lane (box columns must be <= 64):
+// lo / len of the column's sorted points restricted to z cells [z0, z1]
+// (whole columns when the box spans every z cell or the column is small).
+__device__ __forceinline__ void box_ranges(const KnnArgs& a, const Box& b, int cb,
+                                           int lane, uint32_t& lo, uint32_t& len)
 {
     const CloudView& cv = a.cv;
     const GridParams& g = cv.g;
+    const int w = b.x1 - b.x0 + 1;
+    const int ncb = w * (b.y1 - b.y0 + 1);
+    const int ci = cb + lane;
+    lo = 0;
+    len = 0;
+    if (ci >= ncb) return;
+    const int cx = b.x0 + ci % w;
+    const int cy = b.y0 + ci / w;
+    if (cx < 0 || cy < 0 || cx >= static_cast<int>(g.gx) || cy >= static_cast<int>(g.gy))
+        return;
+    const bool zfull = b.z0 <= 0 && b.z1 >= static_cast<int>(g.gz) - 1;
+    const uint32_t col = column_index(g, cx, cy);
+    uint32_t b0 = cv.colstart[col], b1 = cv.colstart[col + 1];
+    if (!zfull && b1 - b0 > static_cast<uint32_t>(a.tune.small_col)) {
+        const uint32_t klo = (col << g.bz) + static_cast<uint32_t>(max(b.z0, 0));
+        const uint32_t khi = (col << g.bz) + static_cast<uint32_t>(min(b.z1, static_cast<int>(g.gz) - 1) + 1);
+        b0 = lower_bound_u32(cv.keys, b0, b1, klo);
+        b1 = lower_bound_u32(cv.keys, b0, b1, khi);
+    }
+    lo = b0;
+    len = b1 - b0;
+}
+
+// Sorted position of candidate t of the packed ranges: the lane whose range
+// holds t is the last lane with excl <= t.
+__device__ __forceinline__ uint32_t candidate_pos(uint32_t lo, uint32_t excl, uint32_t t)
+{
+    int src = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        int p = src + step;
+        uint32_t e = __shfl(excl, p & 63);
+        if (p < 64 && e <= t) src = p;
+    }
+    const uint32_t slo = __shfl(lo, src);
+    const uint32_t sex = __shfl(excl, src);
+    return slo + (t - sex);
+}
+
+__device__ __forceinline__ Ent make_ent(double qx, double qy, double qz, double px,
+                                        double py, double pz, uint32_t id, uint32_t j)
+{
+    const double dx = qx - px;
+    const double dy = qy - py;
+    const double dz = qz - pz;
+    double d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    return Ent{static_cast<uint64_t>(__double_as_longlong(d2)), id, j};
+}
+
+// Selection over at most 256 candidate entries held in registers (4 per
+// lane).  First a threshold E with need <= #{d2 <= E} <= 64 (need = fetch + 1,
+// so the (fetch+1)-th entry that decides ties is kept): the caller's hint (an
+// upper bound on the need-th distance, from the previous query of the group by
+// the triangle inequality) is tried first, then a bisection on d2 whose
+// counts are ballots.  The survivors are compacted into LDS and ranked: each
+// lane counts the survivors that precede its own entry in (d2, id) order,
+// reading them as LDS broadcasts, and stores itself at that rank.  Every
+// candidate with d2 equal to the boundary distance is below E, so the first
+// fetch+1 ranked survivors are exactly the fetch+1 smallest (d2, id) pairs.
+// When no such E exists (more than 64 points at one distance) the register
+// batches are merged one by one with bitonic networks instead.
+__device__ __forceinline__ void select_entries(Ent (&c)[kRegBatches], uint32_t tot, int fetch, int lane,
+                               uint32_t* scratch, double hint, Ent& best, uint64_t* cnt)
+{
+    const uint32_t need = min(static_cast<uint32_t>(fetch + 1), tot);
+    // Survivor window: few enough to rank cheaply, at most one per lane.
+    const uint32_t cap = min(need + 15u, 64u);
+    bool ok = true;
+    double E = INFINITY;
+    cnt[2] += 1;
+    if (tot > cap) {
+        auto count_le = [&](double t) {
+            uint32_t k = 0;
+#pragma unroll
+            for (int b = 0; b < kRegBatches; ++b)
+                k += __popcll(__ballot(c[b].d != kInfD && ent_d2(c[b]) <= t));
+            return k;
+        };
+        double lo_e = 0.0, hi_e = INFINITY;
+        ok = false;
+        if (hint < INFINITY) {
+            const uint32_t k = count_le(hint);
+            if (k >= need && k <= cap) {
+                E = hint;
+                ok = true;
+            } else if (k > cap) {
+                hi_e = hint;
+            } else {
+                lo_e = hint;
+            }
+        }
+        if (!ok && hi_e == INFINITY) {
+            double mx = 0.0;
+#pragma unroll
+            for (int b = 0; b < kRegBatches; ++b)
+                if (c[b].d != kInfD) mx = fmax(mx, ent_d2(c[b]));
+#pragma unroll
+            for (int m = 32; m > 0; m >>= 1) mx = fmax(mx, __shfl_xor(mx, m));
+            hi_e = mx;  // counts every candidate (> cap)
+        }
+        for (int it = 0; it < 80 && !ok; ++it) {
+            const double mid = 0.5 * (lo_e + hi_e);
+            const uint32_t k = count_le(mid);
+            if (k < need) {
+                if (!(mid > lo_e)) break;
+                lo_e = mid;
+            } else if (k > cap) {
+                if (!(mid < hi_e)) break;
+                hi_e = mid;
+            } else {
+                E = mid;
+                ok = true;
+            }
+        }
+    }
+    if (ok) {
+        const uint64_t lt = (1ull << lane) - 1ull;
+        uint32_t S = 0;
+#pragma unroll
+        for (int b = 0; b < kRegBatches; ++b) {
+            const bool s = c[b].d != kInfD && ent_d2(c[b]) <= E;
+            const uint64_t m = __ballot(s);
+            if (s) {
+                uint32_t* e = scratch + 4 * (S + __popcll(m & lt));
+                e[0] = static_cast<uint32_t>(c[b].d);
+                e[1] = static_cast<uint32_t>(c[b].d >> 32);
+                e[2] = c[b].id;
+                e[3] = c[b].j;
+            }
+            S += __popcll(m);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        Ent v = ent_inf();
+        uint32_t rank = 0;
+        if (static_cast<uint32_t>(lane) < S) {
+            const uint32_t* e = scratch + 4 * lane;
+            v.d = static_cast<uint64_t>(e[0]) | (static_cast<uint64_t>(e[1]) << 32);
+            v.id = e[2];
+            v.j = e[3];
+        }
+        const double vd = ent_d2(v);
+#pragma unroll 4
+        for (uint32_t i = 0; i < S; ++i) {
+            const uint4 e = reinterpret_cast<const uint4*>(scratch)[i];
+            const double od = __longlong_as_double(
+                static_cast<long long>(static_cast<uint64_t>(e.x) | (static_cast<uint64_t>(e.y) << 32)));
+            rank += (od < vd || (od == vd && e.z < v.id)) ? 1u : 0u;
+        }
+        uint32_t* sorted = scratch + 256;
+        if (static_cast<uint32_t>(lane) < S) {
+            uint32_t* e = sorted + 4 * rank;
+            e[0] = static_cast<uint32_t>(v.d);
+            e[1] = static_cast<uint32_t>(v.d >> 32);
+            e[2] = v.id;
+            e[3] = v.j;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        best = ent_inf();
+        if (static_cast<uint32_t>(lane) < S) {
+            const uint32_t* e = sorted + 4 * lane;
+            best.d = static_cast<uint64_t>(e[0]) | (static_cast<uint64_t>(e[1]) << 32);
+            best.id = e[2];
+            best.j = e[3];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        return;
+    }
+    best = ent_inf();
+#pragma unroll
+    for (int b = 0; b < kRegBatches; ++b) {
+        if (static_cast<uint32_t>(b * 64) >= tot) break;
+        bitonic_sort64(c[b], lane);
+        merge_sorted(best, c[b], lane);
+        cnt[2] += 1;
+    }
+}
+
+// One query alone, with a growing box starting at half-width r (columns and
+// z cells around the query's cell): candidates stream through the registers
+// (<= 256, one box of <= 64 columns) or through sort-and-merge otherwise.
+__device__ __forceinline__ void search_single(const KnnArgs& a, double qx, double qy, double qz,
+                              int qcx, int qcy, int qcz, int r, int lane,
+                              uint32_t* scratch, Ent& best, bool& incomplete,
+                              uint64_t* cnt)
+{
+    const CloudView& cv = a.cv;
     const int fetch = a.prog.fetch;
-    int r = a.tune.ring0;
     incomplete = false;
     for (;;) {
-        best = ent_inf();
         cnt[0] += 1;
-        const int span = 2 * r + 1;
-        const int ncb = span * span;
-        const int za = max(static_cast<int>(qcz) - r, 0);
-        const int zb = min(static_cast<int>(qcz) + r, static_cast<int>(g.gz) - 1);
-        const bool zfull = (za == 0) && (zb == static_cast<int>(g.gz) - 1);
+        const Box b{qcx - r, qcx + r, qcy - r, qcy + r, qcz - r, qcz + r};
+        const int ncb = (2 * r + 1) * (2 * r + 1);
+        best = ent_inf();
         for (int cb = 0; cb < ncb; cb += 64) {
-            const int ci = cb + lane;
-            uint32_t lo = 0, len = 0;
-            if (ci < ncb) {
-                int cx = static_cast<int>(qcx) + ci % span - r;
-                int cy = static_cast<int>(qcy) + ci / span - r;
-                if (cx >= 0 && cy >= 0 && cx < static_cast<int>(g.gx) &&
-                    cy < static_cast<int>(g.gy)) {
-                    uint32_t col = column_index(g, cx, cy);
-                    uint32_t b0 = cv.colstart[col], b1 = cv.colstart[col + 1];
-                    if (!zfull && b1 - b0 > static_cast<uint32_t>(a.tune.small_col)) {
-                        uint32_t klo = (col << g.bz) + static_cast<uint32_t>(za);
-                        uint32_t khi = (col << g.bz) + static_cast<uint32_t>(zb + 1);
-                        b0 = lower_bound_u32(cv.keys, b0, b1, klo);
-                        b1 = lower_bound_u32(cv.keys, b0, b1, khi);
-                    }
-                    lo = b0;
-                    len = b1 - b0;
-                }
-            }
+            uint32_t lo, len;
+            box_ranges(a, b, cb, lane, lo, len);
             const uint32_t incl = wave_incl_scan_u32(len, lane);
             const uint32_t tot = __shfl(incl, 63);
             const uint32_t excl = incl - len;
             cnt[1] += tot;
+            if (ncb <= 64 && tot <= kRegCand) {
+                Ent c[kRegBatches];
+#pragma unroll
+                for (int bb = 0; bb < kRegBatches; ++bb) {
+                    c[bb] = ent_inf();
+                    if (static_cast<uint32_t>(bb * 64) < tot) {
+                        const uint32_t t = bb * 64 + lane;
+                        const uint32_t j = candidate_pos(lo, excl, t);
+                        if (t < tot)
+                            c[bb] = make_ent(qx, qy, qz, cv.sx[j], cv.sy[j], cv.sz[j],
+                                             cv.perm[j], j);
+                    }
+                }
+                select_entries(c, tot, fetch, lane, scratch, INFINITY, best, cnt);
+                break;
+            }
             for (uint32_t t0 = 0; t0 < tot; t0 += 64) {
                 const uint32_t t = t0 + lane;
-                const bool valid = t < tot;
-                int src = 0;
-#pragma unroll
-                for (int step = 32; step > 0; step >>= 1) {
-                    int p = src + step;
-                    uint32_t e = __shfl(excl, p & 63);
-                    if (p < 64 && e <= t) src = p;
-                }
-                const uint32_t slo = __shfl(lo, src);
-                const uint32_t sex = __shfl(excl, src);
+                const uint32_t j = candidate_pos(lo, excl, t);
                 Ent c = ent_inf();
-                if (valid) {
-                    const uint32_t j = slo + (t - sex);
-                    const double dx = qx - cv.sx[j];
-                    const double dy = qy - cv.sy[j];
-                    const double dz = qz - cv.sz[j];
-                    double d2 = dx * dx;
-                    d2 = d2 + dy * dy;
-                    d2 = d2 + dz * dz;
-                    c.d = static_cast<uint64_t>(__double_as_longlong(d2));
-                    c.id = cv.perm[j];
-                    c.j = j;
-                }
+                if (t < tot)
+                    c = make_ent(qx, qy, qz, cv.sx[j], cv.sy[j], cv.sz[j], cv.perm[j], j);
                 const Ent worst = ent_shfl(best, fetch);
-                const bool surv = valid && ent_less(c, worst);
+                const bool surv = t < tot && ent_less(c, worst);
                 if (__ballot(surv)) {
                     cnt[2] += 1;
                     if (!surv) c = ent_inf();
@@ -222,41 +439,15 @@ __device__ void wave_knn(const KnnArgs& a, double qx, double qy, double qz,
                 }
             }
         }
-        // Certification against the faces of the box that border cells of
-        // the grid (faces on or beyond the grid's edge bound nothing).
         const Ent kth = ent_shfl(best, fetch - 1);
-        double R = INFINITY;
-        if (static_cast<int>(qcx) - r > 0)
-            R = fmin(R, qx - (g.ox + static_cast<double>(static_cast<int>(qcx) - r) * g.c));
-        if (static_cast<int>(qcx) + r < static_cast<int>(g.gx) - 1)
-            R = fmin(R, (g.ox + static_cast<double>(static_cast<int>(qcx) + r + 1) * g.c) - qx);
-        if (static_cast<int>(qcy) - r > 0)
-            R = fmin(R, qy - (g.oy + static_cast<double>(static_cast<int>(qcy) - r) * g.c));
-        if (static_cast<int>(qcy) + r < static_cast<int>(g.gy) - 1)
-            R = fmin(R, (g.oy + static_cast<double>(static_cast<int>(qcy) + r + 1) * g.c) - qy);
-        if (za > 0)
-            R = fmin(R, qz - (g.oz + static_cast<double>(za) * g.cz));
-        if (zb < static_cast<int>(g.gz) - 1)
-            R = fmin(R, (g.oz + static_cast<double>(zb + 1) * g.cz) - qz);
-        bool certified;
-        double kd2 = __longlong_as_double(static_cast<long long>(kth.d));
-        if (R == INFINITY) {
-            certified = true;
-        } else if (kth.d != kInfD) {
-            double Rm = R - a.margin;
-            certified = Rm > 0.0 && kd2 < Rm * Rm;
-        } else {
-            certified = false;
-        }
-        if (certified) return;
+        if (certified(a, b, kth, qx, qy, qz)) return;
         if (r >= a.tune.ring_max) {
             incomplete = true;
             return;
         }
         int rn = r + 1;
         if (kth.d != kInfD) {
-            double need = sqrt(kd2) + a.margin;
-            double rr = ceil(need / g.c);
+            const double rr = ceil((sqrt(ent_d2(kth)) + a.margin) / cv.g.c);
             if (rr > rn) rn = rr > 1e6 ? 1000000 : static_cast<int>(rr);
         } else {
             rn = max(rn, 2 * r);
@@ -265,12 +456,49 @@ __device__ void wave_knn(const KnnArgs& a, double qx, double qy, double qz,
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_knn(KnnArgs a)
+// Tie test and hand-off of one query's result: in neighbor-list mode the kept
+// ids go straight to global memory; in feature mode the kept sorted positions
+// are parked in LDS for the covariance pass.
+__device__ __forceinline__ void finish_query(const KnnArgs& a, const Ent& best,
+                                             bool incomplete, uint64_t s, int qslot,
+                                             int lane, uint32_t* nbr, uint8_t& my_flags)
 {
-    extern __shared__ uint32_t s_nbr[];
+    const int fetch = a.prog.fetch;
+    const int k = a.prog.k;
+    const int stride = a.prog.stride;
+    // Boundary tie: the fetch-th and (fetch+1)-th distances are equal.  With
+    // stride > 1 every adjacent equal pair inside the fetched list can change
+    // which ids are kept, so any such pair flags the row.
+    const Ent nxt = ent_shfl(best, lane + 1 < 64 ? lane + 1 : 63);
+    const bool eq = (lane < fetch) && (lane + 1 < 64) && best.d == nxt.d && best.d != kInfD;
+    const bool tie = stride > 1 ? (__ballot(eq) != 0)
+                                : (__shfl(static_cast<int>(eq), fetch - 1) != 0);
+    const uint8_t flags = (tie ? ST_TIE : 0) | (incomplete ? ST_INCOMPLETE : 0);
+    // Kept list in canonical (d2, id) order: entries 0, stride, 2*stride...
+    const Ent kept = ent_shfl(best, (lane * stride) & 63);
+    if (a.mode == 1) {
+        const uint32_t qid = a.cv.perm[s];
+        if (lane < k) a.out.ids[static_cast<uint64_t>(qid) * k + lane] = kept.id;
+        if (lane == 0) a.out.status[0][qid] = flags;
+        if (lane == 0 && flags) {
+            uint32_t slot = atomicAdd(a.out.nflagged, 1u);
+            a.out.flagged[slot] = static_cast<uint32_t>(s);
+        }
+    } else {
+        // An uncertified row can hold fewer than fetch entries; keep its list
+        // in bounds (the exact search replaces the row).
+        if (lane < k) nbr[qslot * a.lds_stride + lane] = kept.j == kInfU ? 0u : kept.j;
+        if (lane == qslot) my_flags = flags;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_knn(KnnArgs a)
+{
+    extern __shared__ uint32_t s_lds[];
     const int lane = __lane_id();
     const int w = threadIdx.x / kWave;
     const CloudView& cv = a.cv;
+    const GridParams& g = cv.g;
     const uint64_t n = cv.n;
     const int Q = a.q_per_wave;
     const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -279,45 +507,170 @@ __global__ __launch_bounds__(kBlock) void k_knn(KnnArgs a)
     const int nq = static_cast<int>(n - q0 < static_cast<uint64_t>(Q) ? n - q0 : Q);
     const int fetch = a.prog.fetch;
     const int k = a.prog.k;
-    const int stride = a.prog.stride;
-    uint32_t* nbr = s_nbr + static_cast<size_t>(w) * Q * a.lds_stride;
+    // LDS: per-wave neighbor lists (features mode), then per wave a 64-entry
+    // compaction scratch of 16 bytes per entry and a 64-entry pending list.
+    const size_t list_words = a.mode == 1 ? 0 : static_cast<size_t>(kWaves) * Q * a.lds_stride;
+    uint32_t* nbr = s_lds + static_cast<size_t>(w) * Q * a.lds_stride;
+    uint32_t* scratch = s_lds + list_words + static_cast<size_t>(w) * kScratchWords;
+    uint32_t* pending = scratch + 512;  // queries left for the growing search
+    int npend = 0;
+    const uint32_t zmask = (1u << g.bz) - 1u;
 
     uint8_t my_flags = 0;
-    uint64_t cnt[3] = {0, 0, 0};  // search passes, candidates, merges
-    for (int qi = 0; qi < nq; ++qi) {
-        const uint64_t s = q0 + qi;
-        const double qx = cv.sx[s], qy = cv.sy[s], qz = cv.sz[s];
-        const uint32_t qcx = cell_coord(qx, cv.g.ox, cv.g.inv_c, cv.g.gx);
-        const uint32_t qcy = cell_coord(qy, cv.g.oy, cv.g.inv_c, cv.g.gy);
-        const uint32_t qcz = cell_coord(qz, cv.g.oz, cv.g.inv_cz, cv.g.gz);
-        Ent best;
-        bool incomplete;
-        wave_knn(a, qx, qy, qz, qcx, qcy, qcz, lane, best, incomplete, cnt);
-
-        // Boundary tie: the fetch-th and (fetch+1)-th distances are equal.
-        // With stride > 1 every adjacent equal pair inside the fetched list
-        // can change which ids are kept, so any such pair flags the row.
-        const Ent nxt = ent_shfl(best, lane + 1 < 64 ? lane + 1 : 63);
-        bool eq = (lane < fetch) && (lane + 1 < 64) && best.d == nxt.d && best.d != kInfD;
-        bool tie = stride > 1 ? (__ballot(eq) != 0) : (__shfl(static_cast<int>(eq), fetch - 1) != 0);
-        uint8_t flags = (tie ? ST_TIE : 0) | (incomplete ? ST_INCOMPLETE : 0);
-
-        // Kept list in canonical (d2, id) order: entries 0, stride, 2*stride...
-        const Ent kept = ent_shfl(best, (lane * stride) & 63);
-        if (a.mode == 1) {
-            const uint32_t qid = cv.perm[s];
-            if (lane < k) a.out.ids[static_cast<uint64_t>(qid) * k + lane] = kept.id;
-            if (lane == 0) a.out.status[0][qid] = flags;
-            if (lane == 0 && flags) {
-                uint32_t slot = atomicAdd(a.out.nflagged, 1u);
-                a.out.flagged[slot] = static_cast<uint32_t>(s);
+    uint64_t cnt[3] = {0, 0, 0};  // search passes, candidates, selections
+    int qi = 0;
+    int rg_prev = 1;  // box half-width of the previous group (density guess)
+    while (qi < nq) {
+        // The group: the run of queries from qi that lie in one 2x2 block of
+        // columns (one column if the block is too dense) with z cells within
+        // group_z of the first query's.  All of them share one candidate box:
+        // the block grown by rg columns on each side, z cells of the members
+        // grown by rg.
+        const uint64_t s0 = q0 + qi;
+        const bool in_range = qi + lane < nq;
+        const uint32_t key_l = in_range ? cv.keys[s0 + lane] : 0u;
+        const uint32_t key0 = __shfl(key_l, 0);
+        const int cz0 = static_cast<int>(key0 & zmask);
+        const int cz_l = static_cast<int>(key_l & zmask);
+        const bool zok = in_range && abs(cz_l - cz0) <= a.tune.group_z;
+        const double fx = cv.sx[s0], fy = cv.sy[s0];
+        const int qcx = static_cast<int>(cell_coord(fx, g.ox, g.inv_c, g.gx));
+        const int qcy = static_cast<int>(cell_coord(fy, g.oy, g.inv_c, g.gy));
+        const uint32_t need = static_cast<uint32_t>(fetch + 1);
+        int level = 1;
+        int gm = 0, zmin = cz0, zmax = cz0, rg = rg_prev;
+        Box box{};
+        uint32_t lo = 0, len = 0, tot = 0, excl = 0;
+        for (int attempt = 0; attempt < 6; ++attempt) {
+            const uint32_t shift = g.bz + 2u * static_cast<uint32_t>(level);
+            const bool member = zok && (key_l >> shift) == (key0 >> shift);
+            const uint64_t mm = __ballot(member);
+            gm = (~mm == 0ull) ? 64 : __builtin_ctzll(~mm);
+            int zl = lane < gm ? cz_l : cz0, zh = zl;
+#pragma unroll
+            for (int m = 32; m > 0; m >>= 1) {
+                zl = min(zl, __shfl_xor(zl, m));
+                zh = max(zh, __shfl_xor(zh, m));
             }
-        } else {
-            // An uncertified row can hold fewer than fetch entries; keep its
-            // list in bounds (the exact search replaces the row anyway).
-            if (lane < k) nbr[qi * a.lds_stride + lane] = kept.j == kInfU ? 0u : kept.j;
-            if (lane == qi) my_flags = flags;
+            zmin = zl;
+            zmax = zh;
+            const int bx0 = level ? (qcx & ~1) : qcx, by0 = level ? (qcy & ~1) : qcy;
+            const int bw = level ? 2 : 1;
+            box = Box{bx0 - rg, bx0 + bw - 1 + rg, by0 - rg, by0 + bw - 1 + rg, zmin - rg,
+                      zmax + rg};
+            box_ranges(a, box, 0, lane, lo, len);
+            const uint32_t incl = wave_incl_scan_u32(len, lane);
+            tot = __shfl(incl, 63);
+            excl = incl - len;
+            // Half-width the local density asks for: the need-th neighbor
+            // radius of a surface of this 2-D density, with 10% slack.
+            const int width = bw + 2 * rg;
+            const double area = static_cast<double>(width * width) * g.c * g.c;
+            int est = 3;
+            if (tot > 0) {
+                const double rk = sqrt(static_cast<double>(need) * area /
+                                       (3.141592653589793 * static_cast<double>(tot)));
+                est = static_cast<int>(ceil(1.1 * rk / g.c));
+                est = est < 1 ? 1 : (est > 3 ? 3 : est);
+            }
+            if (tot > kRegCand) {
+                if (rg > 1) {
+                    rg = min(rg - 1, est);
+                    continue;
+                }
+                if (level == 1) {
+                    level = 0;
+                    continue;
+                }
+                break;  // one dense column: searched query by query
+            }
+            if (est > rg) {
+                rg = est;
+                continue;
+            }
+            break;
         }
+        rg_prev = rg;
+        const bool cached = tot <= kRegCand;
+
+        // Shared candidates of the group's box, cached in registers.
+        double px[kRegBatches], py[kRegBatches], pz[kRegBatches];
+        uint32_t pid[kRegBatches], pj[kRegBatches];
+#pragma unroll
+        for (int b = 0; b < kRegBatches; ++b) {
+            px[b] = py[b] = pz[b] = 0.0;
+            pid[b] = pj[b] = kInfU;
+            if (cached && static_cast<uint32_t>(b * 64) < tot) {
+                const uint32_t t = b * 64 + lane;
+                const uint32_t j = candidate_pos(lo, excl, t);
+                if (t < tot) {
+                    px[b] = cv.sx[j];
+                    py[b] = cv.sy[j];
+                    pz[b] = cv.sz[j];
+                    pid[b] = cv.perm[j];
+                    pj[b] = j;
+                }
+            }
+        }
+
+        double hint = INFINITY, pqx = 0.0, pqy = 0.0, pqz = 0.0;
+        for (int gi = 0; gi < gm; ++gi) {
+            const uint64_t s = s0 + gi;
+            const int qslot = qi + gi;
+            bool done = false;
+            if (cached) {
+                const double qx = cv.sx[s], qy = cv.sy[s], qz = cv.sz[s];
+                cnt[0] += 1;
+                cnt[1] += tot;
+                Ent c[kRegBatches];
+#pragma unroll
+                for (int b = 0; b < kRegBatches; ++b)
+                    c[b] = pj[b] == kInfU ? ent_inf()
+                                          : make_ent(qx, qy, qz, px[b], py[b], pz[b], pid[b], pj[b]);
+                // The previous query's need nearest lie within its need-th
+                // distance plus the step between the two queries.
+                double h = INFINITY;
+                if (hint < INFINITY) {
+                    const double dx = qx - pqx, dy = qy - pqy, dz = qz - pqz;
+                    const double t = sqrt(hint) + sqrt(dx * dx + dy * dy + dz * dz);
+                    h = t * t * (1.0 + 1e-9) + 1e-300;
+                }
+                Ent best;
+                select_entries(c, tot, fetch, lane, scratch, h, best, cnt);
+                const Ent nth = ent_shfl(best, static_cast<int>(min(need, tot)) - 1);
+                hint = nth.d == kInfD ? INFINITY : ent_d2(nth);
+                pqx = qx;
+                pqy = qy;
+                pqz = qz;
+                done = certified(a, box, ent_shfl(best, fetch - 1), qx, qy, qz);
+                if (done) finish_query(a, best, false, s, qslot, lane, nbr, my_flags);
+            }
+            if (!done) {
+                // Searched alone after the groups, so the group's register
+                // cache is not live at the same time as the growing search.
+                if (lane == 0)
+                    pending[npend] = static_cast<uint32_t>(qslot) |
+                                     (static_cast<uint32_t>(cached ? rg + 1 : 1) << 8);
+                ++npend;
+            }
+        }
+        qi += gm;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int p = 0; p < npend; ++p) {
+        const uint32_t word = pending[p];
+        const int qslot = static_cast<int>(word & 0xFFu);
+        const uint64_t s = q0 + qslot;
+        const double qx = cv.sx[s], qy = cv.sy[s], qz = cv.sz[s];
+        const int qcx = static_cast<int>(cell_coord(qx, g.ox, g.inv_c, g.gx));
+        const int qcy = static_cast<int>(cell_coord(qy, g.oy, g.inv_c, g.gy));
+        const int qcz = static_cast<int>(cell_coord(qz, g.oz, g.inv_cz, g.gz));
+        Ent best;
+        bool incomplete = false;
+        search_single(a, qx, qy, qz, qcx, qcy, qcz, static_cast<int>(word >> 8), lane,
+                      scratch, best, incomplete, cnt);
+        finish_query(a, best, incomplete, s, qslot, lane, nbr, my_flags);
     }
     if (a.out.counters && lane == 0) {
         atomicAdd(a.out.counters + 0, static_cast<unsigned long long>(cnt[0]));
@@ -430,9 +783,9 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
                1e-15 * (fabs(g.ox) + fabs(g.oy) + fabs(g.oz)) + 1e-300;
     const uint64_t per_block = static_cast<uint64_t>(kWaves) * a.q_per_wave;
     const uint32_t blocks = div_up(cv.n, per_block);
-    const size_t lds = mode == 1 ? 0
-                                 : static_cast<size_t>(kWaves) * a.q_per_wave *
-                                       a.lds_stride * sizeof(uint32_t);
+    const size_t list_words =
+        mode == 1 ? 0 : static_cast<size_t>(kWaves) * a.q_per_wave * a.lds_stride;
+    const size_t lds = (list_words + static_cast<size_t>(kWaves) * kScratchWords) * sizeof(uint32_t);
     hipLaunchKernelGGL(k_knn, dim3(blocks), dim3(kBlock), lds, s, a);
 }
 
