@@ -61,7 +61,7 @@ EXPORTED = [
     "gf_cloud_create", "gf_cloud_release", "gf_cloud_size", "gf_stage_columns",
     "gf_run", "gf_last_stats", "gf_knn", "gf_debug_eigen", "gf_debug_grid",
     "gf_synth_defaults", "gf_tile_seed", "gf_synth_cloud", "gf_dev_alloc", "gf_dev_free",
-    "gf_memcpy", "gf_cloud_invalidate",
+    "gf_memcpy", "gf_cloud_invalidate", "gf_debug_kdtree",
 ]
 
 
@@ -91,6 +91,7 @@ def load_library(path=LIB_PATH):
     L.gf_knn.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.c_int]
     L.gf_debug_eigen.argtypes = [vp, _dp, ctypes.c_uint64, _dp, _dp, _i32p]
     L.gf_debug_grid.argtypes = [vp, vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p, _u32p]
+    L.gf_debug_kdtree.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.gf_dev_alloc.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
     L.gf_dev_free.argtypes = [vp, vp]
     L.gf_memcpy.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.c_int]
@@ -278,6 +279,24 @@ class Cloud:
         self.ctx._check(self.ctx.L.gf_knn(self.ctx.h, self.h, k, stride, ids.ctypes.data,
                                           st.ctypes.data, WHERE_HOST), "gf_knn")
         return ids, st
+
+    def kdtree(self):
+        """The exact KD3Index tree: vind and level-order nodes."""
+        m = ctypes.c_uint32()
+        L = self.ctx.L
+        self.ctx._check(L.gf_debug_kdtree(self.ctx.h, self.h, None, ctypes.byref(m), None, None,
+                                          None, None, None), "gf_debug_kdtree")
+        k = m.value
+        vind = np.zeros(self.n, np.uint32)
+        lo = np.zeros(k, np.uint32)
+        hi = np.zeros(k, np.uint32)
+        feat = np.zeros(k, np.int32)
+        dl = np.zeros(k)
+        dh = np.zeros(k)
+        self.ctx._check(L.gf_debug_kdtree(self.ctx.h, self.h, vind.ctypes.data, ctypes.byref(m),
+                                          lo.ctypes.data, hi.ctypes.data, feat.ctypes.data,
+                                          dl.ctypes.data, dh.ctypes.data), "gf_debug_kdtree")
+        return dict(vind=vind, left=lo, right=hi, divfeat=feat, divlow=dl, divhigh=dh)
 
     def grid(self, k):
         ncols = ctypes.c_uint32()

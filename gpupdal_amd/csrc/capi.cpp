@@ -85,6 +85,10 @@ void Context::build_index(Cloud& cl, uint32_t fetch)
                           hipMemcpyDeviceToHost, stream));
     GF_HIP(hipStreamSynchronize(stream));
     GridParams g = make_grid(host_bounds, n, fetch, tune);
+    for (int i = 0; i < 6; ++i) cl.bounds[i] = host_bounds[i];
+    cl.dbounds.ensure(64);
+    GF_HIP(hipMemcpyAsync(cl.dbounds.ptr, bounds.ptr, 6 * sizeof(double),
+                          hipMemcpyDeviceToDevice, stream));
 
     cl.keys_a.ensure(n * 4);
     cl.vals_a.ensure(n * 4);
@@ -220,7 +224,7 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     out.nflagged = counters.as<uint32_t>();
     out.cov = covbuf.as<double>();
     out.flags = flagbuf.as<uint8_t>();
-    launch_fill_u32(out.nflagged, 1, 0u, stream);
+    launch_fill_u32(out.nflagged, 4, 0u, stream);
     if (tune.collect_stats) {
         out.counters = reinterpret_cast<unsigned long long*>(counters.as<uint32_t>() + 8);
         launch_fill_u32(counters.as<uint32_t>() + 8, 8, 0u, stream);
@@ -230,13 +234,12 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     hipEvent_t e0 = ev_begin();
     launch_knn(cv, prog, 0, out, knn_tuning(), stream);
     stats.ms_search = ev_end(e0);
+    resolve_flagged(cl, cv, prog, 0, out);
     e0 = ev_begin();
     launch_features(cv, prog, out, stream);
     stats.ms_features = ev_end(e0);
 
-    uint32_t nflag = 0;
     unsigned long long ctr[4] = {0, 0, 0, 0};
-    GF_HIP(hipMemcpyAsync(&nflag, out.nflagged, 4, hipMemcpyDeviceToHost, stream));
     if (out.counters)
         GF_HIP(hipMemcpyAsync(ctr, out.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
     if (where == GF_WHERE_HOST) {
@@ -249,10 +252,104 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     }
     GF_HIP(hipStreamSynchronize(stream));
     stats.points = n;
-    stats.tie_rows = nflag;
     stats.search_passes = ctr[0];
     stats.candidates = ctr[1];
     stats.merges = ctr[2];
+}
+
+// The exact KD3Index tree of a cloud (built once, reused until invalidated).
+void Context::build_kdtree(Cloud& cl)
+{
+    if (cl.kd_valid) return;
+    const uint64_t n = cl.n;
+    KdBuffers& b = cl.kd;
+    for (;;) {
+        const uint32_t cap = std::max(kd_node_capacity(n), b.cap);
+        b.vind.ensure(n * 4 + 8);
+        b.px.ensure(n * 8 + 8);
+        b.py.ensure(n * 8 + 8);
+        b.pz.ensure(n * 8 + 8);
+        b.segid.ensure(n * 4 + 8);
+        b.nodes_u32.ensure(static_cast<uint64_t>(cap) * 4 * 3);
+        b.nodes_i32.ensure(static_cast<uint64_t>(cap) * 4 * 4);
+        b.which.ensure(cap + 8);
+        b.nodes_f64.ensure(static_cast<uint64_t>(cap) * 8 * 9);
+        b.ext.ensure(static_cast<uint64_t>(cap) * 8 * 6);
+        kd_flag.ensure((n + 1) * 4);
+        kd_s1.ensure((n + 1) * 4);
+        kd_s2.ensure((n + 1) * 4);
+        kd_tmp.ensure((n + 1) * 4);
+        kd_nodes.ensure((static_cast<uint64_t>(cap) + 1) * 8);
+        scan_tmp.ensure(scan_scratch_words(std::max<uint64_t>(n + 1, cap + 1)) * 4);
+        KdView t{};
+        t.vind = b.vind.as<uint32_t>();
+        t.px = b.px.as<double>();
+        t.py = b.py.as<double>();
+        t.pz = b.pz.as<double>();
+        t.segid = b.segid.as<uint32_t>();
+        t.nl = b.nodes_u32.as<uint32_t>();
+        t.nr = t.nl + cap;
+        t.split = t.nr + cap;
+        t.c1 = b.nodes_i32.as<int32_t>();
+        t.c2 = t.c1 + cap;
+        t.feat = t.c2 + cap;
+        t.parent = t.feat + cap;
+        t.which = b.which.as<uint8_t>();
+        t.divlow = b.nodes_f64.as<double>();
+        t.divhigh = t.divlow + cap;
+        t.cutval = t.divhigh + cap;
+        t.box = t.cutval + cap;
+        t.mn = b.ext.as<unsigned long long>();
+        t.mx = t.mn + 3 * static_cast<uint64_t>(cap);
+        t.cap = cap;
+        for (int i = 0; i < 6; ++i) t.root[i] = cl.bounds[i];
+        KdScratch w{};
+        w.flag = kd_flag.as<uint32_t>();
+        w.s1 = kd_s1.as<uint32_t>();
+        w.s2 = kd_s2.as<uint32_t>();
+        w.tmp = kd_tmp.as<uint32_t>();
+        w.is_split = kd_nodes.as<uint32_t>();
+        w.rank = w.is_split + cap + 1;
+        w.scan_tmp = scan_tmp.as<uint32_t>();
+        w.overflow = counters.as<uint32_t>() + 4;
+        uint32_t nodes = 0;
+        if (launch_kd_build(cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), n,
+                            cl.dbounds.as<double>(), t, w, &nodes, stream)) {
+            b.view = t;
+            b.cap = cap;
+            b.nodes = nodes;
+            cl.kd_valid = true;
+            return;
+        }
+        b.cap = cap * 2;  // node capacity overflow: grow and build again
+    }
+}
+
+// Re-solves the rows the grid search flagged (boundary ties, uncertified
+// searches) with the exact KD3Index search.
+void Context::resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog,
+                              int mode, const KnnOutputs& out)
+{
+    uint32_t h[4] = {0, 0, 0, 0};
+    GF_HIP(hipMemcpyAsync(h, out.nflagged, sizeof(h), hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipStreamSynchronize(stream));
+    stats.tie_rows = h[1];
+    stats.incomplete_rows = h[2];
+    if (!h[0]) return;
+    hipEvent_t e0 = ev_begin();
+    build_kdtree(cl);
+    uint32_t* failed = counters.as<uint32_t>() + 5;
+    launch_fill_u32(failed, 1, 0u, stream);
+    launch_kd_resolve(cl.kd.view, cv, prog, mode, out, cl.x.as<double>(), cl.y.as<double>(),
+                      cl.z.as<double>(), h[0], failed, stream);
+    uint32_t nfail = 0;
+    GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipStreamSynchronize(stream));
+    stats.ms_exact = ev_end(e0);
+    if (nfail)
+        throw Error(GF_ERR_UNSUPPORTED,
+                    "exact KD3 search exceeded its depth bound on " + std::to_string(nfail) +
+                        " rows");
 }
 
 hipEvent_t Context::ev_begin()
@@ -384,6 +481,7 @@ int gf_cloud_invalidate(gf_cloud* cloud)
 {
     if (!cloud) return GF_ERR_ARG;
     CL(cloud)->idx_valid = false;
+    CL(cloud)->kd_valid = false;
     return GF_OK;
 }
 
@@ -468,8 +566,10 @@ int gf_knn(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t stride, uint32_t* 
         c->counters.ensure(64);
         out.flagged = c->flagged.as<uint32_t>();
         out.nflagged = c->counters.as<uint32_t>();
-        launch_fill_u32(out.nflagged, 1, 0u, c->stream);
-        launch_knn(c->view(*cl), prog, 1, out, c->knn_tuning(), c->stream);
+        launch_fill_u32(out.nflagged, 4, 0u, c->stream);
+        const CloudView cv = c->view(*cl);
+        launch_knn(cv, prog, 1, out, c->knn_tuning(), c->stream);
+        c->resolve_flagged(*cl, cv, prog, 1, out);
         if (where == GF_WHERE_HOST) {
             GF_HIP(hipMemcpyAsync(ids, out.ids, n * kk * 4, hipMemcpyDeviceToHost, c->stream));
             GF_HIP(hipMemcpyAsync(status, out.status[0], n, hipMemcpyDeviceToHost, c->stream));
@@ -554,6 +654,33 @@ int gf_debug_grid(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t* perm,
             GF_HIP(hipMemcpyAsync(colstart, cl->colstart.ptr,
                                   (static_cast<uint64_t>(cl->g.ncols) + 1) * 4,
                                   hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int gf_debug_kdtree(gf_ctx* ctx, gf_cloud* cloud, uint32_t* vind, uint32_t* nodes,
+                    uint32_t* left, uint32_t* right, int32_t* feat, double* divlow,
+                    double* divhigh)
+{
+    if (!ctx || !cloud || !nodes) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        if (!cl->n) throw Error(GF_ERR_ARG, "empty cloud");
+        if (!cl->idx_valid) c->build_index(*cl, 1);
+        c->build_kdtree(*cl);
+        const KdView& t = cl->kd.view;
+        const uint32_t m = cl->kd.nodes;
+        *nodes = m;
+        auto get = [&](void* dst, const void* src, uint64_t bytes) {
+            if (dst) GF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        };
+        get(vind, t.vind, cl->n * 4);
+        get(left, t.nl, static_cast<uint64_t>(m) * 4);
+        get(right, t.nr, static_cast<uint64_t>(m) * 4);
+        get(feat, t.feat, static_cast<uint64_t>(m) * 4);
+        get(divlow, t.divlow, static_cast<uint64_t>(m) * 8);
+        get(divhigh, t.divhigh, static_cast<uint64_t>(m) * 8);
         GF_HIP(hipStreamSynchronize(c->stream));
     });
 }

@@ -34,7 +34,7 @@ struct KnnOutputs {
     uint8_t* status[kMaxStages];
     uint32_t* ids;           // neighbor-list mode: n x k (original order)
     uint32_t* flagged;       // sorted indices of tie/incomplete rows
-    uint32_t* nflagged;      // counter
+    uint32_t* nflagged;      // [0] flagged rows, [1] tie rows, [2] incomplete rows
     double* cov;             // 6 x n FP64 covariance, sorted order (SoA)
     uint8_t* flags;          // n tie/incomplete flags, sorted order
     unsigned long long* counters; // optional: passes, candidates, merges, queries
@@ -65,6 +65,40 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
 // Eigensystem + stage columns from out.cov for every row.
 void launch_features(const CloudView& cv, const FusedProgram& prog,
                      const KnnOutputs& out, hipStream_t s);
+
+// kdtree.hip -- the exact KD3Index tree (nanoflann 1.3.1 semantics).
+struct KdView {
+    uint32_t* vind;          // n: tree order -> original id
+    double *px, *py, *pz;    // n: coordinates in tree order
+    uint32_t* segid;         // n: deepest node holding the position (build)
+    uint32_t *nl, *nr;       // node segment [nl, nr)
+    uint32_t* split;         // split position of internal nodes
+    int32_t *c1, *c2;        // children (-1: leaf)
+    int32_t* feat;           // cut axis (-1: leaf)
+    int32_t* parent;
+    uint8_t* which;          // 0: child1 of its parent, 1: child2
+    double *divlow, *divhigh, *cutval;
+    double* box;             // 6 per node: the box passed down (lo xyz, hi xyz)
+    unsigned long long *mn, *mx;  // 3 per node: order-encoded extents (build)
+    uint32_t cap;            // node capacity
+    uint32_t pad;
+    double root[6];          // root box (data bounds)
+};
+
+struct KdScratch {
+    uint32_t *flag, *s1, *s2, *tmp;  // n + 1 each
+    uint32_t *is_split, *rank;       // capacity + 1 each
+    uint32_t* scan_tmp;
+    uint32_t* overflow;
+};
+
+uint32_t kd_node_capacity(uint64_t n);
+bool launch_kd_build(const double* x, const double* y, const double* z, uint64_t n,
+                     const double* bounds, KdView t, KdScratch w, uint32_t* nodes,
+                     hipStream_t s);
+void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram& prog, int mode,
+                       const KnnOutputs& out, const double* x, const double* y, const double* z,
+                       uint32_t nflagged, uint32_t* failed, hipStream_t s);
 
 // Debug/parity hook: device eigen solver over m matrices (row-major 3x3).
 void launch_eigen_batch(const double* cov, uint64_t m, double* evals,

@@ -77,6 +77,14 @@ struct Tuning {
 
 struct Context;
 
+// Device storage of a cloud's exact KD3Index tree.
+struct KdBuffers {
+    DevBuf vind, px, py, pz, segid, nodes_u32, nodes_i32, which, nodes_f64, ext;
+    uint32_t cap = 0;
+    uint32_t nodes = 0;
+    KdView view{};
+};
+
 struct Cloud {
     Context* ctx;
     uint64_t n = 0;
@@ -89,6 +97,10 @@ struct Cloud {
     DevBuf keys_a, vals_a, keys_b, vals_b, colstart, sx, sy, sz;
     uint32_t* keys = nullptr;    // sorted keys (into keys_a/keys_b)
     uint32_t* perm = nullptr;    // sorted position -> original id
+    double bounds[6] = {0, 0, 0, 0, 0, 0};  // min xyz, max xyz
+    DevBuf dbounds;
+    bool kd_valid = false;       // exact tree (built on demand)
+    KdBuffers kd;
     explicit Cloud(Context* c) : ctx(c) {}
 };
 
@@ -105,6 +117,7 @@ struct Context {
     gf_stats stats{};
     std::string last_error;
     DevBuf part, bounds, hist, scan_tmp, outbuf, statbuf, flagged, counters, covbuf, flagbuf;
+    DevBuf kd_flag, kd_s1, kd_s2, kd_tmp, kd_nodes;
     double* host_bounds = nullptr;
 
     explicit Context(int dev) : device(dev)
@@ -143,6 +156,9 @@ struct Context {
     CloudView view(const Cloud& cl) const;
     void run_group(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
                    uint8_t* const* status, int where);
+    void build_kdtree(Cloud& cl);
+    void resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog, int mode,
+                         const KnnOutputs& out);
     hipEvent_t ev_begin();
     double ev_end(hipEvent_t e);
 };

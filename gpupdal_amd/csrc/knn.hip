@@ -483,6 +483,7 @@ __device__ __forceinline__ void finish_query(const KnnArgs& a, const Ent& best,
         if (lane == 0 && flags) {
             uint32_t slot = atomicAdd(a.out.nflagged, 1u);
             a.out.flagged[slot] = static_cast<uint32_t>(s);
+            atomicAdd(a.out.nflagged + ((flags & ST_INCOMPLETE) ? 2 : 1), 1u);
         }
     } else {
         // An uncertified row can hold fewer than fetch entries; keep its list
@@ -718,6 +719,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
     if (my_flags) {
         uint32_t slot = atomicAdd(a.out.nflagged, 1u);
         a.out.flagged[slot] = static_cast<uint32_t>(s);
+        atomicAdd(a.out.nflagged + ((my_flags & ST_INCOMPLETE) ? 2 : 1), 1u);
     }
 }
 

@@ -166,6 +166,13 @@ int gf_debug_grid(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, uint32_t *perm,
                   uint32_t *keys, uint32_t *colstart, uint32_t *ncols,
                   uint32_t *key_bits);
 
+/* The cloud's exact KD3Index tree (built on demand): vind (n, tree order ->
+ * point id) and the nodes in level order; pass NULL arrays to query the node
+ * count.  Leaves have feat = -1. */
+int gf_debug_kdtree(gf_ctx *ctx, gf_cloud *cloud, uint32_t *vind, uint32_t *nodes,
+                    uint32_t *left, uint32_t *right, int32_t *feat, double *divlow,
+                    double *divhigh);
+
 /* ---- seeded synthetic clouds (the benchmark generator). */
 typedef struct gf_synth {
     uint64_t seed;          /* tile seed (splitmix64(0x5EED ^ tile) by convention) */

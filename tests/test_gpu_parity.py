@@ -37,21 +37,65 @@ def test_grid_invariants(ctx):
 
 @pytest.mark.parametrize("n,k,seed", [(20000, 11, 1), (30000, 17, 2), (15000, 33, 4)])
 def test_knn_sets_vs_kd3(ctx, oracle, n, k, seed):
+    """Every row's neighbor set equals KD3Index's; rows re-solved by the exact
+    tree search (ties, uncertified) also match its order."""
     x, y, z = small_cloud(n, seed=seed, dup_frac=0.02)
     cl = ctx.cloud(x, y, z)
     ids, st = cl.knn(k)
     tree = oracle.kd3(x, y, z)
     ref = tree.neighbors_all(k, threads=8)
-    flagged = (st & (g.ST_TIE | g.ST_INCOMPLETE)) != 0
     eq = set_equal_rows(ids, ref)
-    # every unflagged row is bit-exact
-    assert np.all(eq[~flagged]), f"{np.count_nonzero(~eq & ~flagged)} unflagged rows differ"
+    assert np.all(eq), f"{np.count_nonzero(~eq)} rows differ from KD3Index"
+    flagged = (st & (g.ST_TIE | g.ST_INCOMPLETE)) != 0
+    assert np.array_equal(ids[flagged], ref[flagged]), "re-solved rows keep KD3 order"
     # every row flagged as a tie really has a boundary tie (brute force)
     tie_rows = np.nonzero(st & g.ST_TIE)[0]
     if len(tie_rows):
         bid, bd2 = oracle.brute_knn(x, y, z, k + 1, queries=tie_rows[:2000])
         assert np.all(bd2[:, k - 1] == bd2[:, k])
-    assert np.count_nonzero(st & g.ST_INCOMPLETE) == 0
+
+
+def test_kdtree_matches_oracle(ctx, oracle):
+    """The GPU-built tree is nanoflann's: same vind order and node splits."""
+    for seed, n, dup in [(21, 30000, 0.05), (22, 5000, 0.3), (23, 150, 0.0)]:
+        x, y, z = small_cloud(n, seed=seed, dup_frac=dup)
+        t = ctx.cloud(x, y, z).kdtree()
+        e = oracle.kd3(x, y, z).export()
+        assert np.array_equal(t["vind"].astype(np.uint64), e["vind"])
+        inner = t["divfeat"] >= 0
+        mine = {(int(a), int(b)): (int(f), lo, hi) for a, b, f, lo, hi in
+                zip(t["left"][inner], t["right"][inner], t["divfeat"][inner],
+                    t["divlow"][inner], t["divhigh"][inner])}
+        ref = {}
+        for i in range(len(e["child1"])):
+            if e["child1"][i] >= 0:
+                ref[(int(e["left"][i]), int(e["right"][i]))] = (
+                    int(e["divfeat"][i]), e["divlow"][i], e["divhigh"][i])
+        assert mine == ref
+        assert len(t["left"]) == len(e["child1"])
+
+
+def test_knn_lattice_ties_vs_kd3(ctx, oracle):
+    """A regular lattice puts many points at equal distances: most rows have
+    boundary ties and go through the exact tree search."""
+    g1 = np.arange(30, dtype=np.float64)
+    X, Y, Z = np.meshgrid(g1, g1, np.arange(6, dtype=np.float64), indexing="ij")
+    x, y, z = X.ravel() * 0.5 + 500000.0, Y.ravel() * 0.5 + 4000000.0, Z.ravel() * 0.5
+    cl = ctx.cloud(x, y, z)
+    for k in (9, 17):
+        ids, st = cl.knn(k)
+        ref = oracle.kd3(x, y, z).neighbors_all(k, threads=8)
+        assert np.count_nonzero(st & g.ST_TIE) > len(x) // 4
+        assert np.all(set_equal_rows(ids, ref))
+
+
+@pytest.mark.parametrize("k,stride", [(5, 2), (8, 3), (11, 2)])
+def test_knn_stride_vs_kd3(ctx, oracle, k, stride):
+    x, y, z = small_cloud(12000, seed=31, dup_frac=0.02)
+    ids, st = ctx.cloud(x, y, z).knn(k, stride=stride)
+    tree = oracle.kd3(x, y, z)
+    ref = tree.neighbors_all(k, stride=stride, threads=8)
+    assert np.all(set_equal_rows(ids, ref))
 
 
 def test_knn_tie_free_rows_vs_bruteforce(ctx, oracle):
@@ -95,7 +139,7 @@ def test_covfeat_all_vs_truth(ctx, oracle, n, knn):
     nbr = tree.neighbors_all(knn + 1, threads=8)
     ost = ol.stage(ol.STAGE_COVFEAT, knn, features=ol.F_ALL10)
     tcols, tstat, gap = oracle.truth_from_lists(x, y, z, nbr, ost)
-    ok = (status == 0) & (tstat == 0)
+    ok = ((status & 0x3C) == 0) & (tstat == 0)
     assert np.count_nonzero(ok) > 0.95 * n
     well = ok & (gap > 1e-3)
     for ci, name in enumerate(ol.FEATURE_NAMES):
