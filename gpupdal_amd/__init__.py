@@ -61,7 +61,7 @@ EXPORTED = [
     "gf_cloud_create", "gf_cloud_release", "gf_cloud_size", "gf_stage_columns",
     "gf_run", "gf_last_stats", "gf_knn", "gf_debug_eigen", "gf_debug_grid",
     "gf_synth_defaults", "gf_tile_seed", "gf_synth_cloud", "gf_dev_alloc", "gf_dev_free",
-    "gf_memcpy", "gf_cloud_invalidate", "gf_debug_kdtree",
+    "gf_memcpy", "gf_cloud_invalidate", "gf_debug_kdtree", "gf_debug_copy",
 ]
 
 
@@ -92,6 +92,7 @@ def load_library(path=LIB_PATH):
     L.gf_debug_eigen.argtypes = [vp, _dp, ctypes.c_uint64, _dp, _dp, _i32p]
     L.gf_debug_grid.argtypes = [vp, vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p, _u32p]
     L.gf_debug_kdtree.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.gf_debug_copy.argtypes = [vp, vp, vp, ctypes.c_uint64]
     L.gf_dev_alloc.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
     L.gf_dev_free.argtypes = [vp, vp]
     L.gf_memcpy.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.c_int]
@@ -205,6 +206,10 @@ class Context:
 
     def free(self, ptr):
         self._check(self.L.gf_dev_free(self.h, ptr), "gf_dev_free")
+
+    def copy(self, dst, src, count):
+        """Plain device copy of count doubles (HBM counter calibration)."""
+        self._check(self.L.gf_debug_copy(self.h, dst, src, count), "gf_debug_copy")
 
     def to_host(self, ptr, arr):
         self._check(self.L.gf_memcpy(self.h, arr.ctypes.data, ptr, arr.nbytes, 2), "gf_memcpy")

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 using namespace gf;
@@ -177,9 +178,6 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     if (fetch > static_cast<uint32_t>(kMaxFetch - 1))
         throw Error(GF_ERR_UNSUPPORTED,
                     "(knn+1)*stride above 63 is outside the device kNN envelope");
-    if (fetch > n)
-        throw Error(GF_ERR_UNSUPPORTED,
-                    "stride*min(n, knn+1) exceeds the point count");
     FusedProgram prog{};
     prog.nstages = ns;
     prog.k = static_cast<int32_t>(k);
@@ -232,7 +230,10 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
 
     const CloudView cv = view(cl);
     hipEvent_t e0 = ev_begin();
-    launch_knn(cv, prog, 0, out, knn_tuning(), stream);
+    if (fetch > n)  // fewer points than the result capacity: all rows exact
+        launch_flag_all(n, cv.perm, 0, out, stream);
+    else
+        launch_knn(cv, prog, 0, out, knn_tuning(), stream);
     stats.ms_search = ev_end(e0);
     resolve_flagged(cl, cv, prog, 0, out);
     e0 = ev_begin();
@@ -437,7 +438,12 @@ int gf_sync(gf_ctx* ctx)
 int gf_set_tuning(gf_ctx* ctx, const char* key, double value)
 {
     if (!ctx || !key) return GF_ERR_ARG;
-    return guarded(ctx, [&] { C(ctx)->tune.set(key, value); });
+    return guarded(ctx, [&] {
+        if (!std::strcmp(key, "timing"))
+            C(ctx)->timing = value != 0.0;  // per-phase device timers (they synchronize)
+        else
+            C(ctx)->tune.set(key, value);
+    });
 }
 
 int gf_cloud_create(gf_ctx* ctx, const double* x, const double* y, const double* z,
@@ -485,7 +491,13 @@ int gf_cloud_invalidate(gf_cloud* cloud)
     return GF_OK;
 }
 
-int gf_stage_columns(const gf_stage* st) { return st ? stage_ncols(*st) : -1; }
+int gf_stage_columns(const gf_stage* st)
+{
+    if (!st) return GF_ERR_ARG;
+    if (st->stage == GF_STAGE_COVFEAT && (st->features & ~0x3FFu)) return GF_ERR_ARG;
+    const int n = stage_ncols(*st);
+    return n < 0 ? GF_ERR_ARG : n;
+}
 
 int gf_run(gf_ctx* ctx, gf_cloud* cloud, const gf_stage* stages, int nstages,
            double* const* cols, uint8_t* const* status, int where)
@@ -544,8 +556,6 @@ int gf_knn(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t stride, uint32_t* 
         const uint32_t fetch = kk * stride;
         if (fetch > static_cast<uint32_t>(kMaxFetch - 1))
             throw Error(GF_ERR_UNSUPPORTED, "k*stride above 63 is outside the device kNN envelope");
-        if (fetch > n)
-            throw Error(GF_ERR_UNSUPPORTED, "stride*min(n, k) exceeds the point count");
         c->build_index(*cl, fetch);
         FusedProgram prog{};
         prog.k = static_cast<int32_t>(kk);
@@ -568,7 +578,10 @@ int gf_knn(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t stride, uint32_t* 
         out.nflagged = c->counters.as<uint32_t>();
         launch_fill_u32(out.nflagged, 4, 0u, c->stream);
         const CloudView cv = c->view(*cl);
-        launch_knn(cv, prog, 1, out, c->knn_tuning(), c->stream);
+        if (fetch > n)
+            launch_flag_all(n, cv.perm, 1, out, c->stream);
+        else
+            launch_knn(cv, prog, 1, out, c->knn_tuning(), c->stream);
         c->resolve_flagged(*cl, cv, prog, 1, out);
         if (where == GF_WHERE_HOST) {
             GF_HIP(hipMemcpyAsync(ids, out.ids, n * kk * 4, hipMemcpyDeviceToHost, c->stream));
@@ -655,6 +668,15 @@ int gf_debug_grid(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t* perm,
                                   (static_cast<uint64_t>(cl->g.ncols) + 1) * 4,
                                   hipMemcpyDeviceToHost, c->stream));
         GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int gf_debug_copy(gf_ctx* ctx, double* dst, const double* src, uint64_t count)
+{
+    if (!ctx || (count && (!dst || !src))) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_copy_f64(src, dst, count, C(ctx)->stream);
+        GF_HIP(hipStreamSynchronize(C(ctx)->stream));
     });
 }
 

@@ -27,6 +27,7 @@ void launch_gather_sorted(const uint32_t* perm, const double* x, const double* y
                           double* sz, hipStream_t s);
 void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+void launch_copy_f64(const double* src, double* dst, uint64_t n, hipStream_t s);
 
 // knn.hip
 struct KnnOutputs {
@@ -93,6 +94,10 @@ struct KdScratch {
 };
 
 uint32_t kd_node_capacity(uint64_t n);
+// Sends every row to the exact search: used when the cloud holds fewer
+// points than the result capacity stride*min(n, k).
+void launch_flag_all(uint64_t n, const uint32_t* perm, int mode, const KnnOutputs& out,
+                     hipStream_t s);
 bool launch_kd_build(const double* x, const double* y, const double* z, uint64_t n,
                      const double* bounds, KdView t, KdScratch w, uint32_t* nodes,
                      hipStream_t s);

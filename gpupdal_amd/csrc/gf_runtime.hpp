@@ -45,7 +45,7 @@ struct DevBuf {
 };
 
 struct Tuning {
-    double cell_factor = 1.25;   // column edge = factor * sqrt(k / (pi * rho2d))
+    double cell_factor = 0.6;    // column edge = factor * sqrt(k / (pi * rho2d))
     double cell_z_factor = 1.0;  // z cell height = factor * column edge
     int ring0 = 1;
     int ring_max = 8;

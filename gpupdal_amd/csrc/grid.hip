@@ -412,6 +412,23 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n, uint3
         p[i] = v;
 }
 
+// A plain streaming copy: the reference point for HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy_f64(const double* __restrict__ src,
+                                                  double* __restrict__ dst, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy_f64(const double* src, double* dst, uint64_t n, hipStream_t s)
+{
+    if (!n) return;
+    uint32_t b = div_up(n, 256);
+    if (b > 65536) b = 65536;
+    hipLaunchKernelGGL(k_copy_f64, dim3(b), dim3(256), 0, s, src, dst, n);
+}
+
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s)
 {
     if (!n) return;

@@ -21,9 +21,12 @@
 // The search restates findNeighbors / searchLevel / KNNResultSet (strict <
 // against the worst distance read at leaf entry, closer child first, the
 // other child when the incremental box distance is <= the worst,
-// nanoflann.hpp:145-208, 1059-1080, 1184-1248) one row per thread.
+// nanoflann.hpp:145-208, 1059-1080, 1184-1248) one row per wavefront.
 #include "gf_internal.hpp"
 #include "gf_kernels.hpp"
+#include "gf_wave.hpp"
+
+#include <algorithm>
 
 namespace gf {
 
@@ -31,7 +34,6 @@ namespace {
 
 constexpr uint32_t kLeafMax = 100;
 constexpr int kKdThreads = 256;
-constexpr int kMaxDepth = 96;
 
 __device__ __forceinline__ unsigned long long enc_order(double d)
 {
@@ -85,40 +87,77 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_minmax_reset(KdView t, uint32
 }
 
 // Min/max of every level node's points: a segmented reduction inside each
-// wave (runs of equal node ids are contiguous), then one atomic per run.
-__global__ __launch_bounds__(kKdThreads) void k_kd_minmax(KdView t, uint32_t n, uint32_t lv0)
+// wave (runs of equal node ids are contiguous), then per run head an atomic:
+// into a per-block LDS table when the level has few nodes (the top levels,
+// where every wave would otherwise hit the same global address), else
+// straight to global memory.
+constexpr uint32_t kLdsNodes = 256;
+
+__global__ __launch_bounds__(kKdThreads) void k_kd_minmax(KdView t, uint32_t n, uint32_t lv0,
+                                                          uint32_t lv1)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = __lane_id();
-    const bool act = p < n && t.segid[p] >= lv0;
-    const uint32_t nd = act ? t.segid[p] : 0xFFFFFFFFu;
-    unsigned long long lo[3], hi[3];
-    const double* a[3] = {t.px, t.py, t.pz};
-    for (int d = 0; d < 3; ++d) {
-        unsigned long long e = act ? enc_order(a[d][p]) : 0ull;
-        lo[d] = act ? e : ~0ull;
-        hi[d] = act ? e : 0ull;
+    __shared__ unsigned long long s_lo[kLdsNodes * 3], s_hi[kLdsNodes * 3];
+    const uint32_t nl = lv1 - lv0;
+    const bool use_lds = nl <= kLdsNodes;
+    if (use_lds) {
+        for (uint32_t i = threadIdx.x; i < nl * 3; i += blockDim.x) {
+            s_lo[i] = ~0ull;
+            s_hi[i] = 0ull;
+        }
+        __syncthreads();
     }
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t ond = __shfl_down(nd, off);
-        const bool same = lane + off < 64 && ond == nd;
+    const int lane = __lane_id();
+    const double* a[3] = {t.px, t.py, t.pz};
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n;
+         base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t p = base + threadIdx.x;
+        const bool act = p < n && t.segid[p] >= lv0;
+        const uint32_t nd = act ? t.segid[p] : 0xFFFFFFFFu;
+        unsigned long long lo[3], hi[3];
         for (int d = 0; d < 3; ++d) {
-            unsigned long long ol = __shfl_down(lo[d], off);
-            unsigned long long oh = __shfl_down(hi[d], off);
-            if (same) {
-                lo[d] = ol < lo[d] ? ol : lo[d];
-                hi[d] = oh > hi[d] ? oh : hi[d];
+            unsigned long long e = act ? enc_order(a[d][p]) : 0ull;
+            lo[d] = act ? e : ~0ull;
+            hi[d] = act ? e : 0ull;
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t ond = __shfl_down(nd, off);
+            const bool same = lane + off < 64 && ond == nd;
+            for (int d = 0; d < 3; ++d) {
+                unsigned long long ol = __shfl_down(lo[d], off);
+                unsigned long long oh = __shfl_down(hi[d], off);
+                if (same) {
+                    lo[d] = ol < lo[d] ? ol : lo[d];
+                    hi[d] = oh > hi[d] ? oh : hi[d];
+                }
+            }
+        }
+        const uint32_t prev = __shfl_up(nd, 1);
+        const bool head = act && (lane == 0 || prev != nd);
+        if (head) {
+            if (use_lds) {
+                for (int d = 0; d < 3; ++d) {
+                    atomicMin(&s_lo[(nd - lv0) * 3 + d], lo[d]);
+                    atomicMax(&s_hi[(nd - lv0) * 3 + d], hi[d]);
+                }
+            } else {
+                for (int d = 0; d < 3; ++d) {
+                    atomicMin(&t.mn[3 * nd + d], lo[d]);
+                    atomicMax(&t.mx[3 * nd + d], hi[d]);
+                }
             }
         }
     }
-    const uint32_t prev = __shfl_up(nd, 1);
-    const bool head = act && (lane == 0 || prev != nd);
-    if (head)
-        for (int d = 0; d < 3; ++d) {
-            atomicMin(&t.mn[3 * nd + d], lo[d]);
-            atomicMax(&t.mx[3 * nd + d], hi[d]);
+    if (use_lds) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
+            if (s_hi[i * 3] == 0ull) continue;  // no point of node i in this block
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(&t.mn[3 * (lv0 + i) + d], s_lo[i * 3 + d]);
+                atomicMax(&t.mx[3 * (lv0 + i) + d], s_hi[i * 3 + d]);
+            }
         }
+    }
 }
 
 // One thread per level node: the parent's divlow/divhigh from this node's
@@ -329,41 +368,52 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_segid(KdView t, uint32_t n, u
 }
 
 // -------------------------------------------------------------- search --
-struct Frame {
-    double mind;
-    double cut;
-    double dst;
-    int32_t node;
-    int32_t other;
-    int32_t idx;
-    int32_t stage;
-};
-
-// KD3Index::neighbors for sorted row s (the row's own coordinates as the
-// query), then the row's covariance (features mode) or id list (list mode).
-__global__ __launch_bounds__(64) void k_kd_resolve(KdView t, CloudView cv, FusedProgram prog,
-                                                   int mode, KnnOutputs out,
-                                                   const double* __restrict__ x,
-                                                   const double* __restrict__ y,
-                                                   const double* __restrict__ z,
-                                                   uint32_t* __restrict__ failed)
+// KD3Index::neighbors for one flagged row per wave, the row's own
+// coordinates as the query.  The traversal (closer child first, the other
+// child when the incremental box distance is <= the worst kept distance) is
+// uniform across the wave; the recursion stack lives in lane-indexed
+// registers (frame i in lane i).  A leaf's points are evaluated in parallel
+// and those below the worst distance read at leaf entry are merged into the
+// lane-held sorted list keyed by (d2, visit order): the capacity smallest by
+// (d2, visit order) is exactly what KNNResultSet::addPoint keeps when the
+// points arrive one by one with strict < (earlier points win ties).
+__device__ __forceinline__ double lane_get(double v, int i)
 {
+    return __shfl(v, i);
+}
+
+__device__ __forceinline__ int lane_get(int v, int i)
+{
+    return __builtin_amdgcn_readlane(v, i);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, FusedProgram prog,
+                                                    int mode, KnnOutputs out,
+                                                    const double* __restrict__ x,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ z,
+                                                    uint32_t* __restrict__ failed)
+{
+    const int lane = __lane_id();
+    const uint32_t waves = gridDim.x * (blockDim.x / 64);
     const uint32_t nflag = *out.nflagged;
-    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nflag;
-         f += gridDim.x * blockDim.x) {
+    const uint32_t cap = static_cast<uint32_t>(prog.fetch);
+    const uint32_t k = static_cast<uint32_t>(prog.k);
+    const uint32_t stride = static_cast<uint32_t>(prog.stride);
+    for (uint32_t f = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; f < nflag; f += waves) {
         const uint32_t s = out.flagged[f];
         const uint32_t qid = cv.perm[s];
-        const double q[3] = {x[qid], y[qid], z[qid]};
-        const uint32_t k = static_cast<uint32_t>(prog.k);
-        const uint32_t cap = static_cast<uint32_t>(prog.fetch);
-        double dist[kMaxFetch];
-        uint32_t ids[kMaxFetch];
-        for (uint32_t i = 0; i < cap; ++i) {
-            dist[i] = 0.0;
-            ids[i] = 0;
-        }
-        dist[cap - 1] = 1.7976931348623157e308;  // KNNResultSet::init
-        uint32_t count = 0;
+        const double q0 = x[qid], q1 = y[qid], q2 = z[qid];
+        const double q[3] = {q0, q1, q2};
+        Ent best = ent_inf();
+        uint32_t seq = 0;
         double dists[3] = {0.0, 0.0, 0.0};
         double mind = 0.0;
         for (int i = 0; i < 3; ++i) {
@@ -376,124 +426,165 @@ __global__ __launch_bounds__(64) void k_kd_resolve(KdView t, CloudView cv, Fused
                 mind += dists[i];
             }
         }
-        Frame st[kMaxDepth];
-        int sp = 0;
-        st[sp++] = Frame{mind, 0.0, 0.0, 0, -1, 0, 0};
+        // Frame registers: lane i holds stack frame i.
+        int f_node = 0, f_other = -1, f_idx = 0, f_stage = 0;
+        double f_mind = mind, f_cut = 0.0, f_dst = 0.0;
+        int sp = 1;
         bool overflow = false;
         while (sp > 0) {
-            Frame& F = st[sp - 1];
-            const int nd = F.node;
-            if (F.stage == 0) {
+            const int top = sp - 1;
+            const int nd = lane_get(f_node, top);
+            const int stage = lane_get(f_stage, top);
+            if (stage == 0) {
                 if (t.c1[nd] < 0) {
-                    const double worst = dist[cap - 1];
-                    for (uint32_t i = t.nl[nd]; i < t.nr[nd]; ++i) {
-                        double d = 0.0;
-                        const double d0 = q[0] - t.px[i];
-                        d += d0 * d0;
-                        const double d1 = q[1] - t.py[i];
-                        d += d1 * d1;
-                        const double d2 = q[2] - t.pz[i];
-                        d += d2 * d2;
-                        if (d < worst) {
-                            // KNNResultSet::addPoint
-                            uint32_t j;
-                            for (j = count; j > 0; --j) {
-                                if (dist[j - 1] > d) {
-                                    if (j < cap) {
-                                        dist[j] = dist[j - 1];
-                                        ids[j] = ids[j - 1];
-                                    }
-                                } else
-                                    break;
-                            }
-                            if (j < cap) {
-                                dist[j] = d;
-                                ids[j] = t.vind[i];
-                            }
-                            if (count < cap) ++count;
+                    const Ent wst = ent_shfl(best, static_cast<int>(cap) - 1);
+                    const double worst = wst.d == kInfD ? INFINITY : ent_d2(wst);
+                    const uint32_t l0 = t.nl[nd], l1 = t.nr[nd];
+                    for (uint32_t i0 = l0; i0 < l1; i0 += 64) {
+                        const uint32_t i = i0 + lane;
+                        Ent c = ent_inf();
+                        if (i < l1) {
+                            double d = 0.0;
+                            const double d0 = q0 - t.px[i];
+                            d += d0 * d0;
+                            const double d1 = q1 - t.py[i];
+                            d += d1 * d1;
+                            const double d2 = q2 - t.pz[i];
+                            d += d2 * d2;
+                            if (d < worst)
+                                c = Ent{static_cast<uint64_t>(__double_as_longlong(d)),
+                                        seq + (i - l0), t.vind[i]};
+                        }
+                        if (__ballot(c.d != kInfD)) {
+                            bitonic_sort64(c, lane);
+                            merge_sorted(best, c, lane);
                         }
                     }
+                    seq += l1 - l0;
                     --sp;
                     continue;
                 }
                 const int idx = t.feat[nd];
                 const double val = q[idx];
-                const double diff1 = val - t.divlow[nd];
-                const double diff2 = val - t.divhigh[nd];
-                int best;
+                const double dl = t.divlow[nd], dh = t.divhigh[nd];
+                const double diff1 = val - dl;
+                const double diff2 = val - dh;
+                int bestc, other;
+                double cut;
                 if ((diff1 + diff2) < 0) {
-                    best = t.c1[nd];
-                    F.other = t.c2[nd];
-                    F.cut = (val - t.divhigh[nd]) * (val - t.divhigh[nd]);
+                    bestc = t.c1[nd];
+                    other = t.c2[nd];
+                    cut = (val - dh) * (val - dh);
                 } else {
-                    best = t.c2[nd];
-                    F.other = t.c1[nd];
-                    F.cut = (val - t.divlow[nd]) * (val - t.divlow[nd]);
+                    bestc = t.c2[nd];
+                    other = t.c1[nd];
+                    cut = (val - dl) * (val - dl);
                 }
-                F.idx = idx;
-                F.stage = 1;
-                if (sp >= kMaxDepth) {
+                const double fm = lane_get(f_mind, top);
+                if (lane == top) {
+                    f_other = other;
+                    f_cut = cut;
+                    f_idx = idx;
+                    f_stage = 1;
+                }
+                if (sp >= 64) {
                     overflow = true;
                     break;
                 }
-                st[sp++] = Frame{F.mind, 0.0, 0.0, best, -1, 0, 0};
-            } else if (F.stage == 1) {
-                F.dst = dists[F.idx];
-                const double m2 = F.mind + F.cut - F.dst;
-                dists[F.idx] = F.cut;
-                F.stage = 2;
-                if (m2 * 1.0 <= dist[cap - 1]) {
-                    if (sp >= kMaxDepth) {
+                if (lane == sp) {
+                    f_node = bestc;
+                    f_stage = 0;
+                    f_mind = fm;
+                }
+                ++sp;
+            } else if (stage == 1) {
+                const int idx = lane_get(f_idx, top);
+                const double cut = lane_get(f_cut, top);
+                const double dst = dists[idx];
+                const double m2 = lane_get(f_mind, top) + cut - dst;
+                dists[idx] = cut;
+                if (lane == top) {
+                    f_dst = dst;
+                    f_stage = 2;
+                }
+                const Ent wst = ent_shfl(best, static_cast<int>(cap) - 1);
+                const double worst = wst.d == kInfD ? INFINITY : ent_d2(wst);
+                if (m2 * 1.0 <= worst) {
+                    if (sp >= 64) {
                         overflow = true;
                         break;
                     }
-                    st[sp++] = Frame{m2, 0.0, 0.0, F.other, -1, 0, 0};
+                    const int other = lane_get(f_other, top);
+                    if (lane == sp) {
+                        f_node = other;
+                        f_stage = 0;
+                        f_mind = m2;
+                    }
+                    ++sp;
                 }
             } else {
-                dists[F.idx] = F.dst;
+                dists[lane_get(f_idx, top)] = lane_get(f_dst, top);
                 --sp;
             }
         }
         if (overflow) {
-            atomicAdd(failed, 1u);
+            if (lane == 0) atomicAdd(failed, 1u);
             continue;
         }
-        // KD3ImplT::neighbors: every stride-th id of the result set.
-        const uint32_t stride = static_cast<uint32_t>(prog.stride);
+        // KD3ImplT::neighbors: every stride-th entry of the result set; slots
+        // past the points found keep the output's zero initialisation.
+        const Ent kept = ent_shfl(best, static_cast<int>((static_cast<uint32_t>(lane) * stride) & 63u));
+        const bool mine = static_cast<uint32_t>(lane) < k;
+        const uint32_t kept_id = kept.d == kInfD ? 0u : kept.j;
         if (mode == 1) {
-            for (uint32_t l = 0; l < k; ++l)
-                out.ids[static_cast<uint64_t>(qid) * k + l] = ids[l * stride];
+            if (mine) out.ids[static_cast<uint64_t>(qid) * k + lane] = kept_id;
             continue;
         }
-        double mx = 0.0, my = 0.0, mz = 0.0;
-        for (uint32_t l = 0; l < k; ++l) {
-            const uint32_t id = ids[l * stride];
-            mx += x[id];
-            my += y[id];
-            mz += z[id];
-        }
+        const uint32_t id = mine ? kept_id : 0u;
+        const double px = mine ? x[id] : 0.0, py = mine ? y[id] : 0.0, pz = mine ? z[id] : 0.0;
         const double inv_k = 1.0 / static_cast<double>(k);
-        mx *= inv_k;
-        my *= inv_k;
-        mz *= inv_k;
-        double c[6] = {0, 0, 0, 0, 0, 0};
-        for (uint32_t l = 0; l < k; ++l) {
-            const uint32_t id = ids[l * stride];
-            const double dx = x[id] - mx, dy = y[id] - my, dz = z[id] - mz;
-            c[0] += dx * dx;
-            c[1] += dx * dy;
-            c[2] += dx * dz;
-            c[3] += dy * dy;
-            c[4] += dy * dz;
-            c[5] += dz * dz;
-        }
+        const double mx = wave_sum(px) * inv_k, my = wave_sum(py) * inv_k,
+                     mz = wave_sum(pz) * inv_k;
+        const double dx = mine ? px - mx : 0.0, dy = mine ? py - my : 0.0,
+                     dz = mine ? pz - mz : 0.0;
         const double den = static_cast<double>(k - 1);
-        for (int i = 0; i < 6; ++i)
-            out.cov[static_cast<uint64_t>(i) * cv.n + s] = c[i] / den;
+        const double c0 = wave_sum(dx * dx) / den, c1 = wave_sum(dx * dy) / den,
+                     c2 = wave_sum(dx * dz) / den, c3 = wave_sum(dy * dy) / den,
+                     c4 = wave_sum(dy * dz) / den, c5 = wave_sum(dz * dz) / den;
+        if (lane == 0) {
+            const uint64_t n = cv.n;
+            out.cov[0 * n + s] = c0;
+            out.cov[1 * n + s] = c1;
+            out.cov[2 * n + s] = c2;
+            out.cov[3 * n + s] = c3;
+            out.cov[4 * n + s] = c4;
+            out.cov[5 * n + s] = c5;
+        }
     }
 }
 
+// Every row to the exact search (clouds smaller than the result capacity).
+__global__ __launch_bounds__(256) void k_flag_all(uint64_t n, const uint32_t* __restrict__ perm,
+                                                  int mode, KnnOutputs out)
+{
+    const uint64_t s = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (s == 0) out.nflagged[0] = static_cast<uint32_t>(n);
+    if (s >= n) return;
+    out.flagged[s] = static_cast<uint32_t>(s);
+    if (mode == 1)
+        out.status[0][perm[s]] = 0;
+    else
+        out.flags[s] = 0;
+}
+
 } // namespace
+
+void launch_flag_all(uint64_t n, const uint32_t* perm, int mode, const KnnOutputs& out,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(k_flag_all, dim3(div_up(std::max<uint64_t>(n, 1), 256)), dim3(256), 0, s,
+                       n, perm, mode, out);
+}
 
 uint32_t kd_node_capacity(uint64_t n) { return static_cast<uint32_t>(n / 16 + 1024); }
 
@@ -514,7 +605,8 @@ bool launch_kd_build(const double* x, const double* y, const double* z, uint64_t
         const uint32_t nl = lv1 - lv0;
         const uint32_t gnodes = div_up(nl, kKdThreads);
         hipLaunchKernelGGL(k_kd_minmax_reset, dim3(gnodes), dim3(kKdThreads), 0, s, t, lv0, lv1);
-        hipLaunchKernelGGL(k_kd_minmax, dim3(nb), dim3(kKdThreads), 0, s, t, nn, lv0);
+        hipLaunchKernelGGL(k_kd_minmax, dim3(nl <= kLdsNodes ? std::min<uint32_t>(nb, 2048) : nb),
+                           dim3(kKdThreads), 0, s, t, nn, lv0, lv1);
         launch_fill_u32(w.is_split, nl + 1, 0u, s);
         hipLaunchKernelGGL(k_kd_split, dim3(gnodes), dim3(kKdThreads), 0, s, t, lv0, lv1,
                            w.is_split);
@@ -557,8 +649,9 @@ void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram&
                        uint32_t nflagged, uint32_t* failed, hipStream_t s)
 {
     if (!nflagged) return;
-    hipLaunchKernelGGL(k_kd_resolve, dim3(div_up(nflagged, 64)), dim3(64), 0, s, t, cv, prog, mode,
-                       out, x, y, z, failed);
+    const uint32_t blocks = std::min<uint32_t>(div_up(nflagged, 4), 4096);
+    hipLaunchKernelGGL(k_kd_resolve, dim3(blocks), dim3(256), 0, s, t, cv, prog, mode, out, x, y,
+                       z, failed);
 }
 
 } // namespace gf

@@ -35,13 +35,12 @@
 #include "gf_eigen.hpp"
 #include "gf_internal.hpp"
 #include "gf_kernels.hpp"
+#include "gf_wave.hpp"
 
 namespace gf {
 
 namespace {
 
-constexpr uint64_t kInfD = ~0ull;
-constexpr uint32_t kInfU = ~0u;
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
 constexpr int kRegBatches = 4;
@@ -49,82 +48,6 @@ constexpr uint32_t kRegCand = kRegBatches * 64;
 // Per-wave LDS scratch: 64 compacted entries (16 B), 64 ranked entries, and a
 // 64-entry list of queries left for the growing search.
 constexpr int kScratchWords = 256 + 256 + 64;
-
-struct Ent {
-    uint64_t d;   // bit pattern of the non-negative FP64 squared distance
-    uint32_t id;  // original point index (tie order between equal distances)
-    uint32_t j;   // sorted position (where the coordinates live)
-};
-
-__device__ __forceinline__ Ent ent_inf() { return Ent{kInfD, kInfU, kInfU}; }
-
-__device__ __forceinline__ bool ent_less(const Ent& a, const Ent& b)
-{
-    return a.d < b.d || (a.d == b.d && a.id < b.id);
-}
-
-__device__ __forceinline__ double ent_d2(const Ent& e)
-{
-    return __longlong_as_double(static_cast<long long>(e.d));
-}
-
-__device__ __forceinline__ Ent ent_shfl_xor(const Ent& e, int m)
-{
-    Ent r;
-    r.d = __shfl_xor(static_cast<unsigned long long>(e.d), m);
-    r.id = __shfl_xor(e.id, m);
-    r.j = __shfl_xor(e.j, m);
-    return r;
-}
-
-__device__ __forceinline__ Ent ent_shfl(const Ent& e, int src)
-{
-    Ent r;
-    r.d = __shfl(static_cast<unsigned long long>(e.d), src);
-    r.id = __shfl(e.id, src);
-    r.j = __shfl(e.j, src);
-    return r;
-}
-
-// Ascending bitonic sort of one entry per lane across the wave.
-__device__ __forceinline__ void bitonic_sort64(Ent& e, int lane)
-{
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            Ent o = ent_shfl_xor(e, j);
-            bool up = (lane & k) == 0;
-            bool lower = (lane & j) == 0;
-            bool take = (lower == up) ? ent_less(o, e) : ent_less(e, o);
-            if (take) e = o;
-        }
-    }
-}
-
-// best (ascending) <- the 64 smallest of best U cand (cand ascending).
-__device__ __forceinline__ void merge_sorted(Ent& best, const Ent& cand, int lane)
-{
-    Ent rev = ent_shfl(cand, 63 - lane);
-    if (ent_less(rev, best)) best = rev;
-#pragma unroll
-    for (int j = 32; j > 0; j >>= 1) {
-        Ent o = ent_shfl_xor(best, j);
-        bool lower = (lane & j) == 0;
-        bool take = lower ? ent_less(o, best) : ent_less(best, o);
-        if (take) best = o;
-    }
-}
-
-__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane)
-{
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t t = __shfl_up(v, d);
-        if (lane >= d) v += t;
-    }
-    return v;
-}
 
 __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ a,
                                                     uint32_t lo, uint32_t hi,

@@ -166,6 +166,9 @@ int gf_debug_grid(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, uint32_t *perm,
                   uint32_t *keys, uint32_t *colstart, uint32_t *ncols,
                   uint32_t *key_bits);
 
+/* A plain device-to-device copy of count doubles (the reference kernel that
+ * calibrates HBM byte counters). */
+int gf_debug_copy(gf_ctx *ctx, double *dst, const double *src, uint64_t count);
 /* The cloud's exact KD3Index tree (built on demand): vind (n, tree order ->
  * point id) and the nodes in level order; pass NULL arrays to query the node
  * count.  Leaves have feat = -1. */

@@ -148,6 +148,13 @@ void orc_stage_run(const orc_kd3 *t, const double *x, const double *y,
                    const double *z, uint64_t n, const orc_stage *st,
                    int threads, double *const *cols, uint8_t *status);
 
+/* orc_stage_run restricted to rows [row_begin, row_end) (the CPU baseline's
+ * bounded sample); the tree still covers the whole cloud. */
+void orc_stage_run_range(const orc_kd3 *t, const double *x, const double *y,
+                         const double *z, uint64_t n, const orc_stage *st,
+                         int threads, uint64_t row_begin, uint64_t row_end,
+                         double *const *cols, uint8_t *status);
+
 /* The same stage arithmetic applied to caller-given neighbor lists
  * (nbr: n x kk, uint32).  Used to replay GPU neighbor sets on the CPU. */
 void orc_stage_from_lists(const double *x, const double *y, const double *z,

@@ -431,6 +431,30 @@ static void *stage_worker(void *arg)
     return NULL;
 }
 
+static void run_jobs_range(stage_job proto, uint64_t b0, uint64_t e0, int threads)
+{
+    if (threads < 1) threads = 1;
+    uint64_t n = e0 - b0;
+    uint64_t chunk = n / (uint64_t)threads + ((n % (uint64_t)threads) ? 1 : 0);
+    pthread_t *th = malloc(sizeof(pthread_t) * (size_t)threads);
+    stage_job *jobs = malloc(sizeof(stage_job) * (size_t)threads);
+    for (int i = 0; i < threads; ++i) {
+        jobs[i] = proto;
+        uint64_t b = (uint64_t)i * chunk;
+        uint64_t e = (i + 1 == threads) ? n : (uint64_t)(i + 1) * chunk;
+        jobs[i].begin = b0 + (b > n ? n : b);
+        jobs[i].end = b0 + (e > n ? n : e);
+        if (threads == 1)
+            stage_worker(&jobs[i]);
+        else
+            pthread_create(&th[i], NULL, stage_worker, &jobs[i]);
+    }
+    if (threads > 1)
+        for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    free(th);
+    free(jobs);
+}
+
 static void run_jobs(stage_job proto, uint64_t n, int threads)
 {
     if (threads < 1) threads = 1;
@@ -469,6 +493,26 @@ void orc_stage_run(const orc_kd3 *t, const double *x, const double *y,
     p.cols = cols;
     p.status = status;
     run_jobs(p, n, threads);
+}
+
+void orc_stage_run_range(const orc_kd3 *t, const double *x, const double *y,
+                         const double *z, uint64_t n, const orc_stage *st,
+                         int threads, uint64_t row_begin, uint64_t row_end,
+                         double *const *cols, uint8_t *status)
+{
+    stage_job p;
+    memset(&p, 0, sizeof(p));
+    p.t = t;
+    p.x = x;
+    p.y = y;
+    p.z = z;
+    p.n = n;
+    p.st = st;
+    p.cols = cols;
+    p.status = status;
+    if (row_end > n) row_end = n;
+    if (row_begin > row_end) row_begin = row_end;
+    run_jobs_range(p, row_begin, row_end, threads);
 }
 
 void orc_stage_from_lists(const double *x, const double *y, const double *z,

@@ -104,6 +104,10 @@ class Oracle:
         L.orc_stage_run.argtypes = [ctypes.c_void_p, _dp, _dp, _dp,
                                     ctypes.c_uint64, sp, ctypes.c_int,
                                     ctypes.POINTER(_dp), _u8p]
+        L.orc_stage_run_range.argtypes = [ctypes.c_void_p, _dp, _dp, _dp,
+                                          ctypes.c_uint64, sp, ctypes.c_int,
+                                          ctypes.c_uint64, ctypes.c_uint64,
+                                          ctypes.POINTER(_dp), _u8p]
         L.orc_stage_from_lists.argtypes = [_dp, _dp, _dp, ctypes.c_uint64,
                                            _u32p, ctypes.c_uint64, sp,
                                            ctypes.c_int, ctypes.POINTER(_dp),
@@ -214,6 +218,16 @@ class Oracle:
         self.L.orc_stage_run(tree.h, _ptr(tree.x, _dp), _ptr(tree.y, _dp),
                              _ptr(tree.z, _dp), n, ctypes.byref(st), threads,
                              arr, _ptr(status, _u8p))
+        return cols, status
+
+    def stage_run_range(self, tree, st, begin, end, threads=1, init=0.0):
+        """PDAL's CPU stage over rows [begin, end) of the tree's cloud."""
+        n = tree.n
+        cols, arr = self._cols(st, n, init)
+        status = np.zeros(n, np.uint8)
+        self.L.orc_stage_run_range(tree.h, _ptr(tree.x, _dp), _ptr(tree.y, _dp),
+                                   _ptr(tree.z, _dp), n, ctypes.byref(st), threads,
+                                   begin, end, arr, _ptr(status, _u8p))
         return cols, status
 
     def stage_from_lists(self, x, y, z, nbr, st, threads=8, init=0.0):
