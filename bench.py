@@ -14,6 +14,14 @@ computeCovariance + the Eigen 3x3 solver + the stage formulas) timed on this
 host: the KD build over the whole cloud plus both stages over a bounded
 sample of rows, single-threaded (PDAL's default), extrapolated to the cloud.
 
+Two more legs run by default and are reported beside the headline value:
+  batched  -- 256 seeded tiles (sizes log-uniform in [10k, 2M], knn in
+              {8, 16, 32}, density 2-50 pts/m^2) as one batch vs the same
+              tiles one at a time through the plain path;
+  serving  -- Poisson arrivals of tile requests (sizes log-uniform in
+              [5k, 4M], knn in {10, 16}) into the serving loop: p50/p99
+              latency (submit -> result on the host) and sustained points/s.
+
 Prints ONE JSON line.
 """
 import argparse
@@ -118,6 +126,197 @@ def gpu_leg(args):
     return out, (x, y, z)
 
 
+def gen_tiles(specs, workers=16):
+    """Seeded tiles [(x, y, z)] for specs [(n, tile_seed_index, side)],
+    generated in parallel threads (the generator releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    import gpupdal_amd as g
+
+    def one(spec):
+        n, idx, side = spec
+        return g.synth_cloud(n, tile=idx, side=side)
+    with ThreadPoolExecutor(max_workers=min(workers, os.cpu_count() or 1)) as ex:
+        return list(ex.map(one, specs))
+
+
+def c4_specs(ntiles, seed=0xC4):
+    """The batched configuration: tile sizes log-uniform in [10k, 2M], knn in
+    {8, 16, 32}, density uniform in [2, 50] points/m^2 (side = sqrt(n/density))."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    specs, knns = [], []
+    for i in range(ntiles):
+        n = int(np.exp(rng.uniform(np.log(10_000), np.log(2_000_000))))
+        knn = int(rng.choice([8, 16, 32]))
+        dens = rng.uniform(2.0, 50.0)
+        specs.append((n, 1000 + i, float(np.sqrt(n / dens))))
+        knns.append(knn)
+    return specs, knns
+
+
+def batched_leg(args):
+    """Batched vs plain: the same tiles as one batch (one index build, one
+    search, one exact re-solve, one feature pass) and one tile at a time
+    through the plain path with a synchronisation per tile.  Inputs and
+    outputs resident on the device; every step rebuilds every index."""
+    import numpy as np
+    import gpupdal_amd as g
+    specs, knns = c4_specs(args.tiles)
+    tiles = gen_tiles(specs)
+    n_total = sum(len(t[0]) for t in tiles)
+    ctx = g.Context(0)
+    ctx.set_tuning("timing", 0)
+    stages = [g.stage(g.STAGE_NORMAL, 16), g.stage(g.STAGE_COVFEAT, 16)]
+    ncols = 8
+    b = ctx.batch(tiles)
+    cols = [ctx.alloc(n_total * 8) for _ in range(ncols)]
+    stat = [ctx.alloc(n_total) for _ in range(2)]
+    for _ in range(1):
+        b.invalidate()
+        b.run_device(stages, cols, stat, tile_knn=knns)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        b.invalidate()
+        b.run_device(stages, cols, stat, tile_knn=knns)
+    ctx.sync()
+    ms_batched = (time.perf_counter() - t0) * 1e3 / args.batch_steps
+    b.close()
+    # Plain: one resident cloud per tile, one gf_run per tile (each returns
+    # after its own synchronisation), outputs into the same device columns.
+    clouds = [ctx.cloud(*t) for t in tiles]
+    offs = np.cumsum([0] + [len(t[0]) for t in tiles])
+    def plain_step():
+        for i, cl in enumerate(clouds):
+            cl.invalidate()
+            o = int(offs[i])
+            st = [g.stage(g.STAGE_NORMAL, knns[i]), g.stage(g.STAGE_COVFEAT, knns[i])]
+            cl.run_device(st, [c + 8 * o for c in cols], [s + o for s in stat])
+    plain_step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        plain_step()
+    ctx.sync()
+    ms_plain = (time.perf_counter() - t0) * 1e3 / args.batch_steps
+    for cl in clouds:
+        cl.close()
+    for p in cols + stat:
+        ctx.free(p)
+    ctx.close()
+    return {"tiles": len(tiles), "points": int(n_total),
+            "workload": "normal + covariancefeatures (Dimensionality), per-tile knn in {8,16,32}, "
+                        "tile sizes log-uniform [10k, 2M], density U[2, 50] pts/m^2",
+            "value": n_total / (ms_batched / 1e3), "unit": "points/s",
+            "ms_per_step": ms_batched,
+            "plain_value": n_total / (ms_plain / 1e3), "plain_ms_per_step": ms_plain,
+            "batched_over_plain": ms_plain / ms_batched}
+
+
+def serving_leg(args):
+    """Serving: Poisson arrivals of tile requests at a fixed rate (sizes
+    log-uniform in [5k, 4M], knn in {10, 16}, normal + covariancefeatures)
+    into the serving loop (two workers, 200 GB HBM budget); latency is
+    submit -> result on the host."""
+    import threading
+    import numpy as np
+    import gpupdal_amd as g
+    rng = np.random.default_rng(0xC5)
+    specs, knns = [], []
+    for i in range(args.requests):
+        n = int(np.exp(rng.uniform(np.log(5_000), np.log(4_000_000))))
+        specs.append((n, 5000 + i, float(np.sqrt(n / 16.0))))
+        knns.append(int(rng.choice([10, 16])))
+    tiles = gen_tiles(specs)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(tiles)))
+    stages = [g.stage(g.STAGE_NORMAL, 10), g.stage(g.STAGE_COVFEAT, 10)]
+    srv = g.Server(workers=2, hbm_budget_bytes=200 << 30)
+    # Warm-up request (contexts, code objects, first allocations).
+    t = srv.submit(*tiles[0], stages, knns[0])
+    srv.result(t)
+    tickets = [None] * len(tiles)
+
+    def producer():
+        t0 = time.perf_counter()
+        for i, tile in enumerate(tiles):
+            delay = arrivals[i] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(*tile, stages, knns[i])
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    lat, first, last, batches = [], None, None, set()
+    for tk in tickets:
+        _, _, (ts, tb, td, bid, bt) = srv.result(tk)
+        lat.append(td - ts)
+        first = ts if first is None else min(first, ts)
+        last = td if last is None else max(last, td)
+        batches.add(bid)
+    srv.close()
+    pts = sum(len(t[0]) for t in tiles)
+    lat_ms = np.array(lat) * 1e3
+    return {"requests": len(tiles), "rate_per_s": args.rate, "points": int(pts),
+            "workload": "Poisson arrivals, sizes log-uniform [5k, 4M], knn in {10, 16}, "
+                        "normal + covariancefeatures (Dimensionality), 2 workers, "
+                        "200 GB HBM budget, host arrays in and out",
+            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+            "max_ms": float(lat_ms.max()),
+            "sustained_points_per_s": pts / (last - first), "batches": len(batches)}
+
+
+def precision_leg(args):
+    """Precision stress: 20M points with 5% exactly planar roofs and 2%
+    collinear wires, eigenvalues + covariancefeatures knn=32 chained; every
+    column against the extended-precision truth computed on the host from
+    the same KD3Index neighbor lists.  Per feature: max and p99 relative
+    error over well-conditioned rows (eigengap > 1e-3, smallest eigenvalue
+    resolved), max absolute error over degenerate rows."""
+    import numpy as np
+    import gpupdal_amd as g
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    n, knn = args.precision_points, 32
+    side = float(np.sqrt(n / 80.0))
+    x, y, z = g.synth_cloud(n, tile=3, side=side, roof_frac=0.05, wire_frac=0.02)
+    ctx = g.Context(0)
+    cl = ctx.cloud(x, y, z)
+    stages = [g.stage(g.STAGE_EIGENVALUES, knn), g.stage(g.STAGE_COVFEAT, knn)]
+    t0 = time.perf_counter()
+    (ev, cf), (evs, cfs) = cl.run(stages, init=np.nan)
+    ms = (time.perf_counter() - t0) * 1e3
+    st = ctx.stats()
+    cl.close()
+    ctx.close()
+    orc = ol.Oracle()
+    tree = orc.kd3(x, y, z)
+    nbr = tree.neighbors_all(knn + 1, threads=os.cpu_count() or 1)
+    del tree
+    report = {"points": n, "knn": knn, "roof_frac": 0.05, "wire_frac": 0.02,
+              "gpu_ms_host_io": ms, "tie_rows": int(st["tie_rows"]),
+              "incomplete_rows": int(st["incomplete_rows"]), "columns": {}}
+    for stage_cols, stat, kind in ((ev, evs, ol.STAGE_EIGENVALUES), (cf, cfs, ol.STAGE_COVFEAT)):
+        ost = ol.stage(kind, knn)
+        tcols, tstat, gap, ratio = orc.truth_from_lists(x, y, z, nbr, ost,
+                                                        threads=os.cpu_count() or 1,
+                                                        with_ratio=True)
+        ok = ((stat & 0x3C) == 0) & (tstat == 0)
+        well = ok & (gap > 1e-3) & (ratio > 1e-10)
+        deg = ok & (ratio <= 1e-10)
+        names = (["Eigenvalue0", "Eigenvalue1", "Eigenvalue2"] if kind == ol.STAGE_EIGENVALUES
+                 else [ol.FEATURE_NAMES[b] for b in range(4)])
+        for name, c, t in zip(names, stage_cols, tcols):
+            rel = np.abs(c[well] - t[well]) / np.maximum(np.abs(t[well]), 1e-6)
+            ab = np.abs(c[deg] - t[deg])
+            report["columns"][name] = {
+                "well_rows": int(well.sum()), "max_rel": float(rel.max()) if rel.size else 0.0,
+                "p99_rel": float(np.percentile(rel, 99)) if rel.size else 0.0,
+                "degenerate_rows": int(deg.sum()),
+                "degenerate_max_abs": float(ab.max()) if ab.size else 0.0}
+        report.setdefault("skipped_rows", {})[names[0][:5]] = int((~ok).sum())
+    return report
+
+
 def cpu_leg(args, xyz):
     """PDAL's CPU path (the oracle's restatement), single-threaded, on a
     bounded sample of rows; the KD build covers the whole cloud."""
@@ -153,17 +352,30 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (counter calibration)")
+    ap.add_argument("--tiles", type=int, default=256, help="tiles of the batched leg")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--requests", type=int, default=64, help="requests of the serving leg")
+    ap.add_argument("--rate", type=float, default=120.0, help="serving arrivals per second")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--precision", action="store_true",
+                    help="only the precision-stress report (20M points, roofs, wires)")
+    ap.add_argument("--precision-points", type=int, default=20_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1. Tiles are "
                           "independent and would shard across GPUs without collectives."}))
         sys.exit(2)
 
+    if args.precision:
+        print(json.dumps({"precision": precision_leg(args)}))
+        return
     gpu, xyz = gpu_leg(args)
+    batched = None if args.no_batched else batched_leg(args)
+    serving = None if args.no_serving else serving_leg(args)
     cpu = None if args.no_cpu else cpu_leg(args, xyz)
     line = {
-        "metric": "points/s kNN+covariance features (normal+covariancefeatures knn=16, "
-                  "4M-point ALS tile, plain single-cloud GPU path)",
+        "metric": "points/s kNN+covariance features, batched vs plain GPU; p50/p99 tile latency",
         "value": gpu["value"],
         "unit": "points/s",
         "n_gpus": 1,
@@ -184,6 +396,8 @@ def main():
         "phases_ms": gpu["phases_ms"],
         "rows_resolved_exactly": gpu["rows"],
         "grid": gpu["grid"],
+        "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     if cpu:

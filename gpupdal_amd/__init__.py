@@ -62,6 +62,10 @@ EXPORTED = [
     "gf_run", "gf_last_stats", "gf_knn", "gf_debug_eigen", "gf_debug_grid",
     "gf_synth_defaults", "gf_tile_seed", "gf_synth_cloud", "gf_dev_alloc", "gf_dev_free",
     "gf_memcpy", "gf_cloud_invalidate", "gf_debug_kdtree", "gf_debug_copy",
+    "gf_batch_create", "gf_batch_set_tile", "gf_batch_size", "gf_batch_tile_start",
+    "gf_batch_invalidate", "gf_batch_run", "gf_batch_release",
+    "gf_server_create", "gf_server_submit", "gf_server_wait", "gf_server_request_info",
+    "gf_server_release", "gf_server_destroy",
 ]
 
 
@@ -100,6 +104,25 @@ def load_library(path=LIB_PATH):
     L.gf_tile_seed.restype = ctypes.c_uint64
     L.gf_tile_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
     L.gf_synth_cloud.argtypes = [ctypes.POINTER(GfSynth), _dp, _dp, _dp]
+    L.gf_batch_create.argtypes = [vp, ctypes.c_uint32, vp, ctypes.POINTER(vp)]
+    L.gf_batch_set_tile.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, ctypes.c_int]
+    L.gf_batch_size.restype = ctypes.c_uint64
+    L.gf_batch_size.argtypes = [vp]
+    L.gf_batch_tile_start.restype = ctypes.c_uint64
+    L.gf_batch_tile_start.argtypes = [vp, ctypes.c_uint32]
+    L.gf_batch_invalidate.argtypes = [vp]
+    L.gf_batch_run.argtypes = [vp, vp, ctypes.POINTER(GfStage), ctypes.c_int, vp,
+                               ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int]
+    L.gf_batch_release.argtypes = [vp]
+    L.gf_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
+                                   ctypes.POINTER(vp)]
+    L.gf_server_submit.argtypes = [vp, vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(GfStage),
+                                   ctypes.c_int, ctypes.c_int32, ctypes.POINTER(vp),
+                                   ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_uint64)]
+    L.gf_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_double]
+    L.gf_server_request_info.argtypes = [vp, ctypes.c_uint64, _dp, _u32p, _u32p]
+    L.gf_server_release.argtypes = [vp, ctypes.c_uint64]
+    L.gf_server_destroy.argtypes = [vp]
     return L
 
 
@@ -198,6 +221,10 @@ class Context:
 
     def cloud(self, x, y, z):
         return Cloud(self, x, y, z)
+
+    def batch(self, tiles):
+        """A batch of tiles [(x, y, z), ...] resident on the device."""
+        return Batch(self, tiles)
 
     def alloc(self, nbytes):
         p = ctypes.c_void_p()
@@ -319,3 +346,135 @@ class Cloud:
                                                  ctypes.byref(ncols), ctypes.byref(bits)),
                         "gf_debug_grid")
         return perm, keys, cs, bits.value
+
+
+class Batch:
+    """Independent tiles run together (one index build, one search, one
+    exact re-solve and one feature pass); each tile's rows equal the tile run
+    alone.  Points of tile t are rows starts[t]:starts[t+1] of every output."""
+
+    def __init__(self, ctx, tiles):
+        self.ctx = ctx
+        L = ctx.L
+        counts = np.array([len(t[0]) for t in tiles], dtype=np.uint64)
+        h = ctypes.c_void_p()
+        ctx._check(L.gf_batch_create(ctx.h, len(tiles), counts.ctypes.data, ctypes.byref(h)),
+                   "gf_batch_create")
+        self.h = h
+        self.ntiles = len(tiles)
+        for i, (x, y, z) in enumerate(tiles):
+            x, y, z = _c64(x), _c64(y), _c64(z)
+            ctx._check(L.gf_batch_set_tile(h, i, x.ctypes.data, y.ctypes.data, z.ctypes.data,
+                                           WHERE_HOST), "gf_batch_set_tile")
+        self.starts = [int(L.gf_batch_tile_start(h, i)) for i in range(self.ntiles + 1)]
+        self.n = int(L.gf_batch_size(h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.gf_batch_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def invalidate(self):
+        self.ctx.L.gf_batch_invalidate(self.h)
+
+    def _knn_arg(self, tile_knn):
+        if tile_knn is None:
+            return None, None
+        a = np.ascontiguousarray(tile_knn, dtype=np.int32)
+        assert len(a) == self.ntiles
+        return a, a.ctypes.data
+
+    def run(self, stages, tile_knn=None, init=0.0):
+        """Returns (per-stage column lists over the whole batch, per-stage
+        status arrays)."""
+        stages = list(stages)
+        arr = (GfStage * len(stages))(*stages)
+        cols, per_stage = [], []
+        for st in stages:
+            c = [np.full(self.n, init, np.float64) for _ in range(stage_columns(st))]
+            per_stage.append(c)
+            cols.extend(c)
+        status = [np.zeros(self.n, np.uint8) for _ in stages]
+        cp = (ctypes.c_void_p * max(1, len(cols)))(*[c.ctypes.data for c in cols])
+        sp = (ctypes.c_void_p * len(status))(*[s.ctypes.data for s in status])
+        keep, kp = self._knn_arg(tile_knn)
+        self.ctx._check(self.ctx.L.gf_batch_run(self.ctx.h, self.h, arr, len(stages), kp, cp, sp,
+                                                WHERE_HOST), "gf_batch_run")
+        return per_stage, status
+
+    def run_device(self, stages, cols, status, tile_knn=None):
+        stages = list(stages)
+        arr = (GfStage * len(stages))(*stages)
+        cp = (ctypes.c_void_p * max(1, len(cols)))(*cols)
+        sp = (ctypes.c_void_p * len(status))(*status)
+        keep, kp = self._knn_arg(tile_knn)
+        self.ctx._check(self.ctx.L.gf_batch_run(self.ctx.h, self.h, arr, len(stages), kp, cp, sp,
+                                                WHERE_DEVICE), "gf_batch_run")
+
+    def tile(self, arr, t):
+        """Rows of tile t in an output array."""
+        return arr[self.starts[t]:self.starts[t + 1]]
+
+
+class Server:
+    """The serving loop: submit tiles at any time, wait for their results.
+    Host arrays passed to submit() are kept alive here until the request is
+    released."""
+
+    def __init__(self, device=0, workers=2, hbm_budget_bytes=0, max_batch_points=0):
+        self.L = lib()
+        h = ctypes.c_void_p()
+        rc = self.L.gf_server_create(device, workers, hbm_budget_bytes, max_batch_points,
+                                     ctypes.byref(h))
+        if rc != GF_OK:
+            raise GfError(rc, "gf_server_create failed (is a GPU visible?)")
+        self.h = h
+        self._held = {}
+
+    def submit(self, x, y, z, stages, knn, init=0.0):
+        """Queues one tile; returns a ticket.  Outputs come from result()."""
+        x, y, z = _c64(x), _c64(y), _c64(z)
+        n = len(x)
+        stages = list(stages)
+        arr = (GfStage * len(stages))(*stages)
+        cols, per_stage = [], []
+        for st in stages:
+            c = [np.full(n, init, np.float64) for _ in range(stage_columns(st))]
+            per_stage.append(c)
+            cols.extend(c)
+        status = [np.zeros(n, np.uint8) for _ in stages]
+        cp = (ctypes.c_void_p * max(1, len(cols)))(*[c.ctypes.data for c in cols])
+        sp = (ctypes.c_void_p * len(status))(*[s.ctypes.data for s in status])
+        t = ctypes.c_uint64()
+        rc = self.L.gf_server_submit(self.h, x.ctypes.data, y.ctypes.data, z.ctypes.data, n, arr,
+                                     len(stages), int(knn), cp, sp, ctypes.byref(t))
+        if rc != GF_OK:
+            raise GfError(rc, "gf_server_submit")
+        self._held[t.value] = (x, y, z, arr, cp, sp, per_stage, status)
+        return t.value
+
+    def result(self, ticket, timeout=-1.0):
+        """Waits; returns (per-stage columns, statuses, times) with times =
+        (submit, batch start, done, batch id, tiles in the batch)."""
+        rc = self.L.gf_server_wait(self.h, ticket, float(timeout))
+        if rc != GF_OK:
+            raise GfError(rc, f"request {ticket}")
+        tm = np.zeros(3)
+        bid = ctypes.c_uint32()
+        bt = ctypes.c_uint32()
+        self.L.gf_server_request_info(self.h, ticket, tm.ctypes.data_as(_dp), ctypes.byref(bid),
+                                      ctypes.byref(bt))
+        held = self._held.pop(ticket)
+        self.L.gf_server_release(self.h, ticket)
+        return held[6], held[7], (tm[0], tm[1], tm[2], bid.value, bt.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gf_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

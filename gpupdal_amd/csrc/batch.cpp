@@ -1,0 +1,456 @@
+// batch.cpp -- batches of independent tiles behind the C ABI: tiles of
+// different sizes and neighborhood sizes share one index build (per-tile
+// grids, a tile-segmented radix sort), one search launch, one exact-tree
+// forest and one feature launch.  A tile's rows depend only on that tile, so
+// every result equals the tile run alone.
+#include "gpupdal_amd.h"
+
+#include "gf_internal.hpp"
+#include "gf_kernels.hpp"
+#include "gf_runtime.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace gf {
+
+namespace {
+
+double dec_order(unsigned long long u)
+{
+    const unsigned long long b = (u >> 63) ? (u ^ 0x8000000000000000ull) : ~u;
+    double d;
+    std::memcpy(&d, &b, 8);
+    return d;
+}
+
+} // namespace
+
+// New tile sizes: tile starts, segment blocks (each tile cut into blocks of
+// at most 4096 points) and grow-only point buffers.  The index is dropped.
+void Batch::reset(uint32_t ntiles, const uint64_t* counts, hipStream_t s)
+{
+    T = ntiles;
+    starts.assign(static_cast<size_t>(ntiles) + 1, 0);
+    for (uint32_t t = 0; t < ntiles; ++t) starts[t + 1] = starts[t] + counts[t];
+    n = starts[ntiles];
+    if (n >= (1ull << 32) - 1)
+        throw Error(GF_ERR_UNSUPPORTED, "batches are limited to 2^32-2 points");
+    blocks.clear();
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        const uint64_t nt = counts[t];
+        const uint32_t nbt = static_cast<uint32_t>(div_up(nt, kSegBlockPoints));
+        const uint32_t first = static_cast<uint32_t>(blocks.size());
+        for (uint32_t i = 0; i < nbt; ++i) {
+            SegBlock sb{};
+            sb.begin = static_cast<uint32_t>(starts[t] + static_cast<uint64_t>(i) * kSegBlockPoints);
+            sb.count = static_cast<uint32_t>(
+                std::min<uint64_t>(kSegBlockPoints, nt - static_cast<uint64_t>(i) * kSegBlockPoints));
+            sb.tile = t;
+            sb.hbase = 256u * first;
+            sb.nbt = nbt;
+            sb.lb = i;
+            blocks.push_back(sb);
+        }
+    }
+    x.ensure(n * 8 + 8);
+    y.ensure(n * 8 + 8);
+    z.ensure(n * 8 + 8);
+    dstarts.ensure(starts.size() * 8 + 8);
+    GF_HIP(hipMemcpyAsync(dstarts.ptr, starts.data(), starts.size() * 8, hipMemcpyHostToDevice, s));
+    dblocks.ensure(blocks.size() * sizeof(SegBlock) + 8);
+    if (!blocks.empty())
+        GF_HIP(hipMemcpyAsync(dblocks.ptr, blocks.data(), blocks.size() * sizeof(SegBlock),
+                              hipMemcpyHostToDevice, s));
+    GF_HIP(hipStreamSynchronize(s));
+    idx_valid = false;
+    kd_valid = false;
+}
+
+void Context::build_batch_index(Batch& b, const std::vector<uint32_t>& fetch)
+{
+    if (b.idx_valid && b.idx_fetch == fetch && b.idx_tune == tune.signature()) return;
+    b.idx_valid = false;
+    b.kd_valid = false;
+    const uint32_t T = b.T;
+    const uint64_t N = b.n;
+    const uint32_t nb = static_cast<uint32_t>(b.blocks.size());
+    hipEvent_t e0 = ev_begin();
+
+    // Per-tile bounds (order-encoded min/max), read back for the grids.
+    std::vector<unsigned long long> init(static_cast<size_t>(T) * 6);
+    for (uint32_t t = 0; t < T; ++t)
+        for (int d = 0; d < 3; ++d) {
+            init[6 * t + d] = ~0ull;
+            init[6 * t + 3 + d] = 0ull;
+        }
+    b.benc.ensure(init.size() * 8 + 8);
+    GF_HIP(hipMemcpyAsync(b.benc.ptr, init.data(), init.size() * 8, hipMemcpyHostToDevice,
+                          stream));
+    launch_tile_bounds(b.x.as<double>(), b.y.as<double>(), b.z.as<double>(),
+                       b.dblocks.as<SegBlock>(), nb, b.benc.as<unsigned long long>(), stream);
+    GF_HIP(hipMemcpyAsync(init.data(), b.benc.ptr, init.size() * 8, hipMemcpyDeviceToHost,
+                          stream));
+    GF_HIP(hipStreamSynchronize(stream));
+
+    // Grids on the host (the same rule as a single cloud), column bases.
+    b.grids.assign(T, TileGrid{});
+    b.bounds.assign(static_cast<size_t>(T) * 6, 0.0);
+    uint64_t colbase = 0;
+    uint32_t key_bits = 0;
+    for (uint32_t t = 0; t < T; ++t) {
+        double* bd = &b.bounds[6 * t];
+        const uint64_t nt = b.starts[t + 1] - b.starts[t];
+        for (int i = 0; i < 6; ++i) bd[i] = nt ? dec_order(init[6 * t + i]) : 0.0;
+        TileGrid& tg = b.grids[t];
+        tg.g = make_grid(bd, nt, fetch[t], tune);
+        tg.start = b.starts[t];
+        tg.n = nt;
+        tg.colbase = colbase;
+        colbase += static_cast<uint64_t>(tg.g.ncols) + 1;
+        key_bits = std::max(key_bits, tg.g.key_bits);
+    }
+    b.col_entries = colbase;
+    if (colbase >= (1ull << 32))
+        throw Error(GF_ERR_UNSUPPORTED, "batch column table above 2^32 entries");
+    b.dgrids.ensure(static_cast<uint64_t>(T) * sizeof(TileGrid) + 8);
+    GF_HIP(hipMemcpyAsync(b.dgrids.ptr, b.grids.data(), static_cast<uint64_t>(T) * sizeof(TileGrid),
+                          hipMemcpyHostToDevice, stream));
+    b.dbounds.ensure(b.bounds.size() * 8 + 8);
+    GF_HIP(hipMemcpyAsync(b.dbounds.ptr, b.bounds.data(), b.bounds.size() * 8,
+                          hipMemcpyHostToDevice, stream));
+
+    // Keys, tile-segmented sort, column starts, sorted coordinates.
+    b.keys_a.ensure(N * 4 + 8);
+    b.vals_a.ensure(N * 4 + 8);
+    b.keys_b.ensure(N * 4 + 8);
+    b.vals_b.ensure(N * 4 + 8);
+    b.colstart.ensure(colbase * 4 + 8);
+    b.sx.ensure(N * 8 + 8);
+    b.sy.ensure(N * 8 + 8);
+    b.sz.ensure(N * 8 + 8);
+    b.hist.ensure(static_cast<uint64_t>(nb) * 256 * 4 + 8);
+    scan_tmp.ensure(scan_scratch_words(static_cast<uint64_t>(nb) * 256 + 1) * 4);
+    launch_tile_keys(b.x.as<double>(), b.y.as<double>(), b.z.as<double>(),
+                     b.dblocks.as<SegBlock>(), nb, b.dgrids.as<TileGrid>(),
+                     b.keys_a.as<uint32_t>(), b.vals_a.as<uint32_t>(), stream);
+    uint32_t *kres, *vres;
+    launch_seg_radix_sort(b.keys_a.as<uint32_t>(), b.vals_a.as<uint32_t>(),
+                          b.keys_b.as<uint32_t>(), b.vals_b.as<uint32_t>(),
+                          b.dblocks.as<SegBlock>(), nb, key_bits, b.hist.as<uint32_t>(),
+                          scan_tmp.as<uint32_t>(), &kres, &vres, stream);
+    launch_tile_colstart(kres, b.dgrids.as<TileGrid>(), T, colbase, b.colstart.as<uint32_t>(),
+                         stream);
+    launch_gather_sorted(vres, b.x.as<double>(), b.y.as<double>(), b.z.as<double>(), N,
+                         b.sx.as<double>(), b.sy.as<double>(), b.sz.as<double>(), stream);
+    b.keys = kres;
+    b.perm = vres;
+    b.idx_valid = true;
+    b.idx_fetch = fetch;
+    b.idx_tune = tune.signature();
+    stats.ms_index = ev_end(e0);
+}
+
+void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* tile_knn,
+                        double* const* cols, uint8_t* const* status, int where)
+{
+    const uint32_t T = b.T;
+    const uint64_t N = b.n;
+    // One neighborhood group: every stage shares the stride and, per tile,
+    // the knn (tile_knn overrides the stages' knn).
+    const int stride = st[0].stage == GF_STAGE_NORMAL ? 1 : st[0].stride;
+    for (int i = 1; i < ns; ++i) {
+        const int si = st[i].stage == GF_STAGE_NORMAL ? 1 : st[i].stride;
+        if (si != stride || (!tile_knn && st[i].knn != st[0].knn))
+            throw Error(GF_ERR_UNSUPPORTED,
+                        "a batch runs stages that share one neighborhood (knn and stride)");
+    }
+    FusedProgram base{};
+    base.nstages = ns;
+    base.stride = stride;
+    int ncol_total = 0;
+    for (int i = 0; i < ns; ++i) {
+        StageProgram& sp = base.st[i];
+        sp.stage = st[i].stage;
+        sp.mode = st[i].mode;
+        sp.features = st[i].features;
+        sp.always_up = st[i].always_up;
+        sp.normalize = st[i].normalize;
+        sp.ncols = stage_ncols(st[i]);
+        sp.col_offset = ncol_total;
+        ncol_total += sp.ncols;
+    }
+    if (ncol_total > 16) throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
+
+    // Per-tile programs.
+    std::vector<TileInfo> tiles(T);
+    std::vector<uint32_t> fetch(T);
+    int max_k = 1;
+    bool any_exact_tile = false;
+    for (uint32_t t = 0; t < T; ++t) {
+        const uint64_t nt = b.starts[t + 1] - b.starts[t];
+        const int64_t knn = tile_knn ? tile_knn[t] : st[0].knn;
+        if (knn < 1) throw Error(GF_ERR_ARG, "knn must be at least 1");
+        const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(nt, static_cast<uint64_t>(knn) + 1));
+        const uint32_t f = static_cast<uint32_t>(stride) * k;
+        if (f > static_cast<uint32_t>(kMaxFetch - 1))
+            throw Error(GF_ERR_UNSUPPORTED,
+                        "(knn+1)*stride above 63 is outside the device kNN envelope");
+        fetch[t] = std::max<uint32_t>(f, 1);
+        TileInfo& ti = tiles[t];
+        ti.prog = base;
+        ti.prog.k = static_cast<int32_t>(k);
+        ti.prog.fetch = static_cast<int32_t>(f);
+        ti.start = b.starts[t];
+        ti.n = nt;
+        ti.root = t;
+        ti.all_exact = (nt > 0 && f > nt) ? 1u : 0u;
+        any_exact_tile = any_exact_tile || ti.all_exact;
+        max_k = std::max<int>(max_k, static_cast<int>(k));
+    }
+    build_batch_index(b, fetch);
+    const int Q = knn_queries_per_wave(max_k);
+    std::vector<WaveTask> tasks;
+    for (uint32_t t = 0; t < T; ++t) {
+        TileInfo& ti = tiles[t];
+        ti.g = b.grids[t].g;
+        ti.colbase = b.grids[t].colbase;
+        const GridParams& g = ti.g;
+        ti.margin = 1e-6 * std::fmin(g.c, g.cz) +
+                    1e-15 * (std::fabs(g.ox) + std::fabs(g.oy) + std::fabs(g.oz)) + 1e-300;
+        if (ti.all_exact || !ti.n) continue;
+        for (uint64_t q = 0; q < ti.n; q += Q)
+            tasks.push_back(WaveTask{ti.start + q, t,
+                                     static_cast<uint32_t>(std::min<uint64_t>(Q, ti.n - q))});
+    }
+    b.dtiles.ensure(static_cast<uint64_t>(T) * sizeof(TileInfo) + 8);
+    GF_HIP(hipMemcpyAsync(b.dtiles.ptr, tiles.data(), static_cast<uint64_t>(T) * sizeof(TileInfo),
+                          hipMemcpyHostToDevice, stream));
+    b.dtasks.ensure(tasks.size() * sizeof(WaveTask) + 16);
+    if (!tasks.empty())
+        GF_HIP(hipMemcpyAsync(b.dtasks.ptr, tasks.data(), tasks.size() * sizeof(WaveTask),
+                              hipMemcpyHostToDevice, stream));
+    TileTables tt{};
+    tt.tiles = b.dtiles.as<TileInfo>();
+    tt.starts = b.dstarts.as<uint64_t>();
+    tt.ntiles = T;
+    tt.tasks = b.dtasks.as<WaveTask>();
+    tt.ntasks = static_cast<uint32_t>(tasks.size());
+    tt.max_k = max_k;
+    tt.q_per_wave = Q;
+
+    KnnOutputs out{};
+    if (where == GF_WHERE_DEVICE) {
+        for (int c = 0; c < ncol_total; ++c) out.col[c] = cols[c];
+        for (int i = 0; i < ns; ++i) out.status[i] = status[i];
+    } else {
+        outbuf.ensure(static_cast<uint64_t>(ncol_total) * N * 8 + 8);
+        statbuf.ensure(static_cast<uint64_t>(ns) * N + 8);
+        for (int c = 0; c < ncol_total; ++c) {
+            out.col[c] = outbuf.as<double>() + static_cast<uint64_t>(c) * N;
+            GF_HIP(hipMemcpyAsync(out.col[c], cols[c], N * 8, hipMemcpyHostToDevice, stream));
+        }
+        for (int i = 0; i < ns; ++i) out.status[i] = statbuf.as<uint8_t>() + i * N;
+    }
+    flagged.ensure(N * 4 + 4);
+    counters.ensure(64);
+    covbuf.ensure(N * 48 + 8);
+    flagbuf.ensure(N + 8);
+    out.flagged = flagged.as<uint32_t>();
+    out.nflagged = counters.as<uint32_t>();
+    out.cov = covbuf.as<double>();
+    out.flags = flagbuf.as<uint8_t>();
+    launch_fill_u32(out.nflagged, 4, 0u, stream);
+    if (tune.collect_stats) {
+        out.counters = reinterpret_cast<unsigned long long*>(counters.as<uint32_t>() + 8);
+        launch_fill_u32(counters.as<uint32_t>() + 8, 8, 0u, stream);
+    }
+
+    CloudView cv{};
+    cv.sx = b.sx.as<double>();
+    cv.sy = b.sy.as<double>();
+    cv.sz = b.sz.as<double>();
+    cv.perm = b.perm;
+    cv.keys = b.keys;
+    cv.colstart = b.colstart.as<uint32_t>();
+    cv.n = N;
+    cv.g = b.grids.empty() ? GridParams{} : b.grids[0].g;
+    FusedProgram prog0 = tiles.empty() ? base : tiles[0].prog;
+
+    hipEvent_t e0 = ev_begin();
+    launch_knn(cv, prog0, 0, out, knn_tuning(), stream, &tt);
+    if (any_exact_tile) launch_flag_tiles(N, cv.perm, 0, out, tt, stream);
+    stats.ms_search = ev_end(e0);
+
+    // Exact re-solve of flagged rows on the tiles' KD3 trees.
+    uint32_t h[4] = {0, 0, 0, 0};
+    GF_HIP(hipMemcpyAsync(h, out.nflagged, sizeof(h), hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipStreamSynchronize(stream));
+    stats.tie_rows = h[1];
+    stats.incomplete_rows = h[2];
+    if (h[0]) {
+        e0 = ev_begin();
+        if (!b.kd_valid) {
+            build_forest(b.kd, b.x.as<double>(), b.y.as<double>(), b.z.as<double>(), N,
+                         b.dstarts.as<uint64_t>(), T, b.dbounds.as<double>());
+            b.kd_valid = true;
+        }
+        uint32_t* failed = counters.as<uint32_t>() + 5;
+        launch_fill_u32(failed, 1, 0u, stream);
+        launch_kd_resolve(b.kd.view, cv, prog0, 0, out, b.x.as<double>(), b.y.as<double>(),
+                          b.z.as<double>(), h[0], failed, stream, &tt);
+        uint32_t nfail = 0;
+        GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, stream));
+        GF_HIP(hipStreamSynchronize(stream));
+        stats.ms_exact = ev_end(e0);
+        if (nfail)
+            throw Error(GF_ERR_UNSUPPORTED, "exact KD3 search exceeded its depth bound on " +
+                                                std::to_string(nfail) + " rows");
+    }
+    e0 = ev_begin();
+    launch_features(cv, prog0, out, stream, &tt);
+    stats.ms_features = ev_end(e0);
+
+    unsigned long long ctr[4] = {0, 0, 0, 0};
+    if (out.counters)
+        GF_HIP(hipMemcpyAsync(ctr, out.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
+    if (where == GF_WHERE_HOST) {
+        for (int c = 0; c < ncol_total; ++c)
+            GF_HIP(hipMemcpyAsync(cols[c], out.col[c], N * 8, hipMemcpyDeviceToHost, stream));
+        for (int i = 0; i < ns; ++i)
+            GF_HIP(hipMemcpyAsync(status[i], out.status[i], N, hipMemcpyDeviceToHost, stream));
+    }
+    GF_HIP(hipStreamSynchronize(stream));
+    stats.points = N;
+    stats.search_passes = ctr[0];
+    stats.candidates = ctr[1];
+    stats.merges = ctr[2];
+}
+
+} // namespace gf
+
+// ------------------------------------------------------------------ ABI ----
+using namespace gf;
+
+namespace {
+
+template <class F>
+int bguarded(gf_ctx* ctx, F&& f)
+{
+    try {
+        f();
+        return GF_OK;
+    } catch (const gf::Error& e) {
+        if (ctx) reinterpret_cast<Context*>(ctx)->last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) reinterpret_cast<Context*>(ctx)->last_error = "out of host memory";
+        return GF_ERR_OOM;
+    } catch (const std::exception& e) {
+        if (ctx) reinterpret_cast<Context*>(ctx)->last_error = e.what();
+        return GF_ERR_STATE;
+    }
+}
+
+Batch* B(gf_batch* b) { return reinterpret_cast<Batch*>(b); }
+
+} // namespace
+
+extern "C" {
+
+int gf_batch_create(gf_ctx* ctx, uint32_t ntiles, const uint64_t* counts, gf_batch** out)
+{
+    if (!ctx || !out || (ntiles && !counts)) return GF_ERR_ARG;
+    *out = nullptr;
+    return bguarded(ctx, [&] {
+        Context* c = reinterpret_cast<Context*>(ctx);
+        GF_HIP(hipSetDevice(c->device));
+        Batch* b = new Batch(c);
+        try {
+            b->reset(ntiles, counts, c->stream);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = reinterpret_cast<gf_batch*>(b);
+    });
+}
+
+int gf_batch_set_tile(gf_batch* batch, uint32_t tile, const double* x, const double* y,
+                      const double* z, int where)
+{
+    if (!batch) return GF_ERR_ARG;
+    Batch* b = B(batch);
+    gf_ctx* ctx = reinterpret_cast<gf_ctx*>(b->ctx);
+    if (tile >= b->T) return GF_ERR_ARG;
+    const uint64_t nt = b->starts[tile + 1] - b->starts[tile];
+    if (nt && (!x || !y || !z)) return GF_ERR_ARG;
+    return bguarded(ctx, [&] {
+        Context* c = b->ctx;
+        const hipMemcpyKind kind =
+            where == GF_WHERE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        const uint64_t off = b->starts[tile] * 8;
+        if (nt) {
+            GF_HIP(hipMemcpyAsync(static_cast<char*>(b->x.ptr) + off, x, nt * 8, kind, c->stream));
+            GF_HIP(hipMemcpyAsync(static_cast<char*>(b->y.ptr) + off, y, nt * 8, kind, c->stream));
+            GF_HIP(hipMemcpyAsync(static_cast<char*>(b->z.ptr) + off, z, nt * 8, kind, c->stream));
+        }
+        GF_HIP(hipStreamSynchronize(c->stream));
+        b->idx_valid = false;
+        b->kd_valid = false;
+    });
+}
+
+uint64_t gf_batch_size(const gf_batch* batch)
+{
+    return batch ? reinterpret_cast<const Batch*>(batch)->n : 0;
+}
+
+uint64_t gf_batch_tile_start(const gf_batch* batch, uint32_t tile)
+{
+    if (!batch) return 0;
+    const Batch* b = reinterpret_cast<const Batch*>(batch);
+    return tile <= b->T ? b->starts[tile] : b->n;
+}
+
+int gf_batch_invalidate(gf_batch* batch)
+{
+    if (!batch) return GF_ERR_ARG;
+    B(batch)->idx_valid = false;
+    B(batch)->kd_valid = false;
+    return GF_OK;
+}
+
+int gf_batch_run(gf_ctx* ctx, gf_batch* batch, const gf_stage* stages, int nstages,
+                 const int32_t* tile_knn, double* const* cols, uint8_t* const* status,
+                 int where)
+{
+    if (!ctx || !batch || !stages || nstages < 1 || nstages > kMaxStages || !status)
+        return GF_ERR_ARG;
+    return bguarded(ctx, [&] {
+        Context* c = reinterpret_cast<Context*>(ctx);
+        Batch* b = B(batch);
+        if (b->ctx != c) throw Error(GF_ERR_ARG, "batch belongs to another context");
+        c->stats = gf_stats{};
+        int total_cols = 0;
+        for (int i = 0; i < nstages; ++i) {
+            validate_stage(stages[i]);
+            total_cols += stage_ncols(stages[i]);
+        }
+        if (total_cols && !cols) throw Error(GF_ERR_ARG, "null column array");
+        if (b->n == 0) return;
+        GF_HIP(hipSetDevice(c->device));
+        c->run_batch(*b, stages, nstages, tile_knn, cols, status, where);
+    });
+}
+
+int gf_batch_release(gf_batch* batch)
+{
+    if (!batch) return GF_ERR_ARG;
+    delete B(batch);
+    return GF_OK;
+}
+
+} // extern "C"

@@ -258,39 +258,38 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     stats.merges = ctr[2];
 }
 
-// The exact KD3Index tree of a cloud (built once, reused until invalidated).
-void Context::build_kdtree(Cloud& cl)
+// The exact KD3Index trees of T tiles (points [starts[r], starts[r+1]) of
+// x, y, z; bounds 6 per tile), tile r rooted at node r.
+void Context::build_forest(KdBuffers& b, const double* x, const double* y, const double* z,
+                           uint64_t n, const uint64_t* dstarts, uint32_t T,
+                           const double* dbounds)
 {
-    if (cl.kd_valid) return;
-    const uint64_t n = cl.n;
-    KdBuffers& b = cl.kd;
-    for (;;) {
-        const uint32_t cap = std::max(kd_node_capacity(n), b.cap);
+    uint32_t cap = std::max(kd_node_capacity(n) + T, b.cap);
+    uint32_t lcap = std::max(kd_list_capacity(n) + T, b.list_cap);
+    for (int attempt = 0;; ++attempt) {
         b.vind.ensure(n * 4 + 8);
         b.px.ensure(n * 8 + 8);
         b.py.ensure(n * 8 + 8);
         b.pz.ensure(n * 8 + 8);
-        b.segid.ensure(n * 4 + 8);
-        b.nodes_u32.ensure(static_cast<uint64_t>(cap) * 4 * 3);
+        b.nodes_u32.ensure(static_cast<uint64_t>(cap) * 4 * 4);
         b.nodes_i32.ensure(static_cast<uint64_t>(cap) * 4 * 4);
         b.which.ensure(cap + 8);
         b.nodes_f64.ensure(static_cast<uint64_t>(cap) * 8 * 9);
         b.ext.ensure(static_cast<uint64_t>(cap) * 8 * 6);
-        kd_flag.ensure((n + 1) * 4);
         kd_s1.ensure((n + 1) * 4);
         kd_s2.ensure((n + 1) * 4);
         kd_tmp.ensure((n + 1) * 4);
-        kd_nodes.ensure((static_cast<uint64_t>(cap) + 1) * 8);
-        scan_tmp.ensure(scan_scratch_words(std::max<uint64_t>(n + 1, cap + 1)) * 4);
+        kd_lists.ensure((static_cast<uint64_t>(lcap) * 5 + 2 + 8) * 4);
+        scan_tmp.ensure(scan_scratch_words(n + 1) * 4);
         KdView t{};
         t.vind = b.vind.as<uint32_t>();
         t.px = b.px.as<double>();
         t.py = b.py.as<double>();
         t.pz = b.pz.as<double>();
-        t.segid = b.segid.as<uint32_t>();
         t.nl = b.nodes_u32.as<uint32_t>();
         t.nr = t.nl + cap;
         t.split = t.nr + cap;
+        t.aux = t.split + cap;
         t.c1 = b.nodes_i32.as<int32_t>();
         t.c2 = t.c1 + cap;
         t.feat = t.c2 + cap;
@@ -303,27 +302,47 @@ void Context::build_kdtree(Cloud& cl)
         t.mn = b.ext.as<unsigned long long>();
         t.mx = t.mn + 3 * static_cast<uint64_t>(cap);
         t.cap = cap;
-        for (int i = 0; i < 6; ++i) t.root[i] = cl.bounds[i];
         KdScratch w{};
-        w.flag = kd_flag.as<uint32_t>();
         w.s1 = kd_s1.as<uint32_t>();
         w.s2 = kd_s2.as<uint32_t>();
         w.tmp = kd_tmp.as<uint32_t>();
-        w.is_split = kd_nodes.as<uint32_t>();
-        w.rank = w.is_split + cap + 1;
-        w.scan_tmp = scan_tmp.as<uint32_t>();
-        w.overflow = counters.as<uint32_t>() + 4;
+        w.sums = scan_tmp.as<uint32_t>();
+        uint32_t* L = kd_lists.as<uint32_t>();
+        w.hdr = L;
+        w.act[0] = L + 8;
+        w.act[1] = w.act[0] + lcap;
+        w.off[0] = w.act[1] + lcap;
+        w.off[1] = w.off[0] + lcap + 1;
+        w.small = w.off[1] + lcap + 1;
+        w.list_cap = lcap;
         uint32_t nodes = 0;
-        if (launch_kd_build(cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), n,
-                            cl.dbounds.as<double>(), t, w, &nodes, stream)) {
+        KdTuning kt{};
+        kt.subtree_points = static_cast<uint32_t>(tune.kd_subtree);
+        kt.minmax_lds_nodes = static_cast<uint32_t>(tune.kd_minmax_lds);
+        const uint32_t rc =
+            launch_kd_build(x, y, z, n, dstarts, T, dbounds, t, w, kt, &nodes, stream);
+        if (rc == 0) {
             b.view = t;
             b.cap = cap;
+            b.list_cap = lcap;
             b.nodes = nodes;
-            cl.kd_valid = true;
             return;
         }
-        b.cap = cap * 2;  // node capacity overflow: grow and build again
+        if (rc == 3 || attempt >= 8)
+            throw Error(GF_ERR_UNSUPPORTED, "exact KD3 build: table overflow (code " +
+                                                std::to_string(rc) + ")");
+        if (rc == 1) cap *= 2;  // node capacity: grow and build again
+        if (rc == 2) lcap *= 2; // node lists
     }
+}
+
+// The exact KD3Index tree of a cloud (built once, reused until invalidated).
+void Context::build_kdtree(Cloud& cl)
+{
+    if (cl.kd_valid) return;
+    build_forest(cl.kd, cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), cl.n,
+                 cl.dstarts.as<uint64_t>(), 1, cl.dbounds.as<double>());
+    cl.kd_valid = true;
 }
 
 // Re-solves the rows the grid search flagged (boundary ties, uncertified
@@ -459,6 +478,9 @@ int gf_cloud_create(gf_ctx* ctx, const double* x, const double* y, const double*
         cl->x.ensure(n * 8 + 8);
         cl->y.ensure(n * 8 + 8);
         cl->z.ensure(n * 8 + 8);
+        cl->dstarts.ensure(16);
+        const uint64_t se[2] = {0, n};
+        GF_HIP(hipMemcpy(cl->dstarts.ptr, se, sizeof(se), hipMemcpyHostToDevice));
         hipMemcpyKind kind = where == GF_WHERE_DEVICE ? hipMemcpyDeviceToDevice
                                                       : hipMemcpyHostToDevice;
         if (n) {

@@ -15,6 +15,8 @@ void launch_keys(const double* x, const double* y, const double* z, uint64_t n,
 void launch_colstart(const uint32_t* keys, uint64_t n, const GridParams& g,
                      uint32_t* colstart, hipStream_t s);
 uint64_t scan_scratch_words(uint64_t n);
+// Exclusive scan of nb per-tile sums in place (one block).
+void launch_scan_top(uint32_t* sums, uint32_t nb, hipStream_t s);
 void launch_excl_scan(const uint32_t* in, uint32_t* out, uint64_t n,
                       uint32_t* scratch, hipStream_t s);
 uint64_t radix_hist_words(uint64_t n);
@@ -26,6 +28,36 @@ void launch_gather_sorted(const uint32_t* perm, const double* x, const double* y
                           const double* z, uint64_t n, double* sx, double* sy,
                           double* sz, hipStream_t s);
 void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
+
+// Tiled batches: segment blocks of <= 4096 points that never cross a tile,
+// with each block's slots in the tile-major radix histogram.
+constexpr uint32_t kSegBlockPoints = 4096;
+struct SegBlock {
+    uint32_t begin;  // first point
+    uint32_t count;  // <= kSegBlockPoints
+    uint32_t tile;
+    uint32_t hbase;  // 256 * (first block of the tile)
+    uint32_t nbt;    // blocks of the tile
+    uint32_t lb;     // index of this block within the tile
+};
+// A tile's grid and its place in the batch (device copy for the index kernels).
+struct TileGrid {
+    GridParams g;
+    uint64_t start, n;
+    uint64_t colbase;
+};
+void launch_tile_bounds(const double* x, const double* y, const double* z,
+                        const SegBlock* blocks, uint32_t nblocks, unsigned long long* benc,
+                        hipStream_t s);
+void launch_tile_keys(const double* x, const double* y, const double* z, const SegBlock* blocks,
+                      uint32_t nblocks, const TileGrid* grids, uint32_t* keys, uint32_t* ids,
+                      hipStream_t s);
+void launch_seg_radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                           const SegBlock* blocks, uint32_t nblocks, uint32_t bits,
+                           uint32_t* hist, uint32_t* scan_scratch, uint32_t** kres,
+                           uint32_t** vres, hipStream_t s);
+void launch_tile_colstart(const uint32_t* keys, const TileGrid* grids, uint32_t T,
+                          uint64_t entries, uint32_t* colstart, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 void launch_copy_f64(const double* src, double* dst, uint64_t n, hipStream_t s);
 
@@ -59,19 +91,55 @@ struct CloudView {
     GridParams g;
 };
 
+// One tile (an independent cloud) of a batch.  Tiles are contiguous in the
+// batch's point order, and each tile's sorted rows stay in its own range
+// [start, start + n): sorting never crosses a tile boundary.
+struct TileInfo {
+    GridParams g;
+    FusedProgram prog;     // k, stride, fetch of this tile (stages are shared)
+    uint64_t start;        // first point of the tile
+    uint64_t n;            // points
+    uint64_t colbase;      // first entry of the tile's column starts
+    double margin;         // certification margin of its grid
+    uint32_t root;         // root node of its exact tree
+    uint32_t all_exact;    // fewer points than the result capacity: every row exact
+};
+
+// The queries of one wavefront in the tiled search: [q0, q0 + nq) of tile.
+struct WaveTask {
+    uint64_t q0;
+    uint32_t tile;
+    uint32_t nq;
+};
+
+// Batch tables (device memory); tiles == nullptr means a single cloud whose
+// grid and program come from the CloudView / FusedProgram arguments.
+struct TileTables {
+    const TileInfo* tiles;
+    const uint64_t* starts;  // T + 1 tile starts (for row -> tile lookups)
+    uint32_t ntiles;
+    const WaveTask* tasks;
+    uint32_t ntasks;
+    int32_t max_k;           // largest k over the tiles (LDS sizing)
+    int32_t q_per_wave;      // queries per task (the tasks were cut with it)
+};
+
+// Queries per wavefront task for a neighborhood of k (LDS lists scale with k).
+inline int knn_queries_per_wave(int max_k) { return max_k <= 32 ? 64 : 32; }
+
 // mode 0: neighborhood covariance into out.cov / out.flags (sorted order);
 // mode 1: neighbor lists into out.ids.
 void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
-                const KnnOutputs& out, const KnnTuning& tune, hipStream_t s);
+                const KnnOutputs& out, const KnnTuning& tune, hipStream_t s,
+                const TileTables* tt = nullptr);
 // Eigensystem + stage columns from out.cov for every row.
 void launch_features(const CloudView& cv, const FusedProgram& prog,
-                     const KnnOutputs& out, hipStream_t s);
+                     const KnnOutputs& out, hipStream_t s, const TileTables* tt = nullptr);
 
 // kdtree.hip -- the exact KD3Index tree (nanoflann 1.3.1 semantics).
 struct KdView {
     uint32_t* vind;          // n: tree order -> original id
     double *px, *py, *pz;    // n: coordinates in tree order
-    uint32_t* segid;         // n: deepest node holding the position (build)
     uint32_t *nl, *nr;       // node segment [nl, nr)
     uint32_t* split;         // split position of internal nodes
     int32_t *c1, *c2;        // children (-1: leaf)
@@ -81,29 +149,47 @@ struct KdView {
     double *divlow, *divhigh, *cutval;
     double* box;             // 6 per node: the box passed down (lo xyz, hi xyz)
     unsigned long long *mn, *mx;  // 3 per node: order-encoded extents (build)
+    uint32_t* aux;           // per node: index in the next big-node list (build)
     uint32_t cap;            // node capacity
     uint32_t pad;
-    double root[6];          // root box (data bounds)
 };
 
 struct KdScratch {
-    uint32_t *flag, *s1, *s2, *tmp;  // n + 1 each
-    uint32_t *is_split, *rank;       // capacity + 1 each
-    uint32_t* scan_tmp;
-    uint32_t* overflow;
+    uint32_t *s1, *s2, *tmp;   // n + 1 each: pass prefixes (active index), pairing
+    uint32_t* sums;            // per-tile sums of the prefix scans
+    uint32_t *act[2], *off[2]; // big nodes of a level and their point-count prefix
+    uint32_t* small;           // nodes handed to the per-workgroup subtree builder
+    uint32_t list_cap;         // entries of act/small (off: list_cap + 1)
+    uint32_t* hdr;             // 8 words: level header (counts, overflow, nodes)
+};
+
+// Build strategy knobs (the tree does not depend on them).
+struct KdTuning {
+    uint32_t subtree_points;    // nodes up to this size go to one workgroup (<= 4096)
+    uint32_t minmax_lds_nodes;  // levels up to this many big nodes stage extents in LDS
 };
 
 uint32_t kd_node_capacity(uint64_t n);
+uint32_t kd_list_capacity(uint64_t n);
 // Sends every row to the exact search: used when the cloud holds fewer
 // points than the result capacity stride*min(n, k).
 void launch_flag_all(uint64_t n, const uint32_t* perm, int mode, const KnnOutputs& out,
                      hipStream_t s);
-bool launch_kd_build(const double* x, const double* y, const double* z, uint64_t n,
-                     const double* bounds, KdView t, KdScratch w, uint32_t* nodes,
-                     hipStream_t s);
+// The same for the tiles of a batch marked all_exact.
+void launch_flag_tiles(uint64_t n, const uint32_t* perm, int mode, const KnnOutputs& out,
+                       const TileTables& tt, hipStream_t s);
+// 0 on success, else 1 (node capacity) or 2 (node lists) to grow and retry,
+// 3 if a subtree level overflowed its table (cannot happen for leaf size 100).
+// A forest: tile r ([starts[r], starts[r+1]) of the points, bounds at
+// 6 r) is rooted at node r.
+uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint64_t n,
+                         const uint64_t* starts, uint32_t ntiles, const double* bounds,
+                         KdView t, KdScratch w, const KdTuning& kt, uint32_t* nodes,
+                         hipStream_t s);
 void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram& prog, int mode,
                        const KnnOutputs& out, const double* x, const double* y, const double* z,
-                       uint32_t nflagged, uint32_t* failed, hipStream_t s);
+                       uint32_t nflagged, uint32_t* failed, hipStream_t s,
+                       const TileTables* tt = nullptr);
 
 // Debug/parity hook: device eigen solver over m matrices (row-major 3x3).
 void launch_eigen_batch(const double* cov, uint64_t m, double* evals,

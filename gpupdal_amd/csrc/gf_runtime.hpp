@@ -10,6 +10,7 @@
 
 #include <cstring>
 #include <string>
+#include <vector>
 
 namespace gf {
 
@@ -51,6 +52,8 @@ struct Tuning {
     int ring_max = 8;
     int small_col = 0;
     int group_z = 1;
+    int kd_subtree = 4096;       // exact tree: subtree-per-workgroup threshold (points)
+    int kd_minmax_lds = 512;     // exact tree: LDS-staged extents up to this many nodes
     bool collect_stats = false;
 
     void set(const char* key, double v)
@@ -61,6 +64,9 @@ struct Tuning {
         else if (!std::strcmp(key, "ring_max") && v >= 1) ring_max = static_cast<int>(v);
         else if (!std::strcmp(key, "small_col") && v >= 0) small_col = static_cast<int>(v);
         else if (!std::strcmp(key, "group_z") && v >= 0) group_z = static_cast<int>(v);
+        else if (!std::strcmp(key, "kd_subtree") && v >= 101 && v <= 4096)
+            kd_subtree = static_cast<int>(v);
+        else if (!std::strcmp(key, "kd_minmax_lds") && v >= 0) kd_minmax_lds = static_cast<int>(v);
         else if (!std::strcmp(key, "collect_stats")) collect_stats = v != 0.0;
         else throw Error(GF_ERR_ARG, std::string("unknown or invalid tuning key: ") + key);
         if (ring_max < ring0) ring_max = ring0;
@@ -79,8 +85,9 @@ struct Context;
 
 // Device storage of a cloud's exact KD3Index tree.
 struct KdBuffers {
-    DevBuf vind, px, py, pz, segid, nodes_u32, nodes_i32, which, nodes_f64, ext;
+    DevBuf vind, px, py, pz, nodes_u32, nodes_i32, which, nodes_f64, ext;
     uint32_t cap = 0;
+    uint32_t list_cap = 0;
     uint32_t nodes = 0;
     KdView view{};
 };
@@ -99,9 +106,39 @@ struct Cloud {
     uint32_t* perm = nullptr;    // sorted position -> original id
     double bounds[6] = {0, 0, 0, 0, 0, 0};  // min xyz, max xyz
     DevBuf dbounds;
+    DevBuf dstarts;              // {0, n}: the cloud as a one-tile forest
     bool kd_valid = false;       // exact tree (built on demand)
     KdBuffers kd;
     explicit Cloud(Context* c) : ctx(c) {}
+};
+
+// A batch of independent tiles held back to back on the device: one index
+// build, one search, one exact re-solve and one feature pass serve them all.
+struct Batch {
+    Context* ctx;
+    uint32_t T = 0;
+    uint64_t n = 0;
+    std::vector<uint64_t> starts;        // T + 1
+    std::vector<SegBlock> blocks;        // segment blocks (never cross a tile)
+    DevBuf x, y, z;
+    DevBuf dstarts, dblocks;
+    // index, valid for idx_fetch (per tile) and idx_tune
+    bool idx_valid = false;
+    std::vector<uint32_t> idx_fetch;
+    uint64_t idx_tune = 0;
+    std::vector<TileGrid> grids;
+    std::vector<double> bounds;          // 6 per tile
+    DevBuf dgrids, dbounds, benc, hist;
+    DevBuf keys_a, vals_a, keys_b, vals_b, colstart, sx, sy, sz;
+    uint32_t* keys = nullptr;
+    uint32_t* perm = nullptr;
+    uint64_t col_entries = 0;
+    // run tables
+    DevBuf dtiles, dtasks;
+    bool kd_valid = false;
+    KdBuffers kd;
+    explicit Batch(Context* c) : ctx(c) {}
+    void reset(uint32_t ntiles, const uint64_t* counts, hipStream_t s);
 };
 
 GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch, const Tuning& t);
@@ -117,7 +154,7 @@ struct Context {
     gf_stats stats{};
     std::string last_error;
     DevBuf part, bounds, hist, scan_tmp, outbuf, statbuf, flagged, counters, covbuf, flagbuf;
-    DevBuf kd_flag, kd_s1, kd_s2, kd_tmp, kd_nodes;
+    DevBuf kd_s1, kd_s2, kd_tmp, kd_lists;
     double* host_bounds = nullptr;
 
     explicit Context(int dev) : device(dev)
@@ -157,8 +194,13 @@ struct Context {
     void run_group(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
                    uint8_t* const* status, int where);
     void build_kdtree(Cloud& cl);
+    void build_forest(KdBuffers& b, const double* x, const double* y, const double* z,
+                      uint64_t n, const uint64_t* dstarts, uint32_t T, const double* dbounds);
     void resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog, int mode,
                          const KnnOutputs& out);
+    void build_batch_index(Batch& b, const std::vector<uint32_t>& fetch);
+    void run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* tile_knn,
+                   double* const* cols, uint8_t* const* status, int where);
     hipEvent_t ev_begin();
     double ev_end(hipEvent_t e);
 };

@@ -237,6 +237,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(
 
 uint64_t scan_scratch_words(uint64_t n) { return div_up(n ? n : 1, kScanTile) + 1; }
 
+void launch_scan_top(uint32_t* sums, uint32_t nb, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, sums, nb);
+}
+
 void launch_excl_scan(const uint32_t* in, uint32_t* out, uint64_t n,
                       uint32_t* scratch, hipStream_t s)
 {
@@ -374,6 +379,196 @@ void launch_radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
     *vres = va;
 }
 
+// ------------------------------------------------------ tiled batches -----
+// A batch holds independent tiles back to back.  Work is cut into segment
+// blocks of at most 4096 points that never cross a tile, so every per-block
+// histogram belongs to one tile; laid out tile-major (then digit, then block
+// within the tile), one exclusive scan of all histograms gives each tile's
+// scatter offsets inside its own range.
+
+__device__ __forceinline__ unsigned long long ord_enc(double d)
+{
+    unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(d));
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// Per segment block: min/max of x, y, z, then one atomic per value into the
+// tile's order-encoded bounds (6 per tile: min xyz, max xyz).
+__global__ __launch_bounds__(kRsThreads) void k_tile_bounds(
+    const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const SegBlock* __restrict__ blocks,
+    unsigned long long* __restrict__ benc)
+{
+    const SegBlock b = blocks[blockIdx.x];
+    unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
+    for (uint32_t i = threadIdx.x; i < b.count; i += kRsThreads) {
+        const uint64_t p = static_cast<uint64_t>(b.begin) + i;
+        const unsigned long long v[3] = {ord_enc(x[p]), ord_enc(y[p]), ord_enc(z[p])};
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = v[d] < lo[d] ? v[d] : lo[d];
+            hi[d] = v[d] > hi[d] ? v[d] : hi[d];
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) {
+            const unsigned long long ol = __shfl_xor(lo[d], m);
+            const unsigned long long oh = __shfl_xor(hi[d], m);
+            lo[d] = ol < lo[d] ? ol : lo[d];
+            hi[d] = oh > hi[d] ? oh : hi[d];
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&benc[6 * static_cast<uint64_t>(b.tile) + d], lo[d]);
+            atomicMax(&benc[6 * static_cast<uint64_t>(b.tile) + 3 + d], hi[d]);
+        }
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_tile_keys(
+    const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const SegBlock* __restrict__ blocks,
+    const TileGrid* __restrict__ grids, uint32_t* __restrict__ keys,
+    uint32_t* __restrict__ ids)
+{
+    const SegBlock b = blocks[blockIdx.x];
+    const GridParams g = grids[b.tile].g;
+    for (uint32_t i = threadIdx.x; i < b.count; i += kRsThreads) {
+        const uint64_t p = static_cast<uint64_t>(b.begin) + i;
+        const uint32_t cx = cell_coord(x[p], g.ox, g.inv_c, g.gx);
+        const uint32_t cy = cell_coord(y[p], g.oy, g.inv_c, g.gy);
+        const uint32_t cz = cell_coord(z[p], g.oz, g.inv_cz, g.gz);
+        keys[p] = (column_index(g, cx, cy) << g.bz) | cz;
+        ids[p] = static_cast<uint32_t>(p);
+    }
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_seg_radix_up(
+    const uint32_t* __restrict__ kin, const SegBlock* __restrict__ blocks, uint32_t shift,
+    uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t h[kRsBins];
+    const SegBlock b = blocks[blockIdx.x];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < b.count; i += kRsThreads)
+        atomicAdd(&h[(kin[static_cast<uint64_t>(b.begin) + i] >> shift) & 255u], 1u);
+    __syncthreads();
+    hist[static_cast<uint64_t>(b.hbase) + static_cast<uint64_t>(threadIdx.x) * b.nbt + b.lb] =
+        h[threadIdx.x];
+}
+
+// The stable scatter of one segment block (the scheme of k_radix_down with
+// the block's own range and histogram slots).
+__global__ __launch_bounds__(kRsThreads) void k_seg_radix_down(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+    const uint32_t* __restrict__ goff, const SegBlock* __restrict__ blocks, uint32_t shift)
+{
+    __shared__ uint32_t s_cnt[kRsWaves][kRsBins];
+    __shared__ uint32_t s_start[kRsBins];
+    __shared__ uint32_t s_goff[kRsBins];
+    __shared__ uint32_t s_key[kRsTile];
+    __shared__ uint32_t s_val[kRsTile];
+    const SegBlock b = blocks[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t base = b.begin;
+    const uint64_t n = base + b.count;
+    for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads)
+        (&s_cnt[0][0])[i] = 0;
+    s_goff[threadIdx.x] =
+        goff[static_cast<uint64_t>(b.hbase) + static_cast<uint64_t>(threadIdx.x) * b.nbt + b.lb];
+    __syncthreads();
+
+    uint32_t key[kRsItems], val[kRsItems], rank[kRsItems];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int it = 0; it < kRsItems; ++it) {
+        uint64_t pos = base + (uint64_t)(w * kRsItems + it) * 64 + lane;
+        bool valid = pos < n;
+        key[it] = valid ? kin[pos] : 0u;
+        val[it] = valid ? vin[pos] : 0u;
+        uint32_t d = (key[it] >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int bb = 0; bb < 8; ++bb) {
+            uint64_t bits = __ballot((d >> bb) & 1u);
+            m &= ((d >> bb) & 1u) ? bits : ~bits;
+        }
+        uint32_t r = __popcll(m & lt);
+        uint32_t cnt = __popcll(m);
+        uint32_t prior = valid ? s_cnt[w][d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && r == cnt - 1) s_cnt[w][d] = prior + cnt;
+        __builtin_amdgcn_wave_barrier();
+        rank[it] = prior + r;
+    }
+    __syncthreads();
+    {
+        const int t = threadIdx.x;
+        uint32_t c[kRsWaves], tot = 0;
+        for (int i = 0; i < kRsWaves; ++i) {
+            c[i] = s_cnt[i][t];
+            s_cnt[i][t] = tot;
+            tot += c[i];
+        }
+        uint32_t all;
+        s_start[t] = block_excl_scan(tot, &all);
+    }
+    __syncthreads();
+    for (int it = 0; it < kRsItems; ++it) {
+        uint64_t pos = base + (uint64_t)(w * kRsItems + it) * 64 + lane;
+        if (pos < n) {
+            uint32_t d = (key[it] >> shift) & 255u;
+            uint32_t p = s_start[d] + s_cnt[w][d] + rank[it];
+            s_key[p] = key[it];
+            s_val[p] = val[it];
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < b.count; p += kRsThreads) {
+        uint32_t k = s_key[p];
+        uint32_t d = (k >> shift) & 255u;
+        uint32_t g = s_goff[d] + (p - s_start[d]);
+        kout[g] = k;
+        vout[g] = s_val[p];
+    }
+}
+
+// Column starts of every tile: entry e of tile t (colbase <= e <= colbase +
+// ncols) is the first sorted position of the tile whose column is >= e -
+// colbase (the tile's end for the last entry).
+__global__ __launch_bounds__(256) void k_tile_colstart(
+    const uint32_t* __restrict__ keys, const TileGrid* __restrict__ grids, uint32_t T,
+    uint64_t entries, uint32_t* __restrict__ colstart)
+{
+    const uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (e >= entries) return;
+    uint32_t lo_t = 0, hi_t = T;  // largest t with colbase <= e
+    while (hi_t - lo_t > 1) {
+        const uint32_t mid = (lo_t + hi_t) >> 1;
+        if (grids[mid].colbase <= e)
+            lo_t = mid;
+        else
+            hi_t = mid;
+    }
+    const TileGrid& tg = grids[lo_t];
+    const uint64_t c = e - tg.colbase;
+    if (c >= tg.g.ncols) {
+        colstart[e] = static_cast<uint32_t>(tg.start + tg.n);
+        return;
+    }
+    const uint64_t key = c << tg.g.bz;
+    uint64_t lo = tg.start, hi = tg.start + tg.n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (static_cast<uint64_t>(keys[mid]) < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    colstart[e] = static_cast<uint32_t>(lo);
+}
+
 // -------------------------------------------------------------- gather -----
 __global__ __launch_bounds__(256) void k_gather_sorted(
     const uint32_t* __restrict__ perm, const double* __restrict__ x,
@@ -435,6 +630,55 @@ void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s)
     uint32_t b = div_up(n, 256);
     if (b > 65536) b = 65536;
     hipLaunchKernelGGL(k_fill_u32, dim3(b), dim3(256), 0, s, p, n, v);
+}
+
+
+void launch_tile_bounds(const double* x, const double* y, const double* z,
+                        const SegBlock* blocks, uint32_t nblocks, unsigned long long* benc,
+                        hipStream_t s)
+{
+    if (!nblocks) return;
+    hipLaunchKernelGGL(k_tile_bounds, dim3(nblocks), dim3(kRsThreads), 0, s, x, y, z, blocks,
+                       benc);
+}
+
+void launch_tile_keys(const double* x, const double* y, const double* z, const SegBlock* blocks,
+                      uint32_t nblocks, const TileGrid* grids, uint32_t* keys, uint32_t* ids,
+                      hipStream_t s)
+{
+    if (!nblocks) return;
+    hipLaunchKernelGGL(k_tile_keys, dim3(nblocks), dim3(kRsThreads), 0, s, x, y, z, blocks,
+                       grids, keys, ids);
+}
+
+void launch_seg_radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                           const SegBlock* blocks, uint32_t nblocks, uint32_t bits,
+                           uint32_t* hist, uint32_t* scan_scratch, uint32_t** kres,
+                           uint32_t** vres, hipStream_t s)
+{
+    uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
+    if (nblocks) {
+        const uint64_t hw = static_cast<uint64_t>(kRsBins) * nblocks;
+        for (uint32_t shift = 0; shift < bits; shift += 8) {
+            hipLaunchKernelGGL(k_seg_radix_up, dim3(nblocks), dim3(kRsThreads), 0, s, ka, blocks,
+                               shift, hist);
+            launch_excl_scan(hist, hist, hw, scan_scratch, s);
+            hipLaunchKernelGGL(k_seg_radix_down, dim3(nblocks), dim3(kRsThreads), 0, s, ka, va,
+                               kb, vb, hist, blocks, shift);
+            uint32_t* t = ka; ka = kb; kb = t;
+            t = va; va = vb; vb = t;
+        }
+    }
+    *kres = ka;
+    *vres = va;
+}
+
+void launch_tile_colstart(const uint32_t* keys, const TileGrid* grids, uint32_t T,
+                          uint64_t entries, uint32_t* colstart, hipStream_t s)
+{
+    if (!entries) return;
+    hipLaunchKernelGGL(k_tile_colstart, dim3(div_up(entries, 256)), dim3(256), 0, s, keys, grids,
+                       T, entries, colstart);
 }
 
 } // namespace gf

@@ -53,154 +53,68 @@ __device__ __forceinline__ const double* axis(const KdView& t, int f)
 }
 
 // ------------------------------------------------------------- building --
-__global__ void k_kd_root(KdView t, uint32_t n, const double* __restrict__ bounds)
-{
-    t.nl[0] = 0;
-    t.nr[0] = n;
-    t.parent[0] = -1;
-    t.which[0] = 0;
-    for (int i = 0; i < 6; ++i) t.box[i] = bounds[i];
-}
+// Two regimes.  Nodes of more than kSub points ("big") are split level by
+// level over the whole device: every kernel of a level walks only the points
+// of the level's big nodes (an "active index" t maps to a node by binary
+// search over the nodes' point-count prefix, then to its position).  A node
+// of at most kSub points ("small") is handed to one workgroup, which loads
+// its points into LDS and builds its whole subtree there, level by level
+// with block barriers.
+constexpr uint32_t kSub = 4096;           // small-node capacity (points)
+constexpr int kSubThreads = 256;
+constexpr int kSubNodes = 128;            // nodes per subtree level (<= 2 * 4096 / 101)
+constexpr int kPredThreads = 256;
+constexpr int kPredItems = 16;
+constexpr uint32_t kPredTile = kPredThreads * kPredItems;
+constexpr int kChildThreads = 1024;
+constexpr uint32_t kMinMaxLds = 512;      // big children staged per block in LDS
 
-__global__ __launch_bounds__(kKdThreads) void k_kd_init(
-    const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, uint32_t n, KdView t)
-{
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    t.vind[p] = p;
-    t.px[p] = x[p];
-    t.py[p] = y[p];
-    t.pz[p] = z[p];
-    t.segid[p] = 0;
-}
+enum : int { kHdrA = 0, kHdrP = 1, kHdrSmall = 2, kHdrOverflow = 3, kHdrNodes = 4 };
+enum : uint32_t { kOvfNodes = 1, kOvfLists = 2, kOvfSubtree = 3 };
 
-__global__ __launch_bounds__(kKdThreads) void k_kd_minmax_reset(KdView t, uint32_t lv0,
-                                                               uint32_t lv1)
+// Exclusive scan of one value per thread across a block of NT threads.
+template <int NT> __device__ __forceinline__ uint32_t kd_block_scan(uint32_t v, uint32_t* total)
 {
-    uint32_t nd = lv0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (nd >= lv1) return;
-    for (int d = 0; d < 3; ++d) {
-        t.mn[3 * nd + d] = ~0ull;
-        t.mx[3 * nd + d] = 0ull;
-    }
-}
-
-// Min/max of every level node's points: a segmented reduction inside each
-// wave (runs of equal node ids are contiguous), then per run head an atomic:
-// into a per-block LDS table when the level has few nodes (the top levels,
-// where every wave would otherwise hit the same global address), else
-// straight to global memory.
-constexpr uint32_t kLdsNodes = 256;
-
-__global__ __launch_bounds__(kKdThreads) void k_kd_minmax(KdView t, uint32_t n, uint32_t lv0,
-                                                          uint32_t lv1)
-{
-    __shared__ unsigned long long s_lo[kLdsNodes * 3], s_hi[kLdsNodes * 3];
-    const uint32_t nl = lv1 - lv0;
-    const bool use_lds = nl <= kLdsNodes;
-    if (use_lds) {
-        for (uint32_t i = threadIdx.x; i < nl * 3; i += blockDim.x) {
-            s_lo[i] = ~0ull;
-            s_hi[i] = 0ull;
-        }
-        __syncthreads();
-    }
-    const int lane = __lane_id();
-    const double* a[3] = {t.px, t.py, t.pz};
-    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n;
-         base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const uint64_t p = base + threadIdx.x;
-        const bool act = p < n && t.segid[p] >= lv0;
-        const uint32_t nd = act ? t.segid[p] : 0xFFFFFFFFu;
-        unsigned long long lo[3], hi[3];
-        for (int d = 0; d < 3; ++d) {
-            unsigned long long e = act ? enc_order(a[d][p]) : 0ull;
-            lo[d] = act ? e : ~0ull;
-            hi[d] = act ? e : 0ull;
-        }
+    __shared__ uint32_t s_w[NT / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t incl = v;
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t ond = __shfl_down(nd, off);
-            const bool same = lane + off < 64 && ond == nd;
-            for (int d = 0; d < 3; ++d) {
-                unsigned long long ol = __shfl_down(lo[d], off);
-                unsigned long long oh = __shfl_down(hi[d], off);
-                if (same) {
-                    lo[d] = ol < lo[d] ? ol : lo[d];
-                    hi[d] = oh > hi[d] ? oh : hi[d];
-                }
-            }
-        }
-        const uint32_t prev = __shfl_up(nd, 1);
-        const bool head = act && (lane == 0 || prev != nd);
-        if (head) {
-            if (use_lds) {
-                for (int d = 0; d < 3; ++d) {
-                    atomicMin(&s_lo[(nd - lv0) * 3 + d], lo[d]);
-                    atomicMax(&s_hi[(nd - lv0) * 3 + d], hi[d]);
-                }
-            } else {
-                for (int d = 0; d < 3; ++d) {
-                    atomicMin(&t.mn[3 * nd + d], lo[d]);
-                    atomicMax(&t.mx[3 * nd + d], hi[d]);
-                }
-            }
-        }
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d);
+        if (lane >= d) incl += o;
     }
-    if (use_lds) {
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
-            if (s_hi[i * 3] == 0ull) continue;  // no point of node i in this block
-            for (int d = 0; d < 3; ++d) {
-                atomicMin(&t.mn[3 * (lv0 + i) + d], s_lo[i * 3 + d]);
-                atomicMax(&t.mx[3 * (lv0 + i) + d], s_hi[i * 3 + d]);
-            }
-        }
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) {
+        const uint32_t x = s_w[i];
+        if (i < w) off += x;
+        tot += x;
     }
+    __syncthreads();
+    *total = tot;
+    return off + incl - v;
 }
 
-// One thread per level node: the parent's divlow/divhigh from this node's
-// extent, then leaf or split parameters (middleSplit_).
-__global__ __launch_bounds__(kKdThreads) void k_kd_split(KdView t, uint32_t lv0, uint32_t lv1,
-                                                         uint32_t* __restrict__ is_split)
+// middleSplit_: among the axes whose box span is within 1e-5 of the widest,
+// the one with the largest data spread; cut at the box middle clamped to the
+// data range (nanoflann.hpp:779-827).
+__device__ __forceinline__ void kd_choose_split(const double* b, const double mn[3],
+                                                const double mx[3], int& f, double& cv)
 {
-    const uint32_t nd = lv0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (nd >= lv1) return;
-    double mn[3], mx[3];
-    for (int d = 0; d < 3; ++d) {
-        mn[d] = dec_order(t.mn[3 * nd + d]);
-        mx[d] = dec_order(t.mx[3 * nd + d]);
-    }
-    const int par = t.parent[nd];
-    if (par >= 0) {
-        const int f = t.feat[par];
-        if (t.which[nd] == 0)
-            t.divlow[par] = mx[f];
-        else
-            t.divhigh[par] = mn[f];
-    }
-    const uint32_t cnt = t.nr[nd] - t.nl[nd];
-    t.c1[nd] = -1;
-    t.c2[nd] = -1;
-    if (cnt <= kLeafMax) {
-        t.feat[nd] = -1;
-        is_split[nd - lv0] = 0;
-        return;
-    }
-    const double* b = t.box + 6 * static_cast<size_t>(nd);
     const double eps = 0.00001;
     double max_span = b[3] - b[0];
     for (int i = 1; i < 3; ++i) {
-        double span = b[3 + i] - b[i];
+        const double span = b[3 + i] - b[i];
         if (span > max_span) max_span = span;
     }
     double max_spread = -1;
-    int f = 0;
+    f = 0;
     for (int i = 0; i < 3; ++i) {
-        double span = b[3 + i] - b[i];
+        const double span = b[3 + i] - b[i];
         if (span > (1 - eps) * max_span) {
-            double spread = mx[i] - mn[i];
+            const double spread = mx[i] - mn[i];
             if (spread > max_spread) {
                 f = i;
                 max_spread = spread;
@@ -208,90 +122,302 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_split(KdView t, uint32_t lv0,
         }
     }
     const double split = (b[f] + b[3 + f]) / 2;
-    double cv = split;
+    cv = split;
     if (split < mn[f])
         cv = mn[f];
     else if (split > mx[f])
         cv = mx[f];
-    t.feat[nd] = f;
-    t.cutval[nd] = cv;
-    is_split[nd - lv0] = 1;
 }
 
-// flag[p] = 1 when p belongs to a splitting node and sits on the "left" side
-// of the pass: pass 1 tests coord < cut over the node; pass 2 tests
-// coord <= cut over [L, right) of the node (L from pass 1's counts).
-__global__ __launch_bounds__(kKdThreads) void k_kd_flags(KdView t, uint32_t n, uint32_t lv0,
-                                                         int pass,
-                                                         const uint32_t* __restrict__ s1,
-                                                         uint32_t* __restrict__ flag)
+// The parent's divlow (max of child1) or divhigh (min of child2) on its cut
+// axis, from this node's extent (nanoflann.hpp:672-683).
+__device__ __forceinline__ void kd_parent_div(const KdView& t, uint32_t nd, const double mn[3],
+                                              const double mx[3])
+{
+    const int par = t.parent[nd];
+    if (par < 0) return;
+    const int f = t.feat[par];
+    if (t.which[nd] == 0)
+        t.divlow[par] = mx[f];
+    else
+        t.divhigh[par] = mn[f];
+}
+
+// planeSplit's index rule: lim1 if more than half are strictly left, lim2 if
+// fewer than half are left-or-equal, else the middle (nanoflann.hpp:663-670).
+__device__ __forceinline__ uint32_t kd_split_index(uint32_t cnt, uint32_t lim1, uint32_t lim2)
+{
+    if (lim1 > cnt / 2) return lim1;
+    if (lim2 < cnt / 2) return lim2;
+    return cnt / 2;
+}
+
+// Records of the two children of nd: segments [lo, lo+idx) and [lo+idx, hi),
+// boxes cut at the node's cut value, no children yet.
+__device__ __forceinline__ void kd_make_children(const KdView& t, uint32_t nd, uint32_t lo,
+                                                 uint32_t idx, uint32_t hi, uint32_t c1)
+{
+    const uint32_t c2 = c1 + 1;
+    t.c1[nd] = static_cast<int32_t>(c1);
+    t.c2[nd] = static_cast<int32_t>(c2);
+    t.split[nd] = lo + idx;
+    const int f = t.feat[nd];
+    const double cv = t.cutval[nd];
+    const double* b = t.box + 6 * static_cast<size_t>(nd);
+    double* b1 = t.box + 6 * static_cast<size_t>(c1);
+    double* b2 = t.box + 6 * static_cast<size_t>(c2);
+    for (int i = 0; i < 6; ++i) {
+        b1[i] = b[i];
+        b2[i] = b[i];
+    }
+    b1[3 + f] = cv;
+    b2[f] = cv;
+    t.nl[c1] = lo;
+    t.nr[c1] = lo + idx;
+    t.nl[c2] = lo + idx;
+    t.nr[c2] = hi;
+    t.parent[c1] = static_cast<int32_t>(nd);
+    t.parent[c2] = static_cast<int32_t>(nd);
+    t.which[c1] = 0;
+    t.which[c2] = 1;
+    t.c1[c1] = t.c2[c1] = t.c1[c2] = t.c2[c2] = -1;
+    t.feat[c1] = t.feat[c2] = -1;
+}
+
+__global__ __launch_bounds__(kKdThreads) void k_kd_init(
+    const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, uint32_t n, KdView t)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > n) return;
-    uint32_t f = 0;
-    if (p < n) {
-        const uint32_t nd = t.segid[p];
-        if (nd >= lv0 && t.feat[nd] >= 0) {
-            const double v = axis(t, t.feat[nd])[p];
-            const double cv = t.cutval[nd];
-            if (pass == 1) {
-                f = v < cv ? 1u : 0u;
-            } else {
-                const uint32_t s0 = t.nl[nd], e = t.nr[nd];
-                const uint32_t L = s0 + (s1[e] - s1[s0]);
-                f = (p >= L && v <= cv) ? 1u : 0u;
+    if (p >= n) return;
+    t.vind[p] = p;
+    t.px[p] = x[p];
+    t.py[p] = y[p];
+    t.pz[p] = z[p];
+}
+
+// Node t = the root of tile t (the whole tile; its extent is the tile's
+// bounds).  Roots of more than sub points start the big-node levels, in tile
+// order; the others go straight to the subtree builder.
+__global__ __launch_bounds__(kChildThreads) void k_kd_roots(KdView t, const uint64_t* __restrict__ starts,
+                                                            uint32_t T,
+                                                            const double* __restrict__ bounds,
+                                                            KdScratch w, uint32_t sub)
+{
+    uint32_t nbig = 0, nsmall = 0, psum = 0;
+    for (uint32_t r0 = 0; r0 < T; r0 += kChildThreads) {
+        const uint32_t r = r0 + threadIdx.x;
+        const bool ok = r < T;
+        uint32_t cnt = 0;
+        if (ok) {
+            const uint32_t lo = static_cast<uint32_t>(starts[r]);
+            const uint32_t hi = static_cast<uint32_t>(starts[r + 1]);
+            cnt = hi - lo;
+            t.nl[r] = lo;
+            t.nr[r] = hi;
+            t.parent[r] = -1;
+            t.which[r] = 0;
+            t.c1[r] = t.c2[r] = -1;
+            t.feat[r] = -1;
+            const double* b = bounds + 6 * static_cast<size_t>(r);
+            for (int i = 0; i < 6; ++i) t.box[6 * static_cast<size_t>(r) + i] = b[i];
+            for (int d = 0; d < 3; ++d) {
+                t.mn[3 * r + d] = enc_order(b[d]);
+                t.mx[3 * r + d] = enc_order(b[3 + d]);
             }
         }
+        const bool big = ok && cnt > sub;
+        uint32_t tb, ts, tz;
+        const uint32_t eb = kd_block_scan<kChildThreads>(big ? 1u : 0u, &tb);
+        const uint32_t es = kd_block_scan<kChildThreads>((ok && !big) ? 1u : 0u, &ts);
+        const uint32_t ez = kd_block_scan<kChildThreads>(big ? cnt : 0u, &tz);
+        if (big) {
+            if (nbig + eb < w.list_cap) {
+                w.act[0][nbig + eb] = r;
+                w.off[0][nbig + eb] = psum + ez;
+            } else {
+                w.hdr[kHdrOverflow] = kOvfLists;
+            }
+        } else if (ok) {
+            if (nsmall + es < w.list_cap)
+                w.small[nsmall + es] = r;
+            else
+                w.hdr[kHdrOverflow] = kOvfLists;
+        }
+        nbig += tb;
+        nsmall += ts;
+        psum += tz;
     }
-    flag[p] = f;
+    if (threadIdx.x == 0) {
+        if (nbig <= w.list_cap) w.off[0][nbig] = psum;
+        w.hdr[kHdrA] = nbig;
+        w.hdr[kHdrP] = psum;
+        w.hdr[kHdrSmall] = nsmall;
+        w.hdr[kHdrNodes] = T;
+    }
 }
 
-// The pairing of the closed-form Hoare pass: every misplaced right-side
-// element publishes its position under its rank counted from the node's
-// end; every misplaced left-side element then swaps with the partner of the
-// same rank.  [a, e) is the pass's range, split at m = a + #flag in [a, e).
-__device__ __forceinline__ void kd_pass_range(const KdView& t, uint32_t nd, int pass,
-                                              const uint32_t* s1, const uint32_t* sc,
-                                              uint32_t& a, uint32_t& e, uint32_t& m)
+// Largest k in [0, A) with off[k] <= t (big nodes are never empty).
+__device__ __forceinline__ uint32_t act_find(const uint32_t* __restrict__ off, uint32_t A,
+                                             uint32_t t)
 {
-    const uint32_t s0 = t.nl[nd];
-    e = t.nr[nd];
-    a = pass == 1 ? s0 : s0 + (s1[e] - s1[s0]);
-    m = a + (sc[e] - sc[a]);
+    uint32_t lo = 0, hi = A;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
 }
 
-__global__ __launch_bounds__(kKdThreads) void k_kd_publish(KdView t, uint32_t n, uint32_t lv0,
-                                                           int pass,
-                                                           const uint32_t* __restrict__ s1,
-                                                           const uint32_t* __restrict__ sc,
-                                                           const uint32_t* __restrict__ flag,
-                                                           uint32_t* __restrict__ tmp)
+// One thread per big node: parent's div from this node's extent, then the
+// split (big nodes always split).
+__global__ __launch_bounds__(kKdThreads) void k_kd_big_split(KdView t,
+                                                             const uint32_t* __restrict__ act,
+                                                             uint32_t A)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t nd = t.segid[p];
-    if (nd < lv0 || t.feat[nd] < 0 || !flag[p]) return;
-    uint32_t a, e, m;
-    kd_pass_range(t, nd, pass, s1, sc, a, e, m);
-    if (p >= m) tmp[a + (sc[e] - sc[p + 1])] = p;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A) return;
+    const uint32_t nd = act[k];
+    double mn[3], mx[3];
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = dec_order(t.mn[3 * nd + d]);
+        mx[d] = dec_order(t.mx[3 * nd + d]);
+    }
+    kd_parent_div(t, nd, mn, mx);
+    int f;
+    double cv;
+    kd_choose_split(t.box + 6 * static_cast<size_t>(nd), mn, mx, f, cv);
+    t.feat[nd] = f;
+    t.cutval[nd] = cv;
 }
 
-__global__ __launch_bounds__(kKdThreads) void k_kd_swap(KdView t, uint32_t n, uint32_t lv0,
-                                                        int pass,
-                                                        const uint32_t* __restrict__ s1,
-                                                        const uint32_t* __restrict__ sc,
-                                                        const uint32_t* __restrict__ flag,
-                                                        const uint32_t* __restrict__ tmp)
+// Active index -> (node, position).
+struct ActPos {
+    uint32_t nd, lo, t0, t1, p;
+};
+
+__device__ __forceinline__ ActPos act_pos(const KdView& t, const uint32_t* act,
+                                          const uint32_t* off, uint32_t A, uint32_t ti)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t nd = t.segid[p];
-    if (nd < lv0 || t.feat[nd] < 0 || flag[p]) return;
-    uint32_t a, e, m;
-    kd_pass_range(t, nd, pass, s1, sc, a, e, m);
-    if (p < a || p >= m) return;
-    const uint32_t rank = (p - a) - (sc[p] - sc[a]);
+    const uint32_t k = act_find(off, A, ti);
+    ActPos r;
+    r.nd = act[k];
+    r.lo = t.nl[r.nd];
+    r.t0 = off[k];
+    r.t1 = off[k + 1];
+    r.p = r.lo + (ti - r.t0);
+    return r;
+}
+
+// planeSplit's two Hoare passes as predicates: pass 1 "coord < cut" over
+// the node, pass 2 "coord <= cut" over [L, right) where L = lo + #pass 1.
+__device__ __forceinline__ bool kd_pred(const KdView& t, int pass, const ActPos& a,
+                                        const uint32_t* s1)
+{
+    const int f = t.feat[a.nd];
+    const double v = axis(t, f)[a.p];
+    const double cv = t.cutval[a.nd];
+    if (pass == 1) return v < cv;
+    const uint32_t L = a.lo + (s1[a.t1] - s1[a.t0]);
+    return a.p >= L && v <= cv;
+}
+
+__global__ __launch_bounds__(kPredThreads) void k_kd_pred_reduce(
+    KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
+    uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, uint32_t* __restrict__ sums)
+{
+    const uint32_t base = blockIdx.x * kPredTile;
+    uint32_t acc = 0;
+    for (int i = 0; i < kPredItems; ++i) {
+        const uint32_t ti = base + i * kPredThreads + threadIdx.x;
+        if (ti < P && kd_pred(t, pass, act_pos(t, act, off, A, ti), s1)) ++acc;
+    }
+    uint32_t tot;
+    kd_block_scan<kPredThreads>(acc, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Exclusive prefix of the predicate over active indices [0, P], written to
+// out (P + 1 entries; the predicate is 0 at P).
+__global__ __launch_bounds__(kPredThreads) void k_kd_pred_down(
+    KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
+    uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, const uint32_t* __restrict__ sums,
+    uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t s[kPredTile];
+    const uint32_t base = blockIdx.x * kPredTile;
+    for (int i = 0; i < kPredItems; ++i) {
+        const uint32_t l = i * kPredThreads + threadIdx.x;
+        const uint32_t ti = base + l;
+        s[l] = (ti < P && kd_pred(t, pass, act_pos(t, act, off, A, ti), s1)) ? 1u : 0u;
+    }
+    __syncthreads();
+    uint32_t v[kPredItems];
+    uint32_t acc = 0;
+    for (int i = 0; i < kPredItems; ++i) {
+        v[i] = s[threadIdx.x * kPredItems + i];
+        acc += v[i];
+    }
+    uint32_t tot;
+    uint32_t ex = kd_block_scan<kPredThreads>(acc, &tot) + sums[blockIdx.x];
+    for (int i = 0; i < kPredItems; ++i) {
+        s[threadIdx.x * kPredItems + i] = ex;
+        ex += v[i];
+    }
+    __syncthreads();
+    for (int i = 0; i < kPredItems; ++i) {
+        const uint32_t l = i * kPredThreads + threadIdx.x;
+        const uint32_t ti = base + l;
+        if (ti <= P) out[ti] = s[l];
+    }
+}
+
+// The pass's range [a, e) of a node and its split m = a + #predicate.
+__device__ __forceinline__ void kd_pass_bounds(const KdView& t, int pass, const ActPos& ap,
+                                               const uint32_t* s1, const uint32_t* sc,
+                                               uint32_t& a, uint32_t& m)
+{
+    a = pass == 1 ? ap.lo : ap.lo + (s1[ap.t1] - s1[ap.t0]);
+    m = a + (sc[ap.t1] - sc[ap.t0]);
+}
+
+// Closed-form Hoare pairing: the r-th misplaced element of [a, m) counted
+// from the left swaps with the r-th misplaced element of [m, e) counted from
+// the right.  Misplaced right elements publish their position under their
+// rank; misplaced left elements then swap with their partner.
+__global__ __launch_bounds__(kKdThreads) void k_kd_publish(
+    KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
+    uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, const uint32_t* __restrict__ sc,
+    uint32_t* __restrict__ tmp)
+{
+    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti >= P) return;
+    const ActPos ap = act_pos(t, act, off, A, ti);
+    if (!kd_pred(t, pass, ap, s1)) return;
+    uint32_t a, m;
+    kd_pass_bounds(t, pass, ap, s1, sc, a, m);
+    if (ap.p >= m) tmp[a + (sc[ap.t1] - sc[ti + 1])] = ap.p;
+}
+
+__global__ __launch_bounds__(kKdThreads) void k_kd_swap(
+    KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
+    uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, const uint32_t* __restrict__ sc,
+    const uint32_t* __restrict__ tmp)
+{
+    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti >= P) return;
+    const ActPos ap = act_pos(t, act, off, A, ti);
+    uint32_t a, m;
+    kd_pass_bounds(t, pass, ap, s1, sc, a, m);
+    if (ap.p < a || ap.p >= m) return;
+    if (kd_pred(t, pass, ap, s1)) return;
+    // The predicate is 0 on [lo, a) in pass 2, so sc at a equals sc at t0.
+    const uint32_t rank = (ap.p - a) - (sc[ti] - sc[ap.t0]);
     const uint32_t q = tmp[a + rank];
+    const uint32_t p = ap.p;
     const uint32_t vi = t.vind[p];
     const double x = t.px[p], y = t.py[p], z = t.pz[p];
     t.vind[p] = t.vind[q];
@@ -304,67 +430,431 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_swap(KdView t, uint32_t n, ui
     t.pz[q] = z;
 }
 
-// Children of every splitting node: ids from the exclusive scan of the
-// level's split flags, segments from the lim1/lim2/count/2 rule, boxes from
-// the cut value.
-__global__ __launch_bounds__(kKdThreads) void k_kd_children(KdView t, uint32_t lv0, uint32_t lv1,
-                                                            const uint32_t* __restrict__ is_split,
-                                                            const uint32_t* __restrict__ rank,
-                                                            const uint32_t* __restrict__ s1,
-                                                            const uint32_t* __restrict__ s2,
-                                                            uint32_t* __restrict__ overflow)
+// One block: children of every big node of the level (ids base + 2k,
+// base + 2k + 1), then the next level's big list with its point-count
+// prefix, and the small children appended to the subtree list -- both in
+// child order, so the tree's numbering is deterministic.
+__global__ __launch_bounds__(kChildThreads) void k_kd_big_children(
+    KdView t, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off, uint32_t A,
+    const uint32_t* __restrict__ s1, const uint32_t* __restrict__ s2, KdScratch w, int nxt,
+    uint32_t sub)
 {
-    const uint32_t nd = lv0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (nd >= lv1 || !is_split[nd - lv0]) return;
-    const uint32_t s0 = t.nl[nd], e = t.nr[nd], cnt = e - s0;
-    const uint32_t L = s0 + (s1[e] - s1[s0]);
-    const uint32_t L2 = L + (s2[e] - s2[L]);
-    const uint32_t lim1 = L - s0, lim2 = L2 - s0;
-    uint32_t idx;
-    if (lim1 > cnt / 2)
-        idx = lim1;
-    else if (lim2 < cnt / 2)
-        idx = lim2;
-    else
-        idx = cnt / 2;
-    const uint32_t c1 = lv1 + 2 * rank[nd - lv0];
-    const uint32_t c2 = c1 + 1;
-    if (c2 >= t.cap) {
-        *overflow = 1;
-        return;
+    const uint32_t base = w.hdr[kHdrNodes];
+    const uint32_t small0 = w.hdr[kHdrSmall];
+    uint32_t nbig = 0, nsmall = 0, pnext = 0;
+    for (uint32_t k0 = 0; k0 < A; k0 += kChildThreads) {
+        const uint32_t k = k0 + threadIdx.x;
+        uint32_t sa = 0, sb = 0, ca = 0;
+        bool ok = k < A;
+        if (ok) {
+            const uint32_t nd = act[k];
+            const uint32_t lo = t.nl[nd], hi = t.nr[nd], cnt = hi - lo;
+            const uint32_t t0 = off[k], t1 = off[k + 1];
+            const uint32_t lim1 = s1[t1] - s1[t0];
+            const uint32_t lim2 = lim1 + (s2[t1] - s2[t0]);
+            const uint32_t idx = kd_split_index(cnt, lim1, lim2);
+            ca = base + 2 * k;
+            if (ca + 1 >= t.cap) {
+                w.hdr[kHdrOverflow] = kOvfNodes;
+                ok = false;
+            } else {
+                kd_make_children(t, nd, lo, idx, hi, ca);
+                sa = idx;
+                sb = cnt - idx;
+            }
+        }
+        const bool ba = ok && sa > sub, bb = ok && sb > sub;
+        const uint32_t cb = (ba ? 1u : 0u) + (bb ? 1u : 0u);
+        const uint32_t cs = ok ? 2u - cb : 0u;
+        const uint32_t sz = (ba ? sa : 0u) + (bb ? sb : 0u);
+        uint32_t tb, ts, tz;
+        const uint32_t eb = kd_block_scan<kChildThreads>(cb, &tb);
+        const uint32_t es = kd_block_scan<kChildThreads>(cs, &ts);
+        const uint32_t ez = kd_block_scan<kChildThreads>(sz, &tz);
+        if (ok) {
+            uint32_t ib = nbig + eb, is = small0 + nsmall + es, pz = pnext + ez;
+            const uint32_t ch[2] = {ca, ca + 1};
+            const uint32_t chs[2] = {sa, sb};
+            const bool chb[2] = {ba, bb};
+            for (int c = 0; c < 2; ++c) {
+                if (chb[c]) {
+                    if (ib >= w.list_cap) {
+                        w.hdr[kHdrOverflow] = kOvfLists;
+                    } else {
+                        w.act[nxt][ib] = ch[c];
+                        w.off[nxt][ib] = pz;
+                        t.aux[ch[c]] = ib;
+                        for (int d = 0; d < 3; ++d) {
+                            t.mn[3 * ch[c] + d] = ~0ull;
+                            t.mx[3 * ch[c] + d] = 0ull;
+                        }
+                    }
+                    ++ib;
+                    pz += chs[c];
+                } else {
+                    if (is >= w.list_cap)
+                        w.hdr[kHdrOverflow] = kOvfLists;
+                    else
+                        w.small[is] = ch[c];
+                    t.aux[ch[c]] = ~0u;
+                    ++is;
+                }
+            }
+        }
+        nbig += tb;
+        nsmall += ts;
+        pnext += tz;
     }
-    t.c1[nd] = static_cast<int32_t>(c1);
-    t.c2[nd] = static_cast<int32_t>(c2);
-    t.split[nd] = s0 + idx;
-    const int f = t.feat[nd];
-    const double cv = t.cutval[nd];
-    const double* b = t.box + 6 * static_cast<size_t>(nd);
-    double* b1 = t.box + 6 * static_cast<size_t>(c1);
-    double* b2 = t.box + 6 * static_cast<size_t>(c2);
-    for (int i = 0; i < 6; ++i) {
-        b1[i] = b[i];
-        b2[i] = b[i];
+    if (threadIdx.x == 0) {
+        if (nbig <= w.list_cap) w.off[nxt][nbig] = pnext;
+        w.hdr[kHdrA] = nbig;
+        w.hdr[kHdrP] = pnext;
+        w.hdr[kHdrSmall] = small0 + nsmall;
+        w.hdr[kHdrNodes] = base + 2 * A;
     }
-    b1[3 + f] = cv;
-    b2[f] = cv;
-    t.nl[c1] = s0;
-    t.nr[c1] = s0 + idx;
-    t.nl[c2] = s0 + idx;
-    t.nr[c2] = e;
-    t.parent[c1] = static_cast<int32_t>(nd);
-    t.parent[c2] = static_cast<int32_t>(nd);
-    t.which[c1] = 0;
-    t.which[c2] = 1;
 }
 
-__global__ __launch_bounds__(kKdThreads) void k_kd_segid(KdView t, uint32_t n, uint32_t lv0)
+// Extents of the next level's big nodes over this level's active points:
+// a segmented reduction per wave (children are contiguous), then per run an
+// atomic into a per-block LDS table when the next level has few big nodes,
+// else into global memory.
+__global__ __launch_bounds__(kKdThreads) void k_kd_big_minmax(
+    KdView t, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off, uint32_t A,
+    uint32_t P, const uint32_t* __restrict__ hdr, const uint32_t* __restrict__ act_next,
+    uint32_t lds_max)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t nd = t.segid[p];
-    if (nd < lv0 || t.c1[nd] < 0) return;
-    t.segid[p] = p < t.split[nd] ? static_cast<uint32_t>(t.c1[nd])
-                                 : static_cast<uint32_t>(t.c2[nd]);
+    __shared__ unsigned long long s_lo[kMinMaxLds * 3], s_hi[kMinMaxLds * 3];
+    const uint32_t An = hdr[kHdrA];
+    // After an overflow the children are incomplete; the build is redone.
+    if (An == 0 || hdr[kHdrOverflow]) return;
+    const bool use_lds = An <= min(lds_max, kMinMaxLds);
+    if (use_lds) {
+        for (uint32_t i = threadIdx.x; i < An * 3; i += blockDim.x) {
+            s_lo[i] = ~0ull;
+            s_hi[i] = 0ull;
+        }
+        __syncthreads();
+    }
+    const int lane = __lane_id();
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < P; b0 += gridDim.x * blockDim.x) {
+        const uint32_t ti = b0 + threadIdx.x;
+        uint32_t j = ~0u;
+        unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
+        if (ti < P) {
+            const ActPos ap = act_pos(t, act, off, A, ti);
+            const uint32_t ch = ap.p < t.split[ap.nd] ? static_cast<uint32_t>(t.c1[ap.nd])
+                                                      : static_cast<uint32_t>(t.c2[ap.nd]);
+            j = t.aux[ch];
+            if (j != ~0u) {
+                const double v[3] = {t.px[ap.p], t.py[ap.p], t.pz[ap.p]};
+                for (int d = 0; d < 3; ++d) lo[d] = hi[d] = enc_order(v[d]);
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t oj = __shfl_down(j, o);
+            const bool same = lane + o < 64 && oj == j;
+            for (int d = 0; d < 3; ++d) {
+                const unsigned long long ol = __shfl_down(lo[d], o);
+                const unsigned long long oh = __shfl_down(hi[d], o);
+                if (same) {
+                    lo[d] = ol < lo[d] ? ol : lo[d];
+                    hi[d] = oh > hi[d] ? oh : hi[d];
+                }
+            }
+        }
+        const uint32_t pj = __shfl_up(j, 1);
+        if (j != ~0u && (lane == 0 || pj != j)) {
+            if (use_lds) {
+                for (int d = 0; d < 3; ++d) {
+                    atomicMin(&s_lo[j * 3 + d], lo[d]);
+                    atomicMax(&s_hi[j * 3 + d], hi[d]);
+                }
+            } else {
+                const uint32_t nd = act_next[j];
+                for (int d = 0; d < 3; ++d) {
+                    atomicMin(&t.mn[3 * nd + d], lo[d]);
+                    atomicMax(&t.mx[3 * nd + d], hi[d]);
+                }
+            }
+        }
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < An; j += blockDim.x) {
+            if (s_hi[j * 3] == 0ull) continue;  // no point of this node in the block
+            const uint32_t nd = act_next[j];
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(&t.mn[3 * nd + d], s_lo[j * 3 + d]);
+                atomicMax(&t.mx[3 * nd + d], s_hi[j * 3 + d]);
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------- small subtrees --
+// One workgroup per small node: its points in LDS (coordinates, ids), the
+// node's subtree built level by level -- extents by segmented reduction and
+// LDS atomics, the two Hoare passes by block prefix sums and the same
+// closed-form pairing, children records written to global memory -- and the
+// permuted points written back.
+struct SubShared {
+    double x[kSub], y[kSub], z[kSub];
+    uint32_t v[kSub];
+    uint32_t pre[kSub + 1];
+    uint16_t tmp[kSub];
+    uint8_t node[kSub];                 // level-local node index, 0xFF: settled
+    unsigned long long mn[kSubNodes * 3], mx[kSubNodes * 3];
+    double cut[kSubNodes];
+    uint32_t nid[kSubNodes];
+    uint16_t lo[kSubNodes], hi[kSubNodes], L[kSubNodes], L2[kSubNodes], sp[kSubNodes];
+    uint8_t child[kSubNodes];
+    int8_t feat[kSubNodes];
+    uint32_t nnid[kSubNodes];
+    uint16_t nlo[kSubNodes], nhi[kSubNodes];
+    uint32_t nc, nnext, any;
+};
+
+__device__ __forceinline__ double sub_coord(const SubShared& S, int f, uint32_t i)
+{
+    return f == 0 ? S.x[i] : (f == 1 ? S.y[i] : S.z[i]);
+}
+
+// Exclusive block prefix of pred(i) over i in [0, kSub) into S.pre (pre[kSub]
+// = total).  Thread tid owns items [16 tid, 16 tid + 16).
+template <class Pred> __device__ __forceinline__ void sub_prefix(SubShared& S, Pred pred)
+{
+    constexpr int kItems = kSub / kSubThreads;
+    const uint32_t i0 = threadIdx.x * kItems;
+    uint32_t bits = 0, cnt = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i)
+        if (pred(i0 + i)) {
+            bits |= 1u << i;
+            ++cnt;
+        }
+    uint32_t tot;
+    uint32_t ex = kd_block_scan<kSubThreads>(cnt, &tot);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        S.pre[i0 + i] = ex;
+        ex += (bits >> i) & 1u;
+    }
+    if (threadIdx.x == 0) S.pre[kSub] = tot;
+    __syncthreads();
+}
+
+__device__ __forceinline__ void sub_swap(SubShared& S, uint32_t i, uint32_t q)
+{
+    const double x = S.x[i], y = S.y[i], z = S.z[i];
+    const uint32_t v = S.v[i];
+    S.x[i] = S.x[q];
+    S.y[i] = S.y[q];
+    S.z[i] = S.z[q];
+    S.v[i] = S.v[q];
+    S.x[q] = x;
+    S.y[q] = y;
+    S.z[q] = z;
+    S.v[q] = v;
+}
+
+__global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
+                                                           const uint32_t* __restrict__ small,
+                                                           uint32_t* __restrict__ hdr)
+{
+    __shared__ SubShared S;
+    const uint32_t tid = threadIdx.x;
+    const int lane = __lane_id(), wv = tid >> 6;
+    const uint32_t nd0 = small[blockIdx.x];
+    const uint32_t a0 = t.nl[nd0];
+    const uint32_t m = t.nr[nd0] - a0;  // <= kSub
+    for (uint32_t i = tid; i < kSub; i += kSubThreads) {
+        if (i < m) {
+            S.x[i] = t.px[a0 + i];
+            S.y[i] = t.py[a0 + i];
+            S.z[i] = t.pz[a0 + i];
+            S.v[i] = t.vind[a0 + i];
+            S.node[i] = 0;
+        } else {
+            S.node[i] = 0xFF;
+        }
+    }
+    if (tid == 0) {
+        S.nc = 1;
+        S.nid[0] = nd0;
+        S.lo[0] = 0;
+        S.hi[0] = static_cast<uint16_t>(m);  // m <= 4096 < 65536
+    }
+    __syncthreads();
+    for (;;) {
+        const uint32_t nc = S.nc;
+        // (a) extents of the level's nodes
+        for (uint32_t i = tid; i < nc * 3; i += kSubThreads) {
+            S.mn[i] = ~0ull;
+            S.mx[i] = 0ull;
+        }
+        if (tid == 0) {
+            S.nnext = 0;
+            S.any = 0;
+        }
+        __syncthreads();
+        for (uint32_t c = wv; c * 64 < m; c += kSubThreads / 64) {
+            const uint32_t i = c * 64 + lane;
+            const uint32_t j = i < m ? S.node[i] : 0xFFu;
+            unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
+            if (j != 0xFFu) {
+                lo[0] = hi[0] = enc_order(S.x[i]);
+                lo[1] = hi[1] = enc_order(S.y[i]);
+                lo[2] = hi[2] = enc_order(S.z[i]);
+            }
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t oj = __shfl_down(j, o);
+                const bool same = lane + o < 64 && oj == j;
+                for (int d = 0; d < 3; ++d) {
+                    const unsigned long long ol = __shfl_down(lo[d], o);
+                    const unsigned long long oh = __shfl_down(hi[d], o);
+                    if (same) {
+                        lo[d] = ol < lo[d] ? ol : lo[d];
+                        hi[d] = oh > hi[d] ? oh : hi[d];
+                    }
+                }
+            }
+            const uint32_t pj = __shfl_up(j, 1);
+            if (j != 0xFFu && (lane == 0 || pj != j))
+                for (int d = 0; d < 3; ++d) {
+                    atomicMin(&S.mn[j * 3 + d], lo[d]);
+                    atomicMax(&S.mx[j * 3 + d], hi[d]);
+                }
+        }
+        __syncthreads();
+        // (b) one thread per node: parent's div, leaf or split
+        if (tid < nc) {
+            const uint32_t nd = S.nid[tid];
+            double mn[3], mx[3];
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = dec_order(S.mn[tid * 3 + d]);
+                mx[d] = dec_order(S.mx[tid * 3 + d]);
+            }
+            kd_parent_div(t, nd, mn, mx);
+            const uint32_t cnt = S.hi[tid] - S.lo[tid];
+            if (cnt <= kLeafMax) {
+                S.feat[tid] = -1;
+            } else {
+                int f;
+                double cv;
+                kd_choose_split(t.box + 6 * static_cast<size_t>(nd), mn, mx, f, cv);
+                t.feat[nd] = f;
+                t.cutval[nd] = cv;
+                S.feat[tid] = static_cast<int8_t>(f);
+                S.cut[tid] = cv;
+                S.any = 1;
+            }
+        }
+        __syncthreads();
+        if (!S.any) break;
+        // (c) pass 1: coord < cut
+        auto p1 = [&](uint32_t i) {
+            const uint32_t j = S.node[i];
+            if (j == 0xFFu || S.feat[j] < 0) return false;
+            return sub_coord(S, S.feat[j], i) < S.cut[j];
+        };
+        sub_prefix(S, p1);
+        if (tid < nc && S.feat[tid] >= 0)
+            S.L[tid] = static_cast<uint16_t>(S.lo[tid] + (S.pre[S.hi[tid]] - S.pre[S.lo[tid]]));
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += kSubThreads) {
+            const uint32_t j = S.node[i];
+            if (j == 0xFFu || S.feat[j] < 0) continue;
+            if (i >= S.L[j] && p1(i)) S.tmp[S.lo[j] + (S.pre[S.hi[j]] - S.pre[i + 1])] = i;
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += kSubThreads) {
+            const uint32_t j = S.node[i];
+            if (j == 0xFFu || S.feat[j] < 0) continue;
+            if (i < S.L[j] && !p1(i)) {
+                const uint32_t r = (i - S.lo[j]) - (S.pre[i] - S.pre[S.lo[j]]);
+                sub_swap(S, i, S.tmp[S.lo[j] + r]);
+            }
+        }
+        __syncthreads();
+        // (d) pass 2: coord <= cut over [L, hi)
+        auto p2 = [&](uint32_t i) {
+            const uint32_t j = S.node[i];
+            if (j == 0xFFu || S.feat[j] < 0 || i < S.L[j]) return false;
+            return sub_coord(S, S.feat[j], i) <= S.cut[j];
+        };
+        sub_prefix(S, p2);
+        if (tid < nc && S.feat[tid] >= 0)
+            S.L2[tid] = static_cast<uint16_t>(S.L[tid] + (S.pre[S.hi[tid]] - S.pre[S.lo[tid]]));
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += kSubThreads) {
+            const uint32_t j = S.node[i];
+            if (j == 0xFFu || S.feat[j] < 0) continue;
+            if (i >= S.L2[j] && p2(i)) S.tmp[S.L[j] + (S.pre[S.hi[j]] - S.pre[i + 1])] = i;
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += kSubThreads) {
+            const uint32_t j = S.node[i];
+            if (j == 0xFFu || S.feat[j] < 0) continue;
+            if (i >= S.L[j] && i < S.L2[j] && !p2(i)) {
+                const uint32_t r = (i - S.L[j]) - (S.pre[i] - S.pre[S.L[j]]);
+                sub_swap(S, i, S.tmp[S.L[j] + r]);
+            }
+        }
+        __syncthreads();
+        // (e) children of the splitting nodes
+        if (tid < nc && S.feat[tid] >= 0) {
+            const uint32_t nd = S.nid[tid];
+            const uint32_t lo = S.lo[tid], hi = S.hi[tid];
+            const uint32_t idx = kd_split_index(hi - lo, S.L[tid] - lo, S.L2[tid] - lo);
+            const uint32_t c1 = atomicAdd(&hdr[kHdrNodes], 2u);
+            const uint32_t slot = atomicAdd(&S.nnext, 2u);
+            if (c1 + 1 >= t.cap) {
+                hdr[kHdrOverflow] = kOvfNodes;
+                S.feat[tid] = -1;  // stop this branch; the build is redone
+            } else if (slot + 1 >= static_cast<uint32_t>(kSubNodes)) {
+                hdr[kHdrOverflow] = kOvfSubtree;
+                S.feat[tid] = -1;
+            } else {
+                kd_make_children(t, nd, a0 + lo, idx, a0 + hi, c1);
+                S.sp[tid] = static_cast<uint16_t>(lo + idx);
+                S.child[tid] = static_cast<uint8_t>(slot);
+                S.nnid[slot] = c1;
+                S.nlo[slot] = static_cast<uint16_t>(lo);
+                S.nhi[slot] = static_cast<uint16_t>(lo + idx);
+                S.nnid[slot + 1] = c1 + 1;
+                S.nlo[slot + 1] = static_cast<uint16_t>(lo + idx);
+                S.nhi[slot + 1] = static_cast<uint16_t>(hi);
+            }
+        }
+        __syncthreads();
+        // (f) points move to their child (leaves settle)
+        for (uint32_t i = tid; i < m; i += kSubThreads) {
+            const uint32_t j = S.node[i];
+            if (j == 0xFFu) continue;
+            if (S.feat[j] < 0)
+                S.node[i] = 0xFF;
+            else
+                S.node[i] = static_cast<uint8_t>(i < S.sp[j] ? S.child[j] : S.child[j] + 1);
+        }
+        __syncthreads();
+        const uint32_t nn = min(S.nnext, static_cast<uint32_t>(kSubNodes));
+        if (tid < nn) {
+            S.nid[tid] = S.nnid[tid];
+            S.lo[tid] = S.nlo[tid];
+            S.hi[tid] = S.nhi[tid];
+        }
+        if (tid == 0) S.nc = nn;
+        __syncthreads();
+        if (nn == 0) break;
+    }
+    for (uint32_t i = tid; i < m; i += kSubThreads) {
+        t.px[a0 + i] = S.x[i];
+        t.py[a0 + i] = S.y[i];
+        t.pz[a0 + i] = S.z[i];
+        t.vind[a0 + i] = S.v[i];
+    }
 }
 
 // -------------------------------------------------------------- search --
@@ -394,21 +884,49 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, FusedProgram prog,
+__device__ __forceinline__ uint32_t kd_tile_of(const uint64_t* __restrict__ starts, uint32_t T,
+                                               uint64_t s)
+{
+    uint32_t lo = 0, hi = T;  // largest t with starts[t] <= s
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (starts[mid] <= s)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, FusedProgram prog0,
                                                     int mode, KnnOutputs out,
                                                     const double* __restrict__ x,
                                                     const double* __restrict__ y,
                                                     const double* __restrict__ z,
-                                                    uint32_t* __restrict__ failed)
+                                                    uint32_t* __restrict__ failed,
+                                                    const TileInfo* __restrict__ tiles,
+                                                    const uint64_t* __restrict__ starts,
+                                                    uint32_t ntiles)
 {
     const int lane = __lane_id();
     const uint32_t waves = gridDim.x * (blockDim.x / 64);
     const uint32_t nflag = *out.nflagged;
-    const uint32_t cap = static_cast<uint32_t>(prog.fetch);
-    const uint32_t k = static_cast<uint32_t>(prog.k);
-    const uint32_t stride = static_cast<uint32_t>(prog.stride);
     for (uint32_t f = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; f < nflag; f += waves) {
         const uint32_t s = out.flagged[f];
+        // The row's tile: its root node (tile r's tree is rooted at node r)
+        // and its neighborhood size.
+        uint32_t root = 0;
+        uint32_t cap = static_cast<uint32_t>(prog0.fetch);
+        uint32_t k = static_cast<uint32_t>(prog0.k);
+        uint32_t stride = static_cast<uint32_t>(prog0.stride);
+        if (tiles) {
+            root = kd_tile_of(starts, ntiles, s);
+            const FusedProgram& pr = tiles[root].prog;
+            cap = static_cast<uint32_t>(pr.fetch);
+            k = static_cast<uint32_t>(pr.k);
+            stride = static_cast<uint32_t>(pr.stride);
+        }
+        const double* rb = t.box + 6 * static_cast<size_t>(root);  // the tile's bounds
         const uint32_t qid = cv.perm[s];
         const double q0 = x[qid], q1 = y[qid], q2 = z[qid];
         const double q[3] = {q0, q1, q2};
@@ -417,17 +935,17 @@ __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, Fuse
         double dists[3] = {0.0, 0.0, 0.0};
         double mind = 0.0;
         for (int i = 0; i < 3; ++i) {
-            if (q[i] < t.root[i]) {
-                dists[i] = (q[i] - t.root[i]) * (q[i] - t.root[i]);
+            if (q[i] < rb[i]) {
+                dists[i] = (q[i] - rb[i]) * (q[i] - rb[i]);
                 mind += dists[i];
             }
-            if (q[i] > t.root[3 + i]) {
-                dists[i] = (q[i] - t.root[3 + i]) * (q[i] - t.root[3 + i]);
+            if (q[i] > rb[3 + i]) {
+                dists[i] = (q[i] - rb[3 + i]) * (q[i] - rb[3 + i]);
                 mind += dists[i];
             }
         }
         // Frame registers: lane i holds stack frame i.
-        int f_node = 0, f_other = -1, f_idx = 0, f_stage = 0;
+        int f_node = static_cast<int>(root), f_other = -1, f_idx = 0, f_stage = 0;
         double f_mind = mind, f_cut = 0.0, f_dst = 0.0;
         int sp = 1;
         bool overflow = false;
@@ -577,7 +1095,34 @@ __global__ __launch_bounds__(256) void k_flag_all(uint64_t n, const uint32_t* __
         out.flags[s] = 0;
 }
 
+// Rows of the tiles with fewer points than their result capacity go to the
+// exact search.
+__global__ __launch_bounds__(256) void k_flag_tiles(uint64_t n, const uint32_t* __restrict__ perm,
+                                                    int mode, KnnOutputs out,
+                                                    const TileInfo* __restrict__ tiles,
+                                                    const uint64_t* __restrict__ starts,
+                                                    uint32_t ntiles)
+{
+    const uint64_t s = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t tl = kd_tile_of(starts, ntiles, s);
+    if (!tiles[tl].all_exact) return;
+    const uint32_t slot = atomicAdd(out.nflagged, 1u);
+    out.flagged[slot] = static_cast<uint32_t>(s);
+    if (mode == 1)
+        out.status[0][perm[s]] = 0;
+    else
+        out.flags[s] = 0;
+}
+
 } // namespace
+
+void launch_flag_tiles(uint64_t n, const uint32_t* perm, int mode, const KnnOutputs& out,
+                       const TileTables& tt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_flag_tiles, dim3(div_up(std::max<uint64_t>(n, 1), 256)), dim3(256), 0,
+                       s, n, perm, mode, out, tt.tiles, tt.starts, tt.ntiles);
+}
 
 void launch_flag_all(uint64_t n, const uint32_t* perm, int mode, const KnnOutputs& out,
                      hipStream_t s)
@@ -588,70 +1133,81 @@ void launch_flag_all(uint64_t n, const uint32_t* perm, int mode, const KnnOutput
 
 uint32_t kd_node_capacity(uint64_t n) { return static_cast<uint32_t>(n / 16 + 1024); }
 
-// Builds the tree level by level; returns false if the node capacity was
-// too small (the caller grows it and builds again).
-bool launch_kd_build(const double* x, const double* y, const double* z, uint64_t n,
-                     const double* bounds, KdView t, KdScratch w, uint32_t* nodes,
-                     hipStream_t s)
+// Builds the tree; returns 0, or an overflow code (kOvfNodes: node
+// capacity, kOvfLists: big/small node lists) after which the caller grows
+// that capacity and builds again.
+uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint64_t n,
+                         const uint64_t* starts, uint32_t ntiles, const double* bounds,
+                         KdView t, KdScratch w, const KdTuning& kt, uint32_t* nodes,
+                         hipStream_t s)
 {
-    const uint32_t nn = static_cast<uint32_t>(n);
-    const uint32_t nb = div_up(n + 1, kKdThreads);
+    const uint32_t sub = std::min<uint32_t>(std::max<uint32_t>(kt.subtree_points, kLeafMax + 1),
+                                            kSub);
     hipLaunchKernelGGL(k_kd_init, dim3(div_up(n, kKdThreads)), dim3(kKdThreads), 0, s, x, y, z,
-                       nn, t);
-    hipLaunchKernelGGL(k_kd_root, dim3(1), dim3(1), 0, s, t, nn, bounds);
-    launch_fill_u32(w.overflow, 1, 0u, s);
-    uint32_t lv0 = 0, lv1 = 1;
-    while (lv1 > lv0) {
-        const uint32_t nl = lv1 - lv0;
-        const uint32_t gnodes = div_up(nl, kKdThreads);
-        hipLaunchKernelGGL(k_kd_minmax_reset, dim3(gnodes), dim3(kKdThreads), 0, s, t, lv0, lv1);
-        hipLaunchKernelGGL(k_kd_minmax, dim3(nl <= kLdsNodes ? std::min<uint32_t>(nb, 2048) : nb),
-                           dim3(kKdThreads), 0, s, t, nn, lv0, lv1);
-        launch_fill_u32(w.is_split, nl + 1, 0u, s);
-        hipLaunchKernelGGL(k_kd_split, dim3(gnodes), dim3(kKdThreads), 0, s, t, lv0, lv1,
-                           w.is_split);
-        // pass 1: coord < cut over each splitting node
-        hipLaunchKernelGGL(k_kd_flags, dim3(nb), dim3(kKdThreads), 0, s, t, nn, lv0, 1, w.s1,
-                           w.flag);
-        launch_excl_scan(w.flag, w.s1, n + 1, w.scan_tmp, s);
-        hipLaunchKernelGGL(k_kd_publish, dim3(nb), dim3(kKdThreads), 0, s, t, nn, lv0, 1, w.s1,
-                           w.s1, w.flag, w.tmp);
-        hipLaunchKernelGGL(k_kd_swap, dim3(nb), dim3(kKdThreads), 0, s, t, nn, lv0, 1, w.s1,
-                           w.s1, w.flag, w.tmp);
-        // pass 2: coord <= cut over [L, right)
-        hipLaunchKernelGGL(k_kd_flags, dim3(nb), dim3(kKdThreads), 0, s, t, nn, lv0, 2, w.s1,
-                           w.flag);
-        launch_excl_scan(w.flag, w.s2, n + 1, w.scan_tmp, s);
-        hipLaunchKernelGGL(k_kd_publish, dim3(nb), dim3(kKdThreads), 0, s, t, nn, lv0, 2, w.s1,
-                           w.s2, w.flag, w.tmp);
-        hipLaunchKernelGGL(k_kd_swap, dim3(nb), dim3(kKdThreads), 0, s, t, nn, lv0, 2, w.s1,
-                           w.s2, w.flag, w.tmp);
-        // children
-        launch_excl_scan(w.is_split, w.rank, nl + 1, w.scan_tmp, s);
-        hipLaunchKernelGGL(k_kd_children, dim3(gnodes), dim3(kKdThreads), 0, s, t, lv0, lv1,
-                           w.is_split, w.rank, w.s1, w.s2, w.overflow);
-        hipLaunchKernelGGL(k_kd_segid, dim3(div_up(n, kKdThreads)), dim3(kKdThreads), 0, s, t,
-                           nn, lv0);
-        uint32_t h[2];
-        GF_HIP(hipMemcpyAsync(&h[0], w.rank + nl, 4, hipMemcpyDeviceToHost, s));
-        GF_HIP(hipMemcpyAsync(&h[1], w.overflow, 4, hipMemcpyDeviceToHost, s));
+                       static_cast<uint32_t>(n), t);
+    launch_fill_u32(w.hdr, 8, 0u, s);
+    hipLaunchKernelGGL(k_kd_roots, dim3(1), dim3(kChildThreads), 0, s, t, starts, ntiles,
+                       bounds, w, sub);
+    uint32_t h[8];
+    GF_HIP(hipMemcpyAsync(h, w.hdr, sizeof(h), hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    if (h[kHdrOverflow]) return h[kHdrOverflow];
+    uint32_t A = h[kHdrA], P = h[kHdrP], nsmall = h[kHdrSmall];
+    int cur = 0;
+    while (A) {
+        hipLaunchKernelGGL(k_kd_big_split, dim3(div_up(A, kKdThreads)), dim3(kKdThreads), 0, s,
+                           t, w.act[cur], A);
+        for (int pass = 1; pass <= 2; ++pass) {
+            uint32_t* sc = pass == 1 ? w.s1 : w.s2;
+            const uint32_t nb = div_up(static_cast<uint64_t>(P) + 1, kPredTile);
+            hipLaunchKernelGGL(k_kd_pred_reduce, dim3(nb), dim3(kPredThreads), 0, s, t, pass,
+                               w.act[cur], w.off[cur], A, P, w.s1, w.sums);
+            launch_scan_top(w.sums, nb, s);
+            hipLaunchKernelGGL(k_kd_pred_down, dim3(nb), dim3(kPredThreads), 0, s, t, pass,
+                               w.act[cur], w.off[cur], A, P, w.s1, w.sums, sc);
+            hipLaunchKernelGGL(k_kd_publish, dim3(div_up(P, kKdThreads)), dim3(kKdThreads), 0,
+                               s, t, pass, w.act[cur], w.off[cur], A, P, w.s1, sc, w.tmp);
+            hipLaunchKernelGGL(k_kd_swap, dim3(div_up(P, kKdThreads)), dim3(kKdThreads), 0, s,
+                               t, pass, w.act[cur], w.off[cur], A, P, w.s1, sc, w.tmp);
+        }
+        hipLaunchKernelGGL(k_kd_big_children, dim3(1), dim3(kChildThreads), 0, s, t,
+                           w.act[cur], w.off[cur], A, w.s1, w.s2, w, cur ^ 1, sub);
+        hipLaunchKernelGGL(k_kd_big_minmax,
+                           dim3(std::min<uint32_t>(div_up(P, kKdThreads), 2048)),
+                           dim3(kKdThreads), 0, s, t, w.act[cur], w.off[cur], A, P, w.hdr,
+                           w.act[cur ^ 1], kt.minmax_lds_nodes);
+        GF_HIP(hipMemcpyAsync(h, w.hdr, sizeof(h), hipMemcpyDeviceToHost, s));
         GF_HIP(hipStreamSynchronize(s));
-        if (h[1]) return false;
-        lv0 = lv1;
-        lv1 = lv1 + 2 * h[0];
+        if (h[kHdrOverflow]) return h[kHdrOverflow];
+        A = h[kHdrA];
+        P = h[kHdrP];
+        nsmall = h[kHdrSmall];
+        cur ^= 1;
     }
-    *nodes = lv1;
-    return true;
+    if (nsmall) {
+        hipLaunchKernelGGL(k_kd_subtree, dim3(nsmall), dim3(kSubThreads), 0, s, t, w.small,
+                           w.hdr);
+        GF_HIP(hipGetLastError());
+    }
+    GF_HIP(hipMemcpyAsync(h, w.hdr, sizeof(h), hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    if (h[kHdrOverflow]) return h[kHdrOverflow];
+    *nodes = h[kHdrNodes];
+    return 0;
 }
+
+uint32_t kd_list_capacity(uint64_t n) { return static_cast<uint32_t>(n / kSub * 4 + 4096); }
 
 void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram& prog, int mode,
                        const KnnOutputs& out, const double* x, const double* y, const double* z,
-                       uint32_t nflagged, uint32_t* failed, hipStream_t s)
+                       uint32_t nflagged, uint32_t* failed, hipStream_t s,
+                       const TileTables* tt)
 {
     if (!nflagged) return;
     const uint32_t blocks = std::min<uint32_t>(div_up(nflagged, 4), 4096);
     hipLaunchKernelGGL(k_kd_resolve, dim3(blocks), dim3(256), 0, s, t, cv, prog, mode, out, x, y,
-                       z, failed);
+                       z, failed, tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
+                       tt ? tt->ntiles : 0u);
 }
 
 } // namespace gf

@@ -74,14 +74,17 @@ __device__ __forceinline__ uint32_t xcd_logical_block(uint32_t b, uint32_t nb)
 }
 
 struct KnnArgs {
-    CloudView cv;
-    FusedProgram prog;
+    CloudView cv;       // tiled: global arrays; colstart of all tiles
+    FusedProgram prog;  // tiled: replaced per wave by the tile's
     KnnOutputs out;
     KnnTuning tune;
     int32_t mode;     // 0 features, 1 neighbor lists
     int32_t q_per_wave;
     int32_t lds_stride;
     double margin;
+    const TileInfo* tiles;   // tiled search only
+    const WaveTask* tasks;
+    uint32_t ntasks;
 };
 
 // The examined region: columns [x0, x1] x [y0, y1], z cells [z0, z1].
@@ -416,25 +419,44 @@ __device__ __forceinline__ void finish_query(const KnnArgs& a, const Ent& best,
     }
 }
 
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_knn(KnnArgs a)
+template <bool kTiled>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_knn(KnnArgs a_in)
 {
     extern __shared__ uint32_t s_lds[];
     const int lane = __lane_id();
     const int w = threadIdx.x / kWave;
+    const int Q = a_in.q_per_wave;
+    const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const uint64_t task = static_cast<uint64_t>(lb) * kWaves + w;
+    KnnArgs a = a_in;
+    uint64_t q0;
+    int nq;
+    if (kTiled) {
+        // The wave's queries and its tile's grid and program.
+        if (task >= a_in.ntasks) return;
+        const WaveTask tk = a_in.tasks[task];
+        const TileInfo& ti = a_in.tiles[tk.tile];
+        a.cv.g = ti.g;
+        a.cv.colstart = a_in.cv.colstart + ti.colbase;
+        a.prog = ti.prog;
+        a.margin = ti.margin;
+        q0 = tk.q0;
+        nq = static_cast<int>(tk.nq);
+    } else {
+        const uint64_t n = a_in.cv.n;
+        q0 = task * Q;
+        if (q0 >= n) return;
+        nq = static_cast<int>(n - q0 < static_cast<uint64_t>(Q) ? n - q0 : Q);
+    }
     const CloudView& cv = a.cv;
     const GridParams& g = cv.g;
-    const uint64_t n = cv.n;
-    const int Q = a.q_per_wave;
-    const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const uint64_t q0 = (static_cast<uint64_t>(lb) * kWaves + w) * Q;
-    if (q0 >= n) return;
-    const int nq = static_cast<int>(n - q0 < static_cast<uint64_t>(Q) ? n - q0 : Q);
+    const uint64_t n = cv.n;  // rows of the whole batch (plane stride of cov)
     const int fetch = a.prog.fetch;
     const int k = a.prog.k;
     // LDS: per-wave neighbor lists (features mode), then per wave a 64-entry
     // compaction scratch of 16 bytes per entry and a 64-entry pending list.
     const size_t list_words = a.mode == 1 ? 0 : static_cast<size_t>(kWaves) * Q * a.lds_stride;
-    uint32_t* nbr = s_lds + static_cast<size_t>(w) * Q * a.lds_stride;
+    uint32_t* nbr = s_lds + static_cast<size_t>(w) * Q * a.lds_stride;  // lds_stride >= k
     uint32_t* scratch = s_lds + list_words + static_cast<size_t>(w) * kScratchWords;
     uint32_t* pending = scratch + 512;  // queries left for the growing search
     int npend = 0;
@@ -648,14 +670,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
 
 // Per sorted row: eigensystem of the stored covariance and every requested
 // stage column, written at the row's original index.
+__device__ __forceinline__ uint32_t tile_of(const uint64_t* __restrict__ starts, uint32_t T,
+                                            uint64_t s)
+{
+    uint32_t lo = 0, hi = T;  // largest t with starts[t] <= s
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (starts[mid] <= s)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov,
                                                   const uint8_t* __restrict__ flags,
                                                   const uint32_t* __restrict__ perm,
                                                   uint64_t n, FusedProgram prog,
-                                                  KnnOutputs out)
+                                                  KnnOutputs out, const TileInfo* __restrict__ tiles,
+                                                  const uint64_t* __restrict__ starts,
+                                                  uint32_t ntiles)
 {
     const uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (s >= n) return;
+    // The neighborhood size of the row's tile (stages are the same in every tile).
+    const int k = tiles ? tiles[tile_of(starts, ntiles, s)].prog.k : prog.k;
     double c[6];
     for (int i = 0; i < 6; ++i) c[i] = cov[static_cast<uint64_t>(i) * n + s];
     const uint8_t f = flags[s];
@@ -667,7 +707,7 @@ __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov
     for (int si = 0; si < prog.nstages; ++si) {
         const StageProgram& sp = prog.st[si];
         double vals[10];
-        uint8_t st = stage_outputs(sp, prog.k, c, zero, info, ev, vec, vals);
+        uint8_t st = stage_outputs(sp, k, c, zero, info, ev, vec, vals);
         if (st == 0)
             for (int cc = 0; cc < sp.ncols; ++cc) out.col[sp.col_offset + cc][qid] = vals[cc];
         out.status[si][qid] = st | f;
@@ -692,7 +732,8 @@ __global__ __launch_bounds__(256) void k_eigen_batch(const double* __restrict__ 
 } // namespace
 
 void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
-                const KnnOutputs& out, const KnnTuning& tune, hipStream_t s)
+                const KnnOutputs& out, const KnnTuning& tune, hipStream_t s,
+                const TileTables* tt)
 {
     if (cv.n == 0) return;
     KnnArgs a;
@@ -701,25 +742,39 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
     a.out = out;
     a.tune = tune;
     a.mode = mode;
-    a.q_per_wave = prog.k <= 32 ? 64 : 32;
-    a.lds_stride = prog.k | 1;  // odd stride: lanes hit distinct LDS banks
+    a.tiles = nullptr;
+    a.tasks = nullptr;
+    a.ntasks = 0;
+    const int max_k = tt ? tt->max_k : prog.k;
+    a.q_per_wave = tt ? tt->q_per_wave : knn_queries_per_wave(prog.k);
+    a.lds_stride = max_k | 1;  // odd stride: lanes hit distinct LDS banks
     const GridParams& g = cv.g;
     a.margin = 1e-6 * fmin(g.c, g.cz) +
                1e-15 * (fabs(g.ox) + fabs(g.oy) + fabs(g.oz)) + 1e-300;
-    const uint64_t per_block = static_cast<uint64_t>(kWaves) * a.q_per_wave;
-    const uint32_t blocks = div_up(cv.n, per_block);
     const size_t list_words =
         mode == 1 ? 0 : static_cast<size_t>(kWaves) * a.q_per_wave * a.lds_stride;
     const size_t lds = (list_words + static_cast<size_t>(kWaves) * kScratchWords) * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_knn, dim3(blocks), dim3(kBlock), lds, s, a);
+    if (tt) {
+        if (!tt->ntasks) return;
+        a.tiles = tt->tiles;
+        a.tasks = tt->tasks;
+        a.ntasks = tt->ntasks;
+        const uint32_t blocks = div_up(tt->ntasks, kWaves);
+        hipLaunchKernelGGL(k_knn<true>, dim3(blocks), dim3(kBlock), lds, s, a);
+        return;
+    }
+    const uint64_t per_block = static_cast<uint64_t>(kWaves) * a.q_per_wave;
+    const uint32_t blocks = div_up(cv.n, per_block);
+    hipLaunchKernelGGL(k_knn<false>, dim3(blocks), dim3(kBlock), lds, s, a);
 }
 
 void launch_features(const CloudView& cv, const FusedProgram& prog,
-                     const KnnOutputs& out, hipStream_t s)
+                     const KnnOutputs& out, hipStream_t s, const TileTables* tt)
 {
     if (cv.n == 0) return;
     hipLaunchKernelGGL(k_features, dim3(div_up(cv.n, 256)), dim3(256), 0, s, out.cov,
-                       out.flags, cv.perm, cv.n, prog, out);
+                       out.flags, cv.perm, cv.n, prog, out, tt ? tt->tiles : nullptr,
+                       tt ? tt->starts : nullptr, tt ? tt->ntiles : 0u);
 }
 
 void launch_eigen_batch(const double* cov, uint64_t m, double* evals,

@@ -1,0 +1,383 @@
+// serve.cpp -- a serving loop over batches: requests (one tile each) arrive
+// at any time; worker threads, each with its own context and stream, admit
+// pending requests under an HBM budget, run them as one batch, and hand the
+// results back.  Requests that arrive while batches are on the device wait
+// only until a worker is free and the budget allows; two workers keep two
+// batches in flight, so uploads, searches and copies of different batches
+// overlap on the device.  A request's result equals the tile run alone.
+#include "gpupdal_amd.h"
+
+#include "gf_internal.hpp"
+#include "gf_kernels.hpp"
+#include "gf_runtime.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace gf {
+
+namespace {
+
+double now_s()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Device bytes a request of n points holds while in flight: the point
+// buffers, the index, the exact tree, the covariance and flags, and the
+// stage outputs.
+uint64_t request_bytes(uint64_t n, int ncols)
+{
+    return n * (256ull + 8ull * static_cast<uint64_t>(ncols)) + (1ull << 20);
+}
+
+struct Request {
+    uint64_t id = 0;
+    const double *x = nullptr, *y = nullptr, *z = nullptr;
+    uint64_t n = 0;
+    gf_stage st[kMaxStages];
+    int ns = 0;
+    int32_t knn = 0;
+    std::vector<double*> cols;
+    std::vector<uint8_t*> status;
+    int ncols = 0;
+    uint64_t bytes = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    uint32_t batch_id = 0, batch_tiles = 0;
+    bool done = false;
+    int rc = GF_OK;
+    std::string error;
+};
+
+bool same_shape(const Request& a, const Request& b)
+{
+    if (a.ns != b.ns) return false;
+    for (int i = 0; i < a.ns; ++i) {
+        const gf_stage &p = a.st[i], &q = b.st[i];
+        if (p.stage != q.stage || p.mode != q.mode || p.features != q.features ||
+            p.always_up != q.always_up || p.normalize != q.normalize ||
+            (p.stage != GF_STAGE_NORMAL && p.stride != q.stride))
+            return false;
+    }
+    return true;
+}
+
+} // namespace
+
+struct Server {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t max_batch_points = 0;
+    uint32_t max_batch_tiles = 256;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> pending;
+    std::map<uint64_t, std::unique_ptr<Request>> requests;
+    uint64_t next_id = 1;
+    uint64_t in_use = 0;
+    uint32_t next_batch = 1;
+    bool stopping = false;
+    std::vector<std::thread> workers;
+    int ready = 0;  // workers that created their context (or failed to)
+    std::string create_error;
+
+    void worker_main(int wi);
+};
+
+void Server::worker_main(int)
+{
+    std::unique_ptr<Context> ctx;
+    try {
+        ctx.reset(new Context(device));
+        ctx->timing = false;
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu);
+        create_error = e.what();
+        ++ready;
+        cv_done.notify_all();
+        return;
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        ++ready;
+    }
+    cv_done.notify_all();
+    Batch batch(ctx.get());
+    DevBuf dcols, dstat;
+    for (;;) {
+        std::vector<Request*> take;
+        uint64_t bytes = 0;
+        uint32_t bid = 0;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            for (;;) {
+                if (stopping && pending.empty()) return;
+                if (!pending.empty()) {
+                    // Admission: requests in arrival order while they fit the
+                    // budget (a request alone larger than the whole budget is
+                    // refused), share the first one's stage shape, and the
+                    // batch stays within its point and tile limits.
+                    Request* first = pending.front();
+                    if (first->bytes > budget) {
+                        pending.pop_front();
+                        first->rc = GF_ERR_OOM;
+                        first->error = "request exceeds the device memory budget";
+                        first->t_done = now_s();
+                        first->done = true;
+                        cv_done.notify_all();
+                        continue;
+                    }
+                    uint64_t pts = 0;
+                    for (auto it = pending.begin(); it != pending.end();) {
+                        Request* r = *it;
+                        if (!same_shape(*r, *first)) {
+                            ++it;
+                            continue;
+                        }
+                        if (in_use + bytes + r->bytes > budget) break;
+                        if (!take.empty() && (pts + r->n > max_batch_points ||
+                                              take.size() >= max_batch_tiles))
+                            break;
+                        take.push_back(r);
+                        bytes += r->bytes;
+                        pts += r->n;
+                        it = pending.erase(it);
+                    }
+                    if (!take.empty()) {
+                        in_use += bytes;
+                        bid = next_batch++;
+                        break;
+                    }
+                }
+                cv_work.wait(lk);
+            }
+        }
+        // Run the batch on this worker's stream.
+        const double t0 = now_s();
+        int rc = GF_OK;
+        std::string err;
+        try {
+            GF_HIP(hipSetDevice(device));
+            const uint32_t T = static_cast<uint32_t>(take.size());
+            std::vector<uint64_t> counts(T);
+            std::vector<int32_t> knn(T);
+            for (uint32_t t = 0; t < T; ++t) {
+                counts[t] = take[t]->n;
+                knn[t] = take[t]->knn;
+            }
+            batch.reset(T, counts.data(), ctx->stream);
+            const uint64_t N = batch.n;
+            const int ncols = take[0]->ncols;
+            const int ns = take[0]->ns;
+            dcols.ensure(static_cast<uint64_t>(ncols) * N * 8 + 8);
+            dstat.ensure(static_cast<uint64_t>(ns) * N + 8);
+            std::vector<double*> cp(ncols);
+            std::vector<uint8_t*> sp(ns);
+            for (int c = 0; c < ncols; ++c) cp[c] = dcols.as<double>() + static_cast<uint64_t>(c) * N;
+            for (int i = 0; i < ns; ++i) sp[i] = dstat.as<uint8_t>() + static_cast<uint64_t>(i) * N;
+            for (uint32_t t = 0; t < T; ++t) {
+                Request* r = take[t];
+                const uint64_t o = batch.starts[t];
+                if (!r->n) continue;
+                GF_HIP(hipMemcpyAsync(batch.x.as<double>() + o, r->x, r->n * 8,
+                                      hipMemcpyHostToDevice, ctx->stream));
+                GF_HIP(hipMemcpyAsync(batch.y.as<double>() + o, r->y, r->n * 8,
+                                      hipMemcpyHostToDevice, ctx->stream));
+                GF_HIP(hipMemcpyAsync(batch.z.as<double>() + o, r->z, r->n * 8,
+                                      hipMemcpyHostToDevice, ctx->stream));
+                // Rows a stage skips keep the caller's prior values.
+                for (int c = 0; c < ncols; ++c)
+                    GF_HIP(hipMemcpyAsync(cp[c] + o, r->cols[c], r->n * 8, hipMemcpyHostToDevice,
+                                          ctx->stream));
+            }
+            if (N) {
+                ctx->stats = gf_stats{};
+                ctx->run_batch(batch, take[0]->st, ns, knn.data(), cp.data(), sp.data(),
+                               GF_WHERE_DEVICE);
+            }
+            for (uint32_t t = 0; t < T; ++t) {
+                Request* r = take[t];
+                const uint64_t o = batch.starts[t];
+                if (!r->n) continue;
+                for (int c = 0; c < ncols; ++c)
+                    GF_HIP(hipMemcpyAsync(r->cols[c], cp[c] + o, r->n * 8, hipMemcpyDeviceToHost,
+                                          ctx->stream));
+                for (int i = 0; i < ns; ++i)
+                    GF_HIP(hipMemcpyAsync(r->status[i], sp[i] + o, r->n, hipMemcpyDeviceToHost,
+                                          ctx->stream));
+            }
+            GF_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (const Error& e) {
+            rc = e.code;
+            err = e.what();
+        } catch (const std::exception& e) {
+            rc = GF_ERR_STATE;
+            err = e.what();
+        }
+        const double t1 = now_s();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            in_use -= bytes;
+            for (Request* r : take) {
+                r->t_start = t0;
+                r->t_done = t1;
+                r->batch_id = bid;
+                r->batch_tiles = static_cast<uint32_t>(take.size());
+                r->rc = rc;
+                r->error = err;
+                r->done = true;
+            }
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+}
+
+} // namespace gf
+
+using namespace gf;
+
+extern "C" {
+
+int gf_server_create(int device, int workers, uint64_t hbm_budget_bytes,
+                     uint64_t max_batch_points, gf_server** out)
+{
+    if (!out || workers < 1 || workers > 8) return GF_ERR_ARG;
+    *out = nullptr;
+    Server* s = new Server();
+    s->device = device;
+    s->budget = hbm_budget_bytes ? hbm_budget_bytes : (200ull << 30);
+    s->max_batch_points = max_batch_points ? max_batch_points : (64ull << 20);
+    for (int i = 0; i < workers; ++i) s->workers.emplace_back(&Server::worker_main, s, i);
+    bool failed;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [&] { return s->ready == workers; });
+        failed = !s->create_error.empty();
+        if (failed) {
+            std::fprintf(stderr, "gf_server_create: %s\n", s->create_error.c_str());
+            s->stopping = true;
+        }
+    }
+    if (failed) {
+        s->cv_work.notify_all();
+        for (auto& t : s->workers) t.join();
+        delete s;
+        return GF_ERR_HIP;
+    }
+    *out = reinterpret_cast<gf_server*>(s);
+    return GF_OK;
+}
+
+int gf_server_submit(gf_server* server, const double* x, const double* y, const double* z,
+                     uint64_t n, const gf_stage* stages, int nstages, int32_t knn,
+                     double* const* cols, uint8_t* const* status, uint64_t* ticket)
+{
+    if (!server || !stages || nstages < 1 || nstages > kMaxStages || !status || !ticket ||
+        (n && (!x || !y || !z)) || knn < 1)
+        return GF_ERR_ARG;
+    Server* s = reinterpret_cast<Server*>(server);
+    std::unique_ptr<Request> r(new Request());
+    int ncols = 0;
+    for (int i = 0; i < nstages; ++i) {
+        const int c = gf_stage_columns(&stages[i]);
+        if (c < 0) return GF_ERR_ARG;
+        r->st[i] = stages[i];
+        r->st[i].knn = knn;
+        ncols += c;
+    }
+    if (ncols && !cols) return GF_ERR_ARG;
+    r->x = x;
+    r->y = y;
+    r->z = z;
+    r->n = n;
+    r->ns = nstages;
+    r->knn = knn;
+    r->ncols = ncols;
+    r->cols.assign(cols, cols + ncols);
+    r->status.assign(status, status + nstages);
+    r->bytes = request_bytes(n, ncols);
+    r->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stopping) return GF_ERR_STATE;
+        r->id = s->next_id++;
+        *ticket = r->id;
+        s->pending.push_back(r.get());
+        s->requests[r->id] = std::move(r);
+    }
+    s->cv_work.notify_one();
+    return GF_OK;
+}
+
+int gf_server_wait(gf_server* server, uint64_t ticket, double timeout_s)
+{
+    if (!server) return GF_ERR_ARG;
+    Server* s = reinterpret_cast<Server*>(server);
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->requests.find(ticket);
+    if (it == s->requests.end()) return GF_ERR_ARG;
+    Request* r = it->second.get();
+    const auto pred = [&] { return r->done; };
+    if (timeout_s < 0)
+        s->cv_done.wait(lk, pred);
+    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
+        return GF_ERR_STATE;  // still running
+    return r->rc;
+}
+
+int gf_server_request_info(gf_server* server, uint64_t ticket, double* times,
+                           uint32_t* batch_id, uint32_t* batch_tiles)
+{
+    if (!server) return GF_ERR_ARG;
+    Server* s = reinterpret_cast<Server*>(server);
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->requests.find(ticket);
+    if (it == s->requests.end()) return GF_ERR_ARG;
+    const Request* r = it->second.get();
+    if (times) {
+        times[0] = r->t_submit;
+        times[1] = r->t_start;
+        times[2] = r->t_done;
+    }
+    if (batch_id) *batch_id = r->batch_id;
+    if (batch_tiles) *batch_tiles = r->batch_tiles;
+    return r->done ? r->rc : GF_ERR_STATE;
+}
+
+int gf_server_release(gf_server* server, uint64_t ticket)
+{
+    if (!server) return GF_ERR_ARG;
+    Server* s = reinterpret_cast<Server*>(server);
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->requests.find(ticket);
+    if (it == s->requests.end() || !it->second->done) return GF_ERR_ARG;
+    s->requests.erase(it);
+    return GF_OK;
+}
+
+int gf_server_destroy(gf_server* server)
+{
+    if (!server) return GF_ERR_ARG;
+    Server* s = reinterpret_cast<Server*>(server);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->workers) t.join();
+    delete s;
+    return GF_OK;
+}
+
+} // extern "C"

@@ -146,6 +146,52 @@ int gf_last_stats(const gf_ctx *ctx, gf_stats *out);
 int gf_knn(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, uint32_t stride,
            uint32_t *ids, uint8_t *status, int where);
 
+/* ---- batches of independent tiles: replaces running the stages tile by
+ *      tile (the reference processes one PointView / one SpatialTile at a time,
+ *      include/pdg/index/SpatialTile.hpp:29-40; src/pdal/PdgNeighborhood.cpp
+ *      :1383-1550).  The tiles of a batch share one index build, one search
+ *      launch, one exact re-solve and one feature launch; every tile's result
+ *      equals the tile run alone.  Points of tile t sit at
+ *      [gf_batch_tile_start(t), gf_batch_tile_start(t+1)) of the batch, and
+ *      gf_batch_run's cols/status use that order. */
+typedef struct gf_batch gf_batch;
+int gf_batch_create(gf_ctx *ctx, uint32_t ntiles, const uint64_t *counts, gf_batch **out);
+int gf_batch_set_tile(gf_batch *batch, uint32_t tile, const double *x, const double *y,
+                      const double *z, int where);
+uint64_t gf_batch_size(const gf_batch *batch);
+uint64_t gf_batch_tile_start(const gf_batch *batch, uint32_t tile);
+int gf_batch_invalidate(gf_batch *batch);
+/* All stages share one neighborhood; tile_knn (NULL: the stages' knn) gives
+ * each tile its own knn, overriding every stage's. */
+int gf_batch_run(gf_ctx *ctx, gf_batch *batch, const gf_stage *stages, int nstages,
+                 const int32_t *tile_knn, double *const *cols, uint8_t *const *status,
+                 int where);
+int gf_batch_release(gf_batch *batch);
+
+/* ---- serving: no counterpart in the reference, which runs one pipeline
+ *      per process (src/cli/main.cpp:1981-2207).  Requests (one tile each,
+ *      host arrays that stay valid until the request is done) are admitted
+ *      under an HBM budget by worker threads, each with its own context and
+ *      stream, and run as batches (continuous batching: a request that arrives
+ *      while batches are on the device joins the next batch a free worker
+ *      forms).  A request's result equals the tile run alone.  Requests of a
+ *      batch share the stage shape (kinds, features, modes, stride); knn may
+ *      differ.  hbm_budget_bytes 0 = 200 GB; max_batch_points 0 = 64 Mi. */
+typedef struct gf_server gf_server;
+int gf_server_create(int device, int workers, uint64_t hbm_budget_bytes,
+                     uint64_t max_batch_points, gf_server **out);
+int gf_server_submit(gf_server *server, const double *x, const double *y, const double *z,
+                     uint64_t n, const gf_stage *stages, int nstages, int32_t knn,
+                     double *const *cols, uint8_t *const *status, uint64_t *ticket);
+/* Blocks until the request is done (timeout_s < 0: no limit); returns its
+ * result code, or GF_ERR_STATE if the timeout expired first. */
+int gf_server_wait(gf_server *server, uint64_t ticket, double timeout_s);
+/* times: submit, batch start, done (seconds, monotonic clock). */
+int gf_server_request_info(gf_server *server, uint64_t ticket, double *times,
+                           uint32_t *batch_id, uint32_t *batch_tiles);
+int gf_server_release(gf_server *server, uint64_t ticket);
+int gf_server_destroy(gf_server *server);
+
 /* ---- device buffers for callers without their own HIP allocator (the
  *      reference's MemoryResource::allocate/deallocate,
  *      include/pdg/Memory.hpp:12-81).  Copies are synchronous. */

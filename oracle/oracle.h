@@ -165,11 +165,13 @@ void orc_stage_from_lists(const double *x, const double *y, const double *z,
 /* Extended-precision truth for the same stage from neighbor lists: long
  * double two-pass covariance without float narrowing, cyclic Jacobi to
  * convergence.  gap (optional, n entries) receives a conditioning figure per
- * row: min over adjacent raw eigenvalue gaps divided by the largest. */
+ * row: min over adjacent raw eigenvalue gaps divided by the largest; ratio
+ * (optional) the smallest eigenvalue (clamped at 0) over the largest. */
 void orc_truth_from_lists(const double *x, const double *y, const double *z,
                           uint64_t n, const uint32_t *nbr, uint64_t kk,
                           const orc_stage *st, int threads,
-                          double *const *cols, uint8_t *status, double *gap);
+                          double *const *cols, uint8_t *status, double *gap,
+                          double *ratio);
 
 /* Extended-precision symmetric 3x3 eigensolve (cyclic Jacobi): ascending
  * values, column-major vectors.  Returns the sweep count. */

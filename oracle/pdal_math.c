@@ -389,11 +389,12 @@ typedef struct {
     double *const *cols;
     uint8_t *status;
     double *gap;
+    double *ratio;
 } stage_job;
 
 static uint8_t truth_row(const orc_stage *st, const double *x, const double *y,
                          const double *z, const uint32_t *ids, uint64_t cnt,
-                         double *vals, double *gap);
+                         double *vals, double *gap, double *ratio);
 
 static void *stage_worker(void *arg)
 {
@@ -414,7 +415,7 @@ static void *stage_worker(void *arg)
             const uint32_t *row = j->lists + i * j->kk;
             if (j->truth)
                 s = truth_row(st, j->x, j->y, j->z, row, j->kk, vals,
-                              j->gap ? j->gap + i : NULL);
+                              j->gap ? j->gap + i : NULL, j->ratio ? j->ratio + i : NULL);
             else
                 s = stage_row(st, j->x, j->y, j->z, row, j->kk, cov_u32, vals);
         } else {
@@ -618,7 +619,7 @@ int orc_jacobi_sym3(const double cov[9], double evals[3], double evecs[9])
 
 static uint8_t truth_row(const orc_stage *st, const double *x, const double *y,
                          const double *z, const uint32_t *ids, uint64_t cnt,
-                         double *vals, double *gap)
+                         double *vals, double *gap, double *ratio)
 {
     if (st->stage == ORC_STAGE_NORMAL && cnt < 3) return ORC_ST_FEW_NEIGHBORS;
     ld m[3] = {0, 0, 0};
@@ -648,6 +649,10 @@ static uint8_t truth_row(const orc_stage *st, const double *x, const double *y,
         ld big = fabsl(ev[2]) > 0 ? fabsl(ev[2]) : 1.0L;
         ld g1 = ev[1] - ev[0], g2 = ev[2] - ev[1];
         *gap = (double)((g1 < g2 ? g1 : g2) / big);
+    }
+    if (ratio) {
+        ld big = fabsl(ev[2]) > 0 ? fabsl(ev[2]) : 1.0L;
+        *ratio = (double)((ev[0] > 0 ? ev[0] : 0) / big);
     }
     if (st->stage == ORC_STAGE_NORMAL) {
         ld sum = ev[0] + ev[1] + ev[2];
@@ -709,7 +714,8 @@ static uint8_t truth_row(const orc_stage *st, const double *x, const double *y,
 void orc_truth_from_lists(const double *x, const double *y, const double *z,
                           uint64_t n, const uint32_t *nbr, uint64_t kk,
                           const orc_stage *st, int threads,
-                          double *const *cols, uint8_t *status, double *gap)
+                          double *const *cols, uint8_t *status, double *gap,
+                          double *ratio)
 {
     stage_job p;
     memset(&p, 0, sizeof(p));
@@ -724,5 +730,6 @@ void orc_truth_from_lists(const double *x, const double *y, const double *z,
     p.cols = cols;
     p.status = status;
     p.gap = gap;
+    p.ratio = ratio;
     run_jobs(p, n, threads);
 }

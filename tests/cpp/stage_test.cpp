@@ -1,0 +1,442 @@
+// stage_test.cpp -- the PDAL-compatible GPU stages (gpupdal_amd/pdal),
+// tested the way the reference tests the CPU stages.
+//
+// Suites named Cpu* need no GPU (option parsing, dimension registration,
+// preconditions); suites named Gpu* execute on GPU 0.  Run as
+//   tests/cpp/stage_test Cpu      or      tests/cpp/stage_test Gpu
+//
+// The known answers are those of test/unit/filters/CovarianceFeaturesTest.cpp
+// :46-188 and test/unit/filters/NormalFilterTest.cpp:47-253.
+#include "gfpdal.hpp"
+#include "mini_test.hpp"
+
+#include <array>
+#include <cmath>
+#include <sstream>
+
+using namespace gfpdal;
+using Id = Dimension::Id;
+
+namespace {
+
+PointViewPtr make_view(PointTable& table, const std::vector<std::array<double, 3>>& pts)
+{
+    PointViewPtr view(new PointView(table));
+    for (size_t i = 0; i < pts.size(); ++i) {
+        view->setField(Id::X, i, pts[i][0]);
+        view->setField(Id::Y, i, pts[i][1]);
+        view->setField(Id::Z, i, pts[i][2]);
+    }
+    return view;
+}
+
+// FauxReader mode=grid over integer bounds: one point per integer coordinate
+// with lo <= v < hi, x fastest (io/FauxReader.cpp), the first count points.
+std::vector<std::array<double, 3>> faux_grid(int x0, int x1, int y0, int y1, int z0, int z1,
+                                             size_t count)
+{
+    std::vector<std::array<double, 3>> p;
+    for (int z = z0; z < std::max(z1, z0 + 1); ++z)
+        for (int y = y0; y < std::max(y1, y0 + 1); ++y)
+            for (int x = x0; x < std::max(x1, x0 + 1); ++x)
+                if (p.size() < count) p.push_back({double(x), double(y), double(z)});
+    return p;
+}
+
+const char* kAllButDensity =
+    "Dimensionality,Omnivariance,Anisotropy,Eigenentropy,EigenvalueSum,SurfaceVariation,"
+    "DemantkeVerticality";
+
+} // namespace
+
+// ------------------------------------------------------------------ CPU --
+TEST(CpuOptions, UnknownOption)
+{
+    PointTable table;
+    CovarianceFeaturesFilter f;
+    Options o;
+    o.add("knnn", 3);
+    f.setOptions(o);
+    EXPECT_THROW_MSG(f.prepare(table), pdal_error,
+                     "filters.covariancefeatures: Unexpected argument 'knnn'.");
+}
+
+TEST(CpuOptions, InvalidMode)
+{
+    PointTable table;
+    CovarianceFeaturesFilter f;
+    Options o;
+    o.add("mode", "cube");
+    f.setOptions(o);
+    EXPECT_THROW_MSG(f.prepare(table), pdal_error,
+                     "filters.covariancefeatures: Invalid value 'cube' for argument 'mode'.");
+}
+
+TEST(CpuOptions, ModeIsCaseInsensitive)
+{
+    for (const char* m : {"RAW", "Sqrt", "normalized"}) {
+        PointTable table;
+        CovarianceFeaturesFilter f;
+        Options o;
+        o.add("mode", m);
+        f.setOptions(o);
+        f.prepare(table);
+    }
+}
+
+TEST(CpuOptions, InvalidKnn)
+{
+    PointTable table;
+    EigenvaluesFilter f;
+    Options o;
+    o.add("knn", "eight");
+    f.setOptions(o);
+    EXPECT_THROW_MSG(f.prepare(table), pdal_error,
+                     "filters.eigenvalues: Invalid value 'eight' for argument 'knn'.");
+}
+
+TEST(CpuOptions, StrideIsUnsigned)
+{
+    PointTable table;
+    CovarianceFeaturesFilter f;
+    Options o;
+    o.add("stride", -2);
+    f.setOptions(o);
+    EXPECT_THROW_MSG(f.prepare(table), pdal_error,
+                     "filters.covariancefeatures: Invalid value '-2' for argument 'stride'.");
+}
+
+TEST(CpuDims, DefaultFeatureSetIsDimensionality)
+{
+    PointTable table;
+    CovarianceFeaturesFilter f;
+    f.prepare(table);
+    const auto& d = table.layout()->dims();
+    ASSERT_EQ(d.size(), size_t(7));
+    ASSERT_TRUE(d[3] == Id::Linearity && d[4] == Id::Planarity && d[5] == Id::Scattering &&
+                d[6] == Id::Verticality);
+}
+
+TEST(CpuDims, FeatureSetOrderAndCase)
+{
+    PointTable table;
+    CovarianceFeaturesFilter f;
+    Options o;
+    o.add("feature_set", " SurfaceVariation, linearity,bogus");
+    f.setOptions(o);
+    f.prepare(table);
+    const auto& d = table.layout()->dims();
+    ASSERT_EQ(d.size(), size_t(5));  // unknown names are ignored, as upstream
+    ASSERT_TRUE(d[3] == Id::SurfaceVariation && d[4] == Id::Linearity);
+}
+
+TEST(CpuDims, NormalAndEigenvalueDims)
+{
+    PointTable table;
+    NormalFilter n;
+    n.prepare(table);
+    EigenvaluesFilter e;
+    e.prepare(table);
+    const auto& d = table.layout()->dims();
+    ASSERT_EQ(d.size(), size_t(10));
+    ASSERT_TRUE(d[3] == Id::NormalX && d[6] == Id::Curvature && d[7] == Id::Eigenvalue0 &&
+                d[9] == Id::Eigenvalue2);
+}
+
+TEST(CpuPrepared, DensityNeedsOptimalDims)
+{
+    PointTable table;
+    CovarianceFeaturesFilter f;
+    Options o;
+    o.add("feature_set", "all");
+    f.setOptions(o);
+    EXPECT_THROW_MSG(f.prepare(table), pdal_error,
+                     "filters.covariancefeatures: Density feature requires OptimalKNN and "
+                     "OptimalRadius dimensions, which are missing in the input PointView.");
+}
+
+TEST(CpuPrepared, OptimizedNeedsOptimalKnn)
+{
+    PointTable table;
+    CovarianceFeaturesFilter f;
+    Options o;
+    o.add("optimized", true);
+    f.setOptions(o);
+    EXPECT_THROW_MSG(f.prepare(table), pdal_error,
+                     "filters.covariancefeatures: Missing OptimalKNN dimension in input "
+                     "PointView.");
+}
+
+TEST(CpuPrepared, NormalKnnAndRadius)
+{
+    PointTable table;
+    NormalFilter f;
+    Options o;
+    o.add("knn", 8);
+    o.add("radius", 1.5);
+    f.setOptions(o);
+    EXPECT_THROW_MSG(f.prepare(table), pdal_error,
+                     "filters.normal: Cannot set both knn and radius.");
+}
+
+TEST(CpuPrepared, RadiusIsNotOnTheGpu)
+{
+    PointTable table;
+    EigenvaluesFilter f;
+    Options o;
+    o.add("radius", 2.0);
+    f.setOptions(o);
+    EXPECT_THROW_MSG(f.prepare(table), pdal_error,
+                     "filters.eigenvalues: radius searches are not supported by the GPU "
+                     "stage; use knn.");
+}
+
+TEST(CpuFactory, StageNames)
+{
+    ASSERT_EQ(createStage("filters.covariancefeatures")->getName(),
+              std::string("filters.covariancefeatures"));
+    ASSERT_EQ(createStage("filters.normal")->getName(), std::string("filters.normal"));
+    ASSERT_EQ(createStage("filters.eigenvalues")->getName(), std::string("filters.eigenvalues"));
+    ASSERT_TRUE(createStage("filters.outlier") == nullptr);
+}
+
+// ------------------------------------------------------------------ GPU --
+TEST(GpuDimensionality, Linearity)
+{
+    PointTable table;
+    BufferReader reader;
+    CovarianceFeaturesFilter filter;
+    Options ops;
+    ops.add("knn", 3);
+    ops.add("feature_set", kAllButDensity);
+    filter.setInput(reader);
+    filter.setOptions(ops);
+    filter.prepare(table);
+    PointViewPtr view = make_view(table, {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}});
+    reader.addView(view);
+    PointViewSet set = filter.execute(table);
+    PointViewPtr out = *set.begin();
+    for (point_count_t i = 0; i < out->size(); ++i) {
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Linearity, i), 1);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Verticality, i), 1);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Planarity, i), 0);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Scattering, i), 0);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Omnivariance, i), 0);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Anisotropy, i), 1);
+        ASSERT_TRUE(std::isnan(out->getFieldAs<float>(Id::Eigenentropy, i)));
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::EigenvalueSum, i), 1);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::SurfaceVariation, i), 0);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::DemantkeVerticality, i), 1);
+    }
+}
+
+TEST(GpuDimensionality, Planarity)
+{
+    PointTable table;
+    BufferReader reader;
+    CovarianceFeaturesFilter filter;
+    Options ops;
+    ops.add("feature_set", kAllButDensity);
+    filter.setInput(reader);
+    filter.setOptions(ops);
+    filter.prepare(table);
+    reader.addView(make_view(table, {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}}));
+    PointViewPtr out = *filter.execute(table).begin();
+    for (point_count_t i = 0; i < out->size(); ++i) {
+        ASSERT_LE(out->getFieldAs<float>(Id::Linearity, i), 0.5);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Verticality, i), 0);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Planarity, i), 1);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Scattering, i), 0);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Omnivariance, i), 0);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Anisotropy, i), 1);
+        ASSERT_TRUE(std::isnan(out->getFieldAs<float>(Id::Eigenentropy, i)));
+        ASSERT_NEAR(out->getFieldAs<float>(Id::EigenvalueSum, i), 0.667, 0.001);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::SurfaceVariation, i), 0);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::DemantkeVerticality, i), 0);
+    }
+}
+
+TEST(GpuDimensionality, Scattering)
+{
+    PointTable table;
+    BufferReader reader;
+    CovarianceFeaturesFilter filter;
+    Options ops;
+    ops.add("feature_set", kAllButDensity);
+    filter.setInput(reader);
+    filter.setOptions(ops);
+    filter.prepare(table);
+    reader.addView(make_view(table, {{0, 0, 0}, {1, 0, 0}, {1, 1, 1}, {0, 1, 2}}));
+    PointViewPtr out = *filter.execute(table).begin();
+    for (point_count_t i = 0; i < out->size(); ++i) {
+        ASSERT_LE(out->getFieldAs<float>(Id::Linearity, i), 0.5);
+        ASSERT_GE(out->getFieldAs<float>(Id::Verticality, i), 0.5);
+        ASSERT_LE(out->getFieldAs<float>(Id::Planarity, i), 1);
+        ASSERT_GE(out->getFieldAs<float>(Id::Scattering, i), 0.1);
+        ASSERT_NEAR(out->getFieldAs<float>(Id::Omnivariance, i), 0.458, 0.001);
+        ASSERT_NEAR(out->getFieldAs<float>(Id::Anisotropy, i), 0.864, 0.001);
+        ASSERT_NEAR(out->getFieldAs<float>(Id::Eigenentropy, i), 0.488, 0.001);
+        ASSERT_NEAR(out->getFieldAs<float>(Id::EigenvalueSum, i), 1.583, 0.001);
+        ASSERT_NEAR(out->getFieldAs<float>(Id::SurfaceVariation, i), 0.095, 0.001);
+        ASSERT_NEAR(out->getFieldAs<float>(Id::DemantkeVerticality, i), 0.492, 0.001);
+    }
+}
+
+namespace {
+
+void check_plane(const std::vector<std::array<double, 3>>& pts, double nx, double ny, double nz)
+{
+    PointTable table;
+    BufferReader reader;
+    NormalFilter filter;
+    Options ops;
+    ops.add("knn", 3);
+    filter.setInput(reader);
+    filter.setOptions(ops);
+    filter.prepare(table);
+    reader.addView(make_view(table, pts));
+    PointViewPtr out = *filter.execute(table).begin();
+    for (point_count_t i = 0; i < out->size(); ++i) {
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::NormalX, i), nx);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::NormalY, i), ny);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::NormalZ, i), nz);
+        ASSERT_FLOAT_EQ(out->getFieldAs<float>(Id::Curvature, i), 0.0);
+    }
+}
+
+} // namespace
+
+TEST(GpuNormal, XYPlane) { check_plane(faux_grid(0, 2, 0, 2, 0, 0, 4), 0, 0, 1); }
+TEST(GpuNormal, XZPlane) { check_plane(faux_grid(0, 2, 0, 0, 0, 2, 4), 0, 1, 0); }
+TEST(GpuNormal, YZPlane) { check_plane(faux_grid(0, 0, 0, 2, 0, 2, 4), 1, 0, 0); }
+
+TEST(GpuNormal, RampPlaneAlwaysUp)
+{
+    // Points on z = x: normal (-sqrt(2)/2, 0, sqrt(2)/2), NormalFilterTest.cpp:158-205.
+    std::vector<std::array<double, 3>> p;
+    for (int y = 0; y < 3; ++y)
+        for (int x = 0; x < 3; ++x) p.push_back({double(x), double(y), double(x)});
+    PointTable table;
+    BufferReader reader;
+    NormalFilter filter;
+    filter.setInput(reader);
+    filter.prepare(table);
+    reader.addView(make_view(table, p));
+    PointViewPtr out = *filter.execute(table).begin();
+    const double h = std::sqrt(2.0) / 2;
+    for (point_count_t i = 0; i < out->size(); ++i) {
+        ASSERT_NEAR(out->getFieldAs<double>(Id::NormalX, i), -h, 1e-9);
+        ASSERT_NEAR(out->getFieldAs<double>(Id::NormalY, i), 0.0, 1e-9);
+        ASSERT_NEAR(out->getFieldAs<double>(Id::NormalZ, i), h, 1e-9);
+    }
+}
+
+TEST(GpuNormal, ViewpointOrientation)
+{
+    // A horizontal plane seen from below: normals point down.
+    PointTable table;
+    BufferReader reader;
+    NormalFilter filter;
+    Options ops;
+    ops.add("viewpoint", "POINT (1 1 -10)");
+    filter.setInput(reader);
+    filter.setOptions(ops);
+    filter.prepare(table);
+    reader.addView(make_view(table, faux_grid(0, 3, 0, 3, 0, 0, 9)));
+    PointViewPtr out = *filter.execute(table).begin();
+    for (point_count_t i = 0; i < out->size(); ++i)
+        ASSERT_NEAR(out->getFieldAs<double>(Id::NormalZ, i), -1.0, 1e-12);
+}
+
+TEST(GpuEigenvalues, PlaneAndNormalize)
+{
+    PointTable table;
+    BufferReader reader;
+    EigenvaluesFilter filter;
+    Options ops;
+    ops.add("knn", 8);
+    ops.add("normalize", true);
+    filter.setInput(reader);
+    filter.setOptions(ops);
+    filter.prepare(table);
+    reader.addView(make_view(table, faux_grid(0, 3, 0, 3, 0, 0, 9)));
+    PointViewPtr out = *filter.execute(table).begin();
+    for (point_count_t i = 0; i < out->size(); ++i) {
+        ASSERT_NEAR(out->getFieldAs<double>(Id::Eigenvalue0, i), 0.0, 1e-15);
+        ASSERT_NEAR(out->getFieldAs<double>(Id::Eigenvalue1, i) +
+                        out->getFieldAs<double>(Id::Eigenvalue2, i),
+                    1.0, 1e-12);
+    }
+}
+
+TEST(GpuChain, NormalThenFeaturesOnOneView)
+{
+    // Two stages over one view: the second reuses the resident cloud.
+    std::vector<std::array<double, 3>> p;
+    for (int y = 0; y < 20; ++y)
+        for (int x = 0; x < 20; ++x) p.push_back({x * 0.5, y * 0.5, 0.1 * x + 0.01 * (y % 3)});
+    PointTable table;
+    BufferReader reader;
+    NormalFilter normal;
+    CovarianceFeaturesFilter feats;
+    Options a, b;
+    a.add("knn", 8);
+    b.add("knn", 8);
+    normal.setOptions(a);
+    feats.setOptions(b);
+    normal.setInput(reader);
+    feats.setInput(normal);
+    feats.prepare(table);
+    PointViewPtr view = make_view(table, p);
+    reader.addView(view);
+    PointViewPtr out = *feats.execute(table).begin();
+    for (point_count_t i = 0; i < out->size(); ++i) {
+        ASSERT_GE(out->getFieldAs<double>(Id::NormalZ, i), 0.0);
+        ASSERT_GE(out->getFieldAs<double>(Id::Planarity, i), 0.0);
+        ASSERT_LE(out->getFieldAs<double>(Id::Planarity, i), 1.0);
+    }
+}
+
+TEST(GpuSkip, ZeroCovarianceKeepsValuesAndLogs)
+{
+    PointTable table;
+    BufferReader reader;
+    EigenvaluesFilter filter;
+    Options ops;
+    ops.add("knn", 3);
+    filter.setInput(reader);
+    filter.setOptions(ops);
+    std::ostringstream log;
+    filter.setLog(&log, LogLevel::Info);
+    filter.prepare(table);
+    PointViewPtr view = make_view(table, {{5, 5, 5}, {5, 5, 5}, {5, 5, 5}, {5, 5, 5}});
+    for (point_count_t i = 0; i < 4; ++i) view->setField(Id::Eigenvalue0, i, -1.0);
+    reader.addView(view);
+    PointViewPtr out = *filter.execute(table).begin();
+    for (point_count_t i = 0; i < 4; ++i)
+        ASSERT_EQ(out->getFieldAs<double>(Id::Eigenvalue0, i), -1.0);
+    ASSERT_TRUE(log.str().find("Skipping point 3. Covariance matrix is all zeros.") !=
+                std::string::npos);
+}
+
+TEST(GpuSkip, StridePaddingSkipsRowZero)
+{
+    // Two points, knn=1, stride 3: KD3Index pads each strided list with id 0
+    // (KDImpl.hpp:351-381), so row 0 sees only itself (zero covariance, a
+    // skip) and row 1 sees points 1 and 0 (a line).
+    PointTable table;
+    BufferReader reader;
+    CovarianceFeaturesFilter filter;
+    Options ops;
+    ops.add("knn", 1);
+    ops.add("stride", 3);
+    filter.setInput(reader);
+    filter.setOptions(ops);
+    filter.prepare(table);
+    reader.addView(make_view(table, {{0, 0, 0}, {1, 0, 0}}));
+    PointViewPtr out = *filter.execute(table).begin();
+    ASSERT_EQ(out->getFieldAs<double>(Id::Linearity, 0), 0.0);  // skipped, prior value
+    ASSERT_FLOAT_EQ(out->getFieldAs<double>(Id::Linearity, 1), 1.0);
+}
+
+int main(int argc, char** argv) { return mini::run_all(argc, argv); }

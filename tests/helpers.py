@@ -34,3 +34,56 @@ def boundary_tie_rows(oracle, x, y, z, kk, rows=None):
     (brute force, so only for small clouds)."""
     ids, d2 = oracle.brute_knn(x, y, z, kk + 1, queries=rows)
     return d2[:, kk - 1] == d2[:, kk]
+
+
+# Parity bar for stage columns against the extended-precision truth.
+RTOL = 1e-5
+# A row whose smallest eigenvalue is below this fraction of the largest has
+# it at FP64 rounding level (exactly planar roofs, collinear wires): the
+# columns that depend on it are then checked absolutely.
+DEGENERATE_RATIO = 1e-10
+
+
+def column_names(st):
+    """Names of a stage's output columns, in order."""
+    if st.stage == ol.STAGE_NORMAL:
+        return ["NormalX", "NormalY", "NormalZ", "Curvature"]
+    if st.stage == ol.STAGE_EIGENVALUES:
+        return ["Eigenvalue0", "Eigenvalue1", "Eigenvalue2"]
+    return [ol.FEATURE_NAMES[b] for b in range(10) if st.features & (1 << b)]
+
+
+def truth_errors(st, cols, tcols, ok, ratio):
+    """Per column: the largest error against the truth, as a multiple of the
+    allowed error, over the rows in ok (1.0 = at the bar).
+
+    Well-resolved rows (ratio > DEGENERATE_RATIO): |gpu - truth| <=
+    RTOL * max(|truth|, 1e-6).  Degenerate rows: Omnivariance is skipped
+    (the 1/6th power of a rounding-level eigenvalue is ~1e-3 in any FP64
+    arithmetic); other columns must be within max(RTOL*|truth|, atol), with
+    atol = 1e-7 for the unit-scale features (sqrt of a 1e-16 relative
+    eigenvalue is 1.5e-8) and 1e-12 of the row's largest eigenvalue for the
+    Eigenvalue columns."""
+    out = {}
+    names = column_names(st)
+    deg = ok & (ratio <= DEGENERATE_RATIO)
+    good = ok & ~deg
+    lam_max = None
+    if st.stage == ol.STAGE_EIGENVALUES:
+        lam_max = np.abs(tcols[2])
+    for name, c, t in zip(names, cols, tcols):
+        if st.stage == ol.STAGE_NORMAL and name.startswith("Normal") and not st.always_up:
+            c, t = np.abs(c), np.abs(t)  # eigenvector sign is the solver's choice
+        e = np.zeros(len(c))
+        e[good] = np.abs(c[good] - t[good]) / (RTOL * np.maximum(np.abs(t[good]), 1e-6))
+        if name != "Omnivariance":
+            if lam_max is not None:
+                atol = 1e-12 * lam_max[deg]
+            elif name == "Curvature":
+                atol = np.full(np.count_nonzero(deg), 1e-12)
+            else:
+                atol = np.full(np.count_nonzero(deg), 1e-7)
+            allow = np.maximum(RTOL * np.abs(t[deg]), atol)
+            e[deg] = np.abs(c[deg] - t[deg]) / np.maximum(allow, 1e-300)
+        out[name] = float(np.nanmax(e)) if len(e) else 0.0
+    return out

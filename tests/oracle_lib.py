@@ -115,7 +115,7 @@ class Oracle:
         L.orc_truth_from_lists.argtypes = [_dp, _dp, _dp, ctypes.c_uint64,
                                            _u32p, ctypes.c_uint64, sp,
                                            ctypes.c_int, ctypes.POINTER(_dp),
-                                           _u8p, _dp]
+                                           _u8p, _dp, _dp]
         self.L = L
 
     # --------------------------------------------------------------- KD3
@@ -242,17 +242,25 @@ class Oracle:
                                     _ptr(status, _u8p))
         return cols, status
 
-    def truth_from_lists(self, x, y, z, nbr, st, threads=8, init=np.nan):
+    def truth_from_lists(self, x, y, z, nbr, st, threads=8, init=np.nan,
+                         with_ratio=False):
+        """Extended-precision truth; returns (cols, status, gap) or, with
+        with_ratio, (cols, status, gap, ratio) where ratio = lambda_min /
+        lambda_max of the row's covariance."""
         x, y, z = (_c(v, np.float64) for v in (x, y, z))
         nbr = _c(nbr, np.uint32)
         n = len(x)
         cols, arr = self._cols(st, n, init)
         status = np.zeros(n, np.uint8)
         gap = np.zeros(n, np.float64)
+        ratio = np.zeros(n, np.float64)
         self.L.orc_truth_from_lists(_ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp), n,
                                     _ptr(nbr, _u32p), nbr.shape[1],
                                     ctypes.byref(st), threads, arr,
-                                    _ptr(status, _u8p), _ptr(gap, _dp))
+                                    _ptr(status, _u8p), _ptr(gap, _dp),
+                                    _ptr(ratio, _dp))
+        if with_ratio:
+            return cols, status, gap, ratio
         return cols, status, gap
 
 
