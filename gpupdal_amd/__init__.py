@@ -66,6 +66,7 @@ EXPORTED = [
     "gf_batch_invalidate", "gf_batch_run", "gf_batch_release",
     "gf_server_create", "gf_server_submit", "gf_server_wait", "gf_server_request_info",
     "gf_server_release", "gf_server_destroy",
+    "gf_autotune", "gf_tuning_save", "gf_tuning_load",
 ]
 
 
@@ -123,6 +124,10 @@ def load_library(path=LIB_PATH):
     L.gf_server_request_info.argtypes = [vp, ctypes.c_uint64, _dp, _u32p, _u32p]
     L.gf_server_release.argtypes = [vp, ctypes.c_uint64]
     L.gf_server_destroy.argtypes = [vp]
+    L.gf_autotune.argtypes = [vp, vp, ctypes.POINTER(GfStage), ctypes.c_int, _dp, ctypes.c_int,
+                              _dp, _dp]
+    L.gf_tuning_save.argtypes = [vp, ctypes.c_char_p]
+    L.gf_tuning_load.argtypes = [vp, ctypes.c_char_p]
     return L
 
 
@@ -221,6 +226,24 @@ class Context:
 
     def cloud(self, x, y, z):
         return Cloud(self, x, y, z)
+
+    def autotune(self, cloud, stages, factors=(0.4, 0.5, 0.6, 0.75, 0.9)):
+        """Times the run for each column-size factor; the fastest is kept for
+        the cloud's (density, k) bucket.  Returns (factor, ms)."""
+        stages = list(stages)
+        arr = (GfStage * len(stages))(*stages)
+        f = np.ascontiguousarray(factors, dtype=np.float64)
+        bf = ctypes.c_double()
+        bm = ctypes.c_double()
+        self._check(self.L.gf_autotune(self.h, cloud.h, arr, len(stages), f.ctypes.data_as(_dp),
+                                       len(f), ctypes.byref(bf), ctypes.byref(bm)), "gf_autotune")
+        return bf.value, bm.value
+
+    def save_tuning(self, path):
+        self._check(self.L.gf_tuning_save(self.h, str(path).encode()), "gf_tuning_save")
+
+    def load_tuning(self, path):
+        self._check(self.L.gf_tuning_load(self.h, str(path).encode()), "gf_tuning_load")
 
     def batch(self, tiles):
         """A batch of tiles [(x, y, z), ...] resident on the device."""

@@ -7,6 +7,7 @@
 #include "gf_runtime.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -28,7 +29,7 @@ GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch,
     const double tiny = std::max(big * 1e-9, 1e-9);
     const double ax = std::max(sx, tiny), ay = std::max(sy, tiny);
     const double rho = static_cast<double>(std::max<uint64_t>(n, 1)) / (ax * ay);
-    double c = t.cell_factor *
+    double c = t.factor_for(rho, fetch) *
                std::sqrt(static_cast<double>(std::max<uint32_t>(fetch, 1)) /
                          (3.141592653589793 * rho));
     if (!(c > tiny)) c = tiny;
@@ -726,6 +727,118 @@ int gf_debug_kdtree(gf_ctx* ctx, gf_cloud* cloud, uint32_t* vind, uint32_t* node
         get(divlow, t.divlow, static_cast<uint64_t>(m) * 8);
         get(divhigh, t.divhigh, static_cast<uint64_t>(m) * 8);
         GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int gf_autotune(gf_ctx* ctx, gf_cloud* cloud, const gf_stage* stages, int nstages,
+                const double* factors, int nfactors, double* best_factor, double* best_ms)
+{
+    if (!ctx || !cloud || !stages || nstages < 1 || nstages > kMaxStages || !factors ||
+        nfactors < 1)
+        return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        for (int i = 0; i < nstages; ++i) validate_stage(stages[i]);
+        if (!cl->n) throw Error(GF_ERR_ARG, "empty cloud");
+        const uint64_t n = cl->n;
+        int ncols = 0;
+        for (int i = 0; i < nstages; ++i) ncols += stage_ncols(stages[i]);
+        DevBuf dcols, dstat;
+        dcols.ensure(static_cast<uint64_t>(std::max(ncols, 1)) * n * 8);
+        dstat.ensure(static_cast<uint64_t>(nstages) * n);
+        std::vector<double*> cp(std::max(ncols, 1));
+        std::vector<uint8_t*> sp(nstages);
+        for (int k = 0; k < ncols; ++k) cp[k] = dcols.as<double>() + static_cast<uint64_t>(k) * n;
+        for (int k = 0; k < nstages; ++k) sp[k] = dstat.as<uint8_t>() + static_cast<uint64_t>(k) * n;
+        const bool timing = c->timing;
+        c->timing = false;
+        double best = 1e300, bestf = factors[0];
+        try {
+            for (int fi = 0; fi < nfactors; ++fi) {
+                if (!(factors[fi] > 0)) throw Error(GF_ERR_ARG, "factors must be positive");
+                c->tune.forced_factor = factors[fi];
+                double t_min = 1e300;
+                for (int rep = 0; rep < 2; ++rep) {
+                    cl->idx_valid = false;
+                    cl->kd_valid = false;
+                    GF_HIP(hipStreamSynchronize(c->stream));
+                    const auto t0 = std::chrono::steady_clock::now();
+                    if (gf_run(ctx, cloud, stages, nstages, cp.data(), sp.data(),
+                               GF_WHERE_DEVICE) != GF_OK)
+                        throw Error(GF_ERR_STATE, c->last_error);
+                    const double ms = std::chrono::duration<double, std::milli>(
+                                          std::chrono::steady_clock::now() - t0)
+                                          .count();
+                    t_min = std::min(t_min, ms);
+                }
+                if (t_min < best) {
+                    best = t_min;
+                    bestf = factors[fi];
+                }
+            }
+        } catch (...) {
+            c->tune.forced_factor = 0.0;
+            c->timing = timing;
+            throw;
+        }
+        c->tune.forced_factor = 0.0;
+        c->timing = timing;
+        // Remember the winner for this cloud's bucket.
+        const double* b = cl->bounds;
+        const double sx = b[3] - b[0], sy = b[4] - b[1];
+        const double big = std::max(std::max(sx, sy), std::max(b[5] - b[2], 0.0));
+        const double tiny = std::max(big * 1e-9, 1e-9);
+        const double rho = static_cast<double>(n) / (std::max(sx, tiny) * std::max(sy, tiny));
+        const int stride = stages[0].stage == GF_STAGE_NORMAL ? 1 : stages[0].stride;
+        const uint32_t fetch = static_cast<uint32_t>(stride) *
+                               static_cast<uint32_t>(std::min<uint64_t>(n, stages[0].knn + 1));
+        c->tune.remember(rho, fetch, bestf);
+        cl->idx_valid = false;
+        cl->kd_valid = false;
+        if (best_factor) *best_factor = bestf;
+        if (best_ms) *best_ms = best;
+    });
+}
+
+int gf_tuning_save(gf_ctx* ctx, const char* path)
+{
+    if (!ctx || !path) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        FILE* f = std::fopen(path, "w");
+        if (!f) throw Error(GF_ERR_ARG, std::string("cannot write ") + path);
+        std::fprintf(f, "# gpupdal_amd autotune: rho_bucket k_bucket cell_factor\n");
+        for (const TuneEntry& e : C(ctx)->tune.table)
+            std::fprintf(f, "%d %d %.17g\n", e.rho_bucket, e.k_bucket, e.cell_factor);
+        std::fclose(f);
+    });
+}
+
+int gf_tuning_load(gf_ctx* ctx, const char* path)
+{
+    if (!ctx || !path) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        FILE* f = std::fopen(path, "r");
+        if (!f) throw Error(GF_ERR_ARG, std::string("cannot read ") + path);
+        char line[256];
+        std::vector<TuneEntry> t;
+        while (std::fgets(line, sizeof(line), f)) {
+            if (line[0] == '#') continue;
+            TuneEntry e{};
+            if (std::sscanf(line, "%d %d %lf", &e.rho_bucket, &e.k_bucket, &e.cell_factor) == 3 &&
+                e.cell_factor > 0)
+                t.push_back(e);
+        }
+        std::fclose(f);
+        for (const TuneEntry& e : t) {
+            bool found = false;
+            for (TuneEntry& x : C(ctx)->tune.table)
+                if (x.rho_bucket == e.rho_bucket && x.k_bucket == e.k_bucket) {
+                    x.cell_factor = e.cell_factor;
+                    found = true;
+                }
+            if (!found) C(ctx)->tune.table.push_back(e);
+        }
     });
 }
 

@@ -8,6 +8,7 @@
 #include "gf_internal.hpp"
 #include "gf_kernels.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -45,8 +46,21 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Autotuned column-size factors, per (2-D density, result capacity) bucket:
+// rho_bucket = floor(log2(points per m^2)), k_bucket = ceil(log2(fetch)).
+struct TuneEntry {
+    int rho_bucket;
+    int k_bucket;
+    double cell_factor;
+};
+
+inline int tune_rho_bucket(double rho) { return static_cast<int>(std::floor(std::log2(rho > 1e-300 ? rho : 1e-300))); }
+inline int tune_k_bucket(uint32_t fetch) { return static_cast<int>(std::ceil(std::log2(static_cast<double>(fetch < 1 ? 1 : fetch)))); }
+
 struct Tuning {
     double cell_factor = 0.6;    // column edge = factor * sqrt(k / (pi * rho2d))
+    std::vector<TuneEntry> table;  // autotuned factors (override cell_factor)
+    double forced_factor = 0.0;    // > 0: used for every grid (autotuning runs)
     double cell_z_factor = 1.0;  // z cell height = factor * column edge
     int ring0 = 1;
     int ring_max = 8;
@@ -71,13 +85,40 @@ struct Tuning {
         else throw Error(GF_ERR_ARG, std::string("unknown or invalid tuning key: ") + key);
         if (ring_max < ring0) ring_max = ring0;
     }
+    // The factor for a grid of this density and result capacity.
+    double factor_for(double rho, uint32_t fetch) const
+    {
+        if (forced_factor > 0) return forced_factor;
+        const int rb = tune_rho_bucket(rho), kb = tune_k_bucket(fetch);
+        for (const TuneEntry& e : table)
+            if (e.rho_bucket == rb && e.k_bucket == kb) return e.cell_factor;
+        return cell_factor;
+    }
+    void remember(double rho, uint32_t fetch, double factor)
+    {
+        const int rb = tune_rho_bucket(rho), kb = tune_k_bucket(fetch);
+        for (TuneEntry& e : table)
+            if (e.rho_bucket == rb && e.k_bucket == kb) {
+                e.cell_factor = factor;
+                return;
+            }
+        table.push_back(TuneEntry{rb, kb, factor});
+    }
     // Grid-shaping knobs only (the ring knobs do not change the index).
     uint64_t signature() const
     {
-        uint64_t a, b;
+        uint64_t a, b, c;
         std::memcpy(&a, &cell_factor, 8);
         std::memcpy(&b, &cell_z_factor, 8);
-        return a * 1000003ull ^ b;
+        std::memcpy(&c, &forced_factor, 8);
+        uint64_t h = a * 1000003ull ^ b ^ (c * 7919ull);
+        for (const TuneEntry& e : table) {
+            uint64_t f;
+            std::memcpy(&f, &e.cell_factor, 8);
+            h = h * 1099511628211ull ^ (static_cast<uint64_t>(e.rho_bucket + 4096) << 20) ^
+                static_cast<uint64_t>(e.k_bucket) ^ f;
+        }
+        return h;
     }
 };
 

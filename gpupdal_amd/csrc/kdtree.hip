@@ -916,11 +916,13 @@ __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, Fuse
         // The row's tile: its root node (tile r's tree is rooted at node r)
         // and its neighborhood size.
         uint32_t root = 0;
+        uint32_t pad_id = 0;  // KD3Index pads short strided lists with the view's first point
         uint32_t cap = static_cast<uint32_t>(prog0.fetch);
         uint32_t k = static_cast<uint32_t>(prog0.k);
         uint32_t stride = static_cast<uint32_t>(prog0.stride);
         if (tiles) {
             root = kd_tile_of(starts, ntiles, s);
+            pad_id = static_cast<uint32_t>(starts[root]);
             const FusedProgram& pr = tiles[root].prog;
             cap = static_cast<uint32_t>(pr.fetch);
             k = static_cast<uint32_t>(pr.k);
@@ -1053,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, Fuse
         // past the points found keep the output's zero initialisation.
         const Ent kept = ent_shfl(best, static_cast<int>((static_cast<uint32_t>(lane) * stride) & 63u));
         const bool mine = static_cast<uint32_t>(lane) < k;
-        const uint32_t kept_id = kept.d == kInfD ? 0u : kept.j;
+        const uint32_t kept_id = kept.d == kInfD ? pad_id : kept.j;
         if (mode == 1) {
             if (mine) out.ids[static_cast<uint64_t>(qid) * k + lane] = kept_id;
             continue;

@@ -201,11 +201,13 @@ __device__ __forceinline__ void select_entries(Ent (&c)[kRegBatches], uint32_t t
     double E = INFINITY;
     cnt[2] += 1;
     if (tot > cap) {
+        const uint32_t nbat = (tot + 63u) >> 6;  // register batches holding candidates
         auto count_le = [&](double t) {
             uint32_t k = 0;
 #pragma unroll
             for (int b = 0; b < kRegBatches; ++b)
-                k += __popcll(__ballot(c[b].d != kInfD && ent_d2(c[b]) <= t));
+                if (static_cast<uint32_t>(b) < nbat)
+                    k += __popcll(__ballot(c[b].d != kInfD && ent_d2(c[b]) <= t));
             return k;
         };
         double lo_e = 0.0, hi_e = INFINITY;
@@ -225,7 +227,7 @@ __device__ __forceinline__ void select_entries(Ent (&c)[kRegBatches], uint32_t t
             double mx = 0.0;
 #pragma unroll
             for (int b = 0; b < kRegBatches; ++b)
-                if (c[b].d != kInfD) mx = fmax(mx, ent_d2(c[b]));
+                if (static_cast<uint32_t>(b) < nbat && c[b].d != kInfD) mx = fmax(mx, ent_d2(c[b]));
 #pragma unroll
             for (int m = 32; m > 0; m >>= 1) mx = fmax(mx, __shfl_xor(mx, m));
             hi_e = mx;  // counts every candidate (> cap)
@@ -248,8 +250,10 @@ __device__ __forceinline__ void select_entries(Ent (&c)[kRegBatches], uint32_t t
     if (ok) {
         const uint64_t lt = (1ull << lane) - 1ull;
         uint32_t S = 0;
+        const uint32_t nbat_s = (tot + 63u) >> 6;
 #pragma unroll
         for (int b = 0; b < kRegBatches; ++b) {
+            if (static_cast<uint32_t>(b) >= nbat_s) break;
             const bool s = c[b].d != kInfD && ent_d2(c[b]) <= E;
             const uint64_t m = __ballot(s);
             if (s) {
@@ -570,9 +574,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                 cnt[1] += tot;
                 Ent c[kRegBatches];
 #pragma unroll
-                for (int b = 0; b < kRegBatches; ++b)
-                    c[b] = pj[b] == kInfU ? ent_inf()
-                                          : make_ent(qx, qy, qz, px[b], py[b], pz[b], pid[b], pj[b]);
+                for (int b = 0; b < kRegBatches; ++b) {
+                    c[b] = ent_inf();
+                    if (static_cast<uint32_t>(b) * 64u < tot && pj[b] != kInfU)
+                        c[b] = make_ent(qx, qy, qz, px[b], py[b], pz[b], pid[b], pj[b]);
+                }
                 // The previous query's need nearest lie within its need-th
                 // distance plus the step between the two queries.
                 double h = INFINITY;

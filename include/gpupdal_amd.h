@@ -114,6 +114,19 @@ int gf_sync(gf_ctx *ctx);
  * "small_col", "collect_stats".  None of them changes any row's result. */
 int gf_set_tuning(gf_ctx *ctx, const char *key, double value);
 
+/* ---- autotuning: replaces the reference's grid-configuration heuristics
+ *      (makeKnnGridConfig / makeAdaptiveKnnConfig, src/index/SpatialIndex.cpp
+ *      :434-631).  gf_autotune times the whole run over a cloud for each
+ *      candidate column-size factor (column edge = factor * sqrt(k / (pi *
+ *      2-D density))) and keeps the fastest for the cloud's (density, k)
+ *      bucket; later grids in that bucket, for clouds and batch tiles, use
+ *      it.  No row's result depends on the factor.  The table is saved and
+ *      loaded as text. */
+int gf_autotune(gf_ctx *ctx, gf_cloud *cloud, const gf_stage *stages, int nstages,
+                const double *factors, int nfactors, double *best_factor, double *best_ms);
+int gf_tuning_save(gf_ctx *ctx, const char *path);
+int gf_tuning_load(gf_ctx *ctx, const char *path);
+
 /* ---- resident clouds: replaces the PointBatch XYZ upload of
  *      makeResidentNeighborhood (src/pdal/PdgNeighborhood.cpp:1383-1550). */
 int gf_cloud_create(gf_ctx *ctx, const double *x, const double *y,

@@ -231,9 +231,10 @@ class Oracle:
         return cols, status
 
     def stage_from_lists(self, x, y, z, nbr, st, threads=8, init=0.0):
+        """Rows are nbr's rows (ids index x, y, z)."""
         x, y, z = (_c(v, np.float64) for v in (x, y, z))
         nbr = _c(nbr, np.uint32)
-        n = len(x)
+        n = nbr.shape[0]
         cols, arr = self._cols(st, n, init)
         status = np.zeros(n, np.uint8)
         self.L.orc_stage_from_lists(_ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp), n,
@@ -246,10 +247,11 @@ class Oracle:
                          with_ratio=False):
         """Extended-precision truth; returns (cols, status, gap) or, with
         with_ratio, (cols, status, gap, ratio) where ratio = lambda_min /
-        lambda_max of the row's covariance."""
+        lambda_max of the row's covariance.  Rows are nbr's rows (ids index
+        x, y, z)."""
         x, y, z = (_c(v, np.float64) for v in (x, y, z))
         nbr = _c(nbr, np.uint32)
-        n = len(x)
+        n = nbr.shape[0]
         cols, arr = self._cols(st, n, init)
         status = np.zeros(n, np.uint8)
         gap = np.zeros(n, np.float64)

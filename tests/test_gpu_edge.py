@@ -193,3 +193,23 @@ def test_argument_errors(ctx):
     # The context stays usable after an error.
     cols, status = gpu_stage(ctx, x, y, z, g.stage(g.STAGE_NORMAL, 8))
     assert np.count_nonzero(status & 0x3C) < len(x)
+
+
+# ------------------------------------------------------------- autotuner --
+def test_autotune_changes_speed_not_results(ctx, tmp_path):
+    x, y, z = small_cloud(60000, seed=70)
+    st = [g.stage(g.STAGE_NORMAL, 16), g.stage(g.STAGE_COVFEAT, 16)]
+    cl = ctx.cloud(x, y, z)
+    before = cl.run(st, init=np.nan)
+    f, ms = ctx.autotune(cl, st, factors=(0.45, 0.6, 0.9))
+    assert f in (0.45, 0.6, 0.9) and ms > 0
+    after = cl.run(st, init=np.nan)
+    for sb, sa in zip(before[1], after[1]):
+        assert np.array_equal(sb & ~np.uint8(INFO), sa & ~np.uint8(INFO))
+    for cb, ca in zip(before[0][0] + before[0][1], after[0][0] + after[0][1]):
+        np.testing.assert_allclose(ca, cb, rtol=1e-12, atol=1e-15, equal_nan=True)
+    path = tmp_path / "tune.txt"
+    ctx.save_tuning(path)
+    text = path.read_text()
+    assert len([l for l in text.splitlines() if not l.startswith("#")]) >= 1
+    ctx.load_tuning(path)

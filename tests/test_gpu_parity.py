@@ -177,3 +177,45 @@ def test_normal_eigen_chain_vs_pdal_oracle(ctx, oracle):
     assert np.all(np.sign(ncols[2][ok]) >= 0)
     e = rel_err(ncols[3][ok], on[3][ok], 1e-9)
     assert np.percentile(e, 99) < 1e-4
+
+
+# ------------------------------------------------- full-size configurations --
+def _full_check(ctx, oracle, x, y, z, stages, knn, sample):
+    cl = ctx.cloud(x, y, z)
+    ids, st = cl.knn(knn + 1)
+    tree = oracle.kd3(x, y, z)
+    ref = tree.neighbors_all(knn + 1, threads=16)
+    eq = set_equal_rows(ids, ref)
+    assert np.all(eq), f"{np.count_nonzero(~eq)} rows differ from KD3Index"
+    flagged = (st & (g.ST_TIE | g.ST_INCOMPLETE)) != 0
+    assert np.array_equal(ids[flagged], ref[flagged])
+    res_cols, res_st = cl.run(stages, init=np.nan)
+    rows = np.random.default_rng(0).choice(len(x), size=min(sample, len(x)), replace=False)
+    rows.sort()
+    for st_, cols, status in zip(stages, res_cols, res_st):
+        ost = ol.stage(st_.stage, st_.knn, st_.stride, st_.mode, st_.features, st_.always_up,
+                       st_.normalize)
+        tc, ts, gap, ratio = oracle.truth_from_lists(x, y, z, ref[rows], ost, threads=16,
+                                                     with_ratio=True)
+        ok = ((status[rows] & 0x3C) == 0) & (ts == 0) & (gap > 1e-3)
+        assert np.count_nonzero(ok) > 0.9 * len(rows)
+        errs = truth_errors(ost, [c[rows] for c in cols], tc, ok, ratio)
+        assert max(errs.values()) <= 1.0, errs
+
+
+def test_config_plain_cpu_runnable_full(ctx, oracle):
+    """250,000 points over 500 m x 500 m, covariancefeatures knn=10 with the
+    ten non-density features: every row's neighbor set equals KD3Index's,
+    features within the 1e-5 bar on a 50k-row sample."""
+    x, y, z = g.synth_cloud(250_000, tile=0, side=500.0)
+    _full_check(ctx, oracle, x, y, z, [g.stage(g.STAGE_COVFEAT, 10, features=g.F_ALL10)], 10,
+                50_000)
+
+
+def test_config_plain_gpu_full(ctx, oracle):
+    """4,000,000 points, normal + covariancefeatures knn=16: every row's
+    neighbor set equals KD3Index's; columns within the 1e-5 bar on a 100k-row
+    sample."""
+    x, y, z = g.synth_cloud(4_000_000, tile=0, side=500.0)
+    _full_check(ctx, oracle, x, y, z,
+                [g.stage(g.STAGE_NORMAL, 16), g.stage(g.STAGE_COVFEAT, 16)], 16, 100_000)
