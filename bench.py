@@ -55,28 +55,39 @@ def gpu_leg(args):
     cols = [ctx.alloc(n * 8) for _ in range(ncols)]
     status = [ctx.alloc(n) for _ in stages]
 
-    # Timed loop: no per-phase timers (they synchronize between kernels).
-    ctx.set_tuning("timing", 0)
+    # Timed loop.  Per-phase device times come from events recorded on the
+    # library's streams and read after each call's final synchronisation,
+    # so they add no wait between kernels.
+    ctx.set_tuning("timing", 1)
     for _ in range(args.warmup):
         cloud.invalidate()
         cloud.run_device(stages, cols, status)
     ctx.sync()
+    per = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         cloud.invalidate()  # a new cloud every step: the index is rebuilt
         cloud.run_device(stages, cols, status)
+        per.append(ctx.stats())
     ctx.sync()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
 
-    # Phase breakdown and the dominant kernel's duration (events on the
-    # library's stream), plus search counters, in separate passes.
-    ctx.set_tuning("timing", 1)
-    per = []
-    for _ in range(3):
-        cloud.invalidate()
-        cloud.run_device(stages, cols, status)
-        per.append(ctx.stats())
+    serial = None
+    if args.serial:
+        # The same step with the exact tree built after the search instead
+        # of beside it.
+        ctx.set_tuning("kd_overlap_min", 0)
+        ctx.sync()
+        s0 = time.perf_counter()
+        for _ in range(3):
+            cloud.invalidate()
+            cloud.run_device(stages, cols, status)
+        ctx.sync()
+        serial = (time.perf_counter() - s0) * 1e3 / 3
+        ctx.set_tuning("kd_overlap_min", 131072)
+
+    # Search counters in a separate pass.
     ctx.set_tuning("collect_stats", 1)
     cloud.invalidate()
     cloud.run_device(stages, cols, status)
@@ -103,11 +114,14 @@ def gpu_leg(args):
                        "candidates_per_point": counters["candidates"] / n,
                        "flops_per_candidate": FLOPS_PER_CANDIDATE},
         phases_ms={"index": mean("ms_index"), "search": ms_search,
-                   "exact": mean("ms_exact"), "features": mean("ms_features")},
+                   "exact_after_search": mean("ms_exact"), "features": mean("ms_features"),
+                   "tree_beside_search": mean("ms_tree"), "device_total": mean("ms_total")},
         rows={"tie": int(st["tie_rows"]), "incomplete": int(st["incomplete_rows"])},
         grid={"cell_m": st["cell"], "cell_z_m": st["cell_z"], "gx": st["grid_x"],
               "gy": st["grid_y"], "gz": st["grid_z"], "key_bits": st["key_bits"]},
     )
+    if serial is not None:
+        out["ms_per_step_tree_after_search"] = serial
     if args.copy_calibration:
         # A plain copy of known size: FETCH_SIZE / WRITE_SIZE counters on
         # this kernel give the scale factor for the search kernel's counters.
@@ -361,6 +375,10 @@ def main():
     ap.add_argument("--precision", action="store_true",
                     help="only the precision-stress report (20M points, roofs, wires)")
     ap.add_argument("--precision-points", type=int, default=20_000_000)
+    ap.add_argument("--serial", action="store_true",
+                    help="also time the step with the exact tree built after the search")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="only the batched-vs-plain leg (for profiling it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1. Tiles are "
@@ -369,6 +387,9 @@ def main():
 
     if args.precision:
         print(json.dumps({"precision": precision_leg(args)}))
+        return
+    if args.only_batched:
+        print(json.dumps({"batched": batched_leg(args)}))
         return
     gpu, xyz = gpu_leg(args)
     batched = None if args.no_batched else batched_leg(args)
@@ -395,6 +416,8 @@ def main():
         "roofline_fp64": gpu["roofline_fp64"],
         "phases_ms": gpu["phases_ms"],
         "rows_resolved_exactly": gpu["rows"],
+        **({"ms_per_step_tree_after_search": gpu["ms_per_step_tree_after_search"]}
+           if "ms_per_step_tree_after_search" in gpu else {}),
         "grid": gpu["grid"],
         "batched": batched,
         "serving": serving,

@@ -77,7 +77,7 @@ void Context::build_batch_index(Batch& b, const std::vector<uint32_t>& fetch)
     const uint32_t T = b.T;
     const uint64_t N = b.n;
     const uint32_t nb = static_cast<uint32_t>(b.blocks.size());
-    hipEvent_t e0 = ev_begin();
+    hipEvent_t e0 = mark();
 
     // Per-tile bounds (order-encoded min/max), read back for the grids.
     std::vector<unsigned long long> init(static_cast<size_t>(T) * 6);
@@ -145,12 +145,14 @@ void Context::build_batch_index(Batch& b, const std::vector<uint32_t>& fetch)
                          stream);
     launch_gather_sorted(vres, b.x.as<double>(), b.y.as<double>(), b.z.as<double>(), N,
                          b.sx.as<double>(), b.sy.as<double>(), b.sz.as<double>(), stream);
+    b.inv.ensure(N * 4 + 8);
+    launch_invert(vres, N, b.inv.as<uint32_t>(), stream);
     b.keys = kres;
     b.perm = vres;
     b.idx_valid = true;
     b.idx_fetch = fetch;
     b.idx_tune = tune.signature();
-    stats.ms_index = ev_end(e0);
+    span(e0, mark(), &stats.ms_index);
 }
 
 void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* tile_knn,
@@ -183,6 +185,8 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         ncol_total += sp.ncols;
     }
     if (ncol_total > 16) throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
+    reset_spans();
+    hipEvent_t t_all = mark();
 
     // Per-tile programs.
     std::vector<TileInfo> tiles(T);
@@ -256,12 +260,10 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     }
     flagged.ensure(N * 4 + 4);
     counters.ensure(64);
-    covbuf.ensure(N * 48 + 8);
-    flagbuf.ensure(N + 8);
+    covbuf.ensure(N * 8 * kCovRecord + 8);
     out.flagged = flagged.as<uint32_t>();
     out.nflagged = counters.as<uint32_t>();
     out.cov = covbuf.as<double>();
-    out.flags = flagbuf.as<uint8_t>();
     launch_fill_u32(out.nflagged, 4, 0u, stream);
     if (tune.collect_stats) {
         out.counters = reinterpret_cast<unsigned long long*>(counters.as<uint32_t>() + 8);
@@ -279,10 +281,22 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     cv.g = b.grids.empty() ? GridParams{} : b.grids[0].g;
     FusedProgram prog0 = tiles.empty() ? base : tiles[0].prog;
 
-    hipEvent_t e0 = ev_begin();
+    // The tiles' exact trees depend only on the points: on large batches
+    // they are built on the side stream while the search runs.
+    const bool early = !b.kd_valid && tune.kd_overlap_min && N >= tune.kd_overlap_min;
+    if (early) fork_side();
+    hipEvent_t e0 = mark();
     launch_knn(cv, prog0, 0, out, knn_tuning(), stream, &tt);
     if (any_exact_tile) launch_flag_tiles(N, cv.perm, 0, out, tt, stream);
-    stats.ms_search = ev_end(e0);
+    hipEvent_t e1 = mark();
+    span(e0, e1, &stats.ms_search);
+    auto forest = [&](hipStream_t s) {
+        if (b.kd_valid) return;
+        build_forest(b.kd, b.x.as<double>(), b.y.as<double>(), b.z.as<double>(), N,
+                     b.dstarts.as<uint64_t>(), T, b.dbounds.as<double>(), s);
+        b.kd_valid = true;
+    };
+    if (early) forest(side);
 
     // Exact re-solve of flagged rows on the tiles' KD3 trees.
     uint32_t h[4] = {0, 0, 0, 0};
@@ -291,12 +305,7 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     stats.tie_rows = h[1];
     stats.incomplete_rows = h[2];
     if (h[0]) {
-        e0 = ev_begin();
-        if (!b.kd_valid) {
-            build_forest(b.kd, b.x.as<double>(), b.y.as<double>(), b.z.as<double>(), N,
-                         b.dstarts.as<uint64_t>(), T, b.dbounds.as<double>());
-            b.kd_valid = true;
-        }
+        forest(stream);
         uint32_t* failed = counters.as<uint32_t>() + 5;
         launch_fill_u32(failed, 1, 0u, stream);
         launch_kd_resolve(b.kd.view, cv, prog0, 0, out, b.x.as<double>(), b.y.as<double>(),
@@ -304,14 +313,16 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         uint32_t nfail = 0;
         GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, stream));
         GF_HIP(hipStreamSynchronize(stream));
-        stats.ms_exact = ev_end(e0);
+        span(e1, mark(), &stats.ms_exact);
         if (nfail)
             throw Error(GF_ERR_UNSUPPORTED, "exact KD3 search exceeded its depth bound on " +
                                                 std::to_string(nfail) + " rows");
     }
-    e0 = ev_begin();
-    launch_features(cv, prog0, out, stream, &tt);
-    stats.ms_features = ev_end(e0);
+    e0 = mark();
+    launch_features(cv, prog0, out, b.inv.as<uint32_t>(), stream, &tt);
+    e1 = mark();
+    span(e0, e1, &stats.ms_features);
+    span(t_all, e1, &stats.ms_total);
 
     unsigned long long ctr[4] = {0, 0, 0, 0};
     if (out.counters)
@@ -323,6 +334,7 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
             GF_HIP(hipMemcpyAsync(status[i], out.status[i], N, hipMemcpyDeviceToHost, stream));
     }
     GF_HIP(hipStreamSynchronize(stream));
+    collect_spans();
     stats.points = N;
     stats.search_passes = ctr[0];
     stats.candidates = ctr[1];
@@ -340,6 +352,7 @@ template <class F>
 int bguarded(gf_ctx* ctx, F&& f)
 {
     try {
+        if (ctx) reinterpret_cast<Context*>(ctx)->reset_spans();
         f();
         return GF_OK;
     } catch (const gf::Error& e) {

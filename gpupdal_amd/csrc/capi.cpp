@@ -78,7 +78,7 @@ void Context::build_index(Cloud& cl, uint32_t fetch)
     if (cl.idx_valid && cl.idx_fetch == fetch && cl.idx_tune == tune.signature())
         return;
     cl.idx_valid = false;
-    hipEvent_t e0 = ev_begin();
+    hipEvent_t e0 = mark();
     part.ensure(kBoundsMaxParts * 6 * sizeof(double));
     bounds.ensure(6 * sizeof(double));
     launch_bounds(cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), n,
@@ -114,13 +114,15 @@ void Context::build_index(Cloud& cl, uint32_t fetch)
     launch_gather_sorted(vres, cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(),
                          n, cl.sx.as<double>(), cl.sy.as<double>(), cl.sz.as<double>(),
                          stream);
+    cl.inv.ensure(n * 4 + 8);
+    launch_invert(vres, n, cl.inv.as<uint32_t>(), stream);
     cl.keys = kres;
     cl.perm = vres;
     cl.g = g;
     cl.idx_valid = true;
     cl.idx_fetch = fetch;
     cl.idx_tune = tune.signature();
-    stats.ms_index = ev_end(e0);
+    span(e0, mark(), &stats.ms_index);
     stats.cell = g.c;
     stats.cell_z = g.cz;
     stats.grid_x = g.gx;
@@ -198,6 +200,8 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     }
     if (ncol_total > 16) throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
 
+    reset_spans();
+    hipEvent_t t_all = mark();
     build_index(cl, fetch);
 
     KnnOutputs out{};
@@ -217,12 +221,10 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     }
     flagged.ensure(n * 4 + 4);
     counters.ensure(64);
-    covbuf.ensure(n * 48 + 8);
-    flagbuf.ensure(n + 8);
+    covbuf.ensure(n * 8 * kCovRecord + 8);
     out.flagged = flagged.as<uint32_t>();
     out.nflagged = counters.as<uint32_t>();
     out.cov = covbuf.as<double>();
-    out.flags = flagbuf.as<uint8_t>();
     launch_fill_u32(out.nflagged, 4, 0u, stream);
     if (tune.collect_stats) {
         out.counters = reinterpret_cast<unsigned long long*>(counters.as<uint32_t>() + 8);
@@ -230,16 +232,26 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     }
 
     const CloudView cv = view(cl);
-    hipEvent_t e0 = ev_begin();
+    // The exact tree depends only on the points: on large clouds, which
+    // nearly always have a flagged row, it is built on the side stream while
+    // the search runs.
+    const bool early = fetch <= n && !cl.kd_valid && tune.kd_overlap_min &&
+                       n >= tune.kd_overlap_min;
+    if (early) fork_side();
+    hipEvent_t e0 = mark();
     if (fetch > n)  // fewer points than the result capacity: all rows exact
         launch_flag_all(n, cv.perm, 0, out, stream);
     else
         launch_knn(cv, prog, 0, out, knn_tuning(), stream);
-    stats.ms_search = ev_end(e0);
-    resolve_flagged(cl, cv, prog, 0, out);
-    e0 = ev_begin();
-    launch_features(cv, prog, out, stream);
-    stats.ms_features = ev_end(e0);
+    hipEvent_t e1 = mark();
+    span(e0, e1, &stats.ms_search);
+    if (early) build_kdtree(cl, side);
+    resolve_flagged(cl, cv, prog, 0, out, e1);
+    e0 = mark();
+    launch_features(cv, prog, out, cl.inv.as<uint32_t>(), stream);
+    e1 = mark();
+    span(e0, e1, &stats.ms_features);
+    span(t_all, e1, &stats.ms_total);
 
     unsigned long long ctr[4] = {0, 0, 0, 0};
     if (out.counters)
@@ -253,6 +265,7 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
                                   stream));
     }
     GF_HIP(hipStreamSynchronize(stream));
+    collect_spans();
     stats.points = n;
     stats.search_passes = ctr[0];
     stats.candidates = ctr[1];
@@ -263,8 +276,9 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
 // x, y, z; bounds 6 per tile), tile r rooted at node r.
 void Context::build_forest(KdBuffers& b, const double* x, const double* y, const double* z,
                            uint64_t n, const uint64_t* dstarts, uint32_t T,
-                           const double* dbounds)
+                           const double* dbounds, hipStream_t s)
 {
+    hipEvent_t t0 = mark(s);
     uint32_t cap = std::max(kd_node_capacity(n) + T, b.cap);
     uint32_t lcap = std::max(kd_list_capacity(n) + T, b.list_cap);
     for (int attempt = 0;; ++attempt) {
@@ -321,12 +335,13 @@ void Context::build_forest(KdBuffers& b, const double* x, const double* y, const
         kt.subtree_points = static_cast<uint32_t>(tune.kd_subtree);
         kt.minmax_lds_nodes = static_cast<uint32_t>(tune.kd_minmax_lds);
         const uint32_t rc =
-            launch_kd_build(x, y, z, n, dstarts, T, dbounds, t, w, kt, &nodes, stream);
+            launch_kd_build(x, y, z, n, dstarts, T, dbounds, t, w, kt, &nodes, s);
         if (rc == 0) {
             b.view = t;
             b.cap = cap;
             b.list_cap = lcap;
             b.nodes = nodes;
+            span(t0, mark(s), &stats.ms_tree);
             return;
         }
         if (rc == 3 || attempt >= 8)
@@ -337,19 +352,27 @@ void Context::build_forest(KdBuffers& b, const double* x, const double* y, const
     }
 }
 
-// The exact KD3Index tree of a cloud (built once, reused until invalidated).
-void Context::build_kdtree(Cloud& cl)
+// The exact KD3Index tree of a cloud (built once, reused until
+// invalidated), complete on return.
+void Context::build_kdtree(Cloud& cl, hipStream_t s)
 {
     if (cl.kd_valid) return;
     build_forest(cl.kd, cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), cl.n,
-                 cl.dstarts.as<uint64_t>(), 1, cl.dbounds.as<double>());
+                 cl.dstarts.as<uint64_t>(), 1, cl.dbounds.as<double>(), s);
     cl.kd_valid = true;
 }
 
+void Context::fork_side()
+{
+    GF_HIP(hipEventRecord(join, stream));
+    GF_HIP(hipStreamWaitEvent(side, join, 0));
+}
+
 // Re-solves the rows the grid search flagged (boundary ties, uncertified
-// searches) with the exact KD3Index search.
+// searches) with the exact KD3Index search.  ms_exact runs from `from` (the
+// end of the search) to the end of the re-solve.
 void Context::resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog,
-                              int mode, const KnnOutputs& out)
+                              int mode, const KnnOutputs& out, hipEvent_t from)
 {
     uint32_t h[4] = {0, 0, 0, 0};
     GF_HIP(hipMemcpyAsync(h, out.nflagged, sizeof(h), hipMemcpyDeviceToHost, stream));
@@ -357,7 +380,7 @@ void Context::resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram
     stats.tie_rows = h[1];
     stats.incomplete_rows = h[2];
     if (!h[0]) return;
-    hipEvent_t e0 = ev_begin();
+    hipEvent_t e0 = from ? from : mark();
     build_kdtree(cl);
     uint32_t* failed = counters.as<uint32_t>() + 5;
     launch_fill_u32(failed, 1, 0u, stream);
@@ -366,28 +389,46 @@ void Context::resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram
     uint32_t nfail = 0;
     GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, stream));
     GF_HIP(hipStreamSynchronize(stream));
-    stats.ms_exact = ev_end(e0);
+    span(e0, mark(), &stats.ms_exact);
     if (nfail)
         throw Error(GF_ERR_UNSUPPORTED,
                     "exact KD3 search exceeded its depth bound on " + std::to_string(nfail) +
                         " rows");
 }
 
-hipEvent_t Context::ev_begin()
+hipEvent_t Context::mark(hipStream_t s)
 {
     if (!timing) return nullptr;
-    GF_HIP(hipEventRecord(ev0, stream));
-    return ev0;
+    if (evnext == evpool.size()) {
+        hipEvent_t e;
+        GF_HIP(hipEventCreate(&e));
+        evpool.push_back(e);
+    }
+    hipEvent_t e = evpool[evnext++];
+    GF_HIP(hipEventRecord(e, s));
+    return e;
 }
 
-double Context::ev_end(hipEvent_t e)
+void Context::span(hipEvent_t a, hipEvent_t b, double* dst)
 {
-    if (!e) return 0.0;
-    GF_HIP(hipEventRecord(ev1, stream));
-    GF_HIP(hipEventSynchronize(ev1));
-    float ms = 0.f;
-    GF_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-    return ms;
+    if (a && b) spans.push_back(Span{a, b, dst});
+}
+
+void Context::collect_spans()
+{
+    for (const Span& sp : spans) {
+        float ms = 0.f;
+        GF_HIP(hipEventSynchronize(sp.b));
+        GF_HIP(hipEventElapsedTime(&ms, sp.a, sp.b));
+        *sp.dst = ms;
+    }
+    reset_spans();
+}
+
+void Context::reset_spans()
+{
+    spans.clear();
+    evnext = 0;
 }
 
 } // namespace gf
@@ -399,6 +440,7 @@ template <class F>
 int guarded(gf_ctx* ctx, F&& f)
 {
     try {
+        if (ctx) reinterpret_cast<Context*>(ctx)->reset_spans();
         f();
         return GF_OK;
     } catch (const gf::Error& e) {
@@ -601,11 +643,15 @@ int gf_knn(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t stride, uint32_t* 
         out.nflagged = c->counters.as<uint32_t>();
         launch_fill_u32(out.nflagged, 4, 0u, c->stream);
         const CloudView cv = c->view(*cl);
+        const bool early = fetch <= n && !cl->kd_valid && c->tune.kd_overlap_min &&
+                           n >= c->tune.kd_overlap_min;
+        if (early) c->fork_side();
         if (fetch > n)
             launch_flag_all(n, cv.perm, 1, out, c->stream);
         else
             launch_knn(cv, prog, 1, out, c->knn_tuning(), c->stream);
-        c->resolve_flagged(*cl, cv, prog, 1, out);
+        if (early) c->build_kdtree(*cl, c->side);
+        c->resolve_flagged(*cl, cv, prog, 1, out, nullptr);
         if (where == GF_WHERE_HOST) {
             GF_HIP(hipMemcpyAsync(ids, out.ids, n * kk * 4, hipMemcpyDeviceToHost, c->stream));
             GF_HIP(hipMemcpyAsync(status, out.status[0], n, hipMemcpyDeviceToHost, c->stream));

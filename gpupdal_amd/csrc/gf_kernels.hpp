@@ -68,8 +68,8 @@ struct KnnOutputs {
     uint32_t* ids;           // neighbor-list mode: n x k (original order)
     uint32_t* flagged;       // sorted indices of tie/incomplete rows
     uint32_t* nflagged;      // [0] flagged rows, [1] tie rows, [2] incomplete rows
-    double* cov;             // 6 x n FP64 covariance, sorted order (SoA)
-    uint8_t* flags;          // n tie/incomplete flags, sorted order
+    double* cov;             // per sorted row a kCovRecord-double record: the 6 unique
+                             // covariance entries, then the row flags (as a uint64)
     unsigned long long* counters; // optional: passes, candidates, merges, queries
 };
 
@@ -90,6 +90,9 @@ struct CloudView {
     uint64_t n;
     GridParams g;
 };
+
+// Doubles per covariance record (64 bytes: one cache line per row).
+constexpr uint32_t kCovRecord = 8;
 
 // One tile (an independent cloud) of a batch.  Tiles are contiguous in the
 // batch's point order, and each tile's sorted rows stay in its own range
@@ -132,9 +135,14 @@ inline int knn_queries_per_wave(int max_k) { return max_k <= 32 ? 64 : 32; }
 void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
                 const KnnOutputs& out, const KnnTuning& tune, hipStream_t s,
                 const TileTables* tt = nullptr);
-// Eigensystem + stage columns from out.cov for every row.
+// Eigensystem + stage columns from out.cov for every row, in original point
+// order (inv: original id -> sorted position), so outputs are written
+// contiguously.
 void launch_features(const CloudView& cv, const FusedProgram& prog,
-                     const KnnOutputs& out, hipStream_t s, const TileTables* tt = nullptr);
+                     const KnnOutputs& out, const uint32_t* inv, hipStream_t s,
+                     const TileTables* tt = nullptr);
+// inv[perm[s]] = s.
+void launch_invert(const uint32_t* perm, uint64_t n, uint32_t* inv, hipStream_t s);
 
 // kdtree.hip -- the exact KD3Index tree (nanoflann 1.3.1 semantics).
 struct KdView {

@@ -68,6 +68,10 @@ struct Tuning {
     int group_z = 1;
     int kd_subtree = 4096;       // exact tree: subtree-per-workgroup threshold (points)
     int kd_minmax_lds = 512;     // exact tree: LDS-staged extents up to this many nodes
+    // Clouds (batches) of at least this many points build the exact tree on
+    // a side stream during the search, before knowing whether a row needs
+    // it; smaller ones build it only when a row is flagged.  0: never early.
+    uint64_t kd_overlap_min = 131072;
     bool collect_stats = false;
 
     void set(const char* key, double v)
@@ -81,6 +85,8 @@ struct Tuning {
         else if (!std::strcmp(key, "kd_subtree") && v >= 101 && v <= 4096)
             kd_subtree = static_cast<int>(v);
         else if (!std::strcmp(key, "kd_minmax_lds") && v >= 0) kd_minmax_lds = static_cast<int>(v);
+        else if (!std::strcmp(key, "kd_overlap_min") && v >= 0)
+            kd_overlap_min = static_cast<uint64_t>(v);
         else if (!std::strcmp(key, "collect_stats")) collect_stats = v != 0.0;
         else throw Error(GF_ERR_ARG, std::string("unknown or invalid tuning key: ") + key);
         if (ring_max < ring0) ring_max = ring0;
@@ -145,6 +151,7 @@ struct Cloud {
     DevBuf keys_a, vals_a, keys_b, vals_b, colstart, sx, sy, sz;
     uint32_t* keys = nullptr;    // sorted keys (into keys_a/keys_b)
     uint32_t* perm = nullptr;    // sorted position -> original id
+    DevBuf inv;                  // original id -> sorted position
     double bounds[6] = {0, 0, 0, 0, 0, 0};  // min xyz, max xyz
     DevBuf dbounds;
     DevBuf dstarts;              // {0, n}: the cloud as a one-tile forest
@@ -170,7 +177,7 @@ struct Batch {
     std::vector<TileGrid> grids;
     std::vector<double> bounds;          // 6 per tile
     DevBuf dgrids, dbounds, benc, hist;
-    DevBuf keys_a, vals_a, keys_b, vals_b, colstart, sx, sy, sz;
+    DevBuf keys_a, vals_a, keys_b, vals_b, colstart, sx, sy, sz, inv;
     uint32_t* keys = nullptr;
     uint32_t* perm = nullptr;
     uint64_t col_entries = 0;
@@ -186,15 +193,28 @@ GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch, const Tuning
 int stage_ncols(const gf_stage& s);
 void validate_stage(const gf_stage& s);
 
+// A device-time interval between two events, read after the call's final
+// synchronisation (timing adds no host wait between kernels).
+struct Span {
+    hipEvent_t a, b;
+    double* dst;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Second default-priority stream: the exact tree, which depends only on
+    // the points, is built on it while the search runs on `stream`.
+    hipStream_t side = nullptr;
+    hipEvent_t join = nullptr;
+    std::vector<hipEvent_t> evpool;
+    size_t evnext = 0;
+    std::vector<Span> spans;
     bool timing = true;
     Tuning tune;
     gf_stats stats{};
     std::string last_error;
-    DevBuf part, bounds, hist, scan_tmp, outbuf, statbuf, flagged, counters, covbuf, flagbuf;
+    DevBuf part, bounds, hist, scan_tmp, outbuf, statbuf, flagged, counters, covbuf;
     DevBuf kd_s1, kd_s2, kd_tmp, kd_lists;
     double* host_bounds = nullptr;
 
@@ -206,17 +226,19 @@ struct Context {
             throw Error(GF_ERR_ARG, "device index out of range");
         GF_HIP(hipSetDevice(dev));
         GF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        GF_HIP(hipEventCreate(&ev0));
-        GF_HIP(hipEventCreate(&ev1));
+        GF_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        GF_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
         GF_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_bounds), 64, hipHostMallocDefault));
     }
     ~Context()
     {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
+        if (side) (void)hipStreamSynchronize(side);
         if (host_bounds) (void)hipHostFree(host_bounds);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -234,16 +256,26 @@ struct Context {
     CloudView view(const Cloud& cl) const;
     void run_group(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
                    uint8_t* const* status, int where);
-    void build_kdtree(Cloud& cl);
+    void build_kdtree(Cloud& cl, hipStream_t s);
+    void build_kdtree(Cloud& cl) { build_kdtree(cl, stream); }
     void build_forest(KdBuffers& b, const double* x, const double* y, const double* z,
-                      uint64_t n, const uint64_t* dstarts, uint32_t T, const double* dbounds);
+                      uint64_t n, const uint64_t* dstarts, uint32_t T, const double* dbounds,
+                      hipStream_t s);
+    // Makes the side stream wait for the work queued on `stream` so far.
+    void fork_side();
     void resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog, int mode,
-                         const KnnOutputs& out);
+                         const KnnOutputs& out, hipEvent_t from);
     void build_batch_index(Batch& b, const std::vector<uint32_t>& fetch);
     void run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* tile_knn,
                    double* const* cols, uint8_t* const* status, int where);
-    hipEvent_t ev_begin();
-    double ev_end(hipEvent_t e);
+    // Timing: mark() records a pooled event on a stream (nullptr when timing
+    // is off); span() registers an interval; collect_spans(), after the
+    // call's final synchronisation, fills every registered interval.
+    hipEvent_t mark(hipStream_t s);
+    hipEvent_t mark() { return mark(stream); }
+    void span(hipEvent_t a, hipEvent_t b, double* dst);
+    void collect_spans();
+    void reset_spans();
 };
 
 } // namespace gf

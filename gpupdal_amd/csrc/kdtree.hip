@@ -61,7 +61,7 @@ __device__ __forceinline__ const double* axis(const KdView& t, int f)
 // its points into LDS and builds its whole subtree there, level by level
 // with block barriers.
 constexpr uint32_t kSub = 4096;           // small-node capacity (points)
-constexpr int kSubThreads = 256;
+constexpr int kSubThreads = 1024;
 constexpr int kSubNodes = 128;            // nodes per subtree level (<= 2 * 4096 / 101)
 constexpr int kPredThreads = 256;
 constexpr int kPredItems = 16;
@@ -97,6 +97,11 @@ template <int NT> __device__ __forceinline__ uint32_t kd_block_scan(uint32_t v, 
     return off + incl - v;
 }
 
+__device__ __forceinline__ double pick3(const double v[3], int f)
+{
+    return f == 0 ? v[0] : (f == 1 ? v[1] : v[2]);  // no dynamic register indexing
+}
+
 // middleSplit_: among the axes whose box span is within 1e-5 of the widest,
 // the one with the largest data spread; cut at the box middle clamped to the
 // data range (nanoflann.hpp:779-827).
@@ -122,11 +127,12 @@ __device__ __forceinline__ void kd_choose_split(const double* b, const double mn
         }
     }
     const double split = (b[f] + b[3 + f]) / 2;
+    const double lo = pick3(mn, f), hi = pick3(mx, f);
     cv = split;
-    if (split < mn[f])
-        cv = mn[f];
-    else if (split > mx[f])
-        cv = mx[f];
+    if (split < lo)
+        cv = lo;
+    else if (split > hi)
+        cv = hi;
 }
 
 // The parent's divlow (max of child1) or divhigh (min of child2) on its cut
@@ -138,9 +144,9 @@ __device__ __forceinline__ void kd_parent_div(const KdView& t, uint32_t nd, cons
     if (par < 0) return;
     const int f = t.feat[par];
     if (t.which[nd] == 0)
-        t.divlow[par] = mx[f];
+        t.divlow[par] = pick3(mx, f);
     else
-        t.divhigh[par] = mn[f];
+        t.divhigh[par] = pick3(mn, f);
 }
 
 // planeSplit's index rule: lim1 if more than half are strictly left, lim2 if
@@ -299,10 +305,15 @@ struct ActPos {
     uint32_t nd, lo, t0, t1, p;
 };
 
+// The node of active index ti, searching forward from node kb (the node of
+// an index <= ti, found once per block): big nodes hold more points than a
+// block covers, so this is one or two steps.
 __device__ __forceinline__ ActPos act_pos(const KdView& t, const uint32_t* act,
-                                          const uint32_t* off, uint32_t A, uint32_t ti)
+                                          const uint32_t* off, uint32_t A, uint32_t kb,
+                                          uint32_t ti)
 {
-    const uint32_t k = act_find(off, A, ti);
+    uint32_t k = kb;
+    while (k + 1 < A && off[k + 1] <= ti) ++k;
     ActPos r;
     r.nd = act[k];
     r.lo = t.nl[r.nd];
@@ -330,10 +341,11 @@ __global__ __launch_bounds__(kPredThreads) void k_kd_pred_reduce(
     uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, uint32_t* __restrict__ sums)
 {
     const uint32_t base = blockIdx.x * kPredTile;
+    const uint32_t kb = act_find(off, A, base);  // uniform: one search per block
     uint32_t acc = 0;
     for (int i = 0; i < kPredItems; ++i) {
         const uint32_t ti = base + i * kPredThreads + threadIdx.x;
-        if (ti < P && kd_pred(t, pass, act_pos(t, act, off, A, ti), s1)) ++acc;
+        if (ti < P && kd_pred(t, pass, act_pos(t, act, off, A, kb, ti), s1)) ++acc;
     }
     uint32_t tot;
     kd_block_scan<kPredThreads>(acc, &tot);
@@ -349,10 +361,11 @@ __global__ __launch_bounds__(kPredThreads) void k_kd_pred_down(
 {
     __shared__ uint32_t s[kPredTile];
     const uint32_t base = blockIdx.x * kPredTile;
+    const uint32_t kb = act_find(off, A, base < P ? base : (P ? P - 1 : 0));
     for (int i = 0; i < kPredItems; ++i) {
         const uint32_t l = i * kPredThreads + threadIdx.x;
         const uint32_t ti = base + l;
-        s[l] = (ti < P && kd_pred(t, pass, act_pos(t, act, off, A, ti), s1)) ? 1u : 0u;
+        s[l] = (ti < P && kd_pred(t, pass, act_pos(t, act, off, A, kb, ti), s1)) ? 1u : 0u;
     }
     __syncthreads();
     uint32_t v[kPredItems];
@@ -393,9 +406,11 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_publish(
     uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, const uint32_t* __restrict__ sc,
     uint32_t* __restrict__ tmp)
 {
-    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b0 = blockIdx.x * blockDim.x;
+    const uint32_t kb = act_find(off, A, b0);
+    const uint32_t ti = b0 + threadIdx.x;
     if (ti >= P) return;
-    const ActPos ap = act_pos(t, act, off, A, ti);
+    const ActPos ap = act_pos(t, act, off, A, kb, ti);
     if (!kd_pred(t, pass, ap, s1)) return;
     uint32_t a, m;
     kd_pass_bounds(t, pass, ap, s1, sc, a, m);
@@ -407,9 +422,11 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_swap(
     uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, const uint32_t* __restrict__ sc,
     const uint32_t* __restrict__ tmp)
 {
-    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b0 = blockIdx.x * blockDim.x;
+    const uint32_t kb = act_find(off, A, b0);
+    const uint32_t ti = b0 + threadIdx.x;
     if (ti >= P) return;
-    const ActPos ap = act_pos(t, act, off, A, ti);
+    const ActPos ap = act_pos(t, act, off, A, kb, ti);
     uint32_t a, m;
     kd_pass_bounds(t, pass, ap, s1, sc, a, m);
     if (ap.p < a || ap.p >= m) return;
@@ -537,11 +554,12 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_big_minmax(
     }
     const int lane = __lane_id();
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < P; b0 += gridDim.x * blockDim.x) {
+        const uint32_t kb = act_find(off, A, b0);
         const uint32_t ti = b0 + threadIdx.x;
         uint32_t j = ~0u;
         unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
         if (ti < P) {
-            const ActPos ap = act_pos(t, act, off, A, ti);
+            const ActPos ap = act_pos(t, act, off, A, kb, ti);
             const uint32_t ch = ap.p < t.split[ap.nd] ? static_cast<uint32_t>(t.c1[ap.nd])
                                                       : static_cast<uint32_t>(t.c2[ap.nd]);
             j = t.aux[ch];
@@ -1072,13 +1090,13 @@ __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, Fuse
                      c2 = wave_sum(dx * dz) / den, c3 = wave_sum(dy * dy) / den,
                      c4 = wave_sum(dy * dz) / den, c5 = wave_sum(dz * dz) / den;
         if (lane == 0) {
-            const uint64_t n = cv.n;
-            out.cov[0 * n + s] = c0;
-            out.cov[1 * n + s] = c1;
-            out.cov[2 * n + s] = c2;
-            out.cov[3 * n + s] = c3;
-            out.cov[4 * n + s] = c4;
-            out.cov[5 * n + s] = c5;
+            double* rec = out.cov + kCovRecord * static_cast<uint64_t>(s);
+            rec[0] = c0;
+            rec[1] = c1;
+            rec[2] = c2;
+            rec[3] = c3;
+            rec[4] = c4;
+            rec[5] = c5;
         }
     }
 }
@@ -1094,7 +1112,7 @@ __global__ __launch_bounds__(256) void k_flag_all(uint64_t n, const uint32_t* __
     if (mode == 1)
         out.status[0][perm[s]] = 0;
     else
-        out.flags[s] = 0;
+        reinterpret_cast<uint64_t*>(out.cov)[kCovRecord * s + 6] = 0;
 }
 
 // Rows of the tiles with fewer points than their result capacity go to the
@@ -1114,7 +1132,7 @@ __global__ __launch_bounds__(256) void k_flag_tiles(uint64_t n, const uint32_t* 
     if (mode == 1)
         out.status[0][perm[s]] = 0;
     else
-        out.flags[s] = 0;
+        reinterpret_cast<uint64_t*>(out.cov)[kCovRecord * s + 6] = 0;
 }
 
 } // namespace

@@ -87,6 +87,15 @@ struct KnnArgs {
     uint32_t ntasks;
 };
 
+// What a wavefront needs of its cloud or tile: the grid, the column starts,
+// the certification margin and the neighborhood sizes.
+struct WaveGrid {
+    GridParams g;
+    const uint32_t* colstart;
+    double margin;
+    int fetch, k, stride;
+};
+
 // The examined region: columns [x0, x1] x [y0, y1], z cells [z0, z1].
 struct Box {
     int x0, x1, y0, y1, z0, z1;
@@ -110,24 +119,24 @@ __device__ __forceinline__ double box_clearance(const GridParams& g, const Box& 
     return R;
 }
 
-__device__ __forceinline__ bool certified(const KnnArgs& a, const Box& b, const Ent& kth,
+__device__ __forceinline__ bool certified(const WaveGrid& wg, const Box& b, const Ent& kth,
                                           double qx, double qy, double qz)
 {
-    const double R = box_clearance(a.cv.g, b, qx, qy, qz);
+    const double R = box_clearance(wg.g, b, qx, qy, qz);
     if (R == INFINITY) return true;
     if (kth.d == kInfD) return false;
-    const double Rm = R - a.margin;
+    const double Rm = R - wg.margin;
     return Rm > 0.0 && ent_d2(kth) < Rm * Rm;
 }
 
 // Column ranges of a box, one column per lane (box columns must be <= 64):
 // lo / len of the column's sorted points restricted to z cells [z0, z1]
 // (whole columns when the box spans every z cell or the column is small).
-__device__ __forceinline__ void box_ranges(const KnnArgs& a, const Box& b, int cb,
-                                           int lane, uint32_t& lo, uint32_t& len)
+__device__ __forceinline__ void box_ranges(const KnnArgs& a, const WaveGrid& wg, const Box& b,
+                                           int cb, int lane, uint32_t& lo, uint32_t& len)
 {
     const CloudView& cv = a.cv;
-    const GridParams& g = cv.g;
+    const GridParams& g = wg.g;
     const int w = b.x1 - b.x0 + 1;
     const int ncb = w * (b.y1 - b.y0 + 1);
     const int ci = cb + lane;
@@ -140,7 +149,7 @@ __device__ __forceinline__ void box_ranges(const KnnArgs& a, const Box& b, int c
         return;
     const bool zfull = b.z0 <= 0 && b.z1 >= static_cast<int>(g.gz) - 1;
     const uint32_t col = column_index(g, cx, cy);
-    uint32_t b0 = cv.colstart[col], b1 = cv.colstart[col + 1];
+    uint32_t b0 = wg.colstart[col], b1 = wg.colstart[col + 1];
     if (!zfull && b1 - b0 > static_cast<uint32_t>(a.tune.small_col)) {
         const uint32_t klo = (col << g.bz) + static_cast<uint32_t>(max(b.z0, 0));
         const uint32_t khi = (col << g.bz) + static_cast<uint32_t>(min(b.z1, static_cast<int>(g.gz) - 1) + 1);
@@ -317,13 +326,13 @@ __device__ __forceinline__ void select_entries(Ent (&c)[kRegBatches], uint32_t t
 // One query alone, with a growing box starting at half-width r (columns and
 // z cells around the query's cell): candidates stream through the registers
 // (<= 256, one box of <= 64 columns) or through sort-and-merge otherwise.
-__device__ __forceinline__ void search_single(const KnnArgs& a, double qx, double qy, double qz,
-                              int qcx, int qcy, int qcz, int r, int lane,
+__device__ __forceinline__ void search_single(const KnnArgs& a, const WaveGrid& wg, double qx,
+                              double qy, double qz, int qcx, int qcy, int qcz, int r, int lane,
                               uint32_t* scratch, Ent& best, bool& incomplete,
                               uint64_t* cnt)
 {
     const CloudView& cv = a.cv;
-    const int fetch = a.prog.fetch;
+    const int fetch = wg.fetch;
     incomplete = false;
     for (;;) {
         cnt[0] += 1;
@@ -332,7 +341,7 @@ __device__ __forceinline__ void search_single(const KnnArgs& a, double qx, doubl
         best = ent_inf();
         for (int cb = 0; cb < ncb; cb += 64) {
             uint32_t lo, len;
-            box_ranges(a, b, cb, lane, lo, len);
+            box_ranges(a, wg, b, cb, lane, lo, len);
             const uint32_t incl = wave_incl_scan_u32(len, lane);
             const uint32_t tot = __shfl(incl, 63);
             const uint32_t excl = incl - len;
@@ -370,14 +379,14 @@ __device__ __forceinline__ void search_single(const KnnArgs& a, double qx, doubl
             }
         }
         const Ent kth = ent_shfl(best, fetch - 1);
-        if (certified(a, b, kth, qx, qy, qz)) return;
+        if (certified(wg, b, kth, qx, qy, qz)) return;
         if (r >= a.tune.ring_max) {
             incomplete = true;
             return;
         }
         int rn = r + 1;
         if (kth.d != kInfD) {
-            const double rr = ceil((sqrt(ent_d2(kth)) + a.margin) / cv.g.c);
+            const double rr = ceil((sqrt(ent_d2(kth)) + wg.margin) / wg.g.c);
             if (rr > rn) rn = rr > 1e6 ? 1000000 : static_cast<int>(rr);
         } else {
             rn = max(rn, 2 * r);
@@ -389,13 +398,13 @@ __device__ __forceinline__ void search_single(const KnnArgs& a, double qx, doubl
 // Tie test and hand-off of one query's result: in neighbor-list mode the kept
 // ids go straight to global memory; in feature mode the kept sorted positions
 // are parked in LDS for the covariance pass.
-__device__ __forceinline__ void finish_query(const KnnArgs& a, const Ent& best,
+__device__ __forceinline__ void finish_query(const KnnArgs& a, const WaveGrid& wg, const Ent& best,
                                              bool incomplete, uint64_t s, int qslot,
                                              int lane, uint32_t* nbr, uint8_t& my_flags)
 {
-    const int fetch = a.prog.fetch;
-    const int k = a.prog.k;
-    const int stride = a.prog.stride;
+    const int fetch = wg.fetch;
+    const int k = wg.k;
+    const int stride = wg.stride;
     // Boundary tie: the fetch-th and (fetch+1)-th distances are equal.  With
     // stride > 1 every adjacent equal pair inside the fetched list can change
     // which ids are kept, so any such pair flags the row.
@@ -428,11 +437,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
 {
     extern __shared__ uint32_t s_lds[];
     const int lane = __lane_id();
-    const int w = threadIdx.x / kWave;
+    // Wave-uniform by construction; said so, so that per-wave tables load
+    // into scalar registers.
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kWave));
     const int Q = a_in.q_per_wave;
     const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint64_t task = static_cast<uint64_t>(lb) * kWaves + w;
-    KnnArgs a = a_in;
+    const KnnArgs& a = a_in;
+    WaveGrid wg;
     uint64_t q0;
     int nq;
     if (kTiled) {
@@ -440,23 +452,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
         if (task >= a_in.ntasks) return;
         const WaveTask tk = a_in.tasks[task];
         const TileInfo& ti = a_in.tiles[tk.tile];
-        a.cv.g = ti.g;
-        a.cv.colstart = a_in.cv.colstart + ti.colbase;
-        a.prog = ti.prog;
-        a.margin = ti.margin;
+        wg.g = ti.g;
+        wg.colstart = a_in.cv.colstart + ti.colbase;
+        wg.margin = ti.margin;
+        wg.fetch = ti.prog.fetch;
+        wg.k = ti.prog.k;
+        wg.stride = ti.prog.stride;
         q0 = tk.q0;
         nq = static_cast<int>(tk.nq);
     } else {
+        wg.g = a_in.cv.g;
+        wg.colstart = a_in.cv.colstart;
+        wg.margin = a_in.margin;
+        wg.fetch = a_in.prog.fetch;
+        wg.k = a_in.prog.k;
+        wg.stride = a_in.prog.stride;
         const uint64_t n = a_in.cv.n;
         q0 = task * Q;
         if (q0 >= n) return;
         nq = static_cast<int>(n - q0 < static_cast<uint64_t>(Q) ? n - q0 : Q);
     }
     const CloudView& cv = a.cv;
-    const GridParams& g = cv.g;
-    const uint64_t n = cv.n;  // rows of the whole batch (plane stride of cov)
-    const int fetch = a.prog.fetch;
-    const int k = a.prog.k;
+    const GridParams& g = wg.g;
+    const int fetch = wg.fetch;
+    const int k = wg.k;
     // LDS: per-wave neighbor lists (features mode), then per wave a 64-entry
     // compaction scratch of 16 bytes per entry and a 64-entry pending list.
     const size_t list_words = a.mode == 1 ? 0 : static_cast<size_t>(kWaves) * Q * a.lds_stride;
@@ -508,7 +527,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
             const int bw = level ? 2 : 1;
             box = Box{bx0 - rg, bx0 + bw - 1 + rg, by0 - rg, by0 + bw - 1 + rg, zmin - rg,
                       zmax + rg};
-            box_ranges(a, box, 0, lane, lo, len);
+            box_ranges(a, wg, box, 0, lane, lo, len);
             const uint32_t incl = wave_incl_scan_u32(len, lane);
             tot = __shfl(incl, 63);
             excl = incl - len;
@@ -594,8 +613,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                 pqx = qx;
                 pqy = qy;
                 pqz = qz;
-                done = certified(a, box, ent_shfl(best, fetch - 1), qx, qy, qz);
-                if (done) finish_query(a, best, false, s, qslot, lane, nbr, my_flags);
+                done = certified(wg, box, ent_shfl(best, fetch - 1), qx, qy, qz);
+                if (done) finish_query(a, wg, best, false, s, qslot, lane, nbr, my_flags);
             }
             if (!done) {
                 // Searched alone after the groups, so the group's register
@@ -620,9 +639,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
         const int qcz = static_cast<int>(cell_coord(qz, g.oz, g.inv_cz, g.gz));
         Ent best;
         bool incomplete = false;
-        search_single(a, qx, qy, qz, qcx, qcy, qcz, static_cast<int>(word >> 8), lane,
+        search_single(a, wg, qx, qy, qz, qcx, qcy, qcz, static_cast<int>(word >> 8), lane,
                       scratch, best, incomplete, cnt);
-        finish_query(a, best, incomplete, s, qslot, lane, nbr, my_flags);
+        finish_query(a, wg, best, incomplete, s, qslot, lane, nbr, my_flags);
     }
     if (a.out.counters && lane == 0) {
         atomicAdd(a.out.counters + 0, static_cast<unsigned long long>(cnt[0]));
@@ -665,8 +684,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
         c[5] += dz * dz;
     }
     const double den = static_cast<double>(k - 1);
-    for (int i = 0; i < 6; ++i) a.out.cov[static_cast<uint64_t>(i) * n + s] = c[i] / den;
-    a.out.flags[s] = my_flags;
+    double* rec = a.out.cov + kCovRecord * s;
+    for (int i = 0; i < 6; ++i) rec[i] = c[i] / den;
+    reinterpret_cast<uint64_t*>(rec)[6] = my_flags;
     if (my_flags) {
         uint32_t slot = atomicAdd(a.out.nflagged, 1u);
         a.out.flagged[slot] = static_cast<uint32_t>(s);
@@ -691,21 +711,21 @@ __device__ __forceinline__ uint32_t tile_of(const uint64_t* __restrict__ starts,
 }
 
 __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov,
-                                                  const uint8_t* __restrict__ flags,
-                                                  const uint32_t* __restrict__ perm,
+                                                  const uint32_t* __restrict__ inv,
                                                   uint64_t n, FusedProgram prog,
                                                   KnnOutputs out, const TileInfo* __restrict__ tiles,
                                                   const uint64_t* __restrict__ starts,
                                                   uint32_t ntiles)
 {
-    const uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (s >= n) return;
+    const uint64_t qid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (qid >= n) return;
+    const uint64_t s = inv[qid];  // the row's sorted position: one record read
     // The neighborhood size of the row's tile (stages are the same in every tile).
     const int k = tiles ? tiles[tile_of(starts, ntiles, s)].prog.k : prog.k;
+    const double* rec = cov + kCovRecord * s;
     double c[6];
-    for (int i = 0; i < 6; ++i) c[i] = cov[static_cast<uint64_t>(i) * n + s];
-    const uint8_t f = flags[s];
-    const uint32_t qid = perm[s];
+    for (int i = 0; i < 6; ++i) c[i] = rec[i];
+    const uint8_t f = static_cast<uint8_t>(reinterpret_cast<const uint64_t*>(rec)[6]);
     const bool zero = cov_is_zero(c);
     double ev[3] = {0, 0, 0}, vec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int info = 0;
@@ -718,6 +738,13 @@ __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov
             for (int cc = 0; cc < sp.ncols; ++cc) out.col[sp.col_offset + cc][qid] = vals[cc];
         out.status[si][qid] = st | f;
     }
+}
+
+__global__ __launch_bounds__(256) void k_invert(const uint32_t* __restrict__ perm, uint64_t n,
+                                                uint32_t* __restrict__ inv)
+{
+    const uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (s < n) inv[perm[s]] = static_cast<uint32_t>(s);
 }
 
 __global__ __launch_bounds__(256) void k_eigen_batch(const double* __restrict__ cov,
@@ -775,12 +802,19 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
 }
 
 void launch_features(const CloudView& cv, const FusedProgram& prog,
-                     const KnnOutputs& out, hipStream_t s, const TileTables* tt)
+                     const KnnOutputs& out, const uint32_t* inv, hipStream_t s,
+                     const TileTables* tt)
 {
     if (cv.n == 0) return;
-    hipLaunchKernelGGL(k_features, dim3(div_up(cv.n, 256)), dim3(256), 0, s, out.cov,
-                       out.flags, cv.perm, cv.n, prog, out, tt ? tt->tiles : nullptr,
-                       tt ? tt->starts : nullptr, tt ? tt->ntiles : 0u);
+    hipLaunchKernelGGL(k_features, dim3(div_up(cv.n, 256)), dim3(256), 0, s, out.cov, inv,
+                       cv.n, prog, out, tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
+                       tt ? tt->ntiles : 0u);
+}
+
+void launch_invert(const uint32_t* perm, uint64_t n, uint32_t* inv, hipStream_t s)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_invert, dim3(div_up(n, 256)), dim3(256), 0, s, perm, n, inv);
 }
 
 void launch_eigen_batch(const double* cov, uint64_t m, double* evals,

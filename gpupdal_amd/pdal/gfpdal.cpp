@@ -347,10 +347,10 @@ void Stage::throwError(const std::string& s) const
 void Stage::prepare(PointTable& table)
 {
     if (m_input) m_input->prepare(table);
-    ProgramArgs args;
+    m_args.reset(new ProgramArgs);
     try {
-        addArgs(args);
-        args.parse(m_options);
+        addArgs(*m_args);
+        m_args->parse(m_options);
     } catch (const ArgError& e) {
         throw pdal_error(getName() + ": " + e.msg);
     }

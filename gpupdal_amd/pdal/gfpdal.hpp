@@ -186,6 +186,9 @@ protected:
 
 private:
     Options m_options;
+    // The stage's arguments live as long as the stage: stages keep pointers
+    // to them (to ask later whether an option was set).
+    std::unique_ptr<ProgramArgs> m_args;
     Log m_log;
     bool m_prepared = false;
 };

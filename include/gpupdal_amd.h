@@ -93,7 +93,8 @@ typedef struct gf_stats {
     double ms_index;          /* grid build, device time */
     double ms_search;         /* kNN search + covariance kernel, device time */
     double ms_features;       /* eigensolve + stage columns kernel */
-    double ms_exact;          /* exact KD3 build + re-solve, device time */
+    double ms_exact;          /* exact KD3 re-solve, plus the part of the tree build
+                                 not hidden behind the search, device time */
     double cell;              /* column edge used (m) */
     double cell_z;            /* z cell height used (m) */
     uint32_t grid_x, grid_y, grid_z;
@@ -102,6 +103,9 @@ typedef struct gf_stats {
     uint64_t search_passes;   /* box searches (1 per row when the first box certifies) */
     uint64_t candidates;      /* candidate points evaluated */
     uint64_t merges;          /* wave-wide sorted merges */
+    double ms_tree;           /* exact KD3 tree build on its own stream (it overlaps
+                                 the search; 0 when the tree was cached) */
+    double ms_total;          /* the call's device time, index build to features */
 } gf_stats;
 
 /* ---- context: replaces pdg::CudaMemoryResource + stream ownership
