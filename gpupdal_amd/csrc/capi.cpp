@@ -294,7 +294,7 @@ void Context::build_forest(KdBuffers& b, const double* x, const double* y, const
         kd_s1.ensure((n + 1) * 4);
         kd_s2.ensure((n + 1) * 4);
         kd_tmp.ensure((n + 1) * 4);
-        kd_lists.ensure((static_cast<uint64_t>(lcap) * 5 + 2 + 8) * 4);
+        kd_lists.ensure((static_cast<uint64_t>(lcap) * 5 + 2 + kKdHeaderWords) * 4);
         scan_tmp.ensure(scan_scratch_words(n + 1) * 4);
         KdView t{};
         t.vind = b.vind.as<uint32_t>();
@@ -324,7 +324,7 @@ void Context::build_forest(KdBuffers& b, const double* x, const double* y, const
         w.sums = scan_tmp.as<uint32_t>();
         uint32_t* L = kd_lists.as<uint32_t>();
         w.hdr = L;
-        w.act[0] = L + 8;
+        w.act[0] = L + kKdHeaderWords;
         w.act[1] = w.act[0] + lcap;
         w.off[0] = w.act[1] + lcap;
         w.off[1] = w.off[0] + lcap + 1;

@@ -177,6 +177,8 @@ struct KdTuning {
     uint32_t minmax_lds_nodes;  // levels up to this many big nodes stage extents in LDS
 };
 
+// Words of the exact-tree build header at the front of the node-list buffer.
+constexpr uint32_t kKdHeaderWords = 16;
 uint32_t kd_node_capacity(uint64_t n);
 uint32_t kd_list_capacity(uint64_t n);
 // Sends every row to the exact search: used when the cloud holds fewer

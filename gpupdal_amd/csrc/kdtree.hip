@@ -69,7 +69,14 @@ constexpr uint32_t kPredTile = kPredThreads * kPredItems;
 constexpr int kChildThreads = 1024;
 constexpr uint32_t kMinMaxLds = 512;      // big children staged per block in LDS
 
-enum : int { kHdrA = 0, kHdrP = 1, kHdrSmall = 2, kHdrOverflow = 3, kHdrNodes = 4 };
+// Build header (device): the last level's big-node count and point count,
+// subtree list length, overflow code, node count, and per level parity
+// (A, P) at kHdrLvl + 2 * parity, so levels run back to back without the
+// host reading their sizes.
+enum : int {
+    kHdrA = 0, kHdrP = 1, kHdrSmall = 2, kHdrOverflow = 3, kHdrNodes = 4, kHdrLvl = 8,
+    kHdrWords = static_cast<int>(kKdHeaderWords)
+};
 enum : uint32_t { kOvfNodes = 1, kOvfLists = 2, kOvfSubtree = 3 };
 
 // Exclusive scan of one value per thread across a block of NT threads.
@@ -260,6 +267,9 @@ __global__ __launch_bounds__(kChildThreads) void k_kd_roots(KdView t, const uint
         w.hdr[kHdrP] = psum;
         w.hdr[kHdrSmall] = nsmall;
         w.hdr[kHdrNodes] = T;
+        const bool ovf = w.hdr[kHdrOverflow] != 0;
+        w.hdr[kHdrLvl] = ovf ? 0u : nbig;
+        w.hdr[kHdrLvl + 1] = ovf ? 0u : psum;
     }
 }
 
@@ -278,15 +288,10 @@ __device__ __forceinline__ uint32_t act_find(const uint32_t* __restrict__ off, u
     return lo;
 }
 
-// One thread per big node: parent's div from this node's extent, then the
-// split (big nodes always split).
-__global__ __launch_bounds__(kKdThreads) void k_kd_big_split(KdView t,
-                                                             const uint32_t* __restrict__ act,
-                                                             uint32_t A)
+// The parent's div from node nd's extent, then nd's split (big nodes always
+// split).
+__device__ __forceinline__ void kd_split_node(const KdView& t, uint32_t nd)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= A) return;
-    const uint32_t nd = act[k];
     double mn[3], mx[3];
     for (int d = 0; d < 3; ++d) {
         mn[d] = dec_order(t.mn[3 * nd + d]);
@@ -298,6 +303,17 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_big_split(KdView t,
     kd_choose_split(t.box + 6 * static_cast<size_t>(nd), mn, mx, f, cv);
     t.feat[nd] = f;
     t.cutval[nd] = cv;
+}
+
+// One thread per big node of the level (A read from the level's header).
+__global__ __launch_bounds__(kKdThreads) void k_kd_big_split(KdView t,
+                                                             const uint32_t* __restrict__ act,
+                                                             const uint32_t* __restrict__ lvl)
+{
+    const uint32_t A = lvl[0];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < A;
+         k += gridDim.x * blockDim.x)
+        kd_split_node(t, act[k]);
 }
 
 // Active index -> (node, position).
@@ -338,8 +354,10 @@ __device__ __forceinline__ bool kd_pred(const KdView& t, int pass, const ActPos&
 
 __global__ __launch_bounds__(kPredThreads) void k_kd_pred_reduce(
     KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
-    uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, uint32_t* __restrict__ sums)
+    const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ s1,
+    uint32_t* __restrict__ sums)
 {
+    const uint32_t A = lvl[0], P = lvl[1];
     const uint32_t base = blockIdx.x * kPredTile;
     const uint32_t kb = act_find(off, A, base);  // uniform: one search per block
     uint32_t acc = 0;
@@ -356,9 +374,10 @@ __global__ __launch_bounds__(kPredThreads) void k_kd_pred_reduce(
 // out (P + 1 entries; the predicate is 0 at P).
 __global__ __launch_bounds__(kPredThreads) void k_kd_pred_down(
     KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
-    uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, const uint32_t* __restrict__ sums,
-    uint32_t* __restrict__ out)
+    const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ s1,
+    const uint32_t* __restrict__ sums, uint32_t* __restrict__ out)
 {
+    const uint32_t A = lvl[0], P = lvl[1];
     __shared__ uint32_t s[kPredTile];
     const uint32_t base = blockIdx.x * kPredTile;
     const uint32_t kb = act_find(off, A, base < P ? base : (P ? P - 1 : 0));
@@ -403,9 +422,10 @@ __device__ __forceinline__ void kd_pass_bounds(const KdView& t, int pass, const 
 // rank; misplaced left elements then swap with their partner.
 __global__ __launch_bounds__(kKdThreads) void k_kd_publish(
     KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
-    uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, const uint32_t* __restrict__ sc,
-    uint32_t* __restrict__ tmp)
+    const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ s1,
+    const uint32_t* __restrict__ sc, uint32_t* __restrict__ tmp)
 {
+    const uint32_t A = lvl[0], P = lvl[1];
     const uint32_t b0 = blockIdx.x * blockDim.x;
     const uint32_t kb = act_find(off, A, b0);
     const uint32_t ti = b0 + threadIdx.x;
@@ -419,9 +439,10 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_publish(
 
 __global__ __launch_bounds__(kKdThreads) void k_kd_swap(
     KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
-    uint32_t A, uint32_t P, const uint32_t* __restrict__ s1, const uint32_t* __restrict__ sc,
-    const uint32_t* __restrict__ tmp)
+    const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ s1,
+    const uint32_t* __restrict__ sc, const uint32_t* __restrict__ tmp)
 {
+    const uint32_t A = lvl[0], P = lvl[1];
     const uint32_t b0 = blockIdx.x * blockDim.x;
     const uint32_t kb = act_find(off, A, b0);
     const uint32_t ti = b0 + threadIdx.x;
@@ -452,10 +473,11 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_swap(
 // prefix, and the small children appended to the subtree list -- both in
 // child order, so the tree's numbering is deterministic.
 __global__ __launch_bounds__(kChildThreads) void k_kd_big_children(
-    KdView t, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off, uint32_t A,
+    KdView t, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ s1, const uint32_t* __restrict__ s2, KdScratch w, int nxt,
     uint32_t sub)
 {
+    const uint32_t A = w.hdr[kHdrLvl + 2 * (nxt ^ 1)];
     const uint32_t base = w.hdr[kHdrNodes];
     const uint32_t small0 = w.hdr[kHdrSmall];
     uint32_t nbig = 0, nsmall = 0, pnext = 0;
@@ -522,25 +544,76 @@ __global__ __launch_bounds__(kChildThreads) void k_kd_big_children(
         nsmall += ts;
         pnext += tz;
     }
+    __syncthreads();  // every thread's overflow store is visible below
     if (threadIdx.x == 0) {
         if (nbig <= w.list_cap) w.off[nxt][nbig] = pnext;
         w.hdr[kHdrA] = nbig;
         w.hdr[kHdrP] = pnext;
         w.hdr[kHdrSmall] = small0 + nsmall;
         w.hdr[kHdrNodes] = base + 2 * A;
+        // After an overflow the next levels see no work (the build is redone).
+        const bool ovf = w.hdr[kHdrOverflow] != 0;
+        w.hdr[kHdrLvl + 2 * nxt] = ovf ? 0u : nbig;
+        w.hdr[kHdrLvl + 2 * nxt + 1] = ovf ? 0u : pnext;
     }
 }
 
-// Extents of the next level's big nodes over this level's active points:
-// a segmented reduction per wave (children are contiguous), then per run an
-// atomic into a per-block LDS table when the next level has few big nodes,
-// else into global memory.
+// Extents of the next level's big nodes over this level's active points.
+// Each wave walks a contiguous chunk of kMinMaxChunk active indices (children
+// are contiguous runs of them), accumulating each lane's points while the
+// lane's child stays the same; the lanes' accumulators are combined by a
+// segmented reduction only at the chunk's end and where a lane crosses into
+// another child, then one atomic per run goes to a per-block LDS table (few
+// big children) or to global memory.
+constexpr uint32_t kMinMaxIter = 16;
+constexpr uint32_t kMinMaxChunk = 64 * kMinMaxIter;
+
+__device__ __forceinline__ void minmax_flush(uint32_t j, unsigned long long (&lo)[3],
+                                             unsigned long long (&hi)[3], int lane,
+                                             bool use_lds, unsigned long long* s_lo,
+                                             unsigned long long* s_hi, const KdView& t,
+                                             const uint32_t* act_next)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t oj = __shfl_down(j, o);
+        const bool same = lane + o < 64 && oj == j;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const unsigned long long ol = __shfl_down(lo[d], o);
+            const unsigned long long oh = __shfl_down(hi[d], o);
+            if (same) {
+                lo[d] = ol < lo[d] ? ol : lo[d];
+                hi[d] = oh > hi[d] ? oh : hi[d];
+            }
+        }
+    }
+    const uint32_t pj = __shfl_up(j, 1);
+    if (j != ~0u && (lane == 0 || pj != j)) {
+        if (use_lds) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(&s_lo[j * 3 + d], lo[d]);
+                atomicMax(&s_hi[j * 3 + d], hi[d]);
+            }
+        } else {
+            const uint32_t nd = act_next[j];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(&t.mn[3 * nd + d], lo[d]);
+                atomicMax(&t.mx[3 * nd + d], hi[d]);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(kKdThreads) void k_kd_big_minmax(
-    KdView t, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off, uint32_t A,
-    uint32_t P, const uint32_t* __restrict__ hdr, const uint32_t* __restrict__ act_next,
-    uint32_t lds_max)
+    KdView t, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
+    const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ hdr,
+    const uint32_t* __restrict__ act_next, uint32_t lds_max)
 {
     __shared__ unsigned long long s_lo[kMinMaxLds * 3], s_hi[kMinMaxLds * 3];
+    const uint32_t A = lvl[0], P = lvl[1];
     const uint32_t An = hdr[kHdrA];
     // After an overflow the children are incomplete; the build is redone.
     if (An == 0 || hdr[kHdrOverflow]) return;
@@ -553,49 +626,49 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_big_minmax(
         __syncthreads();
     }
     const int lane = __lane_id();
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < P; b0 += gridDim.x * blockDim.x) {
-        const uint32_t kb = act_find(off, A, b0);
-        const uint32_t ti = b0 + threadIdx.x;
-        uint32_t j = ~0u;
+    const uint32_t wpb = blockDim.x / 64;
+    const uint32_t wave = blockIdx.x * wpb + threadIdx.x / 64;
+    const uint32_t nwaves = gridDim.x * wpb;
+    const uint32_t nchunks = (P + kMinMaxChunk - 1) / kMinMaxChunk;
+    for (uint32_t c = wave; c < nchunks; c += nwaves) {
+        const uint32_t c0 = c * kMinMaxChunk;
+        uint32_t k = act_find(off, A, c0);  // this lane's node, walked forward
+        uint32_t jc = ~0u;
         unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
-        if (ti < P) {
-            const ActPos ap = act_pos(t, act, off, A, kb, ti);
-            const uint32_t ch = ap.p < t.split[ap.nd] ? static_cast<uint32_t>(t.c1[ap.nd])
-                                                      : static_cast<uint32_t>(t.c2[ap.nd]);
-            j = t.aux[ch];
-            if (j != ~0u) {
-                const double v[3] = {t.px[ap.p], t.py[ap.p], t.pz[ap.p]};
-                for (int d = 0; d < 3; ++d) lo[d] = hi[d] = enc_order(v[d]);
+        for (uint32_t r = 0; r < kMinMaxIter; ++r) {
+            const uint32_t ti = c0 + r * 64 + lane;
+            uint32_t j = ~0u;
+            uint32_t p = 0;
+            if (ti < P) {
+                while (k + 1 < A && off[k + 1] <= ti) ++k;
+                const uint32_t nd = act[k];
+                p = t.nl[nd] + (ti - off[k]);
+                const uint32_t ch = p < t.split[nd] ? static_cast<uint32_t>(t.c1[nd])
+                                                    : static_cast<uint32_t>(t.c2[nd]);
+                j = t.aux[ch];
             }
-        }
+            // A lane entering another child first hands its run to the wave.
+            if (__ballot(jc != ~0u && j != jc)) {
+                minmax_flush(jc, lo, hi, lane, use_lds, s_lo, s_hi, t, act_next);
+                jc = ~0u;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t oj = __shfl_down(j, o);
-            const bool same = lane + o < 64 && oj == j;
-            for (int d = 0; d < 3; ++d) {
-                const unsigned long long ol = __shfl_down(lo[d], o);
-                const unsigned long long oh = __shfl_down(hi[d], o);
-                if (same) {
-                    lo[d] = ol < lo[d] ? ol : lo[d];
-                    hi[d] = oh > hi[d] ? oh : hi[d];
+                for (int d = 0; d < 3; ++d) {
+                    lo[d] = ~0ull;
+                    hi[d] = 0ull;
+                }
+            }
+            if (j != ~0u) {
+                jc = j;
+                const unsigned long long v[3] = {enc_order(t.px[p]), enc_order(t.py[p]),
+                                                 enc_order(t.pz[p])};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    lo[d] = v[d] < lo[d] ? v[d] : lo[d];
+                    hi[d] = v[d] > hi[d] ? v[d] : hi[d];
                 }
             }
         }
-        const uint32_t pj = __shfl_up(j, 1);
-        if (j != ~0u && (lane == 0 || pj != j)) {
-            if (use_lds) {
-                for (int d = 0; d < 3; ++d) {
-                    atomicMin(&s_lo[j * 3 + d], lo[d]);
-                    atomicMax(&s_hi[j * 3 + d], hi[d]);
-                }
-            } else {
-                const uint32_t nd = act_next[j];
-                for (int d = 0; d < 3; ++d) {
-                    atomicMin(&t.mn[3 * nd + d], lo[d]);
-                    atomicMax(&t.mx[3 * nd + d], hi[d]);
-                }
-            }
-        }
+        if (__ballot(jc != ~0u)) minmax_flush(jc, lo, hi, lane, use_lds, s_lo, s_hi, t, act_next);
     }
     if (use_lds) {
         __syncthreads();
@@ -1165,44 +1238,57 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
                                             kSub);
     hipLaunchKernelGGL(k_kd_init, dim3(div_up(n, kKdThreads)), dim3(kKdThreads), 0, s, x, y, z,
                        static_cast<uint32_t>(n), t);
-    launch_fill_u32(w.hdr, 8, 0u, s);
+    launch_fill_u32(w.hdr, kHdrWords, 0u, s);
     hipLaunchKernelGGL(k_kd_roots, dim3(1), dim3(kChildThreads), 0, s, t, starts, ntiles,
                        bounds, w, sub);
-    uint32_t h[8];
+    uint32_t h[kHdrWords];
     GF_HIP(hipMemcpyAsync(h, w.hdr, sizeof(h), hipMemcpyDeviceToHost, s));
     GF_HIP(hipStreamSynchronize(s));
     if (h[kHdrOverflow]) return h[kHdrOverflow];
     uint32_t A = h[kHdrA], P = h[kHdrP], nsmall = h[kHdrSmall];
     int cur = 0;
+    // Levels are queued kLevelsPerRead at a time; each kernel reads its
+    // level's (A, P) from the header.  Grids are sized from bounds known on
+    // the host: P never grows from one level to the next, A at most doubles.
+    constexpr int kLevelsPerRead = 4;
     while (A) {
-        hipLaunchKernelGGL(k_kd_big_split, dim3(div_up(A, kKdThreads)), dim3(kKdThreads), 0, s,
-                           t, w.act[cur], A);
-        for (int pass = 1; pass <= 2; ++pass) {
-            uint32_t* sc = pass == 1 ? w.s1 : w.s2;
-            const uint32_t nb = div_up(static_cast<uint64_t>(P) + 1, kPredTile);
-            hipLaunchKernelGGL(k_kd_pred_reduce, dim3(nb), dim3(kPredThreads), 0, s, t, pass,
-                               w.act[cur], w.off[cur], A, P, w.s1, w.sums);
-            launch_scan_top(w.sums, nb, s);
-            hipLaunchKernelGGL(k_kd_pred_down, dim3(nb), dim3(kPredThreads), 0, s, t, pass,
-                               w.act[cur], w.off[cur], A, P, w.s1, w.sums, sc);
-            hipLaunchKernelGGL(k_kd_publish, dim3(div_up(P, kKdThreads)), dim3(kKdThreads), 0,
-                               s, t, pass, w.act[cur], w.off[cur], A, P, w.s1, sc, w.tmp);
-            hipLaunchKernelGGL(k_kd_swap, dim3(div_up(P, kKdThreads)), dim3(kKdThreads), 0, s,
-                               t, pass, w.act[cur], w.off[cur], A, P, w.s1, sc, w.tmp);
+        uint64_t Ab = A;
+        const uint32_t Pb = P;
+        for (int l = 0; l < kLevelsPerRead; ++l) {
+            const uint32_t* lvl = w.hdr + kHdrLvl + 2 * cur;
+            const uint32_t gsplit = static_cast<uint32_t>(
+                std::min<uint64_t>(div_up(std::min<uint64_t>(Ab, w.list_cap), kKdThreads), 64));
+            hipLaunchKernelGGL(k_kd_big_split, dim3(gsplit), dim3(kKdThreads), 0, s, t,
+                               w.act[cur], lvl);
+            for (int pass = 1; pass <= 2; ++pass) {
+                uint32_t* sc = pass == 1 ? w.s1 : w.s2;
+                const uint32_t nb = div_up(static_cast<uint64_t>(Pb) + 1, kPredTile);
+                hipLaunchKernelGGL(k_kd_pred_reduce, dim3(nb), dim3(kPredThreads), 0, s, t,
+                                   pass, w.act[cur], w.off[cur], lvl, w.s1, w.sums);
+                launch_scan_top(w.sums, nb, s);
+                hipLaunchKernelGGL(k_kd_pred_down, dim3(nb), dim3(kPredThreads), 0, s, t,
+                                   pass, w.act[cur], w.off[cur], lvl, w.s1, w.sums, sc);
+                hipLaunchKernelGGL(k_kd_publish, dim3(div_up(Pb, kKdThreads)), dim3(kKdThreads),
+                                   0, s, t, pass, w.act[cur], w.off[cur], lvl, w.s1, sc, w.tmp);
+                hipLaunchKernelGGL(k_kd_swap, dim3(div_up(Pb, kKdThreads)), dim3(kKdThreads), 0,
+                                   s, t, pass, w.act[cur], w.off[cur], lvl, w.s1, sc, w.tmp);
+            }
+            hipLaunchKernelGGL(k_kd_big_children, dim3(1), dim3(kChildThreads), 0, s, t,
+                               w.act[cur], w.off[cur], w.s1, w.s2, w, cur ^ 1, sub);
+            hipLaunchKernelGGL(
+                k_kd_big_minmax,
+                dim3(std::min<uint32_t>(div_up(Pb, kMinMaxChunk * (kKdThreads / 64)), 1024)),
+                dim3(kKdThreads), 0, s, t, w.act[cur], w.off[cur], lvl, w.hdr, w.act[cur ^ 1],
+                kt.minmax_lds_nodes);
+            cur ^= 1;
+            Ab *= 2;
         }
-        hipLaunchKernelGGL(k_kd_big_children, dim3(1), dim3(kChildThreads), 0, s, t,
-                           w.act[cur], w.off[cur], A, w.s1, w.s2, w, cur ^ 1, sub);
-        hipLaunchKernelGGL(k_kd_big_minmax,
-                           dim3(std::min<uint32_t>(div_up(P, kKdThreads), 2048)),
-                           dim3(kKdThreads), 0, s, t, w.act[cur], w.off[cur], A, P, w.hdr,
-                           w.act[cur ^ 1], kt.minmax_lds_nodes);
         GF_HIP(hipMemcpyAsync(h, w.hdr, sizeof(h), hipMemcpyDeviceToHost, s));
         GF_HIP(hipStreamSynchronize(s));
         if (h[kHdrOverflow]) return h[kHdrOverflow];
-        A = h[kHdrA];
-        P = h[kHdrP];
+        A = h[kHdrLvl + 2 * cur];
+        P = h[kHdrLvl + 2 * cur + 1];
         nsmall = h[kHdrSmall];
-        cur ^= 1;
     }
     if (nsmall) {
         hipLaunchKernelGGL(k_kd_subtree, dim3(nsmall), dim3(kSubThreads), 0, s, t, w.small,

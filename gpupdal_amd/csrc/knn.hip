@@ -540,6 +540,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                 const double rk = sqrt(static_cast<double>(need) * area /
                                        (3.141592653589793 * static_cast<double>(tot)));
                 est = static_cast<int>(ceil(1.1 * rk / g.c));
+                // rg <= 3 keeps the box within 64 columns ((2 + 2 rg)^2),
+                // the one box_ranges chunk this path reads.
                 est = est < 1 ? 1 : (est > 3 ? 3 : est);
             }
             if (tot > kRegCand) {
