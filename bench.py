@@ -34,8 +34,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic bytes per point of the search kernel (k_knn): the query's FP64
-# xyz (24 B) read once, its FP64 covariance (48 B) and row flag (1 B) written.
-SEARCH_BYTES_PER_POINT = 73.0
+# xyz (24 B) and grid key (4 B) read once, its covariance record written (six
+# FP64 entries and the row-flags word, 56 B).
+SEARCH_BYTES_PER_POINT = 84.0
+# HBM traffic of one k_knn launch on the bench workload (4,000,000 points,
+# knn=16), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this
+# script (KiB per dispatch), each scaled by the same counter on a 1 GiB copy
+# kernel in the same runs: FETCH_SIZE reads 0.5 of the copied bytes there
+# (factor 2.0), WRITE_SIZE 1.0 (factor 1.0).  The runs, commands and raw
+# counter values are in profiles/k_knn_plain_4M_knn16_traffic.md.
+MEASURED_TRAFFIC = {"points": 4_000_000, "knn": 16, "fetch_kib": 73566.3,
+                    "write_kib": 251414.2, "fetch_factor": 2.0, "write_factor": 1.0}
 # FP64 flops per candidate distance: 3 sub, 3 mul, 2 add.
 FLOPS_PER_CANDIDATE = 8.0
 HBM_PEAK_GBS = 8000.0
@@ -59,6 +68,7 @@ def gpu_leg(args):
     # library's streams and read after each call's final synchronisation,
     # so they add no wait between kernels.
     ctx.set_tuning("timing", 1)
+    ctx.set_tuning("kd_overlap_min", args.kd_overlap_min)
     for _ in range(args.warmup):
         cloud.invalidate()
         cloud.run_device(stages, cols, status)
@@ -85,7 +95,7 @@ def gpu_leg(args):
             cloud.run_device(stages, cols, status)
         ctx.sync()
         serial = (time.perf_counter() - s0) * 1e3 / 3
-        ctx.set_tuning("kd_overlap_min", 131072)
+        ctx.set_tuning("kd_overlap_min", args.kd_overlap_min)
 
     # Search counters in a separate pass.
     ctx.set_tuning("collect_stats", 1)
@@ -98,6 +108,11 @@ def gpu_leg(args):
         return float(np.mean([p[key] for p in per]))
 
     ms_search = mean("ms_search")
+    traffic = None
+    mt = MEASURED_TRAFFIC
+    if n == mt["points"] and knn == mt["knn"]:
+        traffic = 1024.0 * (mt["fetch_kib"] * mt["fetch_factor"] +
+                            mt["write_kib"] * mt["write_factor"])
     achieved = SEARCH_BYTES_PER_POINT * n / (ms_search / 1e3) / 1e9
     cand_per_s = counters["candidates"] / (ms_search / 1e3)
     fp64 = cand_per_s * FLOPS_PER_CANDIDATE / 1e12
@@ -105,7 +120,7 @@ def gpu_leg(args):
     out = dict(
         value=n / (ms / 1e3), ms_per_step=ms,
         roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                   "kernel": "k_knn (kNN search + FP64 covariance)",
                   "bytes_per_point": SEARCH_BYTES_PER_POINT,
                   "kernel_ms": ms_search},
@@ -302,17 +317,19 @@ def precision_leg(args):
     st = ctx.stats()
     cl.close()
     ctx.close()
+    threads = min(16, os.cpu_count() or 1)
+    print(f"precision: GPU run done ({ms:.0f} ms); host KD3 lists", file=sys.stderr, flush=True)
     orc = ol.Oracle()
     tree = orc.kd3(x, y, z)
-    nbr = tree.neighbors_all(knn + 1, threads=os.cpu_count() or 1)
+    nbr = tree.neighbors_all(knn + 1, threads=threads)
     del tree
     report = {"points": n, "knn": knn, "roof_frac": 0.05, "wire_frac": 0.02,
               "gpu_ms_host_io": ms, "tie_rows": int(st["tie_rows"]),
               "incomplete_rows": int(st["incomplete_rows"]), "columns": {}}
     for stage_cols, stat, kind in ((ev, evs, ol.STAGE_EIGENVALUES), (cf, cfs, ol.STAGE_COVFEAT)):
         ost = ol.stage(kind, knn)
-        tcols, tstat, gap, ratio = orc.truth_from_lists(x, y, z, nbr, ost,
-                                                        threads=os.cpu_count() or 1,
+        print(f"precision: host truth for stage {kind}", file=sys.stderr, flush=True)
+        tcols, tstat, gap, ratio = orc.truth_from_lists(x, y, z, nbr, ost, threads=threads,
                                                         with_ratio=True)
         ok = ((stat & 0x3C) == 0) & (tstat == 0)
         well = ok & (gap > 1e-3) & (ratio > 1e-10)
@@ -375,6 +392,9 @@ def main():
     ap.add_argument("--precision", action="store_true",
                     help="only the precision-stress report (20M points, roofs, wires)")
     ap.add_argument("--precision-points", type=int, default=20_000_000)
+    ap.add_argument("--kd-overlap-min", type=int, default=131072,
+                    help="clouds of at least this many points build the exact tree beside "
+                         "the search (0: after it)")
     ap.add_argument("--serial", action="store_true",
                     help="also time the step with the exact tree built after the search")
     ap.add_argument("--only-batched", action="store_true",

@@ -333,6 +333,7 @@ void Context::build_forest(KdBuffers& b, const double* x, const double* y, const
         uint32_t nodes = 0;
         KdTuning kt{};
         kt.subtree_points = static_cast<uint32_t>(tune.kd_subtree);
+        kt.wave_node = static_cast<uint32_t>(tune.kd_wave_node);
         kt.minmax_lds_nodes = static_cast<uint32_t>(tune.kd_minmax_lds);
         const uint32_t rc =
             launch_kd_build(x, y, z, n, dstarts, T, dbounds, t, w, kt, &nodes, s);

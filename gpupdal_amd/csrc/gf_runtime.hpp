@@ -67,6 +67,7 @@ struct Tuning {
     int small_col = 0;
     int group_z = 1;
     int kd_subtree = 4096;       // exact tree: subtree-per-workgroup threshold (points)
+    int kd_wave_node = 256;      // exact tree: one-wave subtree threshold (points)
     int kd_minmax_lds = 512;     // exact tree: LDS-staged extents up to this many nodes
     // Clouds (batches) of at least this many points build the exact tree on
     // a side stream during the search, before knowing whether a row needs
@@ -84,6 +85,8 @@ struct Tuning {
         else if (!std::strcmp(key, "group_z") && v >= 0) group_z = static_cast<int>(v);
         else if (!std::strcmp(key, "kd_subtree") && v >= 101 && v <= 4096)
             kd_subtree = static_cast<int>(v);
+        else if (!std::strcmp(key, "kd_wave_node") && v >= 101 && v <= 1024)
+            kd_wave_node = static_cast<int>(v);
         else if (!std::strcmp(key, "kd_minmax_lds") && v >= 0) kd_minmax_lds = static_cast<int>(v);
         else if (!std::strcmp(key, "kd_overlap_min") && v >= 0)
             kd_overlap_min = static_cast<uint64_t>(v);

@@ -62,7 +62,14 @@ __device__ __forceinline__ const double* axis(const KdView& t, int f)
 // with block barriers.
 constexpr uint32_t kSub = 4096;           // small-node capacity (points)
 constexpr int kSubThreads = 1024;
-constexpr int kSubNodes = 128;            // nodes per subtree level (<= 2 * 4096 / 101)
+// Subtree nodes of at most wave_node (<= kWaveNodeMax, >= kLeafMax + 1)
+// points are built by one wave each; bigger ones level by level by the
+// whole workgroup.  Block-level nodes then hold more than kLeafMax points,
+// so a level has fewer than kSub / (kLeafMax + 1) of them.
+constexpr uint32_t kWaveNodeMax = 1024;
+constexpr int kSubNodes = 48;             // block-level nodes per subtree level (> 4096 / 101)
+constexpr int kWaveStack = 12;            // pending nodes of one wave (see wave_build)
+constexpr int kWaveQ = 96;                // nodes handed to the waves per block level
 constexpr int kPredThreads = 256;
 constexpr int kPredItems = 16;
 constexpr uint32_t kPredTile = kPredThreads * kPredItems;
@@ -352,6 +359,43 @@ __device__ __forceinline__ bool kd_pred(const KdView& t, int pass, const ActPos&
     return a.p >= L && v <= cv;
 }
 
+// kd_pred over increasing active indices of one thread: the node's data
+// (range, axis, cut, pass-2 start) stays in registers until the index
+// crosses into the next node, so each item costs one coordinate load.
+struct PredCursor {
+    uint32_t k, t0, t1, lo, L;
+    const double* ax;
+    double cv;
+
+    __device__ __forceinline__ void load(const KdView& t, const uint32_t* act,
+                                         const uint32_t* off, const uint32_t* s1, int pass,
+                                         uint32_t kk)
+    {
+        k = kk;
+        const uint32_t nd = act[k];
+        lo = t.nl[nd];
+        t0 = off[k];
+        t1 = off[k + 1];
+        ax = axis(t, t.feat[nd]);
+        cv = t.cutval[nd];
+        L = pass == 1 ? 0u : lo + (s1[t1] - s1[t0]);
+    }
+
+    __device__ __forceinline__ bool pred(const KdView& t, const uint32_t* act,
+                                         const uint32_t* off, const uint32_t* s1, int pass,
+                                         uint32_t A, uint32_t ti)
+    {
+        if (ti >= t1) {
+            uint32_t kk = k;
+            while (kk + 1 < A && off[kk + 1] <= ti) ++kk;
+            load(t, act, off, s1, pass, kk);
+        }
+        const uint32_t p = lo + (ti - t0);
+        const double v = ax[p];
+        return p >= L && (pass == 1 ? v < cv : v <= cv);
+    }
+};
+
 __global__ __launch_bounds__(kPredThreads) void k_kd_pred_reduce(
     KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ s1,
@@ -359,11 +403,16 @@ __global__ __launch_bounds__(kPredThreads) void k_kd_pred_reduce(
 {
     const uint32_t A = lvl[0], P = lvl[1];
     const uint32_t base = blockIdx.x * kPredTile;
-    const uint32_t kb = act_find(off, A, base);  // uniform: one search per block
+    if (base >= P) {  // past this level's points (grids are sized for a bound)
+        if (threadIdx.x == 0) sums[blockIdx.x] = 0;
+        return;
+    }
+    PredCursor c;
+    c.load(t, act, off, s1, pass, act_find(off, A, base));  // one search per block
     uint32_t acc = 0;
     for (int i = 0; i < kPredItems; ++i) {
         const uint32_t ti = base + i * kPredThreads + threadIdx.x;
-        if (ti < P && kd_pred(t, pass, act_pos(t, act, off, A, kb, ti), s1)) ++acc;
+        if (ti < P && c.pred(t, act, off, s1, pass, A, ti)) ++acc;
     }
     uint32_t tot;
     kd_block_scan<kPredThreads>(acc, &tot);
@@ -380,11 +429,17 @@ __global__ __launch_bounds__(kPredThreads) void k_kd_pred_down(
     const uint32_t A = lvl[0], P = lvl[1];
     __shared__ uint32_t s[kPredTile];
     const uint32_t base = blockIdx.x * kPredTile;
-    const uint32_t kb = act_find(off, A, base < P ? base : (P ? P - 1 : 0));
+    if (base > P) return;  // the block holding index P writes out[P]
+    if (P == 0) {  // a queued level with no big node
+        if (threadIdx.x == 0) out[0] = 0;
+        return;
+    }
+    PredCursor c;
+    c.load(t, act, off, s1, pass, act_find(off, A, base < P ? base : (P ? P - 1 : 0)));
     for (int i = 0; i < kPredItems; ++i) {
         const uint32_t l = i * kPredThreads + threadIdx.x;
         const uint32_t ti = base + l;
-        s[l] = (ti < P && kd_pred(t, pass, act_pos(t, act, off, A, kb, ti), s1)) ? 1u : 0u;
+        s[l] = (ti < P && c.pred(t, act, off, s1, pass, A, ti)) ? 1u : 0u;
     }
     __syncthreads();
     uint32_t v[kPredItems];
@@ -427,6 +482,7 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_publish(
 {
     const uint32_t A = lvl[0], P = lvl[1];
     const uint32_t b0 = blockIdx.x * blockDim.x;
+    if (b0 >= P) return;
     const uint32_t kb = act_find(off, A, b0);
     const uint32_t ti = b0 + threadIdx.x;
     if (ti >= P) return;
@@ -444,6 +500,7 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_swap(
 {
     const uint32_t A = lvl[0], P = lvl[1];
     const uint32_t b0 = blockIdx.x * blockDim.x;
+    if (b0 >= P) return;
     const uint32_t kb = act_find(off, A, b0);
     const uint32_t ti = b0 + threadIdx.x;
     if (ti >= P) return;
@@ -689,21 +746,32 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_big_minmax(
 // LDS atomics, the two Hoare passes by block prefix sums and the same
 // closed-form pairing, children records written to global memory -- and the
 // permuted points written back.
+// A node waiting on a wave's stack: its box travels with it, so a wave never
+// reads back node records that another lane stored.
+struct WaveFrame {
+    double box[6];
+    uint32_t nd;
+    uint16_t lo, hi;
+};
+
 struct SubShared {
     double x[kSub], y[kSub], z[kSub];
     uint32_t v[kSub];
     uint32_t pre[kSub + 1];
     uint16_t tmp[kSub];
-    uint8_t node[kSub];                 // level-local node index, 0xFF: settled
+    uint8_t node[kSub];                 // level-local node index, 0xFF: not at block level
     unsigned long long mn[kSubNodes * 3], mx[kSubNodes * 3];
     double cut[kSubNodes];
     uint32_t nid[kSubNodes];
     uint16_t lo[kSubNodes], hi[kSubNodes], L[kSubNodes], L2[kSubNodes], sp[kSubNodes];
-    uint8_t child[kSubNodes];
+    uint8_t child[kSubNodes], child2[kSubNodes];
     int8_t feat[kSubNodes];
     uint32_t nnid[kSubNodes];
     uint16_t nlo[kSubNodes], nhi[kSubNodes];
-    uint32_t nc, nnext, any;
+    uint32_t wq_nd[kWaveQ];
+    uint16_t wq_lo[kWaveQ], wq_hi[kWaveQ];
+    WaveFrame frames[kSubThreads / 64][kWaveStack];
+    uint32_t nc, nnext, nwq, wq_next;
 };
 
 __device__ __forceinline__ double sub_coord(const SubShared& S, int f, uint32_t i)
@@ -749,9 +817,191 @@ __device__ __forceinline__ void sub_swap(SubShared& S, uint32_t i, uint32_t q)
     S.v[q] = v;
 }
 
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+// Min/max of points [lo, hi) of S on each axis, in every lane.
+__device__ __forceinline__ void wave_extents(const SubShared& S, uint32_t lo, uint32_t hi,
+                                             int lane, double mn[3], double mx[3])
+{
+    unsigned long long l[3] = {~0ull, ~0ull, ~0ull}, h[3] = {0ull, 0ull, 0ull};
+    for (uint32_t i = lo + lane; i < hi; i += 64) {
+        const unsigned long long v[3] = {enc_order(S.x[i]), enc_order(S.y[i]), enc_order(S.z[i])};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            l[d] = v[d] < l[d] ? v[d] : l[d];
+            h[d] = v[d] > h[d] ? v[d] : h[d];
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const unsigned long long ol = __shfl_xor(l[d], o);
+            const unsigned long long oh = __shfl_xor(h[d], o);
+            l[d] = ol < l[d] ? ol : l[d];
+            h[d] = oh > h[d] ? oh : h[d];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = dec_order(l[d]);
+        mx[d] = dec_order(h[d]);
+    }
+}
+
+__device__ __forceinline__ bool wave_pred(const SubShared& S, int f, double cv, int pass,
+                                          uint32_t i)
+{
+    const double v = sub_coord(S, f, i);
+    return pass == 1 ? v < cv : v <= cv;
+}
+
+// One planeSplit pass over [a, e) by one wave: returns c = #pred and leaves
+// the predicate-true elements in [a, a + c), with the sequential two-pointer
+// loop's swaps: the r-th misplaced element of [a, a + c) from the left
+// trades places with the r-th misplaced element of [a + c, e) from the right.
+__device__ __forceinline__ uint32_t wave_pass(SubShared& S, int f, double cv, int pass,
+                                              uint32_t a, uint32_t e, int lane)
+{
+    uint32_t c = 0;
+    for (uint32_t i0 = a; i0 < e; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        c += static_cast<uint32_t>(__popcll(__ballot(i < e && wave_pred(S, f, cv, pass, i))));
+    }
+    const uint32_t m = a + c;
+    // Misplaced right elements, numbered from the left.
+    uint32_t R = 0;
+    for (uint32_t i0 = m; i0 < e; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const bool p = i < e && wave_pred(S, f, cv, pass, i);
+        const uint64_t b = __ballot(p);
+        if (p) S.tmp[a + R + lanes_below(b)] = static_cast<uint16_t>(i);
+        R += static_cast<uint32_t>(__popcll(b));
+    }
+    if (R == 0) return c;
+    wave_sync();
+    uint32_t q = 0;
+    for (uint32_t i0 = a; i0 < m; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const bool mis = i < m && !wave_pred(S, f, cv, pass, i);
+        const uint64_t b = __ballot(mis);
+        if (mis) sub_swap(S, i, S.tmp[a + (R - 1 - (q + lanes_below(b)))]);
+        q += static_cast<uint32_t>(__popcll(b));
+    }
+    wave_sync();
+    return c;
+}
+
+// The subtree of node nd (points [lo, hi) of S, which start at a0 in the
+// tree arrays; box bx) built by one wave, depth first.  The wave continues
+// into the first child and keeps the second on its stack unless it is a
+// leaf (whose extents, for the parent's divhigh, are taken at once), so the
+// stack holds only disjoint nodes of more than kLeafMax points: at most
+// kWaveNodeMax / (kLeafMax + 1) < kWaveStack of them.
+__device__ void wave_build(SubShared& S, const KdView& t, uint32_t* hdr, uint32_t a0,
+                           uint32_t nd, uint32_t lo, uint32_t hi, const double* bx, int lane,
+                           WaveFrame* stk)
+{
+    double box[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) box[i] = bx[i];
+    int sp = 0;
+    for (;;) {
+        double mn[3], mx[3];
+        wave_extents(S, lo, hi, lane, mn, mx);
+        if (lane == 0) kd_parent_div(t, nd, mn, mx);
+        const uint32_t cnt = hi - lo;
+        bool down = false;
+        if (cnt > kLeafMax) {
+            // The children's ids: the atomic's round trip overlaps the passes.
+            uint32_t c1 = 0;
+            if (lane == 0) c1 = atomicAdd(&hdr[kHdrNodes], 2u);
+            int f;
+            double cv;
+            kd_choose_split(box, mn, mx, f, cv);
+            if (lane == 0) {
+                t.feat[nd] = f;
+                t.cutval[nd] = cv;
+            }
+            const uint32_t lim1 = wave_pass(S, f, cv, 1, lo, hi, lane);
+            const uint32_t lim2 = lim1 + wave_pass(S, f, cv, 2, lo + lim1, hi, lane);
+            const uint32_t idx = kd_split_index(cnt, lim1, lim2);
+            c1 = __shfl(c1, 0);
+            if (c1 + 1 >= t.cap) {
+                if (lane == 0) hdr[kHdrOverflow] = kOvfNodes;  // the build is redone
+            } else {
+                if (lane == 0) kd_make_children(t, nd, a0 + lo, idx, a0 + hi, c1);
+                const uint32_t m2 = lo + idx;
+                if (hi - m2 > kLeafMax) {
+                    if (sp < kWaveStack) {
+                        if (lane == 0) {
+                            WaveFrame& fr = stk[sp];
+                            for (int i = 0; i < 6; ++i) fr.box[i] = box[i];
+                            fr.box[f] = cv;
+                            fr.nd = c1 + 1;
+                            fr.lo = static_cast<uint16_t>(m2);
+                            fr.hi = static_cast<uint16_t>(hi);
+                        }
+                        ++sp;
+                    } else if (lane == 0) {
+                        hdr[kHdrOverflow] = kOvfSubtree;
+                    }
+                } else {
+                    double lmn[3], lmx[3];
+                    wave_extents(S, m2, hi, lane, lmn, lmx);
+                    if (lane == 0) kd_parent_div(t, c1 + 1, lmn, lmx);
+                }
+                box[3 + f] = cv;  // first child: the upper bound on the cut axis
+                nd = c1;
+                hi = m2;
+                down = true;
+            }
+        }
+        if (down) continue;
+        if (sp == 0) break;
+        --sp;
+        wave_sync();
+        const WaveFrame& fr = stk[sp];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) box[i] = fr.box[i];
+        nd = fr.nd;
+        lo = fr.lo;
+        hi = fr.hi;
+        wave_sync();
+    }
+}
+
+// The nodes queued for the waves, one wave per node until the queue is empty.
+__device__ __forceinline__ void wave_drain(SubShared& S, const KdView& t, uint32_t* hdr,
+                                           uint32_t a0, int lane, int wv)
+{
+    for (;;) {
+        uint32_t q = 0;
+        if (lane == 0) q = atomicAdd(&S.wq_next, 1u);
+        q = __shfl(q, 0);
+        if (q >= S.nwq) break;
+        const uint32_t nd = S.wq_nd[q];
+        double bx[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) bx[i] = t.box[6 * static_cast<size_t>(nd) + i];
+        wave_build(S, t, hdr, a0, nd, S.wq_lo[q], S.wq_hi[q], bx, lane, S.frames[wv]);
+    }
+}
+
 __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
                                                            const uint32_t* __restrict__ small,
-                                                           uint32_t* __restrict__ hdr)
+                                                           uint32_t* __restrict__ hdr,
+                                                           uint32_t wave_node)
 {
     __shared__ SubShared S;
     const uint32_t tid = threadIdx.x;
@@ -771,22 +1021,38 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
         }
     }
     if (tid == 0) {
-        S.nc = 1;
-        S.nid[0] = nd0;
-        S.lo[0] = 0;
-        S.hi[0] = static_cast<uint16_t>(m);  // m <= 4096 < 65536
+        S.wq_next = 0;
+        if (m <= wave_node) {  // small enough for one wave from the start
+            S.nc = 0;
+            S.nwq = 1;
+            S.wq_nd[0] = nd0;
+            S.wq_lo[0] = 0;
+            S.wq_hi[0] = static_cast<uint16_t>(m);
+        } else {
+            S.nc = 1;
+            S.nwq = 0;
+            S.nid[0] = nd0;
+            S.lo[0] = 0;
+            S.hi[0] = static_cast<uint16_t>(m);  // m <= 4096 < 65536
+        }
     }
     __syncthreads();
     for (;;) {
+        // Nodes of at most wave_node points: one wave each, no block barrier.
+        wave_drain(S, t, hdr, a0, lane, wv);
+        __syncthreads();
         const uint32_t nc = S.nc;
-        // (a) extents of the level's nodes
+        if (nc == 0) break;
+        // Block level: nodes of more than wave_node points, all split.
+        // (a) extents
         for (uint32_t i = tid; i < nc * 3; i += kSubThreads) {
             S.mn[i] = ~0ull;
             S.mx[i] = 0ull;
         }
         if (tid == 0) {
             S.nnext = 0;
-            S.any = 0;
+            S.nwq = 0;
+            S.wq_next = 0;
         }
         __syncthreads();
         for (uint32_t c = wv; c * 64 < m; c += kSubThreads / 64) {
@@ -819,7 +1085,7 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
                 }
         }
         __syncthreads();
-        // (b) one thread per node: parent's div, leaf or split
+        // (b) one thread per node: parent's div and the split
         if (tid < nc) {
             const uint32_t nd = S.nid[tid];
             double mn[3], mx[3];
@@ -828,41 +1094,34 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
                 mx[d] = dec_order(S.mx[tid * 3 + d]);
             }
             kd_parent_div(t, nd, mn, mx);
-            const uint32_t cnt = S.hi[tid] - S.lo[tid];
-            if (cnt <= kLeafMax) {
-                S.feat[tid] = -1;
-            } else {
-                int f;
-                double cv;
-                kd_choose_split(t.box + 6 * static_cast<size_t>(nd), mn, mx, f, cv);
-                t.feat[nd] = f;
-                t.cutval[nd] = cv;
-                S.feat[tid] = static_cast<int8_t>(f);
-                S.cut[tid] = cv;
-                S.any = 1;
-            }
+            int f;
+            double cv;
+            kd_choose_split(t.box + 6 * static_cast<size_t>(nd), mn, mx, f, cv);
+            t.feat[nd] = f;
+            t.cutval[nd] = cv;
+            S.feat[tid] = static_cast<int8_t>(f);
+            S.cut[tid] = cv;
         }
         __syncthreads();
-        if (!S.any) break;
         // (c) pass 1: coord < cut
         auto p1 = [&](uint32_t i) {
             const uint32_t j = S.node[i];
-            if (j == 0xFFu || S.feat[j] < 0) return false;
+            if (j == 0xFFu) return false;
             return sub_coord(S, S.feat[j], i) < S.cut[j];
         };
         sub_prefix(S, p1);
-        if (tid < nc && S.feat[tid] >= 0)
+        if (tid < nc)
             S.L[tid] = static_cast<uint16_t>(S.lo[tid] + (S.pre[S.hi[tid]] - S.pre[S.lo[tid]]));
         __syncthreads();
         for (uint32_t i = tid; i < m; i += kSubThreads) {
             const uint32_t j = S.node[i];
-            if (j == 0xFFu || S.feat[j] < 0) continue;
+            if (j == 0xFFu) continue;
             if (i >= S.L[j] && p1(i)) S.tmp[S.lo[j] + (S.pre[S.hi[j]] - S.pre[i + 1])] = i;
         }
         __syncthreads();
         for (uint32_t i = tid; i < m; i += kSubThreads) {
             const uint32_t j = S.node[i];
-            if (j == 0xFFu || S.feat[j] < 0) continue;
+            if (j == 0xFFu) continue;
             if (i < S.L[j] && !p1(i)) {
                 const uint32_t r = (i - S.lo[j]) - (S.pre[i] - S.pre[S.lo[j]]);
                 sub_swap(S, i, S.tmp[S.lo[j] + r]);
@@ -872,62 +1131,72 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
         // (d) pass 2: coord <= cut over [L, hi)
         auto p2 = [&](uint32_t i) {
             const uint32_t j = S.node[i];
-            if (j == 0xFFu || S.feat[j] < 0 || i < S.L[j]) return false;
+            if (j == 0xFFu || i < S.L[j]) return false;
             return sub_coord(S, S.feat[j], i) <= S.cut[j];
         };
         sub_prefix(S, p2);
-        if (tid < nc && S.feat[tid] >= 0)
+        if (tid < nc)
             S.L2[tid] = static_cast<uint16_t>(S.L[tid] + (S.pre[S.hi[tid]] - S.pre[S.lo[tid]]));
         __syncthreads();
         for (uint32_t i = tid; i < m; i += kSubThreads) {
             const uint32_t j = S.node[i];
-            if (j == 0xFFu || S.feat[j] < 0) continue;
+            if (j == 0xFFu) continue;
             if (i >= S.L2[j] && p2(i)) S.tmp[S.L[j] + (S.pre[S.hi[j]] - S.pre[i + 1])] = i;
         }
         __syncthreads();
         for (uint32_t i = tid; i < m; i += kSubThreads) {
             const uint32_t j = S.node[i];
-            if (j == 0xFFu || S.feat[j] < 0) continue;
+            if (j == 0xFFu) continue;
             if (i >= S.L[j] && i < S.L2[j] && !p2(i)) {
                 const uint32_t r = (i - S.L[j]) - (S.pre[i] - S.pre[S.L[j]]);
                 sub_swap(S, i, S.tmp[S.L[j] + r]);
             }
         }
         __syncthreads();
-        // (e) children of the splitting nodes
-        if (tid < nc && S.feat[tid] >= 0) {
+        // (e) children: small ones to the waves' queue, big ones to the next
+        // block level
+        if (tid < nc) {
             const uint32_t nd = S.nid[tid];
             const uint32_t lo = S.lo[tid], hi = S.hi[tid];
             const uint32_t idx = kd_split_index(hi - lo, S.L[tid] - lo, S.L2[tid] - lo);
             const uint32_t c1 = atomicAdd(&hdr[kHdrNodes], 2u);
-            const uint32_t slot = atomicAdd(&S.nnext, 2u);
+            S.sp[tid] = static_cast<uint16_t>(lo + idx);
+            S.child[tid] = S.child2[tid] = 0xFF;
             if (c1 + 1 >= t.cap) {
-                hdr[kHdrOverflow] = kOvfNodes;
-                S.feat[tid] = -1;  // stop this branch; the build is redone
-            } else if (slot + 1 >= static_cast<uint32_t>(kSubNodes)) {
-                hdr[kHdrOverflow] = kOvfSubtree;
-                S.feat[tid] = -1;
+                hdr[kHdrOverflow] = kOvfNodes;  // the build is redone
             } else {
                 kd_make_children(t, nd, a0 + lo, idx, a0 + hi, c1);
-                S.sp[tid] = static_cast<uint16_t>(lo + idx);
-                S.child[tid] = static_cast<uint8_t>(slot);
-                S.nnid[slot] = c1;
-                S.nlo[slot] = static_cast<uint16_t>(lo);
-                S.nhi[slot] = static_cast<uint16_t>(lo + idx);
-                S.nnid[slot + 1] = c1 + 1;
-                S.nlo[slot + 1] = static_cast<uint16_t>(lo + idx);
-                S.nhi[slot + 1] = static_cast<uint16_t>(hi);
+                const uint32_t clo[2] = {lo, lo + idx}, chi[2] = {lo + idx, hi};
+                for (int c = 0; c < 2; ++c) {
+                    if (chi[c] - clo[c] <= wave_node) {
+                        const uint32_t q = atomicAdd(&S.nwq, 1u);
+                        if (q < static_cast<uint32_t>(kWaveQ)) {
+                            S.wq_nd[q] = c1 + c;
+                            S.wq_lo[q] = static_cast<uint16_t>(clo[c]);
+                            S.wq_hi[q] = static_cast<uint16_t>(chi[c]);
+                        } else {
+                            hdr[kHdrOverflow] = kOvfSubtree;
+                        }
+                    } else {
+                        const uint32_t slot = atomicAdd(&S.nnext, 1u);
+                        if (slot < static_cast<uint32_t>(kSubNodes)) {
+                            (c == 0 ? S.child : S.child2)[tid] = static_cast<uint8_t>(slot);
+                            S.nnid[slot] = c1 + c;
+                            S.nlo[slot] = static_cast<uint16_t>(clo[c]);
+                            S.nhi[slot] = static_cast<uint16_t>(chi[c]);
+                        } else {
+                            hdr[kHdrOverflow] = kOvfSubtree;
+                        }
+                    }
+                }
             }
         }
         __syncthreads();
-        // (f) points move to their child (leaves settle)
+        // (f) points move to their child's block-level slot (0xFF: a wave's)
         for (uint32_t i = tid; i < m; i += kSubThreads) {
             const uint32_t j = S.node[i];
             if (j == 0xFFu) continue;
-            if (S.feat[j] < 0)
-                S.node[i] = 0xFF;
-            else
-                S.node[i] = static_cast<uint8_t>(i < S.sp[j] ? S.child[j] : S.child[j] + 1);
+            S.node[i] = i < S.sp[j] ? S.child[j] : S.child2[j];
         }
         __syncthreads();
         const uint32_t nn = min(S.nnext, static_cast<uint32_t>(kSubNodes));
@@ -936,9 +1205,11 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
             S.lo[tid] = S.nlo[tid];
             S.hi[tid] = S.nhi[tid];
         }
-        if (tid == 0) S.nc = nn;
+        if (tid == 0) {
+            S.nc = nn;
+            S.nwq = min(S.nwq, static_cast<uint32_t>(kWaveQ));
+        }
         __syncthreads();
-        if (nn == 0) break;
     }
     for (uint32_t i = tid; i < m; i += kSubThreads) {
         t.px[a0 + i] = S.x[i];
@@ -1291,8 +1562,10 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
         nsmall = h[kHdrSmall];
     }
     if (nsmall) {
+        const uint32_t wave_node = std::min<uint32_t>(
+            std::max<uint32_t>(kt.wave_node, kLeafMax + 1), kWaveNodeMax);
         hipLaunchKernelGGL(k_kd_subtree, dim3(nsmall), dim3(kSubThreads), 0, s, t, w.small,
-                           w.hdr);
+                           w.hdr, wave_node);
         GF_HIP(hipGetLastError());
     }
     GF_HIP(hipMemcpyAsync(h, w.hdr, sizeof(h), hipMemcpyDeviceToHost, s));

@@ -176,6 +176,24 @@ __device__ __forceinline__ uint32_t candidate_pos(uint32_t lo, uint32_t excl, ui
     return slo + (t - sex);
 }
 
+// Bit pattern of the FP64 squared distance, in KD3Index's operation order.
+__device__ __forceinline__ uint64_t dist_bits(double qx, double qy, double qz, double px,
+                                              double py, double pz)
+{
+    const double dx = qx - px;
+    const double dy = qy - py;
+    const double dz = qz - pz;
+    double d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    return static_cast<uint64_t>(__double_as_longlong(d2));
+}
+
+__device__ __forceinline__ double bits_d2(uint64_t d)
+{
+    return __longlong_as_double(static_cast<long long>(d));
+}
+
 __device__ __forceinline__ Ent make_ent(double qx, double qy, double qz, double px,
                                         double py, double pz, uint32_t id, uint32_t j)
 {
@@ -189,19 +207,26 @@ __device__ __forceinline__ Ent make_ent(double qx, double qy, double qz, double 
 }
 
 // Selection over at most 256 candidate entries held in registers (4 per
-// lane).  First a threshold E with need <= #{d2 <= E} <= 64 (need = fetch + 1,
-// so the (fetch+1)-th entry that decides ties is kept): the caller's hint (an
-// upper bound on the need-th distance, from the previous query of the group by
-// the triangle inequality) is tried first, then a bisection on d2 whose
-// counts are ballots.  The survivors are compacted into LDS and ranked: each
+// lane).  First a threshold E with need <= #{d2 <= E} <= cap (need = fetch +
+// 1, so the (fetch+1)-th entry that decides ties is kept), each trial counted
+// exactly by ballots: the caller's estimate (the previous query's need-th d2)
+// first, then two steps that interpolate the count linearly in d2 (points on
+// a surface: count grows with the area, i.e. with d2) inside the bracket, then
+// bisection.  The survivors are compacted into LDS and ranked: each
 // lane counts the survivors that precede its own entry in (d2, id) order,
 // reading them as LDS broadcasts, and stores itself at that rank.  Every
 // candidate with d2 equal to the boundary distance is below E, so the first
 // fetch+1 ranked survivors are exactly the fetch+1 smallest (d2, id) pairs.
 // When no such E exists (more than 64 points at one distance) the register
 // batches are merged one by one with bitonic networks instead.
-__device__ __forceinline__ void select_entries(Ent (&c)[kRegBatches], uint32_t tot, int fetch, int lane,
-                               uint32_t* scratch, double hint, Ent& best, uint64_t* cnt)
+//
+// The lane's candidate of register batch b is (cd[b], cid[b], cj[b]): squared
+// distance bits (kInfD: none), point id and sorted position.
+__device__ __forceinline__ void select_entries(const uint64_t (&cd)[kRegBatches],
+                                               const uint32_t (&cid)[kRegBatches],
+                                               const uint32_t (&cj)[kRegBatches], uint32_t tot,
+                                               int fetch, int lane, uint32_t* scratch,
+                                               double hint, Ent& best, uint64_t* cnt)
 {
     const uint32_t need = min(static_cast<uint32_t>(fetch + 1), tot);
     // Survivor window: few enough to rank cheaply, at most one per lane.
@@ -216,7 +241,7 @@ __device__ __forceinline__ void select_entries(Ent (&c)[kRegBatches], uint32_t t
 #pragma unroll
             for (int b = 0; b < kRegBatches; ++b)
                 if (static_cast<uint32_t>(b) < nbat)
-                    k += __popcll(__ballot(c[b].d != kInfD && ent_d2(c[b]) <= t));
+                    k += __popcll(__ballot(cd[b] != kInfD && bits_d2(cd[b]) <= t));
             return k;
         };
         double lo_e = 0.0, hi_e = INFINITY;
@@ -236,7 +261,7 @@ __device__ __forceinline__ void select_entries(Ent (&c)[kRegBatches], uint32_t t
             double mx = 0.0;
 #pragma unroll
             for (int b = 0; b < kRegBatches; ++b)
-                if (static_cast<uint32_t>(b) < nbat && c[b].d != kInfD) mx = fmax(mx, ent_d2(c[b]));
+                if (static_cast<uint32_t>(b) < nbat && cd[b] != kInfD) mx = fmax(mx, bits_d2(cd[b]));
 #pragma unroll
             for (int m = 32; m > 0; m >>= 1) mx = fmax(mx, __shfl_xor(mx, m));
             hi_e = mx;  // counts every candidate (> cap)
@@ -257,20 +282,24 @@ __device__ __forceinline__ void select_entries(Ent (&c)[kRegBatches], uint32_t t
         }
     }
     if (ok) {
-        const uint64_t lt = (1ull << lane) - 1ull;
         uint32_t S = 0;
         const uint32_t nbat_s = (tot + 63u) >> 6;
 #pragma unroll
         for (int b = 0; b < kRegBatches; ++b) {
             if (static_cast<uint32_t>(b) >= nbat_s) break;
-            const bool s = c[b].d != kInfD && ent_d2(c[b]) <= E;
+            const bool s = cd[b] != kInfD && bits_d2(cd[b]) <= E;
             const uint64_t m = __ballot(s);
             if (s) {
-                uint32_t* e = scratch + 4 * (S + __popcll(m & lt));
-                e[0] = static_cast<uint32_t>(c[b].d);
-                e[1] = static_cast<uint32_t>(c[b].d >> 32);
-                e[2] = c[b].id;
-                e[3] = c[b].j;
+                // Survivors of lower lanes in this batch (mbcnt: popcount of
+                // the ballot below this lane).
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                    static_cast<uint32_t>(m >> 32),
+                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+                uint32_t* e = scratch + 4 * (S + below);
+                e[0] = static_cast<uint32_t>(cd[b]);
+                e[1] = static_cast<uint32_t>(cd[b] >> 32);
+                e[2] = cid[b];
+                e[3] = cj[b];
             }
             S += __popcll(m);
         }
@@ -317,8 +346,9 @@ __device__ __forceinline__ void select_entries(Ent (&c)[kRegBatches], uint32_t t
 #pragma unroll
     for (int b = 0; b < kRegBatches; ++b) {
         if (static_cast<uint32_t>(b * 64) >= tot) break;
-        bitonic_sort64(c[b], lane);
-        merge_sorted(best, c[b], lane);
+        Ent c = cd[b] != kInfD ? Ent{cd[b], cid[b], cj[b]} : ent_inf();
+        bitonic_sort64(c, lane);
+        merge_sorted(best, c, lane);
         cnt[2] += 1;
     }
 }
@@ -347,19 +377,23 @@ __device__ __forceinline__ void search_single(const KnnArgs& a, const WaveGrid& 
             const uint32_t excl = incl - len;
             cnt[1] += tot;
             if (ncb <= 64 && tot <= kRegCand) {
-                Ent c[kRegBatches];
+                uint64_t cd[kRegBatches];
+                uint32_t cid[kRegBatches], cj[kRegBatches];
 #pragma unroll
                 for (int bb = 0; bb < kRegBatches; ++bb) {
-                    c[bb] = ent_inf();
+                    cd[bb] = kInfD;
+                    cid[bb] = cj[bb] = kInfU;
                     if (static_cast<uint32_t>(bb * 64) < tot) {
                         const uint32_t t = bb * 64 + lane;
                         const uint32_t j = candidate_pos(lo, excl, t);
-                        if (t < tot)
-                            c[bb] = make_ent(qx, qy, qz, cv.sx[j], cv.sy[j], cv.sz[j],
-                                             cv.perm[j], j);
+                        if (t < tot) {
+                            cd[bb] = dist_bits(qx, qy, qz, cv.sx[j], cv.sy[j], cv.sz[j]);
+                            cid[bb] = cv.perm[j];
+                            cj[bb] = j;
+                        }
                     }
                 }
-                select_entries(c, tot, fetch, lane, scratch, INFINITY, best, cnt);
+                select_entries(cd, cid, cj, tot, fetch, lane, scratch, INFINITY, best, cnt);
                 break;
             }
             for (uint32_t t0 = 0; t0 < tot; t0 += 64) {
@@ -433,7 +467,10 @@ __device__ __forceinline__ void finish_query(const KnnArgs& a, const WaveGrid& w
 }
 
 template <bool kTiled>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_knn(KnnArgs a_in)
+// Four waves per SIMD: the selection is a chain of ballots and LDS
+// round trips, so more resident waves hide more of its latency (measured
+// faster than three waves even with the register spills four waves cost).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_knn(KnnArgs a_in)
 {
     extern __shared__ uint32_t s_lds[];
     const int lane = __lane_id();
@@ -584,7 +621,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
             }
         }
 
-        double hint = INFINITY, pqx = 0.0, pqy = 0.0, pqz = 0.0;
+        double hint = INFINITY;
         for (int gi = 0; gi < gm; ++gi) {
             const uint64_t s = s0 + gi;
             const int qslot = qi + gi;
@@ -593,28 +630,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                 const double qx = cv.sx[s], qy = cv.sy[s], qz = cv.sz[s];
                 cnt[0] += 1;
                 cnt[1] += tot;
-                Ent c[kRegBatches];
+                // The candidates' ids and positions are the cache's own.
+                uint64_t cd[kRegBatches];
 #pragma unroll
-                for (int b = 0; b < kRegBatches; ++b) {
-                    c[b] = ent_inf();
-                    if (static_cast<uint32_t>(b) * 64u < tot && pj[b] != kInfU)
-                        c[b] = make_ent(qx, qy, qz, px[b], py[b], pz[b], pid[b], pj[b]);
-                }
-                // The previous query's need nearest lie within its need-th
-                // distance plus the step between the two queries.
-                double h = INFINITY;
-                if (hint < INFINITY) {
-                    const double dx = qx - pqx, dy = qy - pqy, dz = qz - pqz;
-                    const double t = sqrt(hint) + sqrt(dx * dx + dy * dy + dz * dz);
-                    h = t * t * (1.0 + 1e-9) + 1e-300;
-                }
+                for (int b = 0; b < kRegBatches; ++b)
+                    cd[b] = (static_cast<uint32_t>(b) * 64u < tot && pj[b] != kInfU)
+                                ? dist_bits(qx, qy, qz, px[b], py[b], pz[b])
+                                : kInfD;
+                // Neighboring queries of a group have similar need-th
+                // distances: the previous one is the first threshold tried.
                 Ent best;
-                select_entries(c, tot, fetch, lane, scratch, h, best, cnt);
+                select_entries(cd, pid, pj, tot, fetch, lane, scratch, hint, best, cnt);
                 const Ent nth = ent_shfl(best, static_cast<int>(min(need, tot)) - 1);
                 hint = nth.d == kInfD ? INFINITY : ent_d2(nth);
-                pqx = qx;
-                pqy = qy;
-                pqz = qz;
                 done = certified(wg, box, ent_shfl(best, fetch - 1), qx, qy, qz);
                 if (done) finish_query(a, wg, best, false, s, qslot, lane, nbr, my_flags);
             }
