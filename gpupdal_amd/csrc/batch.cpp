@@ -290,13 +290,13 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     if (any_exact_tile) launch_flag_tiles(N, cv.perm, 0, out, tt, stream);
     hipEvent_t e1 = mark();
     span(e0, e1, &stats.ms_search);
-    auto forest = [&](hipStream_t s) {
+    auto forest = [&](hipStream_t s, bool beside) {
         if (b.kd_valid) return;
         build_forest(b.kd, b.x.as<double>(), b.y.as<double>(), b.z.as<double>(), N,
-                     b.dstarts.as<uint64_t>(), T, b.dbounds.as<double>(), s);
+                     b.dstarts.as<uint64_t>(), T, b.dbounds.as<double>(), s, beside);
         b.kd_valid = true;
     };
-    if (early) forest(side);
+    if (early) forest(side, true);
 
     // Exact re-solve of flagged rows on the tiles' KD3 trees.
     uint32_t h[4] = {0, 0, 0, 0};
@@ -305,7 +305,7 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     stats.tie_rows = h[1];
     stats.incomplete_rows = h[2];
     if (h[0]) {
-        forest(stream);
+        forest(stream, false);
         uint32_t* failed = counters.as<uint32_t>() + 5;
         launch_fill_u32(failed, 1, 0u, stream);
         launch_kd_resolve(b.kd.view, cv, prog0, 0, out, b.x.as<double>(), b.y.as<double>(),

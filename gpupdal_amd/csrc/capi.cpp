@@ -245,7 +245,7 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
         launch_knn(cv, prog, 0, out, knn_tuning(), stream);
     hipEvent_t e1 = mark();
     span(e0, e1, &stats.ms_search);
-    if (early) build_kdtree(cl, side);
+    if (early) build_kdtree(cl, side, true);
     resolve_flagged(cl, cv, prog, 0, out, e1);
     e0 = mark();
     launch_features(cv, prog, out, cl.inv.as<uint32_t>(), stream);
@@ -276,7 +276,7 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
 // x, y, z; bounds 6 per tile), tile r rooted at node r.
 void Context::build_forest(KdBuffers& b, const double* x, const double* y, const double* z,
                            uint64_t n, const uint64_t* dstarts, uint32_t T,
-                           const double* dbounds, hipStream_t s)
+                           const double* dbounds, hipStream_t s, bool beside)
 {
     hipEvent_t t0 = mark(s);
     uint32_t cap = std::max(kd_node_capacity(n) + T, b.cap);
@@ -334,6 +334,8 @@ void Context::build_forest(KdBuffers& b, const double* x, const double* y, const
         KdTuning kt{};
         kt.subtree_points = static_cast<uint32_t>(tune.kd_subtree);
         kt.wave_node = static_cast<uint32_t>(tune.kd_wave_node);
+        kt.small_blocks = tune.kd_small_blocks < 0 ? (beside ? 1u : 0u)
+                                                   : static_cast<uint32_t>(tune.kd_small_blocks);
         kt.minmax_lds_nodes = static_cast<uint32_t>(tune.kd_minmax_lds);
         const uint32_t rc =
             launch_kd_build(x, y, z, n, dstarts, T, dbounds, t, w, kt, &nodes, s);
@@ -355,11 +357,11 @@ void Context::build_forest(KdBuffers& b, const double* x, const double* y, const
 
 // The exact KD3Index tree of a cloud (built once, reused until
 // invalidated), complete on return.
-void Context::build_kdtree(Cloud& cl, hipStream_t s)
+void Context::build_kdtree(Cloud& cl, hipStream_t s, bool beside)
 {
     if (cl.kd_valid) return;
     build_forest(cl.kd, cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), cl.n,
-                 cl.dstarts.as<uint64_t>(), 1, cl.dbounds.as<double>(), s);
+                 cl.dstarts.as<uint64_t>(), 1, cl.dbounds.as<double>(), s, beside);
     cl.kd_valid = true;
 }
 
@@ -651,7 +653,7 @@ int gf_knn(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t stride, uint32_t* 
             launch_flag_all(n, cv.perm, 1, out, c->stream);
         else
             launch_knn(cv, prog, 1, out, c->knn_tuning(), c->stream);
-        if (early) c->build_kdtree(*cl, c->side);
+        if (early) c->build_kdtree(*cl, c->side, true);
         c->resolve_flagged(*cl, cv, prog, 1, out, nullptr);
         if (where == GF_WHERE_HOST) {
             GF_HIP(hipMemcpyAsync(ids, out.ids, n * kk * 4, hipMemcpyDeviceToHost, c->stream));

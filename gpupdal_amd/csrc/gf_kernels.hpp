@@ -176,6 +176,7 @@ struct KdTuning {
     uint32_t subtree_points;    // nodes up to this size go to one workgroup (<= 4096)
     uint32_t minmax_lds_nodes;  // levels up to this many big nodes stage extents in LDS
     uint32_t wave_node;         // subtree nodes up to this size are built by one wave
+    uint32_t small_blocks;      // one-wave subtree workgroups of <= 512 points
 };
 
 // Words of the exact-tree build header at the front of the node-list buffer.

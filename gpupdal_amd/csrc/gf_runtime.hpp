@@ -68,6 +68,9 @@ struct Tuning {
     int group_z = 1;
     int kd_subtree = 4096;       // exact tree: subtree-per-workgroup threshold (points)
     int kd_wave_node = 256;      // exact tree: one-wave subtree threshold (points)
+    // Exact tree: one-wave subtree workgroups (-1: when built beside the
+    // search, 0: never, 1: always).
+    int kd_small_blocks = -1;
     int kd_minmax_lds = 512;     // exact tree: LDS-staged extents up to this many nodes
     // Clouds (batches) of at least this many points build the exact tree on
     // a side stream during the search, before knowing whether a row needs
@@ -87,6 +90,8 @@ struct Tuning {
             kd_subtree = static_cast<int>(v);
         else if (!std::strcmp(key, "kd_wave_node") && v >= 101 && v <= 1024)
             kd_wave_node = static_cast<int>(v);
+        else if (!std::strcmp(key, "kd_small_blocks") && v >= -1 && v <= 1)
+            kd_small_blocks = static_cast<int>(v);
         else if (!std::strcmp(key, "kd_minmax_lds") && v >= 0) kd_minmax_lds = static_cast<int>(v);
         else if (!std::strcmp(key, "kd_overlap_min") && v >= 0)
             kd_overlap_min = static_cast<uint64_t>(v);
@@ -259,11 +264,12 @@ struct Context {
     CloudView view(const Cloud& cl) const;
     void run_group(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
                    uint8_t* const* status, int where);
-    void build_kdtree(Cloud& cl, hipStream_t s);
-    void build_kdtree(Cloud& cl) { build_kdtree(cl, stream); }
+    // beside: built on the side stream while the search runs.
+    void build_kdtree(Cloud& cl, hipStream_t s, bool beside);
+    void build_kdtree(Cloud& cl) { build_kdtree(cl, stream, false); }
     void build_forest(KdBuffers& b, const double* x, const double* y, const double* z,
                       uint64_t n, const uint64_t* dstarts, uint32_t T, const double* dbounds,
-                      hipStream_t s);
+                      hipStream_t s, bool beside);
     // Makes the side stream wait for the work queued on `stream` so far.
     void fork_side();
     void resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog, int mode,

@@ -62,6 +62,11 @@ __device__ __forceinline__ const double* axis(const KdView& t, int f)
 // with block barriers.
 constexpr uint32_t kSub = 4096;           // small-node capacity (points)
 constexpr int kSubThreads = 1024;
+// While the search runs beside the build, its blocks hold every CU; a
+// one-wave workgroup on at most kSubSmall points (about 22 KB of LDS) fits in
+// the slot a finished search block frees, a kSub one (155 KB) does not.
+constexpr uint32_t kSubSmall = 512;
+constexpr int kSubSmallThreads = 64;
 // Subtree nodes of at most wave_node (<= kWaveNodeMax, >= kLeafMax + 1)
 // points are built by one wave each; bigger ones level by level by the
 // whole workgroup.  Block-level nodes then hold more than kLeafMax points,
@@ -73,7 +78,7 @@ constexpr int kWaveQ = 96;                // nodes handed to the waves per block
 constexpr int kPredThreads = 256;
 constexpr int kPredItems = 16;
 constexpr uint32_t kPredTile = kPredThreads * kPredItems;
-constexpr int kChildThreads = 1024;
+constexpr int kChildThreads = 256;       // one block; small enough to run beside the search
 constexpr uint32_t kMinMaxLds = 512;      // big children staged per block in LDS
 
 // Build header (device): the last level's big-node count and point count,
@@ -754,12 +759,15 @@ struct WaveFrame {
     uint16_t lo, hi;
 };
 
-struct SubShared {
-    double x[kSub], y[kSub], z[kSub];
-    uint32_t v[kSub];
-    uint32_t pre[kSub + 1];
-    uint16_t tmp[kSub];
-    uint8_t node[kSub];                 // level-local node index, 0xFF: not at block level
+// LDS of one subtree workgroup: SUB points, NT threads.
+template <uint32_t SUB, int NT> struct SubShared {
+    static constexpr uint32_t kCap = SUB;
+    static constexpr int kThreads = NT;
+    double x[SUB], y[SUB], z[SUB];
+    uint32_t v[SUB];
+    uint32_t pre[SUB + 1];
+    uint16_t tmp[SUB];
+    uint8_t node[SUB];                 // level-local node index, 0xFF: not at block level
     unsigned long long mn[kSubNodes * 3], mx[kSubNodes * 3];
     double cut[kSubNodes];
     uint32_t nid[kSubNodes];
@@ -770,20 +778,20 @@ struct SubShared {
     uint16_t nlo[kSubNodes], nhi[kSubNodes];
     uint32_t wq_nd[kWaveQ];
     uint16_t wq_lo[kWaveQ], wq_hi[kWaveQ];
-    WaveFrame frames[kSubThreads / 64][kWaveStack];
+    WaveFrame frames[NT / 64][kWaveStack];
     uint32_t nc, nnext, nwq, wq_next;
 };
 
-__device__ __forceinline__ double sub_coord(const SubShared& S, int f, uint32_t i)
+template <class SS> __device__ __forceinline__ double sub_coord(const SS& S, int f, uint32_t i)
 {
     return f == 0 ? S.x[i] : (f == 1 ? S.y[i] : S.z[i]);
 }
 
-// Exclusive block prefix of pred(i) over i in [0, kSub) into S.pre (pre[kSub]
+// Exclusive block prefix of pred(i) over i in [0, SUB) into S.pre (pre[SUB]
 // = total).  Thread tid owns items [16 tid, 16 tid + 16).
-template <class Pred> __device__ __forceinline__ void sub_prefix(SubShared& S, Pred pred)
+template <class SS, class Pred> __device__ __forceinline__ void sub_prefix(SS& S, Pred pred)
 {
-    constexpr int kItems = kSub / kSubThreads;
+    constexpr int kItems = SS::kCap / SS::kThreads;
     const uint32_t i0 = threadIdx.x * kItems;
     uint32_t bits = 0, cnt = 0;
 #pragma unroll
@@ -793,17 +801,17 @@ template <class Pred> __device__ __forceinline__ void sub_prefix(SubShared& S, P
             ++cnt;
         }
     uint32_t tot;
-    uint32_t ex = kd_block_scan<kSubThreads>(cnt, &tot);
+    uint32_t ex = kd_block_scan<SS::kThreads>(cnt, &tot);
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         S.pre[i0 + i] = ex;
         ex += (bits >> i) & 1u;
     }
-    if (threadIdx.x == 0) S.pre[kSub] = tot;
+    if (threadIdx.x == 0) S.pre[SS::kCap] = tot;
     __syncthreads();
 }
 
-__device__ __forceinline__ void sub_swap(SubShared& S, uint32_t i, uint32_t q)
+template <class SS> __device__ __forceinline__ void sub_swap(SS& S, uint32_t i, uint32_t q)
 {
     const double x = S.x[i], y = S.y[i], z = S.z[i];
     const uint32_t v = S.v[i];
@@ -830,7 +838,8 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
 }
 
 // Min/max of points [lo, hi) of S on each axis, in every lane.
-__device__ __forceinline__ void wave_extents(const SubShared& S, uint32_t lo, uint32_t hi,
+template <class SS>
+__device__ __forceinline__ void wave_extents(const SS& S, uint32_t lo, uint32_t hi,
                                              int lane, double mn[3], double mx[3])
 {
     unsigned long long l[3] = {~0ull, ~0ull, ~0ull}, h[3] = {0ull, 0ull, 0ull};
@@ -859,7 +868,8 @@ __device__ __forceinline__ void wave_extents(const SubShared& S, uint32_t lo, ui
     }
 }
 
-__device__ __forceinline__ bool wave_pred(const SubShared& S, int f, double cv, int pass,
+template <class SS>
+__device__ __forceinline__ bool wave_pred(const SS& S, int f, double cv, int pass,
                                           uint32_t i)
 {
     const double v = sub_coord(S, f, i);
@@ -870,7 +880,8 @@ __device__ __forceinline__ bool wave_pred(const SubShared& S, int f, double cv, 
 // the predicate-true elements in [a, a + c), with the sequential two-pointer
 // loop's swaps: the r-th misplaced element of [a, a + c) from the left
 // trades places with the r-th misplaced element of [a + c, e) from the right.
-__device__ __forceinline__ uint32_t wave_pass(SubShared& S, int f, double cv, int pass,
+template <class SS>
+__device__ __forceinline__ uint32_t wave_pass(SS& S, int f, double cv, int pass,
                                               uint32_t a, uint32_t e, int lane)
 {
     uint32_t c = 0;
@@ -908,7 +919,8 @@ __device__ __forceinline__ uint32_t wave_pass(SubShared& S, int f, double cv, in
 // leaf (whose extents, for the parent's divhigh, are taken at once), so the
 // stack holds only disjoint nodes of more than kLeafMax points: at most
 // kWaveNodeMax / (kLeafMax + 1) < kWaveStack of them.
-__device__ void wave_build(SubShared& S, const KdView& t, uint32_t* hdr, uint32_t a0,
+template <class SS>
+__device__ void wave_build(SS& S, const KdView& t, uint32_t* hdr, uint32_t a0,
                            uint32_t nd, uint32_t lo, uint32_t hi, const double* bx, int lane,
                            WaveFrame* stk)
 {
@@ -982,7 +994,8 @@ __device__ void wave_build(SubShared& S, const KdView& t, uint32_t* hdr, uint32_
 }
 
 // The nodes queued for the waves, one wave per node until the queue is empty.
-__device__ __forceinline__ void wave_drain(SubShared& S, const KdView& t, uint32_t* hdr,
+template <class SS>
+__device__ __forceinline__ void wave_drain(SS& S, const KdView& t, uint32_t* hdr,
                                            uint32_t a0, int lane, int wv)
 {
     for (;;) {
@@ -998,18 +1011,22 @@ __device__ __forceinline__ void wave_drain(SubShared& S, const KdView& t, uint32
     }
 }
 
-__global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
+// At most 128 VGPRs per lane: a one-wave workgroup then fits in the registers
+// one finished search wave frees.
+template <uint32_t SUB, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(128))) void k_kd_subtree(
+    KdView t,
                                                            const uint32_t* __restrict__ small,
                                                            uint32_t* __restrict__ hdr,
                                                            uint32_t wave_node)
 {
-    __shared__ SubShared S;
+    __shared__ SubShared<SUB, NT> S;
     const uint32_t tid = threadIdx.x;
     const int lane = __lane_id(), wv = tid >> 6;
     const uint32_t nd0 = small[blockIdx.x];
     const uint32_t a0 = t.nl[nd0];
-    const uint32_t m = t.nr[nd0] - a0;  // <= kSub
-    for (uint32_t i = tid; i < kSub; i += kSubThreads) {
+    const uint32_t m = t.nr[nd0] - a0;  // <= SUB
+    for (uint32_t i = tid; i < SUB; i += NT) {
         if (i < m) {
             S.x[i] = t.px[a0 + i];
             S.y[i] = t.py[a0 + i];
@@ -1020,6 +1037,15 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
             S.node[i] = 0xFF;
         }
     }
+    if constexpr (NT == 64) {
+        // One wave: the node's whole subtree, depth first.
+        __syncthreads();
+        double bx[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) bx[i] = t.box[6 * static_cast<size_t>(nd0) + i];
+        wave_build(S, t, hdr, a0, nd0, 0u, m, bx, lane, S.frames[0]);
+        __syncthreads();
+    } else {
     if (tid == 0) {
         S.wq_next = 0;
         if (m <= wave_node) {  // small enough for one wave from the start
@@ -1045,7 +1071,7 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
         if (nc == 0) break;
         // Block level: nodes of more than wave_node points, all split.
         // (a) extents
-        for (uint32_t i = tid; i < nc * 3; i += kSubThreads) {
+        for (uint32_t i = tid; i < nc * 3; i += NT) {
             S.mn[i] = ~0ull;
             S.mx[i] = 0ull;
         }
@@ -1055,7 +1081,7 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
             S.wq_next = 0;
         }
         __syncthreads();
-        for (uint32_t c = wv; c * 64 < m; c += kSubThreads / 64) {
+        for (uint32_t c = wv; c * 64 < m; c += NT / 64) {
             const uint32_t i = c * 64 + lane;
             const uint32_t j = i < m ? S.node[i] : 0xFFu;
             unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
@@ -1113,13 +1139,13 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
         if (tid < nc)
             S.L[tid] = static_cast<uint16_t>(S.lo[tid] + (S.pre[S.hi[tid]] - S.pre[S.lo[tid]]));
         __syncthreads();
-        for (uint32_t i = tid; i < m; i += kSubThreads) {
+        for (uint32_t i = tid; i < m; i += NT) {
             const uint32_t j = S.node[i];
             if (j == 0xFFu) continue;
             if (i >= S.L[j] && p1(i)) S.tmp[S.lo[j] + (S.pre[S.hi[j]] - S.pre[i + 1])] = i;
         }
         __syncthreads();
-        for (uint32_t i = tid; i < m; i += kSubThreads) {
+        for (uint32_t i = tid; i < m; i += NT) {
             const uint32_t j = S.node[i];
             if (j == 0xFFu) continue;
             if (i < S.L[j] && !p1(i)) {
@@ -1138,13 +1164,13 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
         if (tid < nc)
             S.L2[tid] = static_cast<uint16_t>(S.L[tid] + (S.pre[S.hi[tid]] - S.pre[S.lo[tid]]));
         __syncthreads();
-        for (uint32_t i = tid; i < m; i += kSubThreads) {
+        for (uint32_t i = tid; i < m; i += NT) {
             const uint32_t j = S.node[i];
             if (j == 0xFFu) continue;
             if (i >= S.L2[j] && p2(i)) S.tmp[S.L[j] + (S.pre[S.hi[j]] - S.pre[i + 1])] = i;
         }
         __syncthreads();
-        for (uint32_t i = tid; i < m; i += kSubThreads) {
+        for (uint32_t i = tid; i < m; i += NT) {
             const uint32_t j = S.node[i];
             if (j == 0xFFu) continue;
             if (i >= S.L[j] && i < S.L2[j] && !p2(i)) {
@@ -1193,7 +1219,7 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
         }
         __syncthreads();
         // (f) points move to their child's block-level slot (0xFF: a wave's)
-        for (uint32_t i = tid; i < m; i += kSubThreads) {
+        for (uint32_t i = tid; i < m; i += NT) {
             const uint32_t j = S.node[i];
             if (j == 0xFFu) continue;
             S.node[i] = i < S.sp[j] ? S.child[j] : S.child2[j];
@@ -1211,7 +1237,8 @@ __global__ __launch_bounds__(kSubThreads) void k_kd_subtree(KdView t,
         }
         __syncthreads();
     }
-    for (uint32_t i = tid; i < m; i += kSubThreads) {
+    }  // one wave or the whole workgroup
+    for (uint32_t i = tid; i < m; i += NT) {
         t.px[a0 + i] = S.x[i];
         t.py[a0 + i] = S.y[i];
         t.pz[a0 + i] = S.z[i];
@@ -1506,7 +1533,7 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
                          hipStream_t s)
 {
     const uint32_t sub = std::min<uint32_t>(std::max<uint32_t>(kt.subtree_points, kLeafMax + 1),
-                                            kSub);
+                                            kt.small_blocks ? kSubSmall : kSub);
     hipLaunchKernelGGL(k_kd_init, dim3(div_up(n, kKdThreads)), dim3(kKdThreads), 0, s, x, y, z,
                        static_cast<uint32_t>(n), t);
     launch_fill_u32(w.hdr, kHdrWords, 0u, s);
@@ -1564,8 +1591,12 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
     if (nsmall) {
         const uint32_t wave_node = std::min<uint32_t>(
             std::max<uint32_t>(kt.wave_node, kLeafMax + 1), kWaveNodeMax);
-        hipLaunchKernelGGL(k_kd_subtree, dim3(nsmall), dim3(kSubThreads), 0, s, t, w.small,
-                           w.hdr, wave_node);
+        if (kt.small_blocks)
+            hipLaunchKernelGGL((k_kd_subtree<kSubSmall, kSubSmallThreads>), dim3(nsmall),
+                               dim3(kSubSmallThreads), 0, s, t, w.small, w.hdr, wave_node);
+        else
+            hipLaunchKernelGGL((k_kd_subtree<kSub, kSubThreads>), dim3(nsmall),
+                               dim3(kSubThreads), 0, s, t, w.small, w.hdr, wave_node);
         GF_HIP(hipGetLastError());
     }
     GF_HIP(hipMemcpyAsync(h, w.hdr, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -1575,7 +1606,7 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
     return 0;
 }
 
-uint32_t kd_list_capacity(uint64_t n) { return static_cast<uint32_t>(n / kSub * 4 + 4096); }
+uint32_t kd_list_capacity(uint64_t n) { return static_cast<uint32_t>(n / kSubSmall * 4 + 4096); }
 
 void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram& prog, int mode,
                        const KnnOutputs& out, const double* x, const double* y, const double* z,

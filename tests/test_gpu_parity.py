@@ -71,14 +71,19 @@ def _oracle_nodes(e):
     return ref
 
 
-@pytest.mark.parametrize("subtree,lds", [(4096, 512), (200, 0), (1000, 4)],
-                         ids=["default", "global-levels", "mixed"])
-def test_kdtree_matches_oracle(ctx, oracle, subtree, lds):
+@pytest.mark.parametrize("subtree,lds,small,wave", [(4096, 512, 0, 256), (200, 0, 0, 256),
+                                                    (1000, 4, 0, 101), (4096, 512, 0, 1024),
+                                                    (4096, 512, 1, 256)],
+                         ids=["default", "global-levels", "mixed", "wave-1024", "one-wave-blocks"])
+def test_kdtree_matches_oracle(ctx, oracle, subtree, lds, small, wave):
     """The GPU-built tree is nanoflann's: same vind order and node splits,
     whichever way the build divides the work (level-synchronous big nodes,
-    per-workgroup subtrees, LDS-staged or global extents)."""
+    per-workgroup subtrees with workgroup levels and one-wave subtrees, or
+    one-wave workgroups of <= 512 points; LDS-staged or global extents)."""
     ctx.set_tuning("kd_subtree", subtree)
     ctx.set_tuning("kd_minmax_lds", lds)
+    ctx.set_tuning("kd_small_blocks", small)
+    ctx.set_tuning("kd_wave_node", wave)
     try:
         for seed, n, dup in [(21, 30000, 0.05), (22, 5000, 0.3), (23, 150, 0.0),
                              (24, 120000, 0.02)]:
@@ -91,6 +96,8 @@ def test_kdtree_matches_oracle(ctx, oracle, subtree, lds):
     finally:
         ctx.set_tuning("kd_subtree", 4096)
         ctx.set_tuning("kd_minmax_lds", 512)
+        ctx.set_tuning("kd_small_blocks", -1)
+        ctx.set_tuning("kd_wave_node", 256)
 
 
 def test_knn_lattice_ties_vs_kd3(ctx, oracle):
