@@ -121,6 +121,8 @@ void Context::build_batch_index(Batch& b, const std::vector<uint32_t>& fetch)
     b.dbounds.ensure(b.bounds.size() * 8 + 8);
     GF_HIP(hipMemcpyAsync(b.dbounds.ptr, b.bounds.data(), b.bounds.size() * 8,
                           hipMemcpyHostToDevice, stream));
+    GF_HIP(hipEventRecord(join, stream));
+    points_mark = true;
 
     // Keys, tile-segmented sort, column starts, sorted coordinates.
     b.keys_a.ensure(N * 4 + 8);
@@ -214,6 +216,7 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         any_exact_tile = any_exact_tile || ti.all_exact;
         max_k = std::max<int>(max_k, static_cast<int>(k));
     }
+    points_mark = false;
     build_batch_index(b, fetch);
     const int Q = knn_queries_per_wave(max_k);
     std::vector<WaveTask> tasks;

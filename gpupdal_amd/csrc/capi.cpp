@@ -91,6 +91,8 @@ void Context::build_index(Cloud& cl, uint32_t fetch)
     cl.dbounds.ensure(64);
     GF_HIP(hipMemcpyAsync(cl.dbounds.ptr, bounds.ptr, 6 * sizeof(double),
                           hipMemcpyDeviceToDevice, stream));
+    GF_HIP(hipEventRecord(join, stream));
+    points_mark = true;
 
     cl.keys_a.ensure(n * 4);
     cl.vals_a.ensure(n * 4);
@@ -202,6 +204,7 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
 
     reset_spans();
     hipEvent_t t_all = mark();
+    points_mark = false;
     build_index(cl, fetch);
 
     KnnOutputs out{};
@@ -295,7 +298,7 @@ void Context::build_forest(KdBuffers& b, const double* x, const double* y, const
         kd_s2.ensure((n + 1) * 4);
         kd_tmp.ensure((n + 1) * 4);
         kd_lists.ensure((static_cast<uint64_t>(lcap) * 5 + 2 + kKdHeaderWords) * 4);
-        scan_tmp.ensure(scan_scratch_words(n + 1) * 4);
+        kd_sums.ensure(scan_scratch_words(n + 1) * 4);
         KdView t{};
         t.vind = b.vind.as<uint32_t>();
         t.px = b.px.as<double>();
@@ -321,7 +324,7 @@ void Context::build_forest(KdBuffers& b, const double* x, const double* y, const
         w.s1 = kd_s1.as<uint32_t>();
         w.s2 = kd_s2.as<uint32_t>();
         w.tmp = kd_tmp.as<uint32_t>();
-        w.sums = scan_tmp.as<uint32_t>();
+        w.sums = kd_sums.as<uint32_t>();
         uint32_t* L = kd_lists.as<uint32_t>();
         w.hdr = L;
         w.act[0] = L + kKdHeaderWords;
@@ -367,8 +370,9 @@ void Context::build_kdtree(Cloud& cl, hipStream_t s, bool beside)
 
 void Context::fork_side()
 {
-    GF_HIP(hipEventRecord(join, stream));
+    if (!points_mark) GF_HIP(hipEventRecord(join, stream));
     GF_HIP(hipStreamWaitEvent(side, join, 0));
+    points_mark = false;
 }
 
 // Re-solves the rows the grid search flagged (boundary ties, uncertified
@@ -624,6 +628,7 @@ int gf_knn(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t stride, uint32_t* 
         const uint32_t fetch = kk * stride;
         if (fetch > static_cast<uint32_t>(kMaxFetch - 1))
             throw Error(GF_ERR_UNSUPPORTED, "k*stride above 63 is outside the device kNN envelope");
+        c->points_mark = false;
         c->build_index(*cl, fetch);
         FusedProgram prog{};
         prog.k = static_cast<int32_t>(kk);

@@ -215,6 +215,10 @@ struct Context {
     // the points, is built on it while the search runs on `stream`.
     hipStream_t side = nullptr;
     hipEvent_t join = nullptr;
+    // join was recorded on `stream` right after the current points' bounds
+    // reached the device (the tree needs nothing else), so the side stream
+    // may start during the rest of the index build.
+    bool points_mark = false;
     std::vector<hipEvent_t> evpool;
     size_t evnext = 0;
     std::vector<Span> spans;
@@ -223,7 +227,9 @@ struct Context {
     gf_stats stats{};
     std::string last_error;
     DevBuf part, bounds, hist, scan_tmp, outbuf, statbuf, flagged, counters, covbuf;
-    DevBuf kd_s1, kd_s2, kd_tmp, kd_lists;
+    // Exact-tree scratch, its own: the tree may be built on the side stream
+    // while the index build uses scan_tmp on the main stream.
+    DevBuf kd_s1, kd_s2, kd_tmp, kd_lists, kd_sums;
     double* host_bounds = nullptr;
 
     explicit Context(int dev) : device(dev)
@@ -270,7 +276,8 @@ struct Context {
     void build_forest(KdBuffers& b, const double* x, const double* y, const double* z,
                       uint64_t n, const uint64_t* dstarts, uint32_t T, const double* dbounds,
                       hipStream_t s, bool beside);
-    // Makes the side stream wait for the work queued on `stream` so far.
+    // Makes the side stream wait for the points and bounds (points_mark), or
+    // for all work queued on `stream` so far.
     void fork_side();
     void resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog, int mode,
                          const KnnOutputs& out, hipEvent_t from);

@@ -213,3 +213,34 @@ def test_autotune_changes_speed_not_results(ctx, tmp_path):
     text = path.read_text()
     assert len([l for l in text.splitlines() if not l.startswith("#")]) >= 1
     ctx.load_tuning(path)
+
+
+def test_tree_beside_the_search_repeats_exactly(ctx, oracle):
+    """Large clouds build the exact tree on a second stream from the moment
+    their bounds are known, while the index build and the search run on the
+    main stream: repeated runs give bit-identical columns and statuses, the
+    same as with the tree built after the search, and the tree is
+    nanoflann's."""
+    n = 1_000_000
+    x, y, z = g.synth_cloud(n, tile=7, side=250.0)
+    cl = ctx.cloud(x, y, z)
+    st = [g.stage(g.STAGE_NORMAL, 16), g.stage(g.STAGE_COVFEAT, 16)]
+    ref = None
+    try:
+        for overlap in (131072, 131072, 131072, 0):
+            ctx.set_tuning("kd_overlap_min", overlap)
+            cl.invalidate()
+            (nc, fc), (ns, fs) = cl.run(st, init=np.nan)
+            assert ctx.stats()["ms_tree"] > 0.0
+            got = [np.asarray(c) for c in nc + fc] + [ns, fs]
+            if ref is None:
+                ref = got
+            else:
+                for u, v in zip(ref, got):
+                    assert np.array_equal(u, v, equal_nan=True)
+    finally:
+        ctx.set_tuning("kd_overlap_min", 131072)
+    sub = slice(0, 200_000)
+    t = ctx.cloud(x[sub], y[sub], z[sub]).kdtree()
+    e = oracle.kd3(x[sub], y[sub], z[sub]).export()
+    assert np.array_equal(t["vind"].astype(np.uint64), e["vind"])
