@@ -135,9 +135,11 @@ void Context::build_batch_index(Batch& b, const std::vector<uint32_t>& fetch)
     b.sz.ensure(N * 8 + 8);
     b.hist.ensure(static_cast<uint64_t>(nb) * 256 * 4 + 8);
     scan_tmp.ensure(scan_scratch_words(static_cast<uint64_t>(nb) * 256 + 1) * 4);
+    xyz_stage.ensure(N * 24 + 8);
     launch_tile_keys(b.x.as<double>(), b.y.as<double>(), b.z.as<double>(),
                      b.dblocks.as<SegBlock>(), nb, b.dgrids.as<TileGrid>(),
-                     b.keys_a.as<uint32_t>(), b.vals_a.as<uint32_t>(), stream);
+                     b.keys_a.as<uint32_t>(), b.vals_a.as<uint32_t>(), xyz_stage.as<double>(),
+                     stream);
     uint32_t *kres, *vres;
     launch_seg_radix_sort(b.keys_a.as<uint32_t>(), b.vals_a.as<uint32_t>(),
                           b.keys_b.as<uint32_t>(), b.vals_b.as<uint32_t>(),
@@ -145,10 +147,9 @@ void Context::build_batch_index(Batch& b, const std::vector<uint32_t>& fetch)
                           scan_tmp.as<uint32_t>(), &kres, &vres, stream);
     launch_tile_colstart(kres, b.dgrids.as<TileGrid>(), T, colbase, b.colstart.as<uint32_t>(),
                          stream);
-    launch_gather_sorted(vres, b.x.as<double>(), b.y.as<double>(), b.z.as<double>(), N,
-                         b.sx.as<double>(), b.sy.as<double>(), b.sz.as<double>(), stream);
     b.inv.ensure(N * 4 + 8);
-    launch_invert(vres, N, b.inv.as<uint32_t>(), stream);
+    launch_gather_sorted(vres, xyz_stage.as<double>(), N, b.sx.as<double>(), b.sy.as<double>(),
+                         b.sz.as<double>(), b.inv.as<uint32_t>(), stream);
     b.keys = kres;
     b.perm = vres;
     b.idx_valid = true;

@@ -105,19 +105,19 @@ void Context::build_index(Cloud& cl, uint32_t fetch)
     hist.ensure(radix_hist_words(n) * 4);
     scan_tmp.ensure(scan_scratch_words(std::max<uint64_t>(radix_hist_words(n), g.ncols)) * 4);
 
+    xyz_stage.ensure(n * 24 + 8);
     launch_keys(cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), n, g,
-                cl.keys_a.as<uint32_t>(), cl.vals_a.as<uint32_t>(), stream);
+                cl.keys_a.as<uint32_t>(), cl.vals_a.as<uint32_t>(), xyz_stage.as<double>(),
+                stream);
     uint32_t *kres, *vres;
     launch_radix_sort(cl.keys_a.as<uint32_t>(), cl.vals_a.as<uint32_t>(),
                       cl.keys_b.as<uint32_t>(), cl.vals_b.as<uint32_t>(), n,
                       g.key_bits, hist.as<uint32_t>(), scan_tmp.as<uint32_t>(), &kres,
                       &vres, stream);
     launch_colstart(kres, n, g, cl.colstart.as<uint32_t>(), stream);
-    launch_gather_sorted(vres, cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(),
-                         n, cl.sx.as<double>(), cl.sy.as<double>(), cl.sz.as<double>(),
-                         stream);
     cl.inv.ensure(n * 4 + 8);
-    launch_invert(vres, n, cl.inv.as<uint32_t>(), stream);
+    launch_gather_sorted(vres, xyz_stage.as<double>(), n, cl.sx.as<double>(),
+                         cl.sy.as<double>(), cl.sz.as<double>(), cl.inv.as<uint32_t>(), stream);
     cl.keys = kres;
     cl.perm = vres;
     cl.g = g;

@@ -10,8 +10,11 @@ constexpr uint32_t kBoundsMaxParts = 1024;
 // grid.hip
 void launch_bounds(const double* x, const double* y, const double* z,
                    uint64_t n, double* part, double* out, hipStream_t s);
+// Keys and ids, plus an interleaved copy of the points (xyz: 3 doubles per
+// point) for the gather after the sort.
 void launch_keys(const double* x, const double* y, const double* z, uint64_t n,
-                 const GridParams& g, uint32_t* keys, uint32_t* ids, hipStream_t s);
+                 const GridParams& g, uint32_t* keys, uint32_t* ids, double* xyz,
+                 hipStream_t s);
 void launch_colstart(const uint32_t* keys, uint64_t n, const GridParams& g,
                      uint32_t* colstart, hipStream_t s);
 uint64_t scan_scratch_words(uint64_t n);
@@ -24,9 +27,10 @@ void launch_radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                        uint64_t n, uint32_t bits, uint32_t* hist,
                        uint32_t* scan_scratch, uint32_t** kres, uint32_t** vres,
                        hipStream_t s);
-void launch_gather_sorted(const uint32_t* perm, const double* x, const double* y,
-                          const double* z, uint64_t n, double* sx, double* sy,
-                          double* sz, hipStream_t s);
+// Sorted coordinates from the interleaved copy (one cache line per point
+// read at random instead of three), and the inverse permutation.
+void launch_gather_sorted(const uint32_t* perm, const double* xyz, uint64_t n, double* sx,
+                          double* sy, double* sz, uint32_t* inv, hipStream_t s);
 void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 
 // Tiled batches: segment blocks of <= 4096 points that never cross a tile,
@@ -51,7 +55,7 @@ void launch_tile_bounds(const double* x, const double* y, const double* z,
                         hipStream_t s);
 void launch_tile_keys(const double* x, const double* y, const double* z, const SegBlock* blocks,
                       uint32_t nblocks, const TileGrid* grids, uint32_t* keys, uint32_t* ids,
-                      hipStream_t s);
+                      double* xyz, hipStream_t s);
 void launch_seg_radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                            const SegBlock* blocks, uint32_t nblocks, uint32_t bits,
                            uint32_t* hist, uint32_t* scan_scratch, uint32_t** kres,
@@ -141,8 +145,6 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
 void launch_features(const CloudView& cv, const FusedProgram& prog,
                      const KnnOutputs& out, const uint32_t* inv, hipStream_t s,
                      const TileTables* tt = nullptr);
-// inv[perm[s]] = s.
-void launch_invert(const uint32_t* perm, uint64_t n, uint32_t* inv, hipStream_t s);
 
 // kdtree.hip -- the exact KD3Index tree (nanoflann 1.3.1 semantics).
 struct KdView {

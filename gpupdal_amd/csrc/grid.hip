@@ -85,24 +85,29 @@ void launch_bounds(const double* x, const double* y, const double* z,
 __global__ __launch_bounds__(256) void k_keys(
     const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, uint64_t n, GridParams g,
-    uint32_t* __restrict__ keys, uint32_t* __restrict__ ids)
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ ids, double* __restrict__ xyz)
 {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t cx = cell_coord(x[i], g.ox, g.inv_c, g.gx);
-    uint32_t cy = cell_coord(y[i], g.oy, g.inv_c, g.gy);
-    uint32_t cz = cell_coord(z[i], g.oz, g.inv_cz, g.gz);
+    const double px = x[i], py = y[i], pz = z[i];
+    uint32_t cx = cell_coord(px, g.ox, g.inv_c, g.gx);
+    uint32_t cy = cell_coord(py, g.oy, g.inv_c, g.gy);
+    uint32_t cz = cell_coord(pz, g.oz, g.inv_cz, g.gz);
     uint32_t col = column_index(g, cx, cy);
     keys[i] = (col << g.bz) | cz;
     ids[i] = static_cast<uint32_t>(i);
+    xyz[3 * i] = px;
+    xyz[3 * i + 1] = py;
+    xyz[3 * i + 2] = pz;
 }
 
 void launch_keys(const double* x, const double* y, const double* z, uint64_t n,
-                 const GridParams& g, uint32_t* keys, uint32_t* ids, hipStream_t s)
+                 const GridParams& g, uint32_t* keys, uint32_t* ids, double* xyz,
+                 hipStream_t s)
 {
     if (!n) return;
     hipLaunchKernelGGL(k_keys, dim3(div_up(n, 256)), dim3(256), 0, s, x, y, z, n,
-                       g, keys, ids);
+                       g, keys, ids, xyz);
 }
 
 // colstart[c] = first sorted position whose column is >= c (c in [0, ncols]),
@@ -429,17 +434,21 @@ __global__ __launch_bounds__(kRsThreads) void k_tile_keys(
     const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const SegBlock* __restrict__ blocks,
     const TileGrid* __restrict__ grids, uint32_t* __restrict__ keys,
-    uint32_t* __restrict__ ids)
+    uint32_t* __restrict__ ids, double* __restrict__ xyz)
 {
     const SegBlock b = blocks[blockIdx.x];
     const GridParams g = grids[b.tile].g;
     for (uint32_t i = threadIdx.x; i < b.count; i += kRsThreads) {
         const uint64_t p = static_cast<uint64_t>(b.begin) + i;
-        const uint32_t cx = cell_coord(x[p], g.ox, g.inv_c, g.gx);
-        const uint32_t cy = cell_coord(y[p], g.oy, g.inv_c, g.gy);
-        const uint32_t cz = cell_coord(z[p], g.oz, g.inv_cz, g.gz);
+        const double px = x[p], py = y[p], pz = z[p];
+        const uint32_t cx = cell_coord(px, g.ox, g.inv_c, g.gx);
+        const uint32_t cy = cell_coord(py, g.oy, g.inv_c, g.gy);
+        const uint32_t cz = cell_coord(pz, g.oz, g.inv_cz, g.gz);
         keys[p] = (column_index(g, cx, cy) << g.bz) | cz;
         ids[p] = static_cast<uint32_t>(p);
+        xyz[3 * p] = px;
+        xyz[3 * p + 1] = py;
+        xyz[3 * p + 2] = pz;
     }
 }
 
@@ -571,25 +580,26 @@ __global__ __launch_bounds__(256) void k_tile_colstart(
 
 // -------------------------------------------------------------- gather -----
 __global__ __launch_bounds__(256) void k_gather_sorted(
-    const uint32_t* __restrict__ perm, const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z, uint64_t n,
-    double* __restrict__ sx, double* __restrict__ sy, double* __restrict__ sz)
+    const uint32_t* __restrict__ perm, const double* __restrict__ xyz, uint64_t n,
+    double* __restrict__ sx, double* __restrict__ sy, double* __restrict__ sz,
+    uint32_t* __restrict__ inv)
 {
     uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (j >= n) return;
-    uint32_t i = perm[j];
-    sx[j] = x[i];
-    sy[j] = y[i];
-    sz[j] = z[i];
+    const uint32_t i = perm[j];
+    const double* p = xyz + 3 * static_cast<uint64_t>(i);
+    sx[j] = p[0];
+    sy[j] = p[1];
+    sz[j] = p[2];
+    inv[i] = static_cast<uint32_t>(j);
 }
 
-void launch_gather_sorted(const uint32_t* perm, const double* x, const double* y,
-                          const double* z, uint64_t n, double* sx, double* sy,
-                          double* sz, hipStream_t s)
+void launch_gather_sorted(const uint32_t* perm, const double* xyz, uint64_t n, double* sx,
+                          double* sy, double* sz, uint32_t* inv, hipStream_t s)
 {
     if (!n) return;
-    hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(n, 256)), dim3(256), 0, s, perm, x,
-                       y, z, n, sx, sy, sz);
+    hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(n, 256)), dim3(256), 0, s, perm, xyz, n,
+                       sx, sy, sz, inv);
 }
 
 // Writes colstart[ncols] = n after the exclusive scan of the column counts.
@@ -644,11 +654,11 @@ void launch_tile_bounds(const double* x, const double* y, const double* z,
 
 void launch_tile_keys(const double* x, const double* y, const double* z, const SegBlock* blocks,
                       uint32_t nblocks, const TileGrid* grids, uint32_t* keys, uint32_t* ids,
-                      hipStream_t s)
+                      double* xyz, hipStream_t s)
 {
     if (!nblocks) return;
     hipLaunchKernelGGL(k_tile_keys, dim3(nblocks), dim3(kRsThreads), 0, s, x, y, z, blocks,
-                       grids, keys, ids);
+                       grids, keys, ids, xyz);
 }
 
 void launch_seg_radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,

@@ -770,13 +770,6 @@ __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov
     }
 }
 
-__global__ __launch_bounds__(256) void k_invert(const uint32_t* __restrict__ perm, uint64_t n,
-                                                uint32_t* __restrict__ inv)
-{
-    const uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (s < n) inv[perm[s]] = static_cast<uint32_t>(s);
-}
-
 __global__ __launch_bounds__(256) void k_eigen_batch(const double* __restrict__ cov,
                                                      uint64_t m, double* __restrict__ ev,
                                                      double* __restrict__ vec,
@@ -839,12 +832,6 @@ void launch_features(const CloudView& cv, const FusedProgram& prog,
     hipLaunchKernelGGL(k_features, dim3(div_up(cv.n, 256)), dim3(256), 0, s, out.cov, inv,
                        cv.n, prog, out, tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
                        tt ? tt->ntiles : 0u);
-}
-
-void launch_invert(const uint32_t* perm, uint64_t n, uint32_t* inv, hipStream_t s)
-{
-    if (!n) return;
-    hipLaunchKernelGGL(k_invert, dim3(div_up(n, 256)), dim3(256), 0, s, perm, n, inv);
 }
 
 void launch_eigen_batch(const double* cov, uint64_t m, double* evals,
