@@ -220,7 +220,8 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     points_mark = false;
     build_batch_index(b, fetch);
     const int Q = knn_queries_per_wave(max_k);
-    std::vector<WaveTask> tasks;
+    std::vector<uint32_t> task0(static_cast<size_t>(T) + 1, 0u);
+    uint64_t ntasks = 0;
     for (uint32_t t = 0; t < T; ++t) {
         TileInfo& ti = tiles[t];
         ti.g = b.grids[t].g;
@@ -228,24 +229,24 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         const GridParams& g = ti.g;
         ti.margin = 1e-6 * std::fmin(g.c, g.cz) +
                     1e-15 * (std::fabs(g.ox) + std::fabs(g.oy) + std::fabs(g.oz)) + 1e-300;
-        if (ti.all_exact || !ti.n) continue;
-        for (uint64_t q = 0; q < ti.n; q += Q)
-            tasks.push_back(WaveTask{ti.start + q, t,
-                                     static_cast<uint32_t>(std::min<uint64_t>(Q, ti.n - q))});
+        task0[t] = static_cast<uint32_t>(ntasks);
+        if (!ti.all_exact) ntasks += (ti.n + Q - 1) / Q;
     }
+    task0[T] = static_cast<uint32_t>(ntasks);
+    if (ntasks > 0xFFFFFF00ull)
+        throw Error(GF_ERR_UNSUPPORTED, "batch above the search's task range");
     b.dtiles.ensure(static_cast<uint64_t>(T) * sizeof(TileInfo) + 8);
     GF_HIP(hipMemcpyAsync(b.dtiles.ptr, tiles.data(), static_cast<uint64_t>(T) * sizeof(TileInfo),
                           hipMemcpyHostToDevice, stream));
-    b.dtasks.ensure(tasks.size() * sizeof(WaveTask) + 16);
-    if (!tasks.empty())
-        GF_HIP(hipMemcpyAsync(b.dtasks.ptr, tasks.data(), tasks.size() * sizeof(WaveTask),
-                              hipMemcpyHostToDevice, stream));
+    b.dtasks.ensure(task0.size() * sizeof(uint32_t) + 16);
+    GF_HIP(hipMemcpyAsync(b.dtasks.ptr, task0.data(), task0.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, stream));
     TileTables tt{};
     tt.tiles = b.dtiles.as<TileInfo>();
     tt.starts = b.dstarts.as<uint64_t>();
     tt.ntiles = T;
-    tt.tasks = b.dtasks.as<WaveTask>();
-    tt.ntasks = static_cast<uint32_t>(tasks.size());
+    tt.task0 = b.dtasks.as<uint32_t>();
+    tt.ntasks = static_cast<uint32_t>(ntasks);
     tt.max_k = max_k;
     tt.q_per_wave = Q;
 

@@ -113,22 +113,19 @@ struct TileInfo {
 };
 
 // The queries of one wavefront in the tiled search: [q0, q0 + nq) of tile.
-struct WaveTask {
-    uint64_t q0;
-    uint32_t tile;
-    uint32_t nq;
-};
-
 // Batch tables (device memory); tiles == nullptr means a single cloud whose
 // grid and program come from the CloudView / FusedProgram arguments.
 struct TileTables {
     const TileInfo* tiles;
     const uint64_t* starts;  // T + 1 tile starts (for row -> tile lookups)
     uint32_t ntiles;
-    const WaveTask* tasks;
+    // Wave tasks: tile t's rows are cut into runs of q_per_wave queries,
+    // tasks task0[t] .. task0[t+1]-1 (T + 1 entries; 0 tasks for tiles
+    // searched only exactly).
+    const uint32_t* task0;
     uint32_t ntasks;
     int32_t max_k;           // largest k over the tiles (LDS sizing)
-    int32_t q_per_wave;      // queries per task (the tasks were cut with it)
+    int32_t q_per_wave;      // queries per task
 };
 
 // Queries per wavefront task for a neighborhood of k (LDS lists scale with k).

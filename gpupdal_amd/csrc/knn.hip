@@ -83,8 +83,9 @@ struct KnnArgs {
     int32_t lds_stride;
     double margin;
     const TileInfo* tiles;   // tiled search only
-    const WaveTask* tasks;
+    const uint32_t* task0;   // tiled: per-tile first task (ntiles + 1)
     uint32_t ntasks;
+    uint32_t ntiles;
 };
 
 // What a wavefront needs of its cloud or tile: the grid, the column starts,
@@ -487,16 +488,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     if (kTiled) {
         // The wave's queries and its tile's grid and program.
         if (task >= a_in.ntasks) return;
-        const WaveTask tk = a_in.tasks[task];
-        const TileInfo& ti = a_in.tiles[tk.tile];
+        // The task's tile: the last t with task0[t] <= task (tiles with no
+        // task share the next tile's prefix).
+        uint32_t tlo = 0, thi = a_in.ntiles;
+        while (thi - tlo > 1) {
+            const uint32_t mid = (tlo + thi) >> 1;
+            if (a_in.task0[mid] <= task)
+                tlo = mid;
+            else
+                thi = mid;
+        }
+        const TileInfo& ti = a_in.tiles[tlo];
+        const uint64_t first = static_cast<uint64_t>(task - a_in.task0[tlo]) * Q;
         wg.g = ti.g;
         wg.colstart = a_in.cv.colstart + ti.colbase;
         wg.margin = ti.margin;
         wg.fetch = ti.prog.fetch;
         wg.k = ti.prog.k;
         wg.stride = ti.prog.stride;
-        q0 = tk.q0;
-        nq = static_cast<int>(tk.nq);
+        q0 = ti.start + first;
+        nq = static_cast<int>(ti.n - first < static_cast<uint64_t>(Q) ? ti.n - first : Q);
     } else {
         wg.g = a_in.cv.g;
         wg.colstart = a_in.cv.colstart;
@@ -799,8 +810,9 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
     a.tune = tune;
     a.mode = mode;
     a.tiles = nullptr;
-    a.tasks = nullptr;
+    a.task0 = nullptr;
     a.ntasks = 0;
+    a.ntiles = 0;
     const int max_k = tt ? tt->max_k : prog.k;
     a.q_per_wave = tt ? tt->q_per_wave : knn_queries_per_wave(prog.k);
     a.lds_stride = max_k | 1;  // odd stride: lanes hit distinct LDS banks
@@ -813,8 +825,9 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
     if (tt) {
         if (!tt->ntasks) return;
         a.tiles = tt->tiles;
-        a.tasks = tt->tasks;
+        a.task0 = tt->task0;
         a.ntasks = tt->ntasks;
+        a.ntiles = tt->ntiles;
         const uint32_t blocks = div_up(tt->ntasks, kWaves);
         hipLaunchKernelGGL(k_knn<true>, dim3(blocks), dim3(kBlock), lds, s, a);
         return;
