@@ -354,6 +354,31 @@ __device__ __forceinline__ void select_entries(const uint64_t (&cd)[kRegBatches]
     }
 }
 
+__device__ __forceinline__ uint32_t lanes_below64(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+// The n (<= 64) entries waiting in scratch, sorted and merged into best.
+__device__ __forceinline__ void stream_flush(const uint32_t* scratch, uint32_t n, int lane,
+                                             Ent& best)
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    Ent c = ent_inf();
+    if (static_cast<uint32_t>(lane) < n) {
+        const uint32_t* e = scratch + 4 * lane;
+        c.d = static_cast<uint64_t>(e[0]) | (static_cast<uint64_t>(e[1]) << 32);
+        c.id = e[2];
+        c.j = e[3];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    bitonic_sort64(c, lane);
+    merge_sorted(best, c, lane);
+}
+
 // One query alone, with a growing box starting at half-width r (columns and
 // z cells around the query's cell): candidates stream through the registers
 // (<= 256, one box of <= 64 columns) or through sort-and-merge otherwise.
@@ -365,6 +390,7 @@ __device__ __forceinline__ void search_single(const KnnArgs& a, const WaveGrid& 
     const CloudView& cv = a.cv;
     const int fetch = wg.fetch;
     incomplete = false;
+    uint32_t nbuf = 0;  // streamed candidates waiting in LDS
     for (;;) {
         cnt[0] += 1;
         const Box b{qcx - r, qcx + r, qcy - r, qcy + r, qcz - r, qcz + r};
@@ -397,6 +423,10 @@ __device__ __forceinline__ void search_single(const KnnArgs& a, const WaveGrid& 
                 select_entries(cd, cid, cj, tot, fetch, lane, scratch, INFINITY, best, cnt);
                 break;
             }
+            // Streamed: candidates below the current (fetch+1)-th entry are
+            // collected in LDS and sorted into the list 64 at a time.  That
+            // entry only decreases, so a candidate not below it when it
+            // is seen cannot be among the final fetch+1.
             for (uint32_t t0 = 0; t0 < tot; t0 += 64) {
                 const uint32_t t = t0 + lane;
                 const uint32_t j = candidate_pos(lo, excl, t);
@@ -405,13 +435,28 @@ __device__ __forceinline__ void search_single(const KnnArgs& a, const WaveGrid& 
                     c = make_ent(qx, qy, qz, cv.sx[j], cv.sy[j], cv.sz[j], cv.perm[j], j);
                 const Ent worst = ent_shfl(best, fetch);
                 const bool surv = t < tot && ent_less(c, worst);
-                if (__ballot(surv)) {
+                const uint64_t m = __ballot(surv);
+                if (!m) continue;
+                const uint32_t ns = static_cast<uint32_t>(__popcll(m));
+                if (nbuf + ns > 64u) {
+                    stream_flush(scratch, nbuf, lane, best);
                     cnt[2] += 1;
-                    if (!surv) c = ent_inf();
-                    bitonic_sort64(c, lane);
-                    merge_sorted(best, c, lane);
+                    nbuf = 0;
                 }
+                if (surv) {
+                    uint32_t* e = scratch + 4 * (nbuf + lanes_below64(m));
+                    e[0] = static_cast<uint32_t>(c.d);
+                    e[1] = static_cast<uint32_t>(c.d >> 32);
+                    e[2] = c.id;
+                    e[3] = c.j;
+                }
+                nbuf += ns;
             }
+        }
+        if (nbuf) {
+            stream_flush(scratch, nbuf, lane, best);
+            cnt[2] += 1;
+            nbuf = 0;
         }
         const Ent kth = ent_shfl(best, fetch - 1);
         if (certified(wg, b, kth, qx, qy, qz)) return;
