@@ -780,7 +780,41 @@ template <uint32_t SUB, int NT> struct SubShared {
     uint16_t wq_lo[kWaveQ], wq_hi[kWaveQ];
     WaveFrame frames[NT / 64][kWaveStack];
     uint32_t nc, nnext, nwq, wq_next;
+    uint32_t id_base, id_next, id_cap;  // node ids reserved for this subtree
 };
+
+// Node ids.  One device-wide counter handing out two ids per split would be
+// hit by every split of every workgroup; on a chip of eight XCDs each with
+// its own L2, atomics on one address are performed one after another past the
+// L2s, so each workgroup instead reserves about what its subtree needs
+// (KD3 trees of leaf size <= 100 have ~0.035 nodes per point; the reservation
+// is 0.042 per point + 2) and hands ids out from LDS.  A subtree that needs
+// more takes the rest from the device counter, one pair at a time.
+__device__ __forceinline__ uint32_t sub_reservation(uint32_t m)
+{
+    return m > kLeafMax ? 2u * ((m + 47u) / 48u) + 2u : 0u;
+}
+
+template <class SS> __device__ __forceinline__ uint32_t sub_alloc2(SS& S, uint32_t* hdr)
+{
+    const uint32_t k = atomicAdd(&S.id_next, 2u);
+    return k + 2u <= S.id_cap ? S.id_base + k : atomicAdd(&hdr[kHdrNodes], 2u);
+}
+
+// Reserved ids the subtree did not use: records that no node links to, with
+// an empty point range.
+template <class SS>
+__device__ __forceinline__ void sub_blank_unused(const SS& S, const KdView& t, uint32_t tid)
+{
+    for (uint32_t i = min(S.id_next, S.id_cap) + tid; i < S.id_cap; i += SS::kThreads) {
+        const uint32_t nd = S.id_base + i;
+        if (nd >= t.cap) break;
+        t.nl[nd] = t.nr[nd] = 0;
+        t.parent[nd] = -1;
+        t.c1[nd] = t.c2[nd] = -1;
+        t.feat[nd] = -1;
+    }
+}
 
 template <class SS> __device__ __forceinline__ double sub_coord(const SS& S, int f, uint32_t i)
 {
@@ -937,7 +971,7 @@ __device__ void wave_build(SS& S, const KdView& t, uint32_t* hdr, uint32_t a0,
         if (cnt > kLeafMax) {
             // The children's ids: the atomic's round trip overlaps the passes.
             uint32_t c1 = 0;
-            if (lane == 0) c1 = atomicAdd(&hdr[kHdrNodes], 2u);
+            if (lane == 0) c1 = sub_alloc2(S, hdr);
             int f;
             double cv;
             kd_choose_split(box, mn, mx, f, cv);
@@ -1026,6 +1060,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(128))) void k_kd
     const uint32_t nd0 = small[blockIdx.x];
     const uint32_t a0 = t.nl[nd0];
     const uint32_t m = t.nr[nd0] - a0;  // <= SUB
+    if (tid == 0) {
+        S.id_cap = sub_reservation(m);
+        S.id_base = S.id_cap ? atomicAdd(&hdr[kHdrNodes], S.id_cap) : 0u;
+        S.id_next = 0;
+    }
     for (uint32_t i = tid; i < SUB; i += NT) {
         if (i < m) {
             S.x[i] = t.px[a0 + i];
@@ -1185,7 +1224,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(128))) void k_kd
             const uint32_t nd = S.nid[tid];
             const uint32_t lo = S.lo[tid], hi = S.hi[tid];
             const uint32_t idx = kd_split_index(hi - lo, S.L[tid] - lo, S.L2[tid] - lo);
-            const uint32_t c1 = atomicAdd(&hdr[kHdrNodes], 2u);
+            const uint32_t c1 = sub_alloc2(S, hdr);
             S.sp[tid] = static_cast<uint16_t>(lo + idx);
             S.child[tid] = S.child2[tid] = 0xFF;
             if (c1 + 1 >= t.cap) {
@@ -1238,6 +1277,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(128))) void k_kd
         __syncthreads();
     }
     }  // one wave or the whole workgroup
+    sub_blank_unused(S, t, tid);
     for (uint32_t i = tid; i < m; i += NT) {
         t.px[a0 + i] = S.x[i];
         t.py[a0 + i] = S.y[i];
@@ -1522,7 +1562,7 @@ void launch_flag_all(uint64_t n, const uint32_t* perm, int mode, const KnnOutput
                        n, perm, mode, out);
 }
 
-uint32_t kd_node_capacity(uint64_t n) { return static_cast<uint32_t>(n / 16 + 1024); }
+uint32_t kd_node_capacity(uint64_t n) { return static_cast<uint32_t>(n / 12 + 1024); }
 
 // Builds the tree; returns 0, or an overflow code (kOvfNodes: node
 // capacity, kOvfLists: big/small node lists) after which the caller grows
@@ -1602,7 +1642,8 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
     GF_HIP(hipMemcpyAsync(h, w.hdr, sizeof(h), hipMemcpyDeviceToHost, s));
     GF_HIP(hipStreamSynchronize(s));
     if (h[kHdrOverflow]) return h[kHdrOverflow];
-    *nodes = h[kHdrNodes];
+    // Subtree reservations may run past the capacity without using those ids.
+    *nodes = std::min<uint32_t>(h[kHdrNodes], t.cap);
     return 0;
 }
 

@@ -233,8 +233,9 @@ int gf_debug_grid(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, uint32_t *perm,
  * calibrates HBM byte counters). */
 int gf_debug_copy(gf_ctx *ctx, double *dst, const double *src, uint64_t count);
 /* The cloud's exact KD3Index tree (built on demand): vind (n, tree order ->
- * point id) and the nodes in level order; pass NULL arrays to query the node
- * count.  Leaves have feat = -1. */
+ * point id) and the node records; pass NULL arrays to query the record
+ * count.  Leaves have feat = -1; records with an empty range [left, right)
+ * are ids reserved during the build and not used by any node. */
 int gf_debug_kdtree(gf_ctx *ctx, gf_cloud *cloud, uint32_t *vind, uint32_t *nodes,
                     uint32_t *left, uint32_t *right, int32_t *feat, double *divlow,
                     double *divhigh);
