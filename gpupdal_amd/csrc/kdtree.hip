@@ -328,45 +328,11 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_big_split(KdView t,
         kd_split_node(t, act[k]);
 }
 
-// Active index -> (node, position).
-struct ActPos {
-    uint32_t nd, lo, t0, t1, p;
-};
-
-// The node of active index ti, searching forward from node kb (the node of
-// an index <= ti, found once per block): big nodes hold more points than a
-// block covers, so this is one or two steps.
-__device__ __forceinline__ ActPos act_pos(const KdView& t, const uint32_t* act,
-                                          const uint32_t* off, uint32_t A, uint32_t kb,
-                                          uint32_t ti)
-{
-    uint32_t k = kb;
-    while (k + 1 < A && off[k + 1] <= ti) ++k;
-    ActPos r;
-    r.nd = act[k];
-    r.lo = t.nl[r.nd];
-    r.t0 = off[k];
-    r.t1 = off[k + 1];
-    r.p = r.lo + (ti - r.t0);
-    return r;
-}
-
-// planeSplit's two Hoare passes as predicates: pass 1 "coord < cut" over
-// the node, pass 2 "coord <= cut" over [L, right) where L = lo + #pass 1.
-__device__ __forceinline__ bool kd_pred(const KdView& t, int pass, const ActPos& a,
-                                        const uint32_t* s1)
-{
-    const int f = t.feat[a.nd];
-    const double v = axis(t, f)[a.p];
-    const double cv = t.cutval[a.nd];
-    if (pass == 1) return v < cv;
-    const uint32_t L = a.lo + (s1[a.t1] - s1[a.t0]);
-    return a.p >= L && v <= cv;
-}
-
-// kd_pred over increasing active indices of one thread: the node's data
-// (range, axis, cut, pass-2 start) stays in registers until the index
-// crosses into the next node, so each item costs one coordinate load.
+// planeSplit's two Hoare passes as predicates -- pass 1 "coord < cut" over
+// the node, pass 2 "coord <= cut" over [L, right) where L = lo + #pass 1 --
+// over increasing active indices of one thread: the node's data (range, axis,
+// cut, pass-2 start) stays in registers until the index crosses into the next
+// node, so each item costs one coordinate load.
 struct PredCursor {
     uint32_t k, t0, t1, lo, L;
     const double* ax;
@@ -467,14 +433,54 @@ __global__ __launch_bounds__(kPredThreads) void k_kd_pred_down(
     }
 }
 
-// The pass's range [a, e) of a node and its split m = a + #predicate.
-__device__ __forceinline__ void kd_pass_bounds(const KdView& t, int pass, const ActPos& ap,
-                                               const uint32_t* s1, const uint32_t* sc,
-                                               uint32_t& a, uint32_t& m)
-{
-    a = pass == 1 ? ap.lo : ap.lo + (s1[ap.t1] - s1[ap.t0]);
-    m = a + (sc[ap.t1] - sc[ap.t0]);
-}
+// A node's data for one pass, kept in registers while a thread's active
+// indices stay in the node: range start lo and active indices [t0, t1), the
+// pass's range [a, e) and split m = a + #predicate, the cut.
+struct LevelCursor {
+    uint32_t k, t0, t1, lo, a, m, sct0, sct1;
+    const double* ax;
+    double cv;
+
+    __device__ __forceinline__ void load(const KdView& t, const uint32_t* act,
+                                         const uint32_t* off, const uint32_t* s1,
+                                         const uint32_t* sc, int pass, uint32_t kk)
+    {
+        k = kk;
+        const uint32_t nd = act[k];
+        lo = t.nl[nd];
+        t0 = off[k];
+        t1 = off[k + 1];
+        ax = axis(t, t.feat[nd]);
+        cv = t.cutval[nd];
+        a = pass == 1 ? lo : lo + (s1[t1] - s1[t0]);
+        sct0 = sc[t0];
+        sct1 = sc[t1];
+        m = a + (sct1 - sct0);
+    }
+
+    __device__ __forceinline__ void seek(const KdView& t, const uint32_t* act,
+                                         const uint32_t* off, const uint32_t* s1,
+                                         const uint32_t* sc, int pass, uint32_t A, uint32_t ti)
+    {
+        if (ti < t1) return;
+        uint32_t kk = k;
+        while (kk + 1 < A && off[kk + 1] <= ti) ++kk;
+        load(t, act, off, s1, sc, pass, kk);
+    }
+
+    // planeSplit's predicate at position p: pass 1 "coord < cut", pass 2
+    // "coord <= cut" over [a, e).
+    __device__ __forceinline__ bool pred(int pass, uint32_t p, double v) const
+    {
+        return pass == 1 ? v < cv : (p >= a && v <= cv);
+    }
+};
+
+// Publish and swap take kLvlItems active indices per thread, kKdThreads
+// apart: a block covers kLvlTile consecutive indices, and a thread reloads
+// node data only where its indices cross into the next node.
+constexpr int kLvlItems = 8;
+constexpr uint32_t kLvlTile = kKdThreads * kLvlItems;
 
 // Closed-form Hoare pairing: the r-th misplaced element of [a, m) counted
 // from the left swaps with the r-th misplaced element of [m, e) counted from
@@ -486,16 +492,19 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_publish(
     const uint32_t* __restrict__ sc, uint32_t* __restrict__ tmp)
 {
     const uint32_t A = lvl[0], P = lvl[1];
-    const uint32_t b0 = blockIdx.x * blockDim.x;
-    if (b0 >= P) return;
-    const uint32_t kb = act_find(off, A, b0);
-    const uint32_t ti = b0 + threadIdx.x;
-    if (ti >= P) return;
-    const ActPos ap = act_pos(t, act, off, A, kb, ti);
-    if (!kd_pred(t, pass, ap, s1)) return;
-    uint32_t a, m;
-    kd_pass_bounds(t, pass, ap, s1, sc, a, m);
-    if (ap.p >= m) tmp[a + (sc[ap.t1] - sc[ti + 1])] = ap.p;
+    const uint32_t base = blockIdx.x * kLvlTile;
+    if (base >= P) return;
+    LevelCursor c;
+    c.load(t, act, off, s1, sc, pass, act_find(off, A, base));  // one search per block
+#pragma unroll
+    for (int i = 0; i < kLvlItems; ++i) {
+        const uint32_t ti = base + i * kKdThreads + threadIdx.x;
+        if (ti >= P) break;
+        c.seek(t, act, off, s1, sc, pass, A, ti);
+        const uint32_t p = c.lo + (ti - c.t0);
+        if (p < c.m) continue;
+        if (c.pred(pass, p, c.ax[p])) tmp[c.a + (c.sct1 - sc[ti + 1])] = p;
+    }
 }
 
 __global__ __launch_bounds__(kKdThreads) void k_kd_swap(
@@ -504,30 +513,35 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_swap(
     const uint32_t* __restrict__ sc, const uint32_t* __restrict__ tmp)
 {
     const uint32_t A = lvl[0], P = lvl[1];
-    const uint32_t b0 = blockIdx.x * blockDim.x;
-    if (b0 >= P) return;
-    const uint32_t kb = act_find(off, A, b0);
-    const uint32_t ti = b0 + threadIdx.x;
-    if (ti >= P) return;
-    const ActPos ap = act_pos(t, act, off, A, kb, ti);
-    uint32_t a, m;
-    kd_pass_bounds(t, pass, ap, s1, sc, a, m);
-    if (ap.p < a || ap.p >= m) return;
-    if (kd_pred(t, pass, ap, s1)) return;
-    // The predicate is 0 on [lo, a) in pass 2, so sc at a equals sc at t0.
-    const uint32_t rank = (ap.p - a) - (sc[ti] - sc[ap.t0]);
-    const uint32_t q = tmp[a + rank];
-    const uint32_t p = ap.p;
-    const uint32_t vi = t.vind[p];
-    const double x = t.px[p], y = t.py[p], z = t.pz[p];
-    t.vind[p] = t.vind[q];
-    t.px[p] = t.px[q];
-    t.py[p] = t.py[q];
-    t.pz[p] = t.pz[q];
-    t.vind[q] = vi;
-    t.px[q] = x;
-    t.py[q] = y;
-    t.pz[q] = z;
+    const uint32_t base = blockIdx.x * kLvlTile;
+    if (base >= P) return;
+    LevelCursor c;
+    c.load(t, act, off, s1, sc, pass, act_find(off, A, base));
+#pragma unroll
+    for (int i = 0; i < kLvlItems; ++i) {
+        const uint32_t ti = base + i * kKdThreads + threadIdx.x;
+        if (ti >= P) break;
+        c.seek(t, act, off, s1, sc, pass, A, ti);
+        const uint32_t p = c.lo + (ti - c.t0);
+        // Only positions in [a, m) are read here, and each only by its own
+        // thread, before that thread writes it: swaps write [a, m) at the
+        // writer's own position and [m, e) at the partner's.
+        if (p < c.a || p >= c.m) continue;
+        if (c.pred(pass, p, c.ax[p])) continue;
+        // The predicate is 0 on [lo, a) in pass 2, so sc at a equals sc at t0.
+        const uint32_t rank = (p - c.a) - (sc[ti] - c.sct0);
+        const uint32_t q = tmp[c.a + rank];
+        const uint32_t vi = t.vind[p];
+        const double x = t.px[p], y = t.py[p], z = t.pz[p];
+        t.vind[p] = t.vind[q];
+        t.px[p] = t.px[q];
+        t.py[p] = t.py[q];
+        t.pz[p] = t.pz[q];
+        t.vind[q] = vi;
+        t.px[q] = x;
+        t.py[q] = y;
+        t.pz[q] = z;
+    }
 }
 
 // One block: children of every big node of the level (ids base + 2k,
@@ -669,6 +683,25 @@ __device__ __forceinline__ void minmax_flush(uint32_t j, unsigned long long (&lo
     }
 }
 
+// A lane's big node of this level, kept while its active indices stay in
+// it: the index end t1, position = index + delta, the split position and the
+// next-level list slots of the two children (~0u: a small child).
+struct MinMaxCursor {
+    uint32_t k, t1, delta, split, j1, j2;
+
+    __device__ __forceinline__ void load(const KdView& t, const uint32_t* act,
+                                         const uint32_t* off, uint32_t kk)
+    {
+        k = kk;
+        const uint32_t nd = act[k];
+        t1 = off[k + 1];
+        delta = t.nl[nd] - off[k];
+        split = t.split[nd];
+        j1 = t.aux[t.c1[nd]];
+        j2 = t.aux[t.c2[nd]];
+    }
+};
+
 __global__ __launch_bounds__(kKdThreads) void k_kd_big_minmax(
     KdView t, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ hdr,
@@ -694,7 +727,8 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_big_minmax(
     const uint32_t nchunks = (P + kMinMaxChunk - 1) / kMinMaxChunk;
     for (uint32_t c = wave; c < nchunks; c += nwaves) {
         const uint32_t c0 = c * kMinMaxChunk;
-        uint32_t k = act_find(off, A, c0);  // this lane's node, walked forward
+        MinMaxCursor cur;
+        cur.load(t, act, off, act_find(off, A, c0));  // walked forward per lane
         uint32_t jc = ~0u;
         unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
         for (uint32_t r = 0; r < kMinMaxIter; ++r) {
@@ -702,12 +736,13 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_big_minmax(
             uint32_t j = ~0u;
             uint32_t p = 0;
             if (ti < P) {
-                while (k + 1 < A && off[k + 1] <= ti) ++k;
-                const uint32_t nd = act[k];
-                p = t.nl[nd] + (ti - off[k]);
-                const uint32_t ch = p < t.split[nd] ? static_cast<uint32_t>(t.c1[nd])
-                                                    : static_cast<uint32_t>(t.c2[nd]);
-                j = t.aux[ch];
+                if (ti >= cur.t1) {
+                    uint32_t kk = cur.k;
+                    while (kk + 1 < A && off[kk + 1] <= ti) ++kk;
+                    cur.load(t, act, off, kk);
+                }
+                p = ti + cur.delta;
+                j = p < cur.split ? cur.j1 : cur.j2;
             }
             // A lane entering another child first hands its run to the wave.
             if (__ballot(jc != ~0u && j != jc)) {
@@ -1606,9 +1641,9 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
                 launch_scan_top(w.sums, nb, s);
                 hipLaunchKernelGGL(k_kd_pred_down, dim3(nb), dim3(kPredThreads), 0, s, t,
                                    pass, w.act[cur], w.off[cur], lvl, w.s1, w.sums, sc);
-                hipLaunchKernelGGL(k_kd_publish, dim3(div_up(Pb, kKdThreads)), dim3(kKdThreads),
+                hipLaunchKernelGGL(k_kd_publish, dim3(div_up(Pb, kLvlTile)), dim3(kKdThreads),
                                    0, s, t, pass, w.act[cur], w.off[cur], lvl, w.s1, sc, w.tmp);
-                hipLaunchKernelGGL(k_kd_swap, dim3(div_up(Pb, kKdThreads)), dim3(kKdThreads), 0,
+                hipLaunchKernelGGL(k_kd_swap, dim3(div_up(Pb, kLvlTile)), dim3(kKdThreads), 0,
                                    s, t, pass, w.act[cur], w.off[cur], lvl, w.s1, sc, w.tmp);
             }
             hipLaunchKernelGGL(k_kd_big_children, dim3(1), dim3(kChildThreads), 0, s, t,
