@@ -180,15 +180,13 @@ __device__ __forceinline__ uint32_t kd_split_index(uint32_t cnt, uint32_t lim1, 
 // Records of the two children of nd: segments [lo, lo+idx) and [lo+idx, hi),
 // boxes cut at the node's cut value, no children yet.
 __device__ __forceinline__ void kd_make_children(const KdView& t, uint32_t nd, uint32_t lo,
-                                                 uint32_t idx, uint32_t hi, uint32_t c1)
+                                                 uint32_t idx, uint32_t hi, uint32_t c1, int f,
+                                                 double cv, const double* b)
 {
     const uint32_t c2 = c1 + 1;
     t.c1[nd] = static_cast<int32_t>(c1);
     t.c2[nd] = static_cast<int32_t>(c2);
     t.split[nd] = lo + idx;
-    const int f = t.feat[nd];
-    const double cv = t.cutval[nd];
-    const double* b = t.box + 6 * static_cast<size_t>(nd);
     double* b1 = t.box + 6 * static_cast<size_t>(c1);
     double* b2 = t.box + 6 * static_cast<size_t>(c2);
     for (int i = 0; i < 6; ++i) {
@@ -207,6 +205,25 @@ __device__ __forceinline__ void kd_make_children(const KdView& t, uint32_t nd, u
     t.which[c2] = 1;
     t.c1[c1] = t.c2[c1] = t.c1[c2] = t.c2[c2] = -1;
     t.feat[c1] = t.feat[c2] = -1;
+}
+
+// The same with nd's split and box read from its record.
+__device__ __forceinline__ void kd_make_children(const KdView& t, uint32_t nd, uint32_t lo,
+                                                 uint32_t idx, uint32_t hi, uint32_t c1)
+{
+    kd_make_children(t, nd, lo, idx, hi, c1, t.feat[nd], t.cutval[nd],
+                     t.box + 6 * static_cast<size_t>(nd));
+}
+
+// The parent's div as in kd_parent_div, the parent (par, its cut axis pf)
+// and the side (which) known to the caller.
+__device__ __forceinline__ void kd_set_div(const KdView& t, int par, int which, int pf,
+                                           const double mn[3], const double mx[3])
+{
+    if (which == 0)
+        t.divlow[par] = pick3(mx, pf);
+    else
+        t.divhigh[par] = pick3(mn, pf);
 }
 
 __global__ __launch_bounds__(kKdThreads) void k_kd_init(
@@ -792,7 +809,12 @@ struct WaveFrame {
     double box[6];
     uint32_t nd;
     uint16_t lo, hi;
+    int32_t par, pf;  // parent and its cut axis (the node is its child2)
 };
+
+// Parent argument of wave_build for a node whose parent is to be read from
+// its record.
+constexpr int kParentLookup = -2;
 
 // LDS of one subtree workgroup: SUB points, NT threads.
 template <uint32_t SUB, int NT> struct SubShared {
@@ -813,6 +835,8 @@ template <uint32_t SUB, int NT> struct SubShared {
     uint16_t nlo[kSubNodes], nhi[kSubNodes];
     uint32_t wq_nd[kWaveQ];
     uint16_t wq_lo[kWaveQ], wq_hi[kWaveQ];
+    int32_t wq_par[kWaveQ];                 // parent (kParentLookup: read it)
+    int8_t wq_which[kWaveQ], wq_pf[kWaveQ];  // side, parent's cut axis
     WaveFrame frames[NT / 64][kWaveStack];
     uint32_t nc, nnext, nwq, wq_next;
     uint32_t id_base, id_next, id_cap;  // node ids reserved for this subtree
@@ -987,11 +1011,14 @@ __device__ __forceinline__ uint32_t wave_pass(SS& S, int f, double cv, int pass,
 // into the first child and keeps the second on its stack unless it is a
 // leaf (whose extents, for the parent's divhigh, are taken at once), so the
 // stack holds only disjoint nodes of more than kLeafMax points: at most
-// kWaveNodeMax / (kLeafMax + 1) < kWaveStack of them.
+// kWaveNodeMax / (kLeafMax + 1) < kWaveStack of them.  The node's parent
+// par (kParentLookup: read from the record), its side and the parent's cut
+// axis pf travel with it, as do box, cut and axis of the node being split,
+// so the records written here are never read back.
 template <class SS>
 __device__ void wave_build(SS& S, const KdView& t, uint32_t* hdr, uint32_t a0,
-                           uint32_t nd, uint32_t lo, uint32_t hi, const double* bx, int lane,
-                           WaveFrame* stk)
+                           uint32_t nd, uint32_t lo, uint32_t hi, const double* bx, int par,
+                           int which, int pf, int lane, WaveFrame* stk)
 {
     double box[6];
 #pragma unroll
@@ -1000,7 +1027,12 @@ __device__ void wave_build(SS& S, const KdView& t, uint32_t* hdr, uint32_t a0,
     for (;;) {
         double mn[3], mx[3];
         wave_extents(S, lo, hi, lane, mn, mx);
-        if (lane == 0) kd_parent_div(t, nd, mn, mx);
+        if (lane == 0) {
+            if (par == kParentLookup)
+                kd_parent_div(t, nd, mn, mx);
+            else if (par >= 0)
+                kd_set_div(t, par, which, pf, mn, mx);
+        }
         const uint32_t cnt = hi - lo;
         bool down = false;
         if (cnt > kLeafMax) {
@@ -1021,7 +1053,7 @@ __device__ void wave_build(SS& S, const KdView& t, uint32_t* hdr, uint32_t a0,
             if (c1 + 1 >= t.cap) {
                 if (lane == 0) hdr[kHdrOverflow] = kOvfNodes;  // the build is redone
             } else {
-                if (lane == 0) kd_make_children(t, nd, a0 + lo, idx, a0 + hi, c1);
+                if (lane == 0) kd_make_children(t, nd, a0 + lo, idx, a0 + hi, c1, f, cv, box);
                 const uint32_t m2 = lo + idx;
                 if (hi - m2 > kLeafMax) {
                     if (sp < kWaveStack) {
@@ -1032,6 +1064,8 @@ __device__ void wave_build(SS& S, const KdView& t, uint32_t* hdr, uint32_t a0,
                             fr.nd = c1 + 1;
                             fr.lo = static_cast<uint16_t>(m2);
                             fr.hi = static_cast<uint16_t>(hi);
+                            fr.par = static_cast<int32_t>(nd);
+                            fr.pf = f;
                         }
                         ++sp;
                     } else if (lane == 0) {
@@ -1040,9 +1074,12 @@ __device__ void wave_build(SS& S, const KdView& t, uint32_t* hdr, uint32_t a0,
                 } else {
                     double lmn[3], lmx[3];
                     wave_extents(S, m2, hi, lane, lmn, lmx);
-                    if (lane == 0) kd_parent_div(t, c1 + 1, lmn, lmx);
+                    if (lane == 0) kd_set_div(t, static_cast<int>(nd), 1, f, lmn, lmx);
                 }
                 box[3 + f] = cv;  // first child: the upper bound on the cut axis
+                par = static_cast<int>(nd);
+                which = 0;
+                pf = f;
                 nd = c1;
                 hi = m2;
                 down = true;
@@ -1058,6 +1095,9 @@ __device__ void wave_build(SS& S, const KdView& t, uint32_t* hdr, uint32_t a0,
         nd = fr.nd;
         lo = fr.lo;
         hi = fr.hi;
+        par = fr.par;
+        which = 1;
+        pf = fr.pf;
         wave_sync();
     }
 }
@@ -1076,7 +1116,8 @@ __device__ __forceinline__ void wave_drain(SS& S, const KdView& t, uint32_t* hdr
         double bx[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) bx[i] = t.box[6 * static_cast<size_t>(nd) + i];
-        wave_build(S, t, hdr, a0, nd, S.wq_lo[q], S.wq_hi[q], bx, lane, S.frames[wv]);
+        wave_build(S, t, hdr, a0, nd, S.wq_lo[q], S.wq_hi[q], bx, S.wq_par[q], S.wq_which[q],
+                   S.wq_pf[q], lane, S.frames[wv]);
     }
 }
 
@@ -1117,7 +1158,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(128))) void k_kd
         double bx[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) bx[i] = t.box[6 * static_cast<size_t>(nd0) + i];
-        wave_build(S, t, hdr, a0, nd0, 0u, m, bx, lane, S.frames[0]);
+        wave_build(S, t, hdr, a0, nd0, 0u, m, bx, kParentLookup, 0, 0, lane, S.frames[0]);
         __syncthreads();
     } else {
     if (tid == 0) {
@@ -1126,6 +1167,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(128))) void k_kd
             S.nc = 0;
             S.nwq = 1;
             S.wq_nd[0] = nd0;
+            S.wq_par[0] = kParentLookup;
+            S.wq_which[0] = 0;
+            S.wq_pf[0] = 0;
             S.wq_lo[0] = 0;
             S.wq_hi[0] = static_cast<uint16_t>(m);
         } else {
@@ -1265,13 +1309,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(128))) void k_kd
             if (c1 + 1 >= t.cap) {
                 hdr[kHdrOverflow] = kOvfNodes;  // the build is redone
             } else {
-                kd_make_children(t, nd, a0 + lo, idx, a0 + hi, c1);
+                kd_make_children(t, nd, a0 + lo, idx, a0 + hi, c1, S.feat[tid], S.cut[tid],
+                                 t.box + 6 * static_cast<size_t>(nd));
                 const uint32_t clo[2] = {lo, lo + idx}, chi[2] = {lo + idx, hi};
                 for (int c = 0; c < 2; ++c) {
                     if (chi[c] - clo[c] <= wave_node) {
                         const uint32_t q = atomicAdd(&S.nwq, 1u);
                         if (q < static_cast<uint32_t>(kWaveQ)) {
                             S.wq_nd[q] = c1 + c;
+                            S.wq_par[q] = static_cast<int32_t>(nd);
+                            S.wq_which[q] = static_cast<int8_t>(c);
+                            S.wq_pf[q] = S.feat[tid];
                             S.wq_lo[q] = static_cast<uint16_t>(clo[c]);
                             S.wq_hi[q] = static_cast<uint16_t>(chi[c]);
                         } else {
