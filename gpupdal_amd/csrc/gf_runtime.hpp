@@ -66,11 +66,13 @@ struct Tuning {
     int ring_max = 8;
     int small_col = 0;
     int group_z = 3;             // measured: flat for knn 16, 4% faster search for knn 32
-    int kd_subtree = 4096;       // exact tree: subtree-per-workgroup threshold (points)
+    // Exact tree: subtree-per-workgroup threshold (points; 2048-point
+    // workgroups, two per CU, measured fastest alone and beside the search).
+    int kd_subtree = 2048;
     int kd_wave_node = 256;      // exact tree: one-wave subtree threshold (points)
-    // Exact tree: one-wave subtree workgroups (-1: when built beside the
-    // search, 0: never, 1: always).
-    int kd_small_blocks = -1;
+    // Exact tree: one-wave subtree workgroups of <= 512 points (-1: when built
+    // beside the search, 0: never, 1: always).
+    int kd_small_blocks = 0;
     int kd_minmax_lds = 512;     // exact tree: LDS-staged extents up to this many nodes
     // Clouds (batches) of at least this many points build the exact tree on
     // a side stream during the search, before knowing whether a row needs

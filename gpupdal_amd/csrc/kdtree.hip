@@ -62,6 +62,9 @@ __device__ __forceinline__ const double* axis(const KdView& t, int f)
 // with block barriers.
 constexpr uint32_t kSub = 4096;           // small-node capacity (points)
 constexpr int kSubThreads = 1024;
+// Smaller subtree sizes use workgroups sized to them (four points per
+// thread): 2048 points in about 80 KB of LDS and 1024 in about 41 KB, two
+// and four workgroups per CU.
 // While the search runs beside the build, its blocks hold every CU; a
 // one-wave workgroup on at most kSubSmall points (about 22 KB of LDS) fits in
 // the slot a finished search block frees, a kSub one (155 KB) does not.
@@ -821,8 +824,8 @@ template <uint32_t SUB, int NT> struct SubShared {
     static constexpr uint32_t kCap = SUB;
     static constexpr int kThreads = NT;
     double x[SUB], y[SUB], z[SUB];
-    uint32_t v[SUB];
-    uint32_t pre[SUB + 1];
+    uint16_t v[SUB];                   // the point's slot at load (its id is read back at the end)
+    uint16_t pre[SUB + 1];
     uint16_t tmp[SUB];
     uint8_t node[SUB];                 // level-local node index, 0xFF: not at block level
     unsigned long long mn[kSubNodes * 3], mx[kSubNodes * 3];
@@ -897,17 +900,17 @@ template <class SS, class Pred> __device__ __forceinline__ void sub_prefix(SS& S
     uint32_t ex = kd_block_scan<SS::kThreads>(cnt, &tot);
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        S.pre[i0 + i] = ex;
+        S.pre[i0 + i] = static_cast<uint16_t>(ex);
         ex += (bits >> i) & 1u;
     }
-    if (threadIdx.x == 0) S.pre[SS::kCap] = tot;
+    if (threadIdx.x == 0) S.pre[SS::kCap] = static_cast<uint16_t>(tot);
     __syncthreads();
 }
 
 template <class SS> __device__ __forceinline__ void sub_swap(SS& S, uint32_t i, uint32_t q)
 {
     const double x = S.x[i], y = S.y[i], z = S.z[i];
-    const uint32_t v = S.v[i];
+    const uint16_t v = S.v[i];
     S.x[i] = S.x[q];
     S.y[i] = S.y[q];
     S.z[i] = S.z[q];
@@ -1146,7 +1149,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(128))) void k_kd
             S.x[i] = t.px[a0 + i];
             S.y[i] = t.py[a0 + i];
             S.z[i] = t.pz[a0 + i];
-            S.v[i] = t.vind[a0 + i];
+            S.v[i] = static_cast<uint16_t>(i);
             S.node[i] = 0;
         } else {
             S.node[i] = 0xFF;
@@ -1361,11 +1364,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(128))) void k_kd
     }
     }  // one wave or the whole workgroup
     sub_blank_unused(S, t, tid);
-    for (uint32_t i = tid; i < m; i += NT) {
+    // The workgroup owns [a0, a0 + m) of the point arrays: every old id is
+    // read (through the slot each position now holds) before any is written.
+    constexpr int kPer = static_cast<int>(SUB) / NT;
+    uint32_t ids[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t i = tid + static_cast<uint32_t>(r * NT);
+        ids[r] = i < m ? t.vind[a0 + S.v[i]] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t i = tid + static_cast<uint32_t>(r * NT);
+        if (i >= m) continue;
         t.px[a0 + i] = S.x[i];
         t.py[a0 + i] = S.y[i];
         t.pz[a0 + i] = S.z[i];
-        t.vind[a0 + i] = S.v[i];
+        t.vind[a0 + i] = ids[r];
     }
 }
 
@@ -1717,6 +1733,12 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
         if (kt.small_blocks)
             hipLaunchKernelGGL((k_kd_subtree<kSubSmall, kSubSmallThreads>), dim3(nsmall),
                                dim3(kSubSmallThreads), 0, s, t, w.small, w.hdr, wave_node);
+        else if (sub <= kSub / 4)
+            hipLaunchKernelGGL((k_kd_subtree<kSub / 4, kSubThreads / 4>), dim3(nsmall),
+                               dim3(kSubThreads / 4), 0, s, t, w.small, w.hdr, wave_node);
+        else if (sub <= kSub / 2)
+            hipLaunchKernelGGL((k_kd_subtree<kSub / 2, kSubThreads / 2>), dim3(nsmall),
+                               dim3(kSubThreads / 2), 0, s, t, w.small, w.hdr, wave_node);
         else
             hipLaunchKernelGGL((k_kd_subtree<kSub, kSubThreads>), dim3(nsmall),
                                dim3(kSubThreads), 0, s, t, w.small, w.hdr, wave_node);

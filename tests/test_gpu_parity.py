@@ -73,13 +73,15 @@ def _oracle_nodes(e):
 
 @pytest.mark.parametrize("subtree,lds,small,wave", [(4096, 512, 0, 256), (200, 0, 0, 256),
                                                     (1000, 4, 0, 101), (4096, 512, 0, 1024),
-                                                    (4096, 512, 1, 256)],
-                         ids=["default", "global-levels", "mixed", "wave-1024", "one-wave-blocks"])
+                                                    (4096, 512, 1, 256), (2048, 512, 0, 256)],
+                         ids=["default", "global-levels", "mixed", "wave-1024", "one-wave-blocks",
+                              "half-size-blocks"])
 def test_kdtree_matches_oracle(ctx, oracle, subtree, lds, small, wave):
     """The GPU-built tree is nanoflann's: same vind order and node splits,
     whichever way the build divides the work (level-synchronous big nodes,
-    per-workgroup subtrees with workgroup levels and one-wave subtrees, or
-    one-wave workgroups of <= 512 points; LDS-staged or global extents)."""
+    per-workgroup subtrees of 4096, 2048 or 1024 points with workgroup levels
+    and one-wave subtrees, or one-wave workgroups of <= 512 points; LDS-staged
+    or global extents)."""
     ctx.set_tuning("kd_subtree", subtree)
     ctx.set_tuning("kd_minmax_lds", lds)
     ctx.set_tuning("kd_small_blocks", small)
@@ -95,9 +97,9 @@ def test_kdtree_matches_oracle(ctx, oracle, subtree, lds, small, wave):
             # Ids a subtree reserved and did not use hold empty ranges.
             assert np.count_nonzero(t["right"] > t["left"]) == len(e["child1"])
     finally:
-        ctx.set_tuning("kd_subtree", 4096)
+        ctx.set_tuning("kd_subtree", 2048)
         ctx.set_tuning("kd_minmax_lds", 512)
-        ctx.set_tuning("kd_small_blocks", -1)
+        ctx.set_tuning("kd_small_blocks", 0)
         ctx.set_tuning("kd_wave_node", 256)
 
 
