@@ -128,8 +128,11 @@ def gpu_leg(args):
                        "unit": "TFLOP/s", "frac": fp64 / FP64_VECTOR_PEAK_TFLOPS,
                        "candidates_per_point": counters["candidates"] / n,
                        "flops_per_candidate": FLOPS_PER_CANDIDATE},
+        # features: the rows the search did not flag, right after the search;
+        # exact_tail: from there to the end of the flagged rows' features (tree
+        # not hidden, exact re-solve, flagged rows' features).
         phases_ms={"index": mean("ms_index"), "search": ms_search,
-                   "exact_after_search": mean("ms_exact"), "features": mean("ms_features"),
+                   "features": mean("ms_features"), "exact_tail": mean("ms_exact"),
                    "tree_beside_search": mean("ms_tree"), "device_total": mean("ms_total")},
         rows={"tie": int(st["tie_rows"]), "incomplete": int(st["incomplete_rows"])},
         grid={"cell_m": st["cell"], "cell_z_m": st["cell_z"], "gx": st["grid_x"],

@@ -295,6 +295,11 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     if (any_exact_tile) launch_flag_tiles(N, cv.perm, 0, out, tt, stream);
     hipEvent_t e1 = mark();
     span(e0, e1, &stats.ms_search);
+    // Rows the search did not flag are final: their features run now, beside
+    // the rest of the tiles' trees; the flagged rows' after their re-solve.
+    launch_features(cv, prog0, out, b.inv.as<uint32_t>(), stream, &tt, kFeaturesUnflagged);
+    hipEvent_t e2 = mark();
+    span(e1, e2, &stats.ms_features);
     auto forest = [&](hipStream_t s, bool beside) {
         if (b.kd_valid) return;
         build_forest(b.kd, b.x.as<double>(), b.y.as<double>(), b.z.as<double>(), N,
@@ -318,15 +323,14 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         uint32_t nfail = 0;
         GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, stream));
         GF_HIP(hipStreamSynchronize(stream));
-        span(e1, mark(), &stats.ms_exact);
         if (nfail)
             throw Error(GF_ERR_UNSUPPORTED, "exact KD3 search exceeded its depth bound on " +
                                                 std::to_string(nfail) + " rows");
+        launch_features(cv, prog0, out, b.inv.as<uint32_t>(), stream, &tt, kFeaturesFlagged,
+                        h[0]);
     }
-    e0 = mark();
-    launch_features(cv, prog0, out, b.inv.as<uint32_t>(), stream, &tt);
     e1 = mark();
-    span(e0, e1, &stats.ms_features);
+    if (h[0]) span(e2, e1, &stats.ms_exact);
     span(t_all, e1, &stats.ms_total);
 
     unsigned long long ctr[4] = {0, 0, 0, 0};

@@ -248,12 +248,18 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
         launch_knn(cv, prog, 0, out, knn_tuning(), stream);
     hipEvent_t e1 = mark();
     span(e0, e1, &stats.ms_search);
+    // Rows the search did not flag are final: their features run now, beside
+    // the rest of the exact tree; the flagged rows' after their re-solve.
+    launch_features(cv, prog, out, cl.inv.as<uint32_t>(), stream, nullptr,
+                    kFeaturesUnflagged);
+    hipEvent_t e2 = mark();
+    span(e1, e2, &stats.ms_features);
     if (early) build_kdtree(cl, side, true);
-    resolve_flagged(cl, cv, prog, 0, out, e1);
-    e0 = mark();
-    launch_features(cv, prog, out, cl.inv.as<uint32_t>(), stream);
+    const uint32_t nflagged = resolve_flagged(cl, cv, prog, 0, out, e2);
+    launch_features(cv, prog, out, cl.inv.as<uint32_t>(), stream, nullptr, kFeaturesFlagged,
+                    nflagged);
     e1 = mark();
-    span(e0, e1, &stats.ms_features);
+    if (nflagged) span(e2, e1, &stats.ms_exact);
     span(t_all, e1, &stats.ms_total);
 
     unsigned long long ctr[4] = {0, 0, 0, 0};
@@ -376,17 +382,17 @@ void Context::fork_side()
 }
 
 // Re-solves the rows the grid search flagged (boundary ties, uncertified
-// searches) with the exact KD3Index search.  ms_exact runs from `from` (the
-// end of the search) to the end of the re-solve.
-void Context::resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog,
-                              int mode, const KnnOutputs& out, hipEvent_t from)
+// searches) with the exact KD3Index search.  ms_exact runs from `from` to the
+// end of the re-solve.
+uint32_t Context::resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog,
+                                  int mode, const KnnOutputs& out, hipEvent_t from)
 {
     uint32_t h[4] = {0, 0, 0, 0};
     GF_HIP(hipMemcpyAsync(h, out.nflagged, sizeof(h), hipMemcpyDeviceToHost, stream));
     GF_HIP(hipStreamSynchronize(stream));
     stats.tie_rows = h[1];
     stats.incomplete_rows = h[2];
-    if (!h[0]) return;
+    if (!h[0]) return 0;
     hipEvent_t e0 = from ? from : mark();
     build_kdtree(cl);
     uint32_t* failed = counters.as<uint32_t>() + 5;
@@ -401,6 +407,7 @@ void Context::resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram
         throw Error(GF_ERR_UNSUPPORTED,
                     "exact KD3 search exceeded its depth bound on " + std::to_string(nfail) +
                         " rows");
+    return h[0];
 }
 
 hipEvent_t Context::mark(hipStream_t s)

@@ -20,8 +20,13 @@ enum : uint8_t {
     ST_ZERO_COV = 4,      // Eigen isZero covariance: PDAL skips the row
     ST_EIGEN_FAIL = 8,    // solver NoConvergence
     ST_FEATURE_INVALID = 16, // covfeat: largest clamped eigenvalue is 0
-    ST_FEW_NEIGHBORS = 32 // normal: fewer than 3 neighbors
+    ST_FEW_NEIGHBORS = 32, // normal: fewer than 3 neighbors
+    // Internal, in a covariance record's flag word only: the row goes to the
+    // exact search without a grid search (a tile or cloud with fewer points
+    // than the result capacity).  Never reported.
+    ST_EXACT_ONLY = 128
 };
+constexpr uint8_t kReportedSearchFlags = ST_TIE | ST_INCOMPLETE;
 
 enum : int32_t { STAGE_COVFEAT = 0, STAGE_NORMAL = 1, STAGE_EIGENVALUES = 2 };
 enum : int32_t { MODE_RAW = 0, MODE_SQRT = 1, MODE_NORMALIZED = 2 };

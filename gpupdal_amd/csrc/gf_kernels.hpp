@@ -136,12 +136,16 @@ inline int knn_queries_per_wave(int max_k) { return max_k <= 32 ? 64 : 32; }
 void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
                 const KnnOutputs& out, const KnnTuning& tune, hipStream_t s,
                 const TileTables* tt = nullptr);
-// Eigensystem + stage columns from out.cov for every row, in original point
-// order (inv: original id -> sorted position), so outputs are written
-// contiguously.
+// Eigensystem + stage columns from out.cov, in original point order (inv:
+// original id -> sorted position), so outputs are written contiguously.
+// rows: kFeaturesAll every row; kFeaturesUnflagged every row the search did
+// not flag (launched right after the search, beside the exact tree);
+// kFeaturesFlagged the nflagged rows of out.flagged, after their re-solve.
+constexpr int kFeaturesAll = 0, kFeaturesUnflagged = 1, kFeaturesFlagged = 2;
 void launch_features(const CloudView& cv, const FusedProgram& prog,
                      const KnnOutputs& out, const uint32_t* inv, hipStream_t s,
-                     const TileTables* tt = nullptr);
+                     const TileTables* tt = nullptr, int rows = kFeaturesAll,
+                     uint64_t nflagged = 0);
 
 // kdtree.hip -- the exact KD3Index tree (nanoflann 1.3.1 semantics).
 struct KdView {

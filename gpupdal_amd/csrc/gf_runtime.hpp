@@ -282,8 +282,9 @@ struct Context {
     // Makes the side stream wait for the points and bounds (points_mark), or
     // for all work queued on `stream` so far.
     void fork_side();
-    void resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog, int mode,
-                         const KnnOutputs& out, hipEvent_t from);
+    // Returns the number of flagged rows.
+    uint32_t resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog,
+                             int mode, const KnnOutputs& out, hipEvent_t from);
     void build_batch_index(Batch& b, const std::vector<uint32_t>& fetch);
     void run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* tile_knn,
                    double* const* cols, uint8_t* const* status, int where);

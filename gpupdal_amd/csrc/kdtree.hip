@@ -1622,7 +1622,7 @@ __global__ __launch_bounds__(256) void k_flag_all(uint64_t n, const uint32_t* __
     if (mode == 1)
         out.status[0][perm[s]] = 0;
     else
-        reinterpret_cast<uint64_t*>(out.cov)[kCovRecord * s + 6] = 0;
+        reinterpret_cast<uint64_t*>(out.cov)[kCovRecord * s + 6] = ST_EXACT_ONLY;
 }
 
 // Rows of the tiles with fewer points than their result capacity go to the
@@ -1642,7 +1642,7 @@ __global__ __launch_bounds__(256) void k_flag_tiles(uint64_t n, const uint32_t* 
     if (mode == 1)
         out.status[0][perm[s]] = 0;
     else
-        reinterpret_cast<uint64_t*>(out.cov)[kCovRecord * s + 6] = 0;
+        reinterpret_cast<uint64_t*>(out.cov)[kCovRecord * s + 6] = ST_EXACT_ONLY;
 }
 
 } // namespace

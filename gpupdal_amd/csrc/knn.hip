@@ -796,22 +796,34 @@ __device__ __forceinline__ uint32_t tile_of(const uint64_t* __restrict__ starts,
     return lo;
 }
 
+// One thread per row: by original id (kFeaturesAll, kFeaturesUnflagged: a
+// row whose record carries a search flag is skipped), or the i-th row of
+// out.flagged by sorted position (kFeaturesFlagged).
+
 __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov,
                                                   const uint32_t* __restrict__ inv,
-                                                  uint64_t n, FusedProgram prog,
+                                                  const uint32_t* __restrict__ perm,
+                                                  uint64_t n, int rows, FusedProgram prog,
                                                   KnnOutputs out, const TileInfo* __restrict__ tiles,
                                                   const uint64_t* __restrict__ starts,
                                                   uint32_t ntiles)
 {
-    const uint64_t qid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (qid >= n) return;
-    const uint64_t s = inv[qid];  // the row's sorted position: one record read
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t qid = i, s;
+    if (rows == kFeaturesFlagged) {
+        s = out.flagged[i];
+        qid = perm[s];
+    } else {
+        s = inv[qid];  // the row's sorted position: one record read
+    }
     // The neighborhood size of the row's tile (stages are the same in every tile).
     const int k = tiles ? tiles[tile_of(starts, ntiles, s)].prog.k : prog.k;
     const double* rec = cov + kCovRecord * s;
     double c[6];
     for (int i = 0; i < 6; ++i) c[i] = rec[i];
     const uint8_t f = static_cast<uint8_t>(reinterpret_cast<const uint64_t*>(rec)[6]);
+    if (rows == kFeaturesUnflagged && f) return;  // written after its exact re-solve
     const bool zero = cov_is_zero(c);
     double ev[3] = {0, 0, 0}, vec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int info = 0;
@@ -822,7 +834,7 @@ __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov
         uint8_t st = stage_outputs(sp, k, c, zero, info, ev, vec, vals);
         if (st == 0)
             for (int cc = 0; cc < sp.ncols; ++cc) out.col[sp.col_offset + cc][qid] = vals[cc];
-        out.status[si][qid] = st | f;
+        out.status[si][qid] = st | (f & kReportedSearchFlags);
     }
 }
 
@@ -884,11 +896,12 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
 
 void launch_features(const CloudView& cv, const FusedProgram& prog,
                      const KnnOutputs& out, const uint32_t* inv, hipStream_t s,
-                     const TileTables* tt)
+                     const TileTables* tt, int rows, uint64_t nflagged)
 {
-    if (cv.n == 0) return;
-    hipLaunchKernelGGL(k_features, dim3(div_up(cv.n, 256)), dim3(256), 0, s, out.cov, inv,
-                       cv.n, prog, out, tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
+    const uint64_t n = rows == kFeaturesFlagged ? nflagged : cv.n;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_features, dim3(div_up(n, 256)), dim3(256), 0, s, out.cov, inv, cv.perm,
+                       n, rows, prog, out, tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
                        tt ? tt->ntiles : 0u);
 }
 

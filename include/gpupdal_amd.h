@@ -92,9 +92,12 @@ typedef struct gf_stats {
     uint64_t incomplete_rows; /* rows re-solved after the grid budget */
     double ms_index;          /* grid build, device time */
     double ms_search;         /* kNN search + covariance kernel, device time */
-    double ms_features;       /* eigensolve + stage columns kernel */
-    double ms_exact;          /* exact KD3 re-solve, plus the part of the tree build
-                                 not hidden behind the search, device time */
+    double ms_features;       /* eigensolve + stage columns of the rows the search
+                                 did not flag (run right after the search) */
+    double ms_exact;          /* from the end of those features to the end of the
+                                 flagged rows' features: the part of the tree build
+                                 not hidden behind search and features, the exact
+                                 KD3 re-solve and the flagged rows' features */
     double cell;              /* column edge used (m) */
     double cell_z;            /* z cell height used (m) */
     uint32_t grid_x, grid_y, grid_z;
