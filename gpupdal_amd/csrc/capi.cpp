@@ -304,7 +304,7 @@ void Context::build_forest(KdBuffers& b, const double* x, const double* y, const
         kd_s2.ensure((n + 1) * 4);
         kd_tmp.ensure((n + 1) * 4);
         kd_lists.ensure((static_cast<uint64_t>(lcap) * 5 + 2 + kKdHeaderWords) * 4);
-        kd_sums.ensure(scan_scratch_words(n + 1) * 4);
+        kd_sums.ensure(kd_sums_words(n) * 4);
         KdView t{};
         t.vind = b.vind.as<uint32_t>();
         t.px = b.px.as<double>();

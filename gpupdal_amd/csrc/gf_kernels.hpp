@@ -185,6 +185,7 @@ struct KdTuning {
 // Words of the exact-tree build header at the front of the node-list buffer.
 constexpr uint32_t kKdHeaderWords = 16;
 uint32_t kd_node_capacity(uint64_t n);
+uint64_t kd_sums_words(uint64_t n);  // per-block sums of the level prefix scans
 uint32_t kd_list_capacity(uint64_t n);
 // Sends every row to the exact search: used when the cloud holds fewer
 // points than the result capacity stride*min(n, k).

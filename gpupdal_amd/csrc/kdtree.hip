@@ -78,11 +78,16 @@ constexpr uint32_t kWaveNodeMax = 1024;
 constexpr int kSubNodes = 48;             // block-level nodes per subtree level (> 4096 / 101)
 constexpr int kWaveStack = 12;            // pending nodes of one wave (see wave_build)
 constexpr int kWaveQ = 96;                // nodes handed to the waves per block level
-constexpr int kPredThreads = 256;
+// The level kernels run in one-wave workgroups: beside the search (whose
+// workgroups are single waves too) a level kernel's workgroup then needs one
+// free wave slot, not four on one CU at once, and is not held back until the
+// search drains.
+constexpr int kLvlThreads = 64;
+constexpr int kPredThreads = kLvlThreads;
 constexpr int kPredItems = 16;
 constexpr uint32_t kPredTile = kPredThreads * kPredItems;
 constexpr int kChildThreads = 256;       // one block; small enough to run beside the search
-constexpr uint32_t kMinMaxLds = 512;      // big children staged per block in LDS
+constexpr uint32_t kMinMaxLds = 128;      // big children staged per block in LDS (6 KB)
 
 // Build header (device): the last level's big-node count and point count,
 // subtree list length, overflow code, node count, and per level parity
@@ -338,7 +343,7 @@ __device__ __forceinline__ void kd_split_node(const KdView& t, uint32_t nd)
 }
 
 // One thread per big node of the level (A read from the level's header).
-__global__ __launch_bounds__(kKdThreads) void k_kd_big_split(KdView t,
+__global__ __launch_bounds__(kLvlThreads) void k_kd_big_split(KdView t,
                                                              const uint32_t* __restrict__ act,
                                                              const uint32_t* __restrict__ lvl)
 {
@@ -408,6 +413,41 @@ __global__ __launch_bounds__(kPredThreads) void k_kd_pred_reduce(
     uint32_t tot;
     kd_block_scan<kPredThreads>(acc, &tot);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// In-place exclusive scan of the nb per-block sums, one workgroup, NT * ITEMS
+// sums per step: one wave (one free slot beside the search) while the sums
+// take at most a few steps, four waves for larger levels.
+template <int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_kd_scan_top(uint32_t* __restrict__ sums, uint32_t nb)
+{
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += NT * ITEMS) {
+        const uint32_t i0 = base + threadIdx.x * ITEMS;
+        uint32_t v[ITEMS];
+        uint32_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            v[i] = i0 + i < nb ? sums[i0 + i] : 0u;
+            acc += v[i];
+        }
+        uint32_t tot;
+        uint32_t ex = kd_block_scan<NT>(acc, &tot) + carry;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (i0 + i < nb) sums[i0 + i] = ex;
+            ex += v[i];
+        }
+        carry += tot;
+    }
+}
+
+void launch_kd_scan_top(uint32_t* sums, uint32_t nb, hipStream_t s)
+{
+    if (nb <= 4u * 64u * 64u)
+        hipLaunchKernelGGL((k_kd_scan_top<64, 64>), dim3(1), dim3(64), 0, s, sums, nb);
+    else
+        hipLaunchKernelGGL((k_kd_scan_top<256, 16>), dim3(1), dim3(256), 0, s, sums, nb);
 }
 
 // Exclusive prefix of the predicate over active indices [0, P], written to
@@ -496,17 +536,17 @@ struct LevelCursor {
     }
 };
 
-// Publish and swap take kLvlItems active indices per thread, kKdThreads
+// Publish and swap take kLvlItems active indices per thread, kLvlThreads
 // apart: a block covers kLvlTile consecutive indices, and a thread reloads
 // node data only where its indices cross into the next node.
 constexpr int kLvlItems = 8;
-constexpr uint32_t kLvlTile = kKdThreads * kLvlItems;
+constexpr uint32_t kLvlTile = kLvlThreads * kLvlItems;
 
 // Closed-form Hoare pairing: the r-th misplaced element of [a, m) counted
 // from the left swaps with the r-th misplaced element of [m, e) counted from
 // the right.  Misplaced right elements publish their position under their
 // rank; misplaced left elements then swap with their partner.
-__global__ __launch_bounds__(kKdThreads) void k_kd_publish(
+__global__ __launch_bounds__(kLvlThreads) void k_kd_publish(
     KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ s1,
     const uint32_t* __restrict__ sc, uint32_t* __restrict__ tmp)
@@ -518,7 +558,7 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_publish(
     c.load(t, act, off, s1, sc, pass, act_find(off, A, base));  // one search per block
 #pragma unroll
     for (int i = 0; i < kLvlItems; ++i) {
-        const uint32_t ti = base + i * kKdThreads + threadIdx.x;
+        const uint32_t ti = base + i * kLvlThreads + threadIdx.x;
         if (ti >= P) break;
         c.seek(t, act, off, s1, sc, pass, A, ti);
         const uint32_t p = c.lo + (ti - c.t0);
@@ -527,7 +567,7 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_publish(
     }
 }
 
-__global__ __launch_bounds__(kKdThreads) void k_kd_swap(
+__global__ __launch_bounds__(kLvlThreads) void k_kd_swap(
     KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ s1,
     const uint32_t* __restrict__ sc, const uint32_t* __restrict__ tmp)
@@ -539,7 +579,7 @@ __global__ __launch_bounds__(kKdThreads) void k_kd_swap(
     c.load(t, act, off, s1, sc, pass, act_find(off, A, base));
 #pragma unroll
     for (int i = 0; i < kLvlItems; ++i) {
-        const uint32_t ti = base + i * kKdThreads + threadIdx.x;
+        const uint32_t ti = base + i * kLvlThreads + threadIdx.x;
         if (ti >= P) break;
         c.seek(t, act, off, s1, sc, pass, A, ti);
         const uint32_t p = c.lo + (ti - c.t0);
@@ -722,7 +762,7 @@ struct MinMaxCursor {
     }
 };
 
-__global__ __launch_bounds__(kKdThreads) void k_kd_big_minmax(
+__global__ __launch_bounds__(kLvlThreads) void k_kd_big_minmax(
     KdView t, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ hdr,
     const uint32_t* __restrict__ act_next, uint32_t lds_max)
@@ -1661,6 +1701,8 @@ void launch_flag_all(uint64_t n, const uint32_t* perm, int mode, const KnnOutput
                        n, perm, mode, out);
 }
 
+uint64_t kd_sums_words(uint64_t n) { return div_up(n + 1, kPredTile) + 2; }
+
 uint32_t kd_node_capacity(uint64_t n) { return static_cast<uint32_t>(n / 12 + 1024); }
 
 // Builds the tree; returns 0, or an overflow code (kOvfNodes: node
@@ -1694,28 +1736,28 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
         for (int l = 0; l < kLevelsPerRead; ++l) {
             const uint32_t* lvl = w.hdr + kHdrLvl + 2 * cur;
             const uint32_t gsplit = static_cast<uint32_t>(
-                std::min<uint64_t>(div_up(std::min<uint64_t>(Ab, w.list_cap), kKdThreads), 64));
-            hipLaunchKernelGGL(k_kd_big_split, dim3(gsplit), dim3(kKdThreads), 0, s, t,
+                std::min<uint64_t>(div_up(std::min<uint64_t>(Ab, w.list_cap), kLvlThreads), 256));
+            hipLaunchKernelGGL(k_kd_big_split, dim3(gsplit), dim3(kLvlThreads), 0, s, t,
                                w.act[cur], lvl);
             for (int pass = 1; pass <= 2; ++pass) {
                 uint32_t* sc = pass == 1 ? w.s1 : w.s2;
                 const uint32_t nb = div_up(static_cast<uint64_t>(Pb) + 1, kPredTile);
                 hipLaunchKernelGGL(k_kd_pred_reduce, dim3(nb), dim3(kPredThreads), 0, s, t,
                                    pass, w.act[cur], w.off[cur], lvl, w.s1, w.sums);
-                launch_scan_top(w.sums, nb, s);
+                launch_kd_scan_top(w.sums, nb, s);
                 hipLaunchKernelGGL(k_kd_pred_down, dim3(nb), dim3(kPredThreads), 0, s, t,
                                    pass, w.act[cur], w.off[cur], lvl, w.s1, w.sums, sc);
-                hipLaunchKernelGGL(k_kd_publish, dim3(div_up(Pb, kLvlTile)), dim3(kKdThreads),
+                hipLaunchKernelGGL(k_kd_publish, dim3(div_up(Pb, kLvlTile)), dim3(kLvlThreads),
                                    0, s, t, pass, w.act[cur], w.off[cur], lvl, w.s1, sc, w.tmp);
-                hipLaunchKernelGGL(k_kd_swap, dim3(div_up(Pb, kLvlTile)), dim3(kKdThreads), 0,
+                hipLaunchKernelGGL(k_kd_swap, dim3(div_up(Pb, kLvlTile)), dim3(kLvlThreads), 0,
                                    s, t, pass, w.act[cur], w.off[cur], lvl, w.s1, sc, w.tmp);
             }
             hipLaunchKernelGGL(k_kd_big_children, dim3(1), dim3(kChildThreads), 0, s, t,
                                w.act[cur], w.off[cur], w.s1, w.s2, w, cur ^ 1, sub);
             hipLaunchKernelGGL(
                 k_kd_big_minmax,
-                dim3(std::min<uint32_t>(div_up(Pb, kMinMaxChunk * (kKdThreads / 64)), 1024)),
-                dim3(kKdThreads), 0, s, t, w.act[cur], w.off[cur], lvl, w.hdr, w.act[cur ^ 1],
+                dim3(std::min<uint32_t>(div_up(Pb, kMinMaxChunk), 4096)),
+                dim3(kLvlThreads), 0, s, t, w.act[cur], w.off[cur], lvl, w.hdr, w.act[cur ^ 1],
                 kt.minmax_lds_nodes);
             cur ^= 1;
             Ab *= 2;

@@ -41,8 +41,11 @@ namespace gf {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / kWave;
+// Waves per workgroup.  One cloud: four, so the tree built beside the search
+// finds the slots a workgroup's early waves leave idle until its last wave
+// ends.  Tiles of a batch: one; a wave's slot and LDS free as soon as it ends
+// (measured: the batch's search 6% faster, the step 7% faster).
+template <bool kTiled> constexpr int knn_waves() { return kTiled ? 1 : 4; }
 constexpr int kRegBatches = 4;
 constexpr uint32_t kRegCand = kRegBatches * 64;
 // Per-wave LDS scratch: 64 compacted entries (16 B), 64 ranked entries, and a
@@ -516,8 +519,9 @@ template <bool kTiled>
 // Four waves per SIMD: the selection is a chain of ballots and LDS
 // round trips, so more resident waves hide more of its latency (measured
 // faster than three waves even with the register spills four waves cost).
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_knn(KnnArgs a_in)
+__global__ __launch_bounds__(knn_waves<kTiled>() * kWave) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_knn(KnnArgs a_in)
 {
+    constexpr int kWaves = knn_waves<kTiled>();
     extern __shared__ uint32_t s_lds[];
     const int lane = __lane_id();
     // Wave-uniform by construction; said so, so that per-wave tables load
@@ -876,22 +880,22 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
     const GridParams& g = cv.g;
     a.margin = 1e-6 * fmin(g.c, g.cz) +
                1e-15 * (fabs(g.ox) + fabs(g.oy) + fabs(g.oz)) + 1e-300;
-    const size_t list_words =
-        mode == 1 ? 0 : static_cast<size_t>(kWaves) * a.q_per_wave * a.lds_stride;
-    const size_t lds = (list_words + static_cast<size_t>(kWaves) * kScratchWords) * sizeof(uint32_t);
+    const size_t waves = static_cast<size_t>(tt ? knn_waves<true>() : knn_waves<false>());
+    const size_t list_words = mode == 1 ? 0 : waves * a.q_per_wave * a.lds_stride;
+    const size_t lds = (list_words + waves * kScratchWords) * sizeof(uint32_t);
     if (tt) {
         if (!tt->ntasks) return;
         a.tiles = tt->tiles;
         a.task0 = tt->task0;
         a.ntasks = tt->ntasks;
         a.ntiles = tt->ntiles;
-        const uint32_t blocks = div_up(tt->ntasks, kWaves);
-        hipLaunchKernelGGL(k_knn<true>, dim3(blocks), dim3(kBlock), lds, s, a);
+        const uint32_t blocks = div_up(tt->ntasks, waves);
+        hipLaunchKernelGGL(k_knn<true>, dim3(blocks), dim3(waves * kWave), lds, s, a);
         return;
     }
-    const uint64_t per_block = static_cast<uint64_t>(kWaves) * a.q_per_wave;
+    const uint64_t per_block = static_cast<uint64_t>(waves) * a.q_per_wave;
     const uint32_t blocks = div_up(cv.n, per_block);
-    hipLaunchKernelGGL(k_knn<false>, dim3(blocks), dim3(kBlock), lds, s, a);
+    hipLaunchKernelGGL(k_knn<false>, dim3(blocks), dim3(waves * kWave), lds, s, a);
 }
 
 void launch_features(const CloudView& cv, const FusedProgram& prog,
