@@ -73,7 +73,7 @@ struct Tuning {
     // Exact tree: one-wave subtree workgroups of <= 512 points (-1: when built
     // beside the search, 0: never, 1: always).
     int kd_small_blocks = 0;
-    int kd_minmax_lds = 512;     // exact tree: LDS-staged extents up to this many nodes
+    int kd_minmax_lds = 128;     // exact tree: LDS-staged extents up to this many nodes (<= 128)
     // Clouds (batches) of at least this many points build the exact tree on
     // a side stream during the search, before knowing whether a row needs
     // it; smaller ones build it only when a row is flagged.  0: never early.

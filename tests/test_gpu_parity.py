@@ -98,7 +98,7 @@ def test_kdtree_matches_oracle(ctx, oracle, subtree, lds, small, wave):
             assert np.count_nonzero(t["right"] > t["left"]) == len(e["child1"])
     finally:
         ctx.set_tuning("kd_subtree", 2048)
-        ctx.set_tuning("kd_minmax_lds", 512)
+        ctx.set_tuning("kd_minmax_lds", 128)
         ctx.set_tuning("kd_small_blocks", 0)
         ctx.set_tuning("kd_wave_node", 256)
 
