@@ -246,6 +246,43 @@ def batched_leg(args):
             "batched_over_plain": ms_plain / ms_batched}
 
 
+def las_leg(args, xyz):
+    """The LAS boundary: the bench cloud as an uncompressed LAS 1.2 file
+    image (point format 1, 28-byte records) in host memory, decoded into a
+    resident cloud.  Device decode time from HIP events; the wall time adds
+    the copy of the records over PCIe."""
+    import numpy as np
+    import gpupdal_amd as g
+    from gpupdal_amd.las import las_image
+    x, y, z = xyz
+    img, _ = las_image(x, y, z, offset=(float(np.floor(x.min())), float(np.floor(y.min())),
+                                        0.0), point_format=1)
+    ctx = g.Context(0)
+    for _ in range(2):
+        ctx.cloud_from_las(img).close()
+    dec, wall = [], []
+    for _ in range(args.las_steps):
+        t0 = time.perf_counter()
+        cl = ctx.cloud_from_las(img)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        dec.append(cl.las_info["ms_decode"])
+        cl.close()
+    ctx.close()
+    n = len(x)
+    L = 28
+    ms_dec = sorted(dec)[len(dec) // 2]
+    ms_wall = sorted(wall)[len(wall) // 2]
+    achieved = (L + 24) * n / (ms_dec / 1e3) / 1e9
+    return {"points": n, "point_format": 1, "record_bytes": L,
+            "workload": "uncompressed LAS 1.2 image of the bench cloud in host memory -> "
+                        "resident FP64 cloud",
+            "ms_decode": ms_dec, "decode_points_per_s": n / (ms_dec / 1e3),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "bytes_per_point": L + 24},
+            "ms_with_upload": ms_wall, "points_per_s_with_upload": n / (ms_wall / 1e3)}
+
+
 def serving_leg(args):
     """Serving: Poisson arrivals of tile requests at a fixed rate (sizes
     log-uniform in [5k, 4M], knn in {10, 16}, normal + covariancefeatures)
@@ -408,6 +445,8 @@ def main():
     ap.add_argument("--requests", type=int, default=64, help="requests of the serving leg")
     ap.add_argument("--rate", type=float, default=120.0, help="serving arrivals per second")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-las", action="store_true", help="skip the LAS decode leg")
+    ap.add_argument("--las-steps", type=int, default=7)
     ap.add_argument("--precision", action="store_true",
                     help="only the precision-stress report (20M points, roofs, wires)")
     ap.add_argument("--precision-points", type=int, default=20_000_000)
@@ -433,6 +472,7 @@ def main():
     gpu, xyz = gpu_leg(args)
     batched = None if args.no_batched else batched_leg(args)
     serving = None if args.no_serving else serving_leg(args)
+    las = None if args.no_las else las_leg(args, xyz)
     cpu, cpu_parallel = (None, None) if args.no_cpu else cpu_leg(args, xyz)
     line = {
         "metric": "points/s kNN+covariance features, batched vs plain GPU; p50/p99 tile latency",
@@ -460,6 +500,7 @@ def main():
         "grid": gpu["grid"],
         "batched": batched,
         "serving": serving,
+        "las_read": las,
         "cpu_baseline": cpu,
         "cpu_baseline_parallel": cpu_parallel,
     }

@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgpupdal_amd.so")
 
 GF_OK = 0
+GF_ERR_ARG, GF_ERR_HIP, GF_ERR_UNSUPPORTED, GF_ERR_OOM, GF_ERR_STATE = -1, -2, -3, -4, -5
 WHERE_HOST, WHERE_DEVICE = 0, 1
 STAGE_COVFEAT, STAGE_NORMAL, STAGE_EIGENVALUES = 0, 1, 2
 MODE_RAW, MODE_SQRT, MODE_NORMALIZED = 0, 1, 2
@@ -34,6 +35,22 @@ class GfStage(ctypes.Structure):
                 ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("features", ctypes.c_uint32), ("always_up", ctypes.c_int32),
                 ("normalize", ctypes.c_int32)]
+
+
+class GfLasInfo(ctypes.Structure):
+    _fields_ = [("version_major", ctypes.c_uint8), ("version_minor", ctypes.c_uint8),
+                ("point_format", ctypes.c_uint8), ("compressed", ctypes.c_uint8),
+                ("header_size", ctypes.c_uint16), ("point_size", ctypes.c_uint16),
+                ("point_offset", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("point_count", ctypes.c_uint64), ("scale", ctypes.c_double * 3),
+                ("offset", ctypes.c_double * 3), ("max", ctypes.c_double * 3),
+                ("min", ctypes.c_double * 3), ("ms_decode", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        for k in ("scale", "offset", "max", "min"):
+            d[k] = tuple(d[k])
+        return d
 
 
 class GfStats(ctypes.Structure):
@@ -68,6 +85,7 @@ EXPORTED = [
     "gf_server_create", "gf_server_submit", "gf_server_wait", "gf_server_request_info",
     "gf_server_release", "gf_server_destroy",
     "gf_autotune", "gf_tuning_save", "gf_tuning_load",
+    "gf_las_parse", "gf_cloud_create_las", "gf_las_pack_extra", "gf_debug_cloud_xyz",
 ]
 
 
@@ -99,6 +117,13 @@ def load_library(path=LIB_PATH):
     L.gf_debug_grid.argtypes = [vp, vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p, _u32p]
     L.gf_debug_kdtree.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.gf_debug_copy.argtypes = [vp, vp, vp, ctypes.c_uint64]
+    L.gf_las_parse.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(GfLasInfo), ctypes.c_char_p,
+                               ctypes.c_uint64]
+    L.gf_cloud_create_las.argtypes = [vp, vp, ctypes.c_uint64, ctypes.POINTER(GfLasInfo),
+                                      ctypes.POINTER(vp)]
+    L.gf_las_pack_extra.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_int, ctypes.c_uint64, vp,
+                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+    L.gf_debug_cloud_xyz.argtypes = [vp, vp, vp, vp, vp]
     L.gf_dev_alloc.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
     L.gf_dev_free.argtypes = [vp, vp]
     L.gf_memcpy.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.c_int]
@@ -169,6 +194,20 @@ def synth_params(n, seed=None, tile=0, side=500.0, ground_frac=0.7, roof_frac=0.
     return p
 
 
+def las_parse(data):
+    """The header of a LAS file image (bytes-like) as a dict; GfError with
+    PDAL's validation message for a file PDAL's reader rejects.  Host only."""
+    L = lib()
+    buf = np.frombuffer(data, dtype=np.uint8)
+    info = GfLasInfo()
+    msg = ctypes.create_string_buffer(256)
+    rc = L.gf_las_parse(buf.ctypes.data if len(buf) else None, len(buf), ctypes.byref(info),
+                        msg, 256)
+    if rc != GF_OK:
+        raise GfError(rc, "gf_las_parse: " + msg.value.decode())
+    return info.as_dict()
+
+
 def synth_cloud(n, **kw):
     """Seeded synthetic ALS tile (x, y, z float64 arrays)."""
     L = lib()
@@ -227,6 +266,30 @@ class Context:
 
     def cloud(self, x, y, z):
         return Cloud(self, x, y, z)
+
+    def cloud_from_las(self, data):
+        """A resident cloud decoded on the device from the point records of an
+        uncompressed LAS file image (bytes-like); its header is `.las_info`."""
+        buf = np.frombuffer(data, dtype=np.uint8)
+        info = GfLasInfo()
+        h = ctypes.c_void_p()
+        self._check(self.L.gf_cloud_create_las(self.h, buf.ctypes.data, len(buf),
+                                               ctypes.byref(info), ctypes.byref(h)),
+                    "gf_cloud_create_las")
+        cl = Cloud.__new__(Cloud)
+        cl.ctx, cl.h, cl.n = self, h, int(info.point_count)
+        cl.las_info = info.as_dict()
+        return cl
+
+    def las_pack_extra(self, cols, records, stride, offset):
+        """Writes the double columns into each stride-byte record of the
+        uint8 array `records` at byte `offset` (host arrays)."""
+        cols = [_c64(c) for c in cols]
+        n = len(cols[0]) if cols else 0
+        cp = (ctypes.c_void_p * max(1, len(cols)))(*[c.ctypes.data for c in cols])
+        self._check(self.L.gf_las_pack_extra(self.h, cp, len(cols), n, records.ctypes.data,
+                                             stride, offset, WHERE_HOST), "gf_las_pack_extra")
+        return records
 
     def autotune(self, cloud, stages, factors=(0.4, 0.5, 0.6, 0.75, 0.9)):
         """Times the run for each column-size factor; the fastest is kept for
@@ -296,6 +359,14 @@ class Cloud:
 
     def __del__(self):
         self.close()
+
+    def xyz(self):
+        """The cloud's resident coordinates, copied to the host."""
+        out = [np.empty(self.n, np.float64) for _ in range(3)]
+        self.ctx._check(self.ctx.L.gf_debug_cloud_xyz(self.ctx.h, self.h, out[0].ctypes.data,
+                                                      out[1].ctypes.data, out[2].ctypes.data),
+                        "gf_debug_cloud_xyz")
+        return tuple(out)
 
     def run(self, stages, init=0.0):
         """Runs a chain of stages; returns (list of per-stage column lists,

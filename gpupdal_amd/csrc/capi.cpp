@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 using namespace gf;
@@ -751,6 +752,125 @@ int gf_debug_grid(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t* perm,
             GF_HIP(hipMemcpyAsync(colstart, cl->colstart.ptr,
                                   (static_cast<uint64_t>(cl->g.ncols) + 1) * 4,
                                   hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int gf_las_parse(const void* bytes, uint64_t nbytes, gf_las_info* info, char* msg,
+                 uint64_t msg_size)
+{
+    if (!bytes || !info) return GF_ERR_ARG;
+    std::string err;
+    const bool ok = gf::las_parse(static_cast<const uint8_t*>(bytes), nbytes, *info, err);
+    if (msg && msg_size) std::snprintf(msg, msg_size, "%s", err.c_str());
+    return ok ? GF_OK : GF_ERR_ARG;
+}
+
+int gf_cloud_create_las(gf_ctx* ctx, const void* bytes, uint64_t nbytes, gf_las_info* info,
+                        gf_cloud** out)
+{
+    if (!ctx || !bytes || !out) return GF_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        gf_las_info h;
+        std::string err;
+        if (!las_parse(static_cast<const uint8_t*>(bytes), nbytes, h, err))
+            throw Error(GF_ERR_ARG, err);
+        if (h.compressed)
+            throw Error(GF_ERR_UNSUPPORTED,
+                        "LAZ-compressed point data: the device decoder reads uncompressed LAS");
+        const uint64_t n = h.point_count;
+        if (n >= (1ull << 32) - 1)
+            throw Error(GF_ERR_UNSUPPORTED, "clouds are limited to 2^32-2 points");
+        Context* c = C(ctx);
+        std::unique_ptr<Cloud> cl(new Cloud(c));
+        cl->n = n;
+        cl->x.ensure(n * 8 + 8);
+        cl->y.ensure(n * 8 + 8);
+        cl->z.ensure(n * 8 + 8);
+        cl->dstarts.ensure(16);
+        const uint64_t se[2] = {0, n};
+        GF_HIP(hipMemcpy(cl->dstarts.ptr, se, sizeof(se), hipMemcpyHostToDevice));
+        const uint64_t L = h.point_size;
+        c->lasbuf.ensure(n * L + 16);  // the decoder reads up to 4 bytes past the last record
+        LasXform xf{};
+        for (int d = 0; d < 3; ++d) {
+            xf.scale[d] = h.scale[d];
+            xf.offset[d] = h.offset[d];
+        }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (n) {
+            GF_HIP(hipMemcpyAsync(c->lasbuf.ptr,
+                                  static_cast<const uint8_t*>(bytes) + h.point_offset, n * L,
+                                  hipMemcpyHostToDevice, c->stream));
+            c->reset_spans();
+            e0 = c->mark();
+            launch_las_xyz(c->lasbuf.as<uint8_t>(), n, static_cast<uint32_t>(L), xf,
+                           cl->x.as<double>(), cl->y.as<double>(), cl->z.as<double>(),
+                           c->stream);
+            e1 = c->mark();
+        }
+        GF_HIP(hipStreamSynchronize(c->stream));
+        h.ms_decode = 0.0;
+        if (e0 && e1) {
+            float ms = 0.f;
+            GF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            h.ms_decode = ms;
+        }
+        c->reset_spans();
+        if (info) *info = h;
+        *out = reinterpret_cast<gf_cloud*>(cl.release());
+    });
+}
+
+int gf_las_pack_extra(gf_ctx* ctx, const double* const* cols, int ncols, uint64_t n,
+                      uint8_t* records, uint32_t stride, uint32_t offset, int where)
+{
+    if (!ctx || ncols < 0 || ncols > 16 || (n && ncols && (!cols || !records)))
+        return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (where != GF_WHERE_HOST && where != GF_WHERE_DEVICE)
+            throw Error(GF_ERR_ARG, "where must be GF_WHERE_HOST or GF_WHERE_DEVICE");
+        if (static_cast<uint64_t>(offset) + 8ull * static_cast<uint64_t>(ncols) > stride)
+            throw Error(GF_ERR_ARG, "the extra dimensions do not fit in the record");
+        if (!n || !ncols) return;
+        for (int i = 0; i < ncols; ++i)
+            if (!cols[i]) throw Error(GF_ERR_ARG, "null column");
+        Context* c = C(ctx);
+        LasColumns lc{};
+        uint8_t* dst = records;
+        const uint64_t bytes = n * stride;
+        if (where == GF_WHERE_HOST) {
+            c->outbuf.ensure(n * 8 * static_cast<uint64_t>(ncols) + 8);
+            for (int i = 0; i < ncols; ++i) {
+                double* d = c->outbuf.as<double>() + static_cast<uint64_t>(i) * n;
+                GF_HIP(hipMemcpyAsync(d, cols[i], n * 8, hipMemcpyHostToDevice, c->stream));
+                lc.col[i] = d;
+            }
+            c->lasbuf.ensure(bytes + 16);
+            GF_HIP(hipMemcpyAsync(c->lasbuf.ptr, records, bytes, hipMemcpyHostToDevice,
+                                  c->stream));
+            dst = c->lasbuf.as<uint8_t>();
+        } else {
+            for (int i = 0; i < ncols; ++i) lc.col[i] = cols[i];
+        }
+        launch_las_pack(lc, ncols, n, dst, stride, offset, c->stream);
+        if (where == GF_WHERE_HOST)
+            GF_HIP(hipMemcpyAsync(records, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int gf_debug_cloud_xyz(gf_ctx* ctx, gf_cloud* cloud, double* x, double* y, double* z)
+{
+    if (!ctx || !cloud || !x || !y || !z) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        if (!cl->n) return;
+        GF_HIP(hipMemcpyAsync(x, cl->x.ptr, cl->n * 8, hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipMemcpyAsync(y, cl->y.ptr, cl->n * 8, hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipMemcpyAsync(z, cl->z.ptr, cl->n * 8, hipMemcpyDeviceToHost, c->stream));
         GF_HIP(hipStreamSynchronize(c->stream));
     });
 }

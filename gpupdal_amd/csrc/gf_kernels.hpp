@@ -2,6 +2,7 @@
 #pragma once
 
 #include "gf_internal.hpp"
+#include "gpupdal_amd.h"
 
 namespace gf {
 
@@ -115,6 +116,20 @@ struct TileInfo {
 // The queries of one wavefront in the tiled search: [q0, q0 + nq) of tile.
 // Batch tables (device memory); tiles == nullptr means a single cloud whose
 // grid and program come from the CloudView / FusedProgram arguments.
+// las.hip -- LAS point records.
+struct LasXform {
+    double scale[3], offset[3];
+};
+struct LasColumns {
+    const double* col[16];
+};
+bool las_parse(const uint8_t* b, uint64_t nbytes, gf_las_info& h, std::string& err);
+// rec: n records of L bytes in device memory, padded by at least 4 bytes.
+void launch_las_xyz(const uint8_t* rec, uint64_t n, uint32_t L, const LasXform& xf, double* x,
+                    double* y, double* z, hipStream_t s);
+void launch_las_pack(const LasColumns& cols, int ncols, uint64_t n, uint8_t* out,
+                     uint32_t stride, uint32_t offset, hipStream_t s);
+
 struct TileTables {
     const TileInfo* tiles;
     const uint64_t* starts;  // T + 1 tile starts (for row -> tile lookups)

@@ -230,6 +230,7 @@ struct Context {
     std::string last_error;
     DevBuf part, bounds, hist, scan_tmp, outbuf, statbuf, flagged, counters, covbuf;
     DevBuf xyz_stage;  // interleaved copy of the points between the keys and the gather
+    DevBuf lasbuf;     // LAS point records (decode input, extra-bytes packing)
     // Exact-tree scratch, its own: the tree may be built on the side stream
     // while the index build uses scan_tmp on the main stream.
     DevBuf kd_s1, kd_s2, kd_tmp, kd_lists, kd_sums;

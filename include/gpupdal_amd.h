@@ -222,6 +222,48 @@ int gf_memcpy(gf_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind)
 #define GF_COPY_D2H 2
 #define GF_COPY_D2D 3
 
+/* ---- LAS point records in and out (the on-disk format either side of the
+ * path).  Replaces GPUPDAL's device LAS hydration, src/pdal/PdgNeighborhood.cpp:
+ * 1067-1193 with src/io/las/LasKernels.cu:306-319, for uncompressed LAS. */
+
+/* A LAS file's public header as PDAL reads it (io/private/las/Header.cpp:
+ * 38-75; point count and format bits, Header.hpp:122-138). */
+typedef struct gf_las_info {
+    uint8_t version_major, version_minor;
+    uint8_t point_format;      /* format bits & 0xF */
+    uint8_t compressed;        /* LAZ bit 0x80 */
+    uint16_t header_size;
+    uint16_t point_size;       /* point record length in bytes */
+    uint32_t point_offset;     /* offset of the first point record */
+    uint32_t reserved;
+    uint64_t point_count;      /* LAS 1.4: the 64-bit count; else the 32-bit one */
+    double scale[3], offset[3];
+    double max[3], min[3];     /* bounds as stored in the header */
+    double ms_decode;          /* gf_cloud_create_las: device time of the decode
+                                  (tuning "timing" 1), else 0 */
+} gf_las_info;
+
+/* Parses and validates the header of a LAS file image (host memory; no GPU
+ * needed).  GF_ERR_ARG with PDAL's validation message in msg (if msg_size >
+ * 0) for a file PDAL's reader rejects (io/private/las/Header.cpp:118-144:
+ * signature, point offset and count against the file size, point formats
+ * 0-3 and 6-8, record length at least the format's, version 1.0-1.4). */
+int gf_las_parse(const void *bytes, uint64_t nbytes, gf_las_info *info, char *msg,
+                 uint64_t msg_size);
+/* A resident cloud from the point records of an uncompressed LAS file image
+ * in host memory: the records are copied to HBM once and X, Y, Z decoded
+ * there as int32 * scale + offset (io/LasReader.cpp:926-932, 1008-1014), in
+ * file order.  LAZ-compressed data: GF_ERR_UNSUPPORTED.  info may be NULL. */
+int gf_cloud_create_las(gf_ctx *ctx, const void *bytes, uint64_t nbytes, gf_las_info *info,
+                        gf_cloud **out);
+/* Writes ncols double columns (n rows each) into n point records of stride
+ * bytes at byte `offset` of each record, one after another, little-endian:
+ * the extra bytes that writers.las extra_dims appends after the base record
+ * (io/LasWriter.cpp).  where: GF_WHERE_HOST (host columns and records) or
+ * GF_WHERE_DEVICE.  offset + 8 * ncols must not exceed stride; ncols <= 16. */
+int gf_las_pack_extra(gf_ctx *ctx, const double *const *cols, int ncols, uint64_t n,
+                      uint8_t *records, uint32_t stride, uint32_t offset, int where);
+
 /* ---- parity hooks (no reference counterpart; used by the test suite). */
 /* The device eigensolver on m row-major 3x3 matrices. */
 int gf_debug_eigen(gf_ctx *ctx, const double *cov, uint64_t m, double *evals,
@@ -232,6 +274,9 @@ int gf_debug_grid(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, uint32_t *perm,
                   uint32_t *keys, uint32_t *colstart, uint32_t *ncols,
                   uint32_t *key_bits);
 
+/* The cloud's resident coordinates (n each, original order) copied to host
+ * arrays. */
+int gf_debug_cloud_xyz(gf_ctx *ctx, gf_cloud *cloud, double *x, double *y, double *z);
 /* A plain device-to-device copy of count doubles (the reference kernel that
  * calibrates HBM byte counters). */
 int gf_debug_copy(gf_ctx *ctx, double *dst, const double *src, uint64_t count);
