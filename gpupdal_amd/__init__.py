@@ -86,6 +86,7 @@ EXPORTED = [
     "gf_server_release", "gf_server_destroy",
     "gf_autotune", "gf_tuning_save", "gf_tuning_load",
     "gf_las_parse", "gf_cloud_create_las", "gf_las_pack_extra", "gf_debug_cloud_xyz",
+    "gf_optimal_neighborhood",
 ]
 
 
@@ -124,6 +125,8 @@ def load_library(path=LIB_PATH):
     L.gf_las_pack_extra.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_int, ctypes.c_uint64, vp,
                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
     L.gf_debug_cloud_xyz.argtypes = [vp, vp, vp, vp, vp]
+    L.gf_optimal_neighborhood.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp,
+                                          ctypes.c_int]
     L.gf_dev_alloc.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
     L.gf_dev_free.argtypes = [vp, vp]
     L.gf_memcpy.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.c_int]
@@ -359,6 +362,16 @@ class Cloud:
 
     def __del__(self):
         self.close()
+
+    def optimal_neighborhood(self, min_k=10, max_k=14):
+        """filters.optimalneighborhood: (OptimalKNN uint64, OptimalRadius)."""
+        kopt = np.zeros(self.n, np.uint64)
+        ropt = np.zeros(self.n, np.float64)
+        self.ctx._check(self.ctx.L.gf_optimal_neighborhood(self.ctx.h, self.h, min_k, max_k,
+                                                           kopt.ctypes.data, ropt.ctypes.data,
+                                                           WHERE_HOST),
+                        "gf_optimal_neighborhood")
+        return kopt, ropt
 
     def xyz(self):
         """The cloud's resident coordinates, copied to the host."""

@@ -676,6 +676,63 @@ int gf_knn(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t stride, uint32_t* 
     });
 }
 
+int gf_optimal_neighborhood(gf_ctx* ctx, gf_cloud* cloud, uint32_t min_k, uint32_t max_k,
+                            uint64_t* optimal_knn, double* optimal_radius, int where)
+{
+    if (!ctx || !cloud || !optimal_knn || !optimal_radius) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (min_k < 2 || min_k > max_k || max_k > static_cast<uint32_t>(kMaxFetch - 1))
+            throw Error(GF_ERR_ARG, "optimalneighborhood needs 2 <= min_k <= max_k <= 63");
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        const uint64_t n = cl->n;
+        if (!n) return;
+        if (n < max_k) throw Error(GF_ERR_ARG, "fewer points than max_k");
+        c->points_mark = false;
+        c->build_index(*cl, max_k);
+        FusedProgram prog{};
+        prog.k = static_cast<int32_t>(max_k);
+        prog.stride = 1;
+        prog.fetch = static_cast<int32_t>(max_k);
+        KnnOutputs out{};
+        c->outbuf.ensure(n * max_k * 4 + n + 8);
+        out.ids = c->outbuf.as<uint32_t>();
+        out.status[0] = reinterpret_cast<uint8_t*>(out.ids + n * max_k);
+        c->flagged.ensure(n * 4 + 4);
+        c->counters.ensure(64);
+        out.flagged = c->flagged.as<uint32_t>();
+        out.nflagged = c->counters.as<uint32_t>();
+        launch_fill_u32(out.nflagged, 8, 0u, c->stream);
+        const CloudView cv = c->view(*cl);
+        const bool early = !cl->kd_valid && c->tune.kd_overlap_min && n >= c->tune.kd_overlap_min;
+        if (early) c->fork_side();
+        launch_knn(cv, prog, 2, out, c->knn_tuning(), c->stream);
+        if (early) c->build_kdtree(*cl, c->side, true);
+        c->resolve_flagged(*cl, cv, prog, 2, out, nullptr);
+        DevBuf res;
+        uint64_t* kopt = optimal_knn;
+        double* ropt = optimal_radius;
+        if (where != GF_WHERE_DEVICE) {
+            res.ensure(n * 16 + 16);
+            kopt = res.as<uint64_t>();
+            ropt = reinterpret_cast<double*>(kopt + n);
+        }
+        uint32_t* failed = c->counters.as<uint32_t>() + 6;
+        launch_optimal_nbhd(out.ids, n, static_cast<int>(min_k), static_cast<int>(max_k),
+                            cl->x.as<double>(), cl->y.as<double>(), cl->z.as<double>(), kopt,
+                            ropt, failed, c->stream);
+        uint32_t nfail = 0;
+        GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, c->stream));
+        if (where != GF_WHERE_DEVICE) {
+            GF_HIP(hipMemcpyAsync(optimal_knn, kopt, n * 8, hipMemcpyDeviceToHost, c->stream));
+            GF_HIP(hipMemcpyAsync(optimal_radius, ropt, n * 8, hipMemcpyDeviceToHost,
+                                  c->stream));
+        }
+        GF_HIP(hipStreamSynchronize(c->stream));
+        if (nfail) throw Error(GF_ERR_UNSUPPORTED, "Cannot perform eigen decomposition.");
+    });
+}
+
 int gf_dev_alloc(gf_ctx* ctx, uint64_t bytes, void** out)
 {
     if (!ctx || !out) return GF_ERR_ARG;

@@ -147,7 +147,9 @@ struct TileTables {
 inline int knn_queries_per_wave(int max_k) { return max_k <= 32 ? 64 : 32; }
 
 // mode 0: neighborhood covariance into out.cov / out.flags (sorted order);
-// mode 1: neighbor lists into out.ids.
+// mode 1: neighbor lists into out.ids; mode 2: the same, every row with two
+// equal distances in its list flagged, so that the lists also follow
+// KD3Index's order among equal distances after the exact re-solve.
 void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
                 const KnnOutputs& out, const KnnTuning& tune, hipStream_t s,
                 const TileTables* tt = nullptr);
@@ -221,6 +223,12 @@ void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram&
                        const KnnOutputs& out, const double* x, const double* y, const double* z,
                        uint32_t nflagged, uint32_t* failed, hipStream_t s,
                        const TileTables* tt = nullptr);
+
+// optnbhd.hip -- filters.optimalneighborhood from n x kmax KD3-ordered lists
+// (original order); failed counts rows whose eigen solve did not converge.
+void launch_optimal_nbhd(const uint32_t* ids, uint64_t n, int kmin, int kmax, const double* x,
+                         const double* y, const double* z, uint64_t* kopt, double* ropt,
+                         uint32_t* failed, hipStream_t s);
 
 // Debug/parity hook: device eigen solver over m matrices (row-major 3x3).
 void launch_eigen_batch(const double* cov, uint64_t m, double* evals,

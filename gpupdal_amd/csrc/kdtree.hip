@@ -1624,7 +1624,7 @@ __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, Fuse
         const Ent kept = ent_shfl(best, static_cast<int>((static_cast<uint32_t>(lane) * stride) & 63u));
         const bool mine = static_cast<uint32_t>(lane) < k;
         const uint32_t kept_id = kept.d == kInfD ? pad_id : kept.j;
-        if (mode == 1) {
+        if (mode != 0) {  // neighbor lists
             if (mine) out.ids[static_cast<uint64_t>(qid) * k + lane] = kept_id;
             continue;
         }
@@ -1659,7 +1659,7 @@ __global__ __launch_bounds__(256) void k_flag_all(uint64_t n, const uint32_t* __
     if (s == 0) out.nflagged[0] = static_cast<uint32_t>(n);
     if (s >= n) return;
     out.flagged[s] = static_cast<uint32_t>(s);
-    if (mode == 1)
+    if (mode != 0)
         out.status[0][perm[s]] = 0;
     else
         reinterpret_cast<uint64_t*>(out.cov)[kCovRecord * s + 6] = ST_EXACT_ONLY;
@@ -1679,7 +1679,7 @@ __global__ __launch_bounds__(256) void k_flag_tiles(uint64_t n, const uint32_t* 
     if (!tiles[tl].all_exact) return;
     const uint32_t slot = atomicAdd(out.nflagged, 1u);
     out.flagged[slot] = static_cast<uint32_t>(s);
-    if (mode == 1)
+    if (mode != 0)
         out.status[0][perm[s]] = 0;
     else
         reinterpret_cast<uint64_t*>(out.cov)[kCovRecord * s + 6] = ST_EXACT_ONLY;

@@ -81,7 +81,7 @@ struct KnnArgs {
     FusedProgram prog;  // tiled: replaced per wave by the tile's
     KnnOutputs out;
     KnnTuning tune;
-    int32_t mode;     // 0 features, 1 neighbor lists
+    int32_t mode;     // 0 features, 1 neighbor lists, 2 lists in KD3Index order
     int32_t q_per_wave;
     int32_t lds_stride;
     double margin;
@@ -490,15 +490,17 @@ __device__ __forceinline__ void finish_query(const KnnArgs& a, const WaveGrid& w
     const int stride = wg.stride;
     // Boundary tie: the fetch-th and (fetch+1)-th distances are equal.  With
     // stride > 1 every adjacent equal pair inside the fetched list can change
-    // which ids are kept, so any such pair flags the row.
+    // which ids are kept, and ordered lists (mode 2) need KD3Index's order
+    // among equal distances, so there any such pair flags the row.
     const Ent nxt = ent_shfl(best, lane + 1 < 64 ? lane + 1 : 63);
     const bool eq = (lane < fetch) && (lane + 1 < 64) && best.d == nxt.d && best.d != kInfD;
-    const bool tie = stride > 1 ? (__ballot(eq) != 0)
-                                : (__shfl(static_cast<int>(eq), fetch - 1) != 0);
+    const bool tie = (stride > 1 || a.mode == 2)
+                         ? (__ballot(eq) != 0)
+                         : (__shfl(static_cast<int>(eq), fetch - 1) != 0);
     const uint8_t flags = (tie ? ST_TIE : 0) | (incomplete ? ST_INCOMPLETE : 0);
     // Kept list in canonical (d2, id) order: entries 0, stride, 2*stride...
     const Ent kept = ent_shfl(best, (lane * stride) & 63);
-    if (a.mode == 1) {
+    if (a.mode != 0) {
         const uint32_t qid = a.cv.perm[s];
         if (lane < k) a.out.ids[static_cast<uint64_t>(qid) * k + lane] = kept.id;
         if (lane == 0) a.out.status[0][qid] = flags;
@@ -575,7 +577,7 @@ __global__ __launch_bounds__(knn_waves<kTiled>() * kWave) __attribute__((amdgpu_
     const int k = wg.k;
     // LDS: per-wave neighbor lists (features mode), then per wave a 64-entry
     // compaction scratch of 16 bytes per entry and a 64-entry pending list.
-    const size_t list_words = a.mode == 1 ? 0 : static_cast<size_t>(kWaves) * Q * a.lds_stride;
+    const size_t list_words = a.mode != 0 ? 0 : static_cast<size_t>(kWaves) * Q * a.lds_stride;
     uint32_t* nbr = s_lds + static_cast<size_t>(w) * Q * a.lds_stride;  // lds_stride >= k
     uint32_t* scratch = s_lds + list_words + static_cast<size_t>(w) * kScratchWords;
     uint32_t* pending = scratch + 512;  // queries left for the growing search
@@ -739,7 +741,7 @@ __global__ __launch_bounds__(knn_waves<kTiled>() * kWave) __attribute__((amdgpu_
         atomicAdd(a.out.counters + 2, static_cast<unsigned long long>(cnt[2]));
         atomicAdd(a.out.counters + 3, static_cast<unsigned long long>(nq));
     }
-    if (a.mode == 1) return;
+    if (a.mode != 0) return;
     // The lists were written by other lanes of this same wave: order those
     // LDS stores before the loads below (no block barrier: waves of this
     // block may already have exited).
@@ -881,7 +883,7 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
     a.margin = 1e-6 * fmin(g.c, g.cz) +
                1e-15 * (fabs(g.ox) + fabs(g.oy) + fabs(g.oz)) + 1e-300;
     const size_t waves = static_cast<size_t>(tt ? knn_waves<true>() : knn_waves<false>());
-    const size_t list_words = mode == 1 ? 0 : waves * a.q_per_wave * a.lds_stride;
+    const size_t list_words = mode != 0 ? 0 : waves * a.q_per_wave * a.lds_stride;
     const size_t lds = (list_words + waves * kScratchWords) * sizeof(uint32_t);
     if (tt) {
         if (!tt->ntasks) return;

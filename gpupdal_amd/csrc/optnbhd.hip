@@ -1,0 +1,90 @@
+// filters.optimalneighborhood: for each point, the k in [min_k, max_k] whose
+// neighborhood has the smallest eigenentropy (filters/OptimalNeighborhoodFilter.cpp:
+// 72-163).  Input: every row's max_k neighbors in KD3Index order (the
+// point itself first), from the list-mode search with its exact re-solve.
+// One thread per row walks its list once: the running mean and scatter
+// matrix are updated neighbor by neighbor in PDAL's order and rounding
+// (:94-129), and from min_k on each prefix's covariance B / (n - 1) goes
+// through the Eigen-semantics solver; the eigenvalues, clamped at zero and
+// normalized by their sum, give -sum(l ln l) (:131-152).  The first k with
+// the strictly smallest entropy wins; OptimalRadius is the square root of
+// that neighbor's squared distance (:154-161).
+#include "gf_eigen.hpp"
+#include "gf_internal.hpp"
+#include "gf_kernels.hpp"
+
+namespace gf {
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_optimal_nbhd(const uint32_t* __restrict__ ids,
+                                                      uint64_t n, int kmin, int kmax,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z,
+                                                      uint64_t* __restrict__ kopt_out,
+                                                      double* __restrict__ ropt_out,
+                                                      uint32_t* __restrict__ failed)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* nb = ids + i * static_cast<uint64_t>(kmax);
+    const double qx = x[i], qy = y[i], qz = z[i];
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    double b00 = 0.0, b01 = 0.0, b02 = 0.0, b11 = 0.0, b12 = 0.0, b22 = 0.0;
+    double minentropy = 1.7976931348623157e308;  // DBL_MAX
+    uint64_t kopt = 0;
+    double ropt = 0.0;
+    for (int k = 0; k < kmax; ++k) {
+        const uint32_t j = nb[k];
+        const double px = x[j], py = y[j], pz = z[j];
+        const double dx = px - mx, dy = py - my, dz = pz - mz;
+        const double cnt = static_cast<double>(k + 1);
+        mx += dx / cnt;
+        my += dy / cnt;
+        mz += dz / cnt;
+        const double s = (cnt - 1.0) / cnt;
+        b00 = b00 + s * dx * dx;
+        b11 = b11 + s * dy * dy;
+        b22 = b22 + s * dz * dz;
+        b01 = b01 + s * dx * dy;
+        b02 = b02 + s * dx * dz;
+        b12 = b12 + s * dy * dz;
+        if (k < kmin - 1) continue;
+        const double den = cnt - 1.0;
+        const double c[6] = {b00 / den, b01 / den, b02 / den, b11 / den, b12 / den, b22 / den};
+        double ev[3], vec[9];
+        if (eigen_sym3(c, ev, vec) != 0) {
+            atomicAdd(failed, 1u);
+            return;
+        }
+        double l0 = fmax(ev[2], 0.0), l1 = fmax(ev[1], 0.0), l2 = fmax(ev[0], 0.0);
+        const double sum = ((0.0 + l0) + l1) + l2;
+        l0 /= sum;
+        l1 /= sum;
+        l2 /= sum;
+        const double entropy = -((l2 * log(l2) + l1 * log(l1)) + l0 * log(l0));
+        if (entropy < minentropy) {
+            minentropy = entropy;
+            kopt = static_cast<uint64_t>(k + 1);
+            // nanoflann's squared distance: query minus point, per axis.
+            const double ex = qx - px, ey = qy - py, ez = qz - pz;
+            ropt = ex * ex + ey * ey + ez * ez;
+        }
+    }
+    kopt_out[i] = kopt;
+    ropt_out[i] = sqrt(ropt);
+}
+
+} // namespace
+
+void launch_optimal_nbhd(const uint32_t* ids, uint64_t n, int kmin, int kmax, const double* x,
+                         const double* y, const double* z, uint64_t* kopt, double* ropt,
+                         uint32_t* failed, hipStream_t s)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_optimal_nbhd, dim3(static_cast<uint32_t>(div_up(n, 256))), dim3(256),
+                       0, s, ids, n, kmin, kmax, x, y, z, kopt, ropt, failed);
+}
+
+} // namespace gf

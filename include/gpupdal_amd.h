@@ -222,6 +222,16 @@ int gf_memcpy(gf_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind)
 #define GF_COPY_D2H 2
 #define GF_COPY_D2D 3
 
+/* filters.optimalneighborhood (filters/OptimalNeighborhoodFilter.cpp:60-163):
+ * for every point the k in [min_k, max_k] (defaults 10, 14; neighbors in
+ * KD3Index order, the point itself first) whose covariance has the smallest
+ * eigenentropy -> optimal_knn (OptimalKNN), and the distance to that k-th
+ * neighbor -> optimal_radius (OptimalRadius); n values each.  Needs
+ * 2 <= min_k <= max_k <= 63 and at least max_k points.  An eigen solve that
+ * does not converge: GF_ERR_UNSUPPORTED with PDAL's message. */
+int gf_optimal_neighborhood(gf_ctx *ctx, gf_cloud *cloud, uint32_t min_k, uint32_t max_k,
+                            uint64_t *optimal_knn, double *optimal_radius, int where);
+
 /* ---- LAS point records in and out (the on-disk format either side of the
  * path).  Replaces GPUPDAL's device LAS hydration, src/pdal/PdgNeighborhood.cpp:
  * 1067-1193 with src/io/las/LasKernels.cu:306-319, for uncompressed LAS. */

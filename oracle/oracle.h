@@ -173,6 +173,18 @@ void orc_truth_from_lists(const double *x, const double *y, const double *z,
                           double *const *cols, uint8_t *status, double *gap,
                           double *ratio);
 
+/* filters.optimalneighborhood (filters/OptimalNeighborhoodFilter.cpp:72-163)
+ * for every point: KD3Index::knnSearch with kmax neighbors, the running
+ * mean and scatter matrix in PDAL's order, the Eigen-semantics solve of
+ * B / (n - 1) for k in [kmin, kmax], eigenentropy with the C library's log;
+ * kopt[i] = OptimalKNN, ropt[i] = OptimalRadius; gap (optional) the
+ * smallest |entropy - best| over the other k with a finite entropy,
+ * relative to |best| (how close the choice was).  Returns the number of rows
+ * whose solve failed (PDAL throws). */
+uint64_t orc_optimal_neighborhood(const orc_kd3 *t, const double *x, const double *y,
+                                  const double *z, uint64_t n, uint64_t kmin, uint64_t kmax,
+                                  int threads, uint64_t *kopt, double *ropt, double *gap);
+
 /* Extended-precision symmetric 3x3 eigensolve (cyclic Jacobi): ascending
  * values, column-major vectors.  Returns the sweep count. */
 int orc_jacobi_sym3(const double cov[9], double evals[3], double evecs[9]);

@@ -733,3 +733,113 @@ void orc_truth_from_lists(const double *x, const double *y, const double *z,
     p.ratio = ratio;
     run_jobs(p, n, threads);
 }
+
+/* ------------------------------------------ optimal neighborhood ------ */
+/* filters/OptimalNeighborhoodFilter.cpp:72-163, one point at a time. */
+typedef struct {
+    const orc_kd3 *t;
+    const double *x, *y, *z;
+    uint64_t kmin, kmax, begin, end;
+    uint64_t *kopt;
+    double *ropt, *gap;
+    uint64_t failed;
+} optn_job;
+
+static void *optn_worker(void *arg)
+{
+    optn_job *j = (optn_job *)arg;
+    uint64_t ids[64];
+    double d2[64];
+    for (uint64_t i = j->begin; i < j->end; ++i) {
+        orc_kd3_neighbors(j->t, j->x[i], j->y[i], j->z[i], j->kmax, 1, ids, d2);
+        double minentropy = DBL_MAX;
+        uint64_t kopt = 0;
+        double ropt = 0.0;
+        double mx = 0.0, my = 0.0, mz = 0.0;
+        double B[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        double ent[64];
+        int nent = 0;
+        for (uint64_t k = 0; k < j->kmax; ++k) {
+            const uint64_t q = ids[k];
+            double dx = j->x[q] - mx;
+            double dy = j->y[q] - my;
+            double dz = j->z[q] - mz;
+            double n = (double)(k + 1);
+            mx += dx / n;
+            my += dy / n;
+            mz += dz / n;
+            double s = (n - 1) / n;
+            B[0][0] = B[0][0] + s * dx * dx;
+            B[1][1] = B[1][1] + s * dy * dy;
+            B[2][2] = B[2][2] + s * dz * dz;
+            B[1][0] = B[0][1] = B[0][1] + s * dx * dy;
+            B[2][0] = B[0][2] = B[0][2] + s * dx * dz;
+            B[1][2] = B[2][1] = B[2][1] + s * dy * dz;
+            if (k + 1 < j->kmin) continue;
+            double cov[9], ev[3], vec[9];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) cov[3 * r + c] = B[r][c] / (n - 1);
+            if (orc_eigen_sym3(cov, ev, vec) != 0) {
+                j->failed++;
+                break;
+            }
+            double lambda[3] = {ev[2] > 0.0 ? ev[2] : 0.0, ev[1] > 0.0 ? ev[1] : 0.0,
+                                ev[0] > 0.0 ? ev[0] : 0.0};
+            double sum = 0.0;
+            for (int m = 0; m < 3; ++m) sum += lambda[m];
+            for (int m = 0; m < 3; ++m) lambda[m] = lambda[m] / sum;
+            double entropy = -(lambda[2] * log(lambda[2]) + lambda[1] * log(lambda[1]) +
+                               lambda[0] * log(lambda[0]));
+            if (entropy == entropy) ent[nent++] = entropy;
+            if (entropy < minentropy) {
+                minentropy = entropy;
+                kopt = k + 1;
+                ropt = d2[k];
+            }
+        }
+        j->kopt[i] = kopt;
+        j->ropt[i] = sqrt(ropt);
+        if (j->gap) {
+            double g = INFINITY;
+            int seen_best = 0;
+            for (int m = 0; m < nent; ++m) {
+                if (ent[m] == minentropy && !seen_best) {
+                    seen_best = 1;
+                    continue;
+                }
+                double dd = fabs(ent[m] - minentropy) / (fabs(minentropy) > 0 ? fabs(minentropy) : 1.0);
+                if (dd < g) g = dd;
+            }
+            j->gap[i] = g;
+        }
+    }
+    return NULL;
+}
+
+uint64_t orc_optimal_neighborhood(const orc_kd3 *t, const double *x, const double *y,
+                                  const double *z, uint64_t n, uint64_t kmin, uint64_t kmax,
+                                  int threads, uint64_t *kopt, double *ropt, double *gap)
+{
+    if (threads < 1) threads = 1;
+    uint64_t chunk = n / (uint64_t)threads + ((n % (uint64_t)threads) ? 1 : 0);
+    pthread_t *th = malloc(sizeof(pthread_t) * (size_t)threads);
+    optn_job *jobs = malloc(sizeof(optn_job) * (size_t)threads);
+    for (int i = 0; i < threads; ++i) {
+        uint64_t b = (uint64_t)i * chunk, e = (i + 1 == threads) ? n : (uint64_t)(i + 1) * chunk;
+        if (b > n) b = n;
+        if (e > n) e = n;
+        jobs[i] = (optn_job){t, x, y, z, kmin, kmax, b, e, kopt, ropt, gap, 0};
+        if (threads == 1)
+            optn_worker(&jobs[i]);
+        else
+            pthread_create(&th[i], NULL, optn_worker, &jobs[i]);
+    }
+    uint64_t failed = 0;
+    for (int i = 0; i < threads; ++i) {
+        if (threads > 1) pthread_join(th[i], NULL);
+        failed += jobs[i].failed;
+    }
+    free(th);
+    free(jobs);
+    return failed;
+}

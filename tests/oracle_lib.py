@@ -78,6 +78,10 @@ class Oracle:
         L.orc_kd3_build.restype = ctypes.c_void_p
         L.orc_kd3_build.argtypes = [_dp, _dp, _dp, ctypes.c_uint64]
         L.orc_kd3_free.argtypes = [ctypes.c_void_p]
+        L.orc_optimal_neighborhood.restype = ctypes.c_uint64
+        L.orc_optimal_neighborhood.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_uint64,
+                                               ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                               _u64p, _dp, _dp]
         L.orc_kd3_neighbors.restype = ctypes.c_uint64
         L.orc_kd3_neighbors.argtypes = [ctypes.c_void_p, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double,
@@ -219,6 +223,19 @@ class Oracle:
                              _ptr(tree.z, _dp), n, ctypes.byref(st), threads,
                              arr, _ptr(status, _u8p))
         return cols, status
+
+    def optimal_neighborhood(self, tree, kmin=10, kmax=14, threads=8):
+        """filters.optimalneighborhood: (OptimalKNN, OptimalRadius, entropy
+        gap of the choice, failed rows)."""
+        n = tree.n
+        kopt = np.zeros(n, np.uint64)
+        ropt = np.zeros(n, np.float64)
+        gap = np.zeros(n, np.float64)
+        failed = self.L.orc_optimal_neighborhood(tree.h, _ptr(tree.x, _dp), _ptr(tree.y, _dp),
+                                                 _ptr(tree.z, _dp), n, kmin, kmax, threads,
+                                                 _ptr(kopt, _u64p), _ptr(ropt, _dp),
+                                                 _ptr(gap, _dp))
+        return kopt, ropt, gap, int(failed)
 
     def stage_run_range(self, tree, st, begin, end, threads=1, init=0.0):
         """PDAL's CPU stage over rows [begin, end) of the tree's cloud."""
