@@ -283,6 +283,32 @@ def las_leg(args, xyz):
             "ms_with_upload": ms_wall, "points_per_s_with_upload": n / (ms_wall / 1e3)}
 
 
+def optimal_leg(args, xyz):
+    """filters.optimalneighborhood (min_k 10, max_k 14) on the bench cloud:
+    index, ordered 14-neighbor lists with their exact re-solve, and the
+    per-prefix eigenentropy; the index and tree are rebuilt every step and
+    the two output columns copied to the host."""
+    import gpupdal_amd as g
+    ctx = g.Context(0)
+    cl = ctx.cloud(*xyz)
+    cl.invalidate()
+    cl.optimal_neighborhood(10, 14)
+    ms = []
+    for _ in range(args.las_steps):
+        cl.invalidate()
+        t0 = time.perf_counter()
+        cl.optimal_neighborhood(10, 14)
+        ms.append((time.perf_counter() - t0) * 1e3)
+    cl.close()
+    ctx.close()
+    med = sorted(ms)[len(ms) // 2]
+    n = len(xyz[0])
+    return {"points": n, "min_k": 10, "max_k": 14, "ms_per_step": med,
+            "points_per_s": n / (med / 1e3),
+            "workload": "filters.optimalneighborhood on the bench cloud, index and tree "
+                        "rebuilt each step, OptimalKNN/OptimalRadius copied to the host"}
+
+
 def serving_leg(args):
     """Serving: Poisson arrivals of tile requests at a fixed rate (sizes
     log-uniform in [5k, 4M], knn in {10, 16}, normal + covariancefeatures)
@@ -445,7 +471,8 @@ def main():
     ap.add_argument("--requests", type=int, default=64, help="requests of the serving leg")
     ap.add_argument("--rate", type=float, default=120.0, help="serving arrivals per second")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
-    ap.add_argument("--no-las", action="store_true", help="skip the LAS decode leg")
+    ap.add_argument("--no-las", action="store_true",
+                    help="skip the LAS decode and optimal-neighborhood legs")
     ap.add_argument("--las-steps", type=int, default=7)
     ap.add_argument("--precision", action="store_true",
                     help="only the precision-stress report (20M points, roofs, wires)")
@@ -473,6 +500,7 @@ def main():
     batched = None if args.no_batched else batched_leg(args)
     serving = None if args.no_serving else serving_leg(args)
     las = None if args.no_las else las_leg(args, xyz)
+    optimal = None if args.no_las else optimal_leg(args, xyz)
     cpu, cpu_parallel = (None, None) if args.no_cpu else cpu_leg(args, xyz)
     line = {
         "metric": "points/s kNN+covariance features, batched vs plain GPU; p50/p99 tile latency",
@@ -501,6 +529,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "las_read": las,
+        "optimal_neighborhood": optimal,
         "cpu_baseline": cpu,
         "cpu_baseline_parallel": cpu_parallel,
     }

@@ -704,8 +704,41 @@ void EigenvaluesFilter::filter(PointView& view)
     }
 }
 
+// ------------------------------------------------- optimalneighborhood --
+void OptimalNeighborhoodFilter::addArgs(ProgramArgs& args)
+{
+    args.add("min_k", "Minimum k-Nearest Neighbors", m_kMin, uint64_t(10));
+    args.add("max_k", "Maximum k-Nearest Neighbors", m_kMax, uint64_t(14));
+}
+
+void OptimalNeighborhoodFilter::addDimensions(PointLayout* layout)
+{
+    layout->registerDim(Dimension::Id::OptimalKNN);
+    layout->registerDim(Dimension::Id::OptimalRadius);
+}
+
+void OptimalNeighborhoodFilter::filter(PointView& view)
+{
+    const uint64_t n = view.size();
+    if (!n) return;
+    std::vector<uint64_t> kopt(n);
+    std::vector<double> ropt(n);
+    auto res = view.resident();
+    gf_ctx* ctx = device_context();
+    const int rc = gf_optimal_neighborhood(ctx, res->cloud, static_cast<uint32_t>(m_kMin),
+                                           static_cast<uint32_t>(m_kMax), kopt.data(),
+                                           ropt.data(), GF_WHERE_HOST);
+    if (rc != GF_OK) throwError(gf_last_error(ctx));
+    for (uint64_t i = 0; i < n; ++i) {
+        view.setField(Dimension::Id::OptimalKNN, i, static_cast<double>(kopt[i]));
+        view.setField(Dimension::Id::OptimalRadius, i, ropt[i]);
+    }
+}
+
 std::unique_ptr<Stage> createStage(const std::string& name)
 {
+    if (name == "filters.optimalneighborhood")
+        return std::unique_ptr<Stage>(new OptimalNeighborhoodFilter);
     if (name == "filters.covariancefeatures")
         return std::unique_ptr<Stage>(new CovarianceFeaturesFilter);
     if (name == "filters.normal") return std::unique_ptr<Stage>(new NormalFilter);

@@ -272,6 +272,23 @@ private:
     int m_minK = 3;
 };
 
+// filters.optimalneighborhood (filters/OptimalNeighborhoodFilter.cpp:53-163):
+// OptimalKNN and OptimalRadius per point, which covariancefeatures' Density
+// and optimized mode read.
+class OptimalNeighborhoodFilter : public Filter {
+public:
+    std::string getName() const override { return "filters.optimalneighborhood"; }
+
+protected:
+    void addArgs(ProgramArgs& args) override;
+    void addDimensions(PointLayout* layout) override;
+    void filter(PointView& view) override;
+
+private:
+    uint64_t m_kMin = 10;
+    uint64_t m_kMax = 14;
+};
+
 // Creates a stage by its PDAL name (nullptr for other names), as
 // StageFactory::createStage does for these three.
 std::unique_ptr<Stage> createStage(const std::string& name);

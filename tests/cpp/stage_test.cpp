@@ -439,4 +439,42 @@ TEST(GpuSkip, StridePaddingSkipsRowZero)
     ASSERT_FLOAT_EQ(out->getFieldAs<double>(Id::Linearity, 1), 1.0);
 }
 
+TEST(GpuChain, OptimalNeighborhoodThenDensity)
+{
+    // filters.optimalneighborhood registers OptimalKNN/OptimalRadius, so
+    // covariancefeatures can compute Density from them
+    // (CovarianceFeaturesFilter.cpp:159-167, 357-362).
+    std::vector<std::array<double, 3>> p;
+    for (int y = 0; y < 15; ++y)
+        for (int x = 0; x < 15; ++x)
+            p.push_back({x * 0.5 + 0.03 * (y % 4), y * 0.5, 0.2 * ((x * 7 + y * 3) % 5)});
+    PointTable table;
+    BufferReader reader;
+    OptimalNeighborhoodFilter opt;
+    CovarianceFeaturesFilter feats;
+    Options b;
+    b.add("knn", 8);
+    b.add("feature_set", "Density");
+    feats.setOptions(b);
+    opt.setInput(reader);
+    feats.setInput(opt);
+    feats.prepare(table);
+    PointViewPtr view = make_view(table, p);
+    reader.addView(view);
+    PointViewPtr out = *feats.execute(table).begin();
+    int chosen = 0;
+    for (point_count_t i = 0; i < out->size(); ++i) {
+        const double k = out->getFieldAs<double>(Id::OptimalKNN, i);
+        const double r = out->getFieldAs<double>(Id::OptimalRadius, i);
+        if (k == 0) continue;  // every entropy NaN: PDAL keeps 0
+        ++chosen;
+        ASSERT_GE(k, 10.0);
+        ASSERT_LE(k, 14.0);
+        ASSERT_TRUE(r > 0.0);
+        ASSERT_NEAR(out->getFieldAs<double>(Id::Density, i),
+                    (k + 1) / (4.0 / 3.0 * 3.14159265 * r * r * r), 1e-9);
+    }
+    ASSERT_TRUE(chosen > 100);
+}
+
 int main(int argc, char** argv) { return mini::run_all(argc, argv); }
