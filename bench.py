@@ -44,8 +44,8 @@ SEARCH_BYTES_PER_POINT = 84.0
 # kernel in the same runs: FETCH_SIZE reads 0.5 of the copied bytes there
 # (factor 2.0), WRITE_SIZE 1.0 (factor 1.0).  The runs, commands and raw
 # counter values are in profiles/k_knn_plain_4M_knn16_traffic.md.
-MEASURED_TRAFFIC = {"points": 4_000_000, "knn": 16, "fetch_kib": 93199.4,
-                    "write_kib": 344261.3, "fetch_factor": 2.0, "write_factor": 1.0}
+MEASURED_TRAFFIC = {"points": 4_000_000, "knn": 16, "fetch_kib": 92925.3,
+                    "write_kib": 344280.5, "fetch_factor": 2.0, "write_factor": 1.0}
 # FP64 flops per candidate distance: 3 sub, 3 mul, 2 add.
 FLOPS_PER_CANDIDATE = 8.0
 HBM_PEAK_GBS = 8000.0
