@@ -708,7 +708,9 @@ int gf_optimal_neighborhood(gf_ctx* ctx, gf_cloud* cloud, uint32_t min_k, uint32
         if (early) c->fork_side();
         launch_knn(cv, prog, 2, out, c->knn_tuning(), c->stream);
         if (early) c->build_kdtree(*cl, c->side, true);
-        c->resolve_flagged(*cl, cv, prog, 2, out, nullptr);
+        const uint32_t nflagged = c->resolve_flagged(*cl, cv, prog, 2, out, nullptr);
+        launch_lists_to_sorted(out.ids, static_cast<int>(max_k), out.flagged, nflagged, cv.perm,
+                               cl->inv.as<uint32_t>(), c->stream);
         DevBuf res;
         uint64_t* kopt = optimal_knn;
         double* ropt = optimal_radius;
@@ -719,8 +721,8 @@ int gf_optimal_neighborhood(gf_ctx* ctx, gf_cloud* cloud, uint32_t min_k, uint32
         }
         uint32_t* failed = c->counters.as<uint32_t>() + 6;
         launch_optimal_nbhd(out.ids, n, static_cast<int>(min_k), static_cast<int>(max_k),
-                            cl->x.as<double>(), cl->y.as<double>(), cl->z.as<double>(), kopt,
-                            ropt, failed, c->stream);
+                            cl->x.as<double>(), cl->y.as<double>(), cl->z.as<double>(),
+                            cv.sx, cv.sy, cv.sz, kopt, ropt, failed, c->stream);
         uint32_t nfail = 0;
         GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, c->stream));
         if (where != GF_WHERE_DEVICE) {

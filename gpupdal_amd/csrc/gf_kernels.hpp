@@ -225,10 +225,16 @@ void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram&
                        const TileTables* tt = nullptr);
 
 // optnbhd.hip -- filters.optimalneighborhood from n x kmax KD3-ordered lists
-// (original order); failed counts rows whose eigen solve did not converge.
+// of sorted positions (rows in original order; x, y, z original order, sx,
+// sy, sz sorted); failed counts rows whose eigen solve did not converge.
 void launch_optimal_nbhd(const uint32_t* ids, uint64_t n, int kmin, int kmax, const double* x,
-                         const double* y, const double* z, uint64_t* kopt, double* ropt,
-                         uint32_t* failed, hipStream_t s);
+                         const double* y, const double* z, const double* sx, const double* sy,
+                         const double* sz, uint64_t* kopt, double* ropt, uint32_t* failed,
+                         hipStream_t s);
+// The flagged rows' lists (point ids from the exact re-solve) to sorted
+// positions.
+void launch_lists_to_sorted(uint32_t* ids, int kmax, const uint32_t* flagged, uint32_t nflagged,
+                            const uint32_t* perm, const uint32_t* inv, hipStream_t s);
 
 // Debug/parity hook: device eigen solver over m matrices (row-major 3x3).
 void launch_eigen_batch(const double* cov, uint64_t m, double* evals,

@@ -502,7 +502,10 @@ __device__ __forceinline__ void finish_query(const KnnArgs& a, const WaveGrid& w
     const Ent kept = ent_shfl(best, (lane * stride) & 63);
     if (a.mode != 0) {
         const uint32_t qid = a.cv.perm[s];
-        if (lane < k) a.out.ids[static_cast<uint64_t>(qid) * k + lane] = kept.id;
+        // Mode 2 keeps sorted positions (the consumer reads the sorted
+        // coordinates, which neighbors share cache lines in).
+        if (lane < k)
+            a.out.ids[static_cast<uint64_t>(qid) * k + lane] = a.mode == 2 ? kept.j : kept.id;
         if (lane == 0) a.out.status[0][qid] = flags;
         if (lane == 0 && flags) {
             uint32_t slot = atomicAdd(a.out.nflagged, 1u);

@@ -1,7 +1,9 @@
 // filters.optimalneighborhood: for each point, the k in [min_k, max_k] whose
 // neighborhood has the smallest eigenentropy (filters/OptimalNeighborhoodFilter.cpp:
 // 72-163).  Input: every row's max_k neighbors in KD3Index order (the
-// point itself first), from the list-mode search with its exact re-solve.
+// point itself first) as sorted positions, from the ordered-list search and
+// its exact re-solve; a row's neighbors are near each other in the sorted
+// coordinates, so their reads share cache lines.
 // One thread per row walks its list once: the running mean and scatter
 // matrix are updated neighbor by neighbor in PDAL's order and rounding
 // (:94-129), and from min_k on each prefix's covariance B / (n - 1) goes
@@ -17,11 +19,31 @@ namespace gf {
 
 namespace {
 
+// The exact re-solve writes point ids: its rows' lists become sorted
+// positions through the inverse permutation.
+__global__ __launch_bounds__(256) void k_lists_to_sorted(uint32_t* __restrict__ ids, int kmax,
+                                                         const uint32_t* __restrict__ flagged,
+                                                         uint32_t nflagged,
+                                                         const uint32_t* __restrict__ perm,
+                                                         const uint32_t* __restrict__ inv)
+{
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (t >= static_cast<uint64_t>(nflagged) * kmax) return;
+    const uint32_t f = static_cast<uint32_t>(t / kmax);
+    const uint32_t j = static_cast<uint32_t>(t % kmax);
+    const uint64_t row = perm[flagged[f]];
+    uint32_t& e = ids[row * kmax + j];
+    e = inv[e];
+}
+
 __global__ __launch_bounds__(256) void k_optimal_nbhd(const uint32_t* __restrict__ ids,
                                                       uint64_t n, int kmin, int kmax,
                                                       const double* __restrict__ x,
                                                       const double* __restrict__ y,
                                                       const double* __restrict__ z,
+                                                      const double* __restrict__ sx,
+                                                      const double* __restrict__ sy,
+                                                      const double* __restrict__ sz,
                                                       uint64_t* __restrict__ kopt_out,
                                                       double* __restrict__ ropt_out,
                                                       uint32_t* __restrict__ failed)
@@ -37,7 +59,7 @@ __global__ __launch_bounds__(256) void k_optimal_nbhd(const uint32_t* __restrict
     double ropt = 0.0;
     for (int k = 0; k < kmax; ++k) {
         const uint32_t j = nb[k];
-        const double px = x[j], py = y[j], pz = z[j];
+        const double px = sx[j], py = sy[j], pz = sz[j];
         const double dx = px - mx, dy = py - my, dz = pz - mz;
         const double cnt = static_cast<double>(k + 1);
         mx += dx / cnt;
@@ -78,13 +100,23 @@ __global__ __launch_bounds__(256) void k_optimal_nbhd(const uint32_t* __restrict
 
 } // namespace
 
+void launch_lists_to_sorted(uint32_t* ids, int kmax, const uint32_t* flagged, uint32_t nflagged,
+                            const uint32_t* perm, const uint32_t* inv, hipStream_t s)
+{
+    if (!nflagged) return;
+    const uint64_t total = static_cast<uint64_t>(nflagged) * static_cast<uint64_t>(kmax);
+    hipLaunchKernelGGL(k_lists_to_sorted, dim3(static_cast<uint32_t>(div_up(total, 256))),
+                       dim3(256), 0, s, ids, kmax, flagged, nflagged, perm, inv);
+}
+
 void launch_optimal_nbhd(const uint32_t* ids, uint64_t n, int kmin, int kmax, const double* x,
-                         const double* y, const double* z, uint64_t* kopt, double* ropt,
-                         uint32_t* failed, hipStream_t s)
+                         const double* y, const double* z, const double* sx, const double* sy,
+                         const double* sz, uint64_t* kopt, double* ropt, uint32_t* failed,
+                         hipStream_t s)
 {
     if (!n) return;
     hipLaunchKernelGGL(k_optimal_nbhd, dim3(static_cast<uint32_t>(div_up(n, 256))), dim3(256),
-                       0, s, ids, n, kmin, kmax, x, y, z, kopt, ropt, failed);
+                       0, s, ids, n, kmin, kmax, x, y, z, sx, sy, sz, kopt, ropt, failed);
 }
 
 } // namespace gf
