@@ -376,15 +376,15 @@ namespace {
 
 // One stage on the GPU: columns start as the view's current values (rows a
 // stage skips keep them), status is one byte per row.
-struct GpuRun {
+struct StageResult {
     std::vector<std::vector<double>> cols;
     std::vector<uint8_t> status;
 };
 
-GpuRun run_on_gpu(PointView& view, const gf_stage& st, const std::vector<Dimension::Id>& outdims,
+StageResult run_on_gpu(PointView& view, const gf_stage& st, const std::vector<Dimension::Id>& outdims,
                   const Stage& stage)
 {
-    GpuRun r;
+    StageResult r;
     const uint64_t n = view.size();
     r.status.assign(n, 0);
     for (auto d : outdims) r.cols.push_back(view.column(d));
@@ -492,7 +492,7 @@ void CovarianceFeaturesFilter::filter(PointView& view)
                                   : (m_mode == Mode::SQRT ? GF_MODE_SQRT : GF_MODE_NORMALIZED);
     st.features = mask;
     st.always_up = 1;
-    GpuRun r = run_on_gpu(view, st, outdims, *this);
+    StageResult r = run_on_gpu(view, st, outdims, *this);
 
     // Rows are applied in order; the first row PDAL would throw on ends the
     // stage after the rows before it were written.
@@ -612,7 +612,7 @@ void NormalFilter::filter(PointView& view)
     st.stride = 1;
     st.always_up = m_up ? 1 : 0;
     const std::vector<Id> outdims{Id::NormalX, Id::NormalY, Id::NormalZ, Id::Curvature};
-    GpuRun r = run_on_gpu(view, st, outdims, *this);
+    StageResult r = run_on_gpu(view, st, outdims, *this);
     double vp[3] = {0, 0, 0};
     const bool toward = m_viewpointArg->set() && parse_viewpoint(m_viewpoint, vp);
     for (uint64_t i = 0; i < n; ++i) {
@@ -688,7 +688,7 @@ void EigenvaluesFilter::filter(PointView& view)
     st.stride = static_cast<int32_t>(m_stride);
     st.normalize = m_normalize ? 1 : 0;
     const std::vector<Id> outdims{Id::Eigenvalue0, Id::Eigenvalue1, Id::Eigenvalue2};
-    GpuRun r = run_on_gpu(view, st, outdims, *this);
+    StageResult r = run_on_gpu(view, st, outdims, *this);
     for (uint64_t i = 0; i < n; ++i) {
         const uint8_t s = r.status[i];
         if (s & GF_ST_EIGEN_FAIL) throwError("Cannot perform eigen decomposition.");

@@ -12,8 +12,7 @@
 // Stage API these stages use is provided here under the same names (Options,
 // PointTable/PointLayout, PointView, BufferReader, Stage::prepare/execute,
 // pdal_error).  A maintainer adding the GPU stages to a real PDAL build keeps
-// the filter classes and swaps this API layer for libpdal's (the binding is
-// shown in INTEGRATION.md).
+// the filter classes and swaps this API layer for libpdal's.
 //
 // Not supported on the GPU (each raises pdal_error naming the option):
 // radius searches (radius/min_k), optimized=true, and filters.normal's
