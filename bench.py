@@ -15,13 +15,17 @@ host: the KD build over the whole cloud plus both stages over a bounded
 sample of rows, single-threaded (PDAL's default), extrapolated to the cloud;
 cpu_baseline_parallel repeats the stages on up to 16 host threads.
 
-Two more legs run by default and are reported beside the headline value:
+More legs run by default and are reported beside the headline value:
   batched  -- 256 seeded tiles (sizes log-uniform in [10k, 2M], knn in
               {8, 16, 32}, density 2-50 pts/m^2) as one batch vs the same
               tiles one at a time through the plain path;
   serving  -- Poisson arrivals of tile requests (sizes log-uniform in
               [5k, 4M], knn in {10, 16}) into the serving loop: p50/p99
-              latency (submit -> result on the host) and sustained points/s.
+              latency (submit -> result on the host) and sustained points/s;
+  precision -- the precision-stress cloud (20M points, 5% planar roofs, 2%
+              collinear wires, eigenvalues + covariancefeatures knn=32): per
+              column max and p99 error against the extended-precision truth
+              on a random sample of rows.
 
 Prints ONE JSON line.
 """
@@ -361,13 +365,15 @@ def serving_leg(args):
             "sustained_points_per_s": pts / (last - first), "batches": len(batches)}
 
 
-def precision_leg(args):
+def precision_leg(args, sample_rows=0):
     """Precision stress: 20M points with 5% exactly planar roofs and 2%
     collinear wires, eigenvalues + covariancefeatures knn=32 chained; every
     column against the extended-precision truth computed on the host from
     the same KD3Index neighbor lists.  Per feature: max and p99 relative
     error over well-conditioned rows (eigengap > 1e-3, smallest eigenvalue
-    resolved), max absolute error over degenerate rows."""
+    resolved), max absolute error over degenerate rows.  sample_rows > 0:
+    the truth on that many random rows (the default run), else on every
+    row."""
     import numpy as np
     import gpupdal_amd as g
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -388,9 +394,19 @@ def precision_leg(args):
     print(f"precision: GPU run done ({ms:.0f} ms); host KD3 lists", file=sys.stderr, flush=True)
     orc = ol.Oracle()
     tree = orc.kd3(x, y, z)
-    nbr = tree.neighbors_all(knn + 1, threads=threads)
+    if sample_rows:
+        rows = np.sort(np.random.default_rng(0x3C).choice(n, min(sample_rows, n),
+                                                          replace=False))
+        nbr = tree.neighbors_rows(rows, knn + 1, threads=threads)
+        ev = [c[rows] for c in ev]
+        cf = [c[rows] for c in cf]
+        evs, cfs = evs[rows], cfs[rows]
+    else:
+        nbr = tree.neighbors_all(knn + 1, threads=threads)
     del tree
     report = {"points": n, "knn": knn, "roof_frac": 0.05, "wire_frac": 0.02,
+              "rows_checked": int(nbr.shape[0]),
+              "rows": "random sample" if sample_rows else "all",
               "gpu_ms_host_io": ms, "tie_rows": int(st["tie_rows"]),
               "incomplete_rows": int(st["incomplete_rows"]), "columns": {}}
     for stage_cols, stat, kind in ((ev, evs, ol.STAGE_EIGENVALUES), (cf, cfs, ol.STAGE_COVFEAT)):
@@ -415,6 +431,22 @@ def precision_leg(args):
     return report
 
 
+def host_cpu():
+    """The host's logical CPU count and model name (from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model,
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else None}
+
+
 def cpu_leg(args, xyz):
     """PDAL's CPU path (the oracle's restatement), single-threaded, on a
     bounded sample of rows; the KD build covers the whole cloud."""
@@ -432,7 +464,8 @@ def cpu_leg(args, xyz):
     orc.stage_run_range(tree, ol.stage(ol.STAGE_COVFEAT, args.knn), 0, rows, threads=1)
     t_rows = time.perf_counter() - t0
     total = t_build + t_rows * n / rows
-    out = {"value": n / total, "unit": "points/s", "cores": 1, "kind": "port",
+    host = host_cpu()
+    out = {"value": n / total, "unit": "points/s", "cores": 1, "kind": "port", **host,
            "sample": f"KD build over all {n} points ({t_build:.2f} s) + normal and "
                      f"covariancefeatures knn={args.knn} on rows [0, {rows}) "
                      f"({t_rows:.2f} s), extrapolated to {n} rows",
@@ -448,6 +481,7 @@ def cpu_leg(args, xyz):
     t_prow = time.perf_counter() - t0
     ptotal = t_build + t_prow * n / prow
     parallel = {"value": n / ptotal, "unit": "points/s", "cores": threads, "kind": "port",
+                **host,
                 "sample": f"as the single-threaded sample, rows [0, {prow}) on {threads} "
                           f"threads ({t_prow:.2f} s), extrapolated to {n} rows",
                 "seconds_estimated": ptotal}
@@ -477,6 +511,9 @@ def main():
     ap.add_argument("--precision", action="store_true",
                     help="only the precision-stress report (20M points, roofs, wires)")
     ap.add_argument("--precision-points", type=int, default=20_000_000)
+    ap.add_argument("--precision-sample", type=int, default=250_000,
+                    help="rows of the precision leg checked against the truth in the default run")
+    ap.add_argument("--no-precision", action="store_true", help="skip the precision leg")
     ap.add_argument("--kd-overlap-min", type=int, default=131072,
                     help="clouds of at least this many points build the exact tree beside "
                          "the search (0: after it)")
@@ -501,6 +538,8 @@ def main():
     serving = None if args.no_serving else serving_leg(args)
     las = None if args.no_las else las_leg(args, xyz)
     optimal = None if args.no_las else optimal_leg(args, xyz)
+    precision = (None if args.no_precision
+                 else precision_leg(args, sample_rows=args.precision_sample))
     cpu, cpu_parallel = (None, None) if args.no_cpu else cpu_leg(args, xyz)
     line = {
         "metric": "points/s kNN+covariance features, batched vs plain GPU; p50/p99 tile latency",
@@ -530,6 +569,7 @@ def main():
         "serving": serving,
         "las_read": las,
         "optimal_neighborhood": optimal,
+        "precision": precision,
         "cpu_baseline": cpu,
         "cpu_baseline_parallel": cpu_parallel,
     }

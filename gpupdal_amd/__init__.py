@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libgpupdal_amd.so")
 
 GF_OK = 0
 GF_ERR_ARG, GF_ERR_HIP, GF_ERR_UNSUPPORTED, GF_ERR_OOM, GF_ERR_STATE = -1, -2, -3, -4, -5
+GF_ERR_TIMEOUT = -6
 WHERE_HOST, WHERE_DEVICE = 0, 1
 STAGE_COVFEAT, STAGE_NORMAL, STAGE_EIGENVALUES = 0, 1, 2
 MODE_RAW, MODE_SQRT, MODE_NORMALIZED = 0, 1, 2
@@ -86,7 +87,7 @@ EXPORTED = [
     "gf_server_release", "gf_server_destroy",
     "gf_autotune", "gf_tuning_save", "gf_tuning_load",
     "gf_las_parse", "gf_cloud_create_las", "gf_las_pack_extra", "gf_debug_cloud_xyz",
-    "gf_optimal_neighborhood",
+    "gf_optimal_neighborhood", "gf_debug_grid_params",
 ]
 
 
@@ -116,6 +117,7 @@ def load_library(path=LIB_PATH):
     L.gf_knn.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.c_int]
     L.gf_debug_eigen.argtypes = [vp, _dp, ctypes.c_uint64, _dp, _dp, _i32p]
     L.gf_debug_grid.argtypes = [vp, vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p, _u32p]
+    L.gf_debug_grid_params.argtypes = [vp, vp, ctypes.c_uint32, _dp, _u32p]
     L.gf_debug_kdtree.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.gf_debug_copy.argtypes = [vp, vp, vp, ctypes.c_uint64]
     L.gf_las_parse.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(GfLasInfo), ctypes.c_char_p,
@@ -455,6 +457,21 @@ class Cloud:
                         "gf_debug_grid")
         return perm, keys, cs, bits.value
 
+    def grid_params(self, k):
+        """The grid of the index built for k: a dict with origin (ox, oy,
+        oz), edges (c, cz), inverse edges (inv_c, inv_cz) and integer fields
+        gx, gy, gz, tiles_x, bz, key_bits, ncols."""
+        f = np.zeros(7)
+        u = np.zeros(8, np.uint32)
+        self.ctx._check(self.ctx.L.gf_debug_grid_params(self.ctx.h, self.h, k,
+                                                        f.ctypes.data_as(_dp),
+                                                        u.ctypes.data_as(_u32p)),
+                        "gf_debug_grid_params")
+        d = dict(zip(["ox", "oy", "oz", "c", "cz", "inv_c", "inv_cz"], f.tolist()))
+        d.update(zip(["gx", "gy", "gz", "tiles_x", "bz", "key_bits", "ncols"],
+                     [int(v) for v in u[:7]]))
+        return d
+
 
 class Batch:
     """Independent tiles run together (one index build, one search, one
@@ -568,7 +585,12 @@ class Server:
         """Waits; returns (per-stage columns, statuses, times) with times =
         (submit, batch start, done, batch id, tiles in the batch)."""
         rc = self.L.gf_server_wait(self.h, ticket, float(timeout))
+        if rc == GF_ERR_TIMEOUT:
+            raise GfError(rc, f"request {ticket}: still running after {timeout} s")
         if rc != GF_OK:
+            # Finished with an error: the request and its arrays are let go.
+            self.L.gf_server_release(self.h, ticket)
+            self._held.pop(ticket, None)
             raise GfError(rc, f"request {ticket}")
         tm = np.zeros(3)
         bid = ctypes.c_uint32()

@@ -187,7 +187,8 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         sp.col_offset = ncol_total;
         ncol_total += sp.ncols;
     }
-    if (ncol_total > 16) throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
+    if (ncol_total > kMaxOutColumns)
+        throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
     reset_spans();
     hipEvent_t t_all = mark();
 
@@ -202,9 +203,7 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         if (knn < 1) throw Error(GF_ERR_ARG, "knn must be at least 1");
         const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(nt, static_cast<uint64_t>(knn) + 1));
         const uint32_t f = static_cast<uint32_t>(stride) * k;
-        if (f > static_cast<uint32_t>(kMaxFetch - 1))
-            throw Error(GF_ERR_UNSUPPORTED,
-                        "(knn+1)*stride above 63 is outside the device kNN envelope");
+        check_neighborhood(nt, knn, stride);
         fetch[t] = std::max<uint32_t>(f, 1);
         TileInfo& ti = tiles[t];
         ti.prog = base;

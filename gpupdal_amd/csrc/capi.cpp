@@ -172,6 +172,14 @@ void validate_stage(const gf_stage& s)
     }
 }
 
+void check_neighborhood(uint64_t n, int64_t knn, int stride)
+{
+    const uint64_t k = std::min<uint64_t>(n, static_cast<uint64_t>(knn) + 1);
+    if (static_cast<uint64_t>(stride) * k > static_cast<uint64_t>(kMaxFetch - 1))
+        throw Error(GF_ERR_UNSUPPORTED,
+                    "(knn+1)*stride above 63 is outside the device kNN envelope");
+}
+
 // Runs one group of stages that share the neighborhood (knn, stride).
 void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
                         uint8_t* const* status, int where)
@@ -181,9 +189,7 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     const uint64_t K = static_cast<uint64_t>(st[0].knn) + 1;
     const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(n, K));
     const uint32_t fetch = static_cast<uint32_t>(stride) * k;
-    if (fetch > static_cast<uint32_t>(kMaxFetch - 1))
-        throw Error(GF_ERR_UNSUPPORTED,
-                    "(knn+1)*stride above 63 is outside the device kNN envelope");
+    check_neighborhood(n, st[0].knn, stride);
     FusedProgram prog{};
     prog.nstages = ns;
     prog.k = static_cast<int32_t>(k);
@@ -201,7 +207,8 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
         sp.col_offset = ncol_total;
         ncol_total += sp.ncols;
     }
-    if (ncol_total > 16) throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
+    if (ncol_total > kMaxOutColumns)
+        throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
 
     reset_spans();
     hipEvent_t t_all = mark();
@@ -812,6 +819,22 @@ int gf_debug_grid(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t* perm,
                                   (static_cast<uint64_t>(cl->g.ncols) + 1) * 4,
                                   hipMemcpyDeviceToHost, c->stream));
         GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int gf_debug_grid_params(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, double* f, uint32_t* u)
+{
+    if (!ctx || !cloud || !f || !u) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        if (!cl->n) throw Error(GF_ERR_ARG, "empty cloud");
+        c->build_index(*cl, k);
+        const GridParams g = c->grid_of(*cl);
+        const double fv[7] = {g.ox, g.oy, g.oz, g.c, g.cz, g.inv_c, g.inv_cz};
+        const uint32_t uv[8] = {g.gx, g.gy, g.gz, g.tiles_x, g.bz, g.key_bits, g.ncols, 0};
+        std::memcpy(f, fv, sizeof(fv));
+        std::memcpy(u, uv, sizeof(uv));
     });
 }
 

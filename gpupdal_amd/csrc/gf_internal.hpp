@@ -34,6 +34,7 @@ enum : int32_t { MODE_RAW = 0, MODE_SQRT = 1, MODE_NORMALIZED = 2 };
 constexpr int kWave = 64;
 constexpr int kMaxStages = 4;
 constexpr int kMaxFetch = 64;     // K*stride held by the wave-wide top list
+constexpr int kMaxOutColumns = 16; // output columns of one fused run
 
 // Uniform column grid over one cloud (or one tile of a batch).  Columns are
 // (cx, cy) cells of edge `c`, stored in 8x8 tiles (row-major tiles, Morton

@@ -202,6 +202,9 @@ struct Batch {
 GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch, const Tuning& t);
 int stage_ncols(const gf_stage& s);
 void validate_stage(const gf_stage& s);
+// Throws GF_ERR_UNSUPPORTED when a cloud (or tile) of n points with this
+// knn and stride is outside what the device search handles.
+void check_neighborhood(uint64_t n, int64_t knn, int stride);
 
 // A device-time interval between two events, read after the call's final
 // synchronisation (timing adds no host wait between kernels).
@@ -271,6 +274,8 @@ struct Context {
     }
 
     void build_index(Cloud& cl, uint32_t fetch);
+    // The grid of the cloud's current index (host copy).
+    GridParams grid_of(const Cloud& cl) const { return cl.g; }
     CloudView view(const Cloud& cl) const;
     void run_group(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
                    uint8_t* const* status, int where);

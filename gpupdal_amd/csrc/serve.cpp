@@ -57,6 +57,8 @@ struct Request {
     bool done = false;
     int rc = GF_OK;
     std::string error;
+    int waiters = 0;        // threads inside gf_server_wait on this request
+    bool released = false;  // released while waited on: the last waiter frees it
 };
 
 bool same_shape(const Request& a, const Request& b)
@@ -289,12 +291,26 @@ int gf_server_submit(gf_server* server, const double* x, const double* y, const 
     Server* s = reinterpret_cast<Server*>(server);
     std::unique_ptr<Request> r(new Request());
     int ncols = 0;
+    // A request is checked here, before it is queued: one the batch would
+    // refuse must not fail the requests batched with it.
     for (int i = 0; i < nstages; ++i) {
-        const int c = gf_stage_columns(&stages[i]);
-        if (c < 0) return GF_ERR_ARG;
         r->st[i] = stages[i];
         r->st[i].knn = knn;
-        ncols += c;
+        try {
+            validate_stage(r->st[i]);
+        } catch (const Error& e) {
+            return e.code;
+        }
+        if (i && (r->st[i].stage == GF_STAGE_NORMAL ? 1 : r->st[i].stride) !=
+                     (r->st[0].stage == GF_STAGE_NORMAL ? 1 : r->st[0].stride))
+            return GF_ERR_ARG;  // the stages of one request share one neighborhood
+        ncols += gf_stage_columns(&r->st[i]);
+    }
+    if (ncols > kMaxOutColumns) return GF_ERR_UNSUPPORTED;
+    try {
+        check_neighborhood(n, knn, r->st[0].stage == GF_STAGE_NORMAL ? 1 : r->st[0].stride);
+    } catch (const Error& e) {
+        return e.code;
     }
     if (ncols && !cols) return GF_ERR_ARG;
     r->x = x;
@@ -328,12 +344,19 @@ int gf_server_wait(gf_server* server, uint64_t ticket, double timeout_s)
     auto it = s->requests.find(ticket);
     if (it == s->requests.end()) return GF_ERR_ARG;
     Request* r = it->second.get();
+    if (r->released) return GF_ERR_ARG;
+    // The request stays alive while this thread waits on it: a release in
+    // the meantime only marks it, and the last waiter frees it.
+    ++r->waiters;
     const auto pred = [&] { return r->done; };
+    bool finished = true;
     if (timeout_s < 0)
         s->cv_done.wait(lk, pred);
-    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
-        return GF_ERR_STATE;  // still running
-    return r->rc;
+    else
+        finished = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
+    const int rc = finished ? r->rc : GF_ERR_TIMEOUT;
+    if (--r->waiters == 0 && r->released) s->requests.erase(r->id);
+    return rc;
 }
 
 int gf_server_request_info(gf_server* server, uint64_t ticket, double* times,
@@ -345,6 +368,7 @@ int gf_server_request_info(gf_server* server, uint64_t ticket, double* times,
     auto it = s->requests.find(ticket);
     if (it == s->requests.end()) return GF_ERR_ARG;
     const Request* r = it->second.get();
+    if (r->released) return GF_ERR_ARG;
     if (times) {
         times[0] = r->t_submit;
         times[1] = r->t_start;
@@ -361,8 +385,11 @@ int gf_server_release(gf_server* server, uint64_t ticket)
     Server* s = reinterpret_cast<Server*>(server);
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->requests.find(ticket);
-    if (it == s->requests.end() || !it->second->done) return GF_ERR_ARG;
-    s->requests.erase(it);
+    if (it == s->requests.end() || !it->second->done || it->second->released) return GF_ERR_ARG;
+    if (it->second->waiters > 0)
+        it->second->released = true;  // freed by its last waiter
+    else
+        s->requests.erase(it);
     return GF_OK;
 }
 

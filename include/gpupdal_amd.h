@@ -32,6 +32,7 @@ extern "C" {
 #define GF_ERR_UNSUPPORTED (-3)
 #define GF_ERR_OOM (-4)
 #define GF_ERR_STATE (-5)
+#define GF_ERR_TIMEOUT (-6)
 
 #define GF_WHERE_HOST 0
 #define GF_WHERE_DEVICE 1
@@ -204,7 +205,11 @@ int gf_server_submit(gf_server *server, const double *x, const double *y, const 
                      uint64_t n, const gf_stage *stages, int nstages, int32_t knn,
                      double *const *cols, uint8_t *const *status, uint64_t *ticket);
 /* Blocks until the request is done (timeout_s < 0: no limit); returns its
- * result code, or GF_ERR_STATE if the timeout expired first. */
+ * result code, or GF_ERR_TIMEOUT if the timeout expired first (the request
+ * is still queued or running).  Requests are checked when submitted: a stage
+ * gf_run would refuse (bad kind, mode, knn or stride; knn and stride outside
+ * the device envelope; more than 16 columns) is refused by gf_server_submit
+ * and never joins a batch. */
 int gf_server_wait(gf_server *server, uint64_t ticket, double timeout_s);
 /* times: submit, batch start, done (seconds, monotonic clock). */
 int gf_server_request_info(gf_server *server, uint64_t ticket, double *times,
@@ -283,6 +288,15 @@ int gf_debug_eigen(gf_ctx *ctx, const double *cov, uint64_t m, double *evals,
 int gf_debug_grid(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, uint32_t *perm,
                   uint32_t *keys, uint32_t *colstart, uint32_t *ncols,
                   uint32_t *key_bits);
+/* The parameters of that grid: f = {origin x, y, z, column edge, z cell
+ * height, 1/column edge, 1/z cell height}, u = {columns along x, along y,
+ * z cells, 8x8 column tiles along x, z bits of the key, key bits, padded
+ * column count, 0}.  A point's key is (column << zbits) | zcell with
+ * column = tile * 64 + the Morton interleave of the low 3 bits of its x and
+ * y columns (x in the even bits), tile = (ycol >> 3) * tiles_x + (xcol >> 3),
+ * and each cell coordinate floor((v - origin) * inverse edge) clamped to
+ * [0, cells - 1] (NaN: 0). */
+int gf_debug_grid_params(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, double *f, uint32_t *u);
 
 /* The cloud's resident coordinates (n each, original order) copied to host
  * arrays. */

@@ -386,6 +386,7 @@ typedef struct {
     const orc_kd3 *t;
     uint64_t k, stride, begin, end;
     uint32_t *out;
+    const uint64_t *rows; /* NULL: row i is point i */
 } kd_job;
 
 static void *kd_worker(void *arg)
@@ -396,7 +397,8 @@ static void *kd_worker(void *arg)
     uint64_t *ids = malloc((k2 ? k2 : 1) * sizeof(uint64_t));
     double *d2 = malloc((k2 ? k2 : 1) * sizeof(double));
     for (uint64_t i = j->begin; i < j->end; ++i) {
-        orc_kd3_neighbors(j->t, j->t->pt[0][i], j->t->pt[1][i], j->t->pt[2][i],
+        uint64_t p = j->rows ? j->rows[i] : i;
+        orc_kd3_neighbors(j->t, j->t->pt[0][p], j->t->pt[1][p], j->t->pt[2][p],
                           j->k, j->stride, ids, d2);
         for (uint64_t c = 0; c < kk; ++c)
             j->out[i * kk + c] = (uint32_t)ids[c];
@@ -406,11 +408,25 @@ static void *kd_worker(void *arg)
     return NULL;
 }
 
+static void kd_run_rows(const orc_kd3 *t, const uint64_t *rows, uint64_t n, uint64_t k,
+                        uint64_t stride, int threads, uint32_t *out);
+
 void orc_kd3_neighbors_all(const orc_kd3 *t, uint64_t k, uint64_t stride,
                            int threads, uint32_t *out)
 {
+    kd_run_rows(t, NULL, t->n, k, stride, threads, out);
+}
+
+void orc_kd3_neighbors_rows(const orc_kd3 *t, const uint64_t *rows, uint64_t nrows,
+                            uint64_t k, uint64_t stride, int threads, uint32_t *out)
+{
+    kd_run_rows(t, rows, nrows, k, stride, threads, out);
+}
+
+static void kd_run_rows(const orc_kd3 *t, const uint64_t *rows, uint64_t n, uint64_t k,
+                        uint64_t stride, int threads, uint32_t *out)
+{
     if (threads < 1) threads = 1;
-    uint64_t n = t->n;
     uint64_t chunk = n / (uint64_t)threads + ((n % (uint64_t)threads) ? 1 : 0);
     pthread_t *th = malloc(sizeof(pthread_t) * (size_t)threads);
     kd_job *jobs = malloc(sizeof(kd_job) * (size_t)threads);
@@ -419,7 +435,7 @@ void orc_kd3_neighbors_all(const orc_kd3 *t, uint64_t k, uint64_t stride,
         uint64_t e = (i + 1 == threads) ? n : (uint64_t)(i + 1) * chunk;
         if (b > n) b = n;
         if (e > n) e = n;
-        jobs[i] = (kd_job){t, k, stride, b, e, out};
+        jobs[i] = (kd_job){t, k, stride, b, e, out, rows};
         if (threads == 1)
             kd_worker(&jobs[i]);
         else

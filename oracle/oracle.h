@@ -66,6 +66,9 @@ uint64_t orc_kd3_neighbors(const orc_kd3 *t, double qx, double qy, double qz,
  * row-major.  threads > 1 splits the rows into contiguous chunks. */
 void orc_kd3_neighbors_all(const orc_kd3 *t, uint64_t k, uint64_t stride,
                            int threads, uint32_t *out);
+/* The same for the points rows[0..nrows): out is nrows x min(n,k). */
+void orc_kd3_neighbors_rows(const orc_kd3 *t, const uint64_t *rows, uint64_t nrows,
+                            uint64_t k, uint64_t stride, int threads, uint32_t *out);
 
 /* Tree introspection (nodes in allocation order = DFS preorder). */
 uint64_t orc_kd3_node_count(const orc_kd3 *t);

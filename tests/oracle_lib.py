@@ -89,6 +89,9 @@ class Oracle:
                                         _u64p, _dp]
         L.orc_kd3_neighbors_all.argtypes = [ctypes.c_void_p, ctypes.c_uint64,
                                             ctypes.c_uint64, ctypes.c_int, _u32p]
+        L.orc_kd3_neighbors_rows.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64,
+                                             ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                             _u32p]
         L.orc_kd3_node_count.restype = ctypes.c_uint64
         L.orc_kd3_node_count.argtypes = [ctypes.c_void_p]
         L.orc_kd3_export.argtypes = [ctypes.c_void_p, _u64p, _i32p, _i32p,
@@ -149,6 +152,15 @@ class Oracle:
             out = np.zeros((self.n, kk), np.uint32)
             self.orc.L.orc_kd3_neighbors_all(self.h, k, stride, threads,
                                              _ptr(out, _u32p))
+            return out
+
+        def neighbors_rows(self, rows, k, stride=1, threads=1):
+            """neighbors(point r, k, stride) for each r in rows."""
+            rows = _c(rows, np.uint64)
+            kk = min(k, self.n)
+            out = np.zeros((len(rows), kk), np.uint32)
+            self.orc.L.orc_kd3_neighbors_rows(self.h, _ptr(rows, _u64p), len(rows), k, stride,
+                                              threads, _ptr(out, _u32p))
             return out
 
         def export(self):

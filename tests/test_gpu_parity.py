@@ -16,23 +16,47 @@ from helpers import small_cloud, set_equal_rows, rel_err, truth_errors
 pytestmark = pytest.mark.gpu
 
 
+def _host_keys(x, y, z, gp):
+    """Each point's grid key from the grid's parameters, by the documented
+    rule: cell coordinate floor((v - origin) / edge) clamped to the grid,
+    column = 8x8 tile (row-major) * 64 + Morton interleave of the low three
+    bits (x in the even bits), key = (column << zbits) | zcell."""
+    def cell(v, o, inv, dim):
+        f = np.floor((v - o) * inv)
+        f = np.where(np.isnan(f) | (f < 0), 0, f)
+        return np.minimum(f, dim - 1).astype(np.uint64)
+    cx = cell(x, gp["ox"], gp["inv_c"], gp["gx"])
+    cy = cell(y, gp["oy"], gp["inv_c"], gp["gy"])
+    cz = cell(z, gp["oz"], gp["inv_cz"], gp["gz"])
+    tile = (cy >> 3) * gp["tiles_x"] + (cx >> 3)
+    mort = np.zeros_like(cx)
+    for b in range(3):
+        mort |= ((cx >> b) & 1) << (2 * b)
+        mort |= ((cy >> b) & 1) << (2 * b + 1)
+    col = tile * 64 + mort
+    return (col << gp["bz"]) | cz, col
+
+
 def test_grid_invariants(ctx):
+    """The index against a host restatement of the grid rule: every point's
+    key, the sorted order (a stable sort by key), and the column starts."""
     x, y, z = small_cloud(50000, seed=3)
     cl = ctx.cloud(x, y, z)
     perm, keys, colstart, bits = cl.grid(11)
+    gp = cl.grid_params(11)
     n = len(x)
-    assert np.array_equal(np.sort(perm), np.arange(n, dtype=np.uint32)), "permutation"
-    assert np.all(np.diff(keys.astype(np.int64)) >= 0), "keys sorted"
-    # stable: within equal keys the original ids increase (LSD radix is stable)
-    same = np.diff(keys.astype(np.int64)) == 0
-    assert np.all(np.diff(perm.astype(np.int64))[same] > 0), "stable order"
-    # column starts are the first sorted position of each column
-    bz = None
-    cols = keys >> (keys.max().bit_length() - 0) if False else None
-    assert colstart[-1] == n and np.all(np.diff(colstart.astype(np.int64)) >= 0)
-    # equality with a host stable sort of the same keys
-    order = np.argsort(keys[np.argsort(perm)], kind="stable")
-    assert np.array_equal(order.astype(np.uint32), perm)
+    assert bits == gp["key_bits"] and len(colstart) == gp["ncols"] + 1
+    hkeys, hcol = _host_keys(x, y, z, gp)
+    assert hkeys.max() < (1 << gp["key_bits"])
+    # the sorted order is a stable sort of the host keys by point id
+    order = np.argsort(hkeys, kind="stable")
+    assert np.array_equal(order.astype(np.uint32), perm), "sorted order"
+    assert np.array_equal(keys.astype(np.uint64), hkeys[order]), "sorted keys"
+    # column starts: first sorted position of each column (n past the last)
+    scol = hcol[order]
+    want = np.searchsorted(scol, np.arange(gp["ncols"] + 1, dtype=np.uint64), side="left")
+    assert np.array_equal(colstart.astype(np.int64), want.astype(np.int64)), "column starts"
+    assert colstart[-1] == n
 
 
 @pytest.mark.parametrize("n,k,seed", [(20000, 11, 1), (30000, 17, 2), (15000, 33, 4)])
@@ -229,3 +253,49 @@ def test_config_plain_gpu_full(ctx, oracle):
     x, y, z = g.synth_cloud(4_000_000, tile=0, side=500.0)
     _full_check(ctx, oracle, x, y, z,
                 [g.stage(g.STAGE_NORMAL, 16), g.stage(g.STAGE_COVFEAT, 16)], 16, 100_000)
+
+
+def test_config_precision_stress_full(ctx, oracle):
+    """20,000,000 points (80 pts/m^2) with 5% exactly planar roof patches and
+    2% exactly collinear wires, filters.eigenvalues + filters.covariancefeatures
+    knn=32: neighbor sets equal KD3Index's on every row the search flagged and
+    on a 2,000,000-row random sample (order too on flagged rows); every
+    column within the 1e-5 bar of the extended-precision truth on a
+    250,000-row sample -- relative on well-conditioned rows, absolute bounds
+    on degenerate rows (smallest eigenvalue at rounding level)."""
+    n, knn = 20_000_000, 32
+    x, y, z = g.synth_cloud(n, tile=3, side=float(np.sqrt(n / 80.0)), roof_frac=0.05,
+                            wire_frac=0.02)
+    cl = ctx.cloud(x, y, z)
+    ids, st = cl.knn(knn + 1)
+    flagged = np.nonzero((st & (g.ST_TIE | g.ST_INCOMPLETE)) != 0)[0]
+    assert len(flagged) > 0  # the roofs and wires do produce boundary ties
+    rng = np.random.default_rng(33)
+    sample = np.union1d(rng.choice(n, 2_000_000, replace=False), flagged)
+    tree = oracle.kd3(x, y, z)
+    ref = tree.neighbors_rows(sample, knn + 1, threads=16)
+    eq = set_equal_rows(ids[sample], ref)
+    assert np.all(eq), f"{np.count_nonzero(~eq)} sampled rows differ from KD3Index"
+    fl = np.isin(sample, flagged)
+    assert np.array_equal(ids[sample[fl]], ref[fl]), "re-solved rows keep KD3 order"
+    del ids, ref
+    stages = [g.stage(g.STAGE_EIGENVALUES, knn), g.stage(g.STAGE_COVFEAT, knn)]
+    res_cols, res_st = cl.run(stages, init=np.nan)
+    cl.close()
+    rows = np.sort(rng.choice(n, 250_000, replace=False))
+    nbr = tree.neighbors_rows(rows, knn + 1, threads=16)
+    degenerate = 0
+    for st_, cols, status in zip(stages, res_cols, res_st):
+        ost = ol.stage(st_.stage, st_.knn, st_.stride, st_.mode, st_.features, st_.always_up,
+                       st_.normalize)
+        tc, ts, gap, ratio = oracle.truth_from_lists(x, y, z, nbr, ost, threads=16,
+                                                     with_ratio=True)
+        done = ((status[rows] & 0x3C) == 0) & (ts == 0)
+        assert np.count_nonzero(done) > 0.95 * len(rows)
+        # Well-conditioned rows need a resolved eigengap; degenerate rows
+        # (flat roofs, wires) are checked absolutely whatever their gap.
+        ok = done & ((gap > 1e-3) | (ratio <= 1e-10))
+        degenerate = max(degenerate, int(np.count_nonzero(ok & (ratio <= 1e-10))))
+        errs = truth_errors(ost, [c[rows] for c in cols], tc, ok, ratio)
+        assert max(errs.values()) <= 1.0, errs
+    assert degenerate > 0.03 * len(rows)  # roofs and wires are in the sample

@@ -57,3 +57,36 @@ def test_request_over_budget_is_refused():
         srv.result(t)
     assert e.value.code == -4
     srv.close()
+
+
+def test_bad_request_refused_at_submit():
+    """A request the batch would refuse (stride 0, an unknown mode, a
+    neighborhood outside the device envelope) is refused by submit itself;
+    the valid requests submitted around it still run and succeed."""
+    srv = g.Server(workers=1)
+    x, y, z = small_cloud(20000, seed=610)
+    good = [srv.submit(x, y, z, stages(), 10)]
+    bad = [g.stage(g.STAGE_COVFEAT, 10, stride=0),
+           g.stage(g.STAGE_COVFEAT, 10, mode=7)]
+    for st in bad:
+        with pytest.raises(g.GfError) as e:
+            srv.submit(x, y, z, [st], 10)
+        assert e.value.code == g.GF_ERR_ARG
+    good.append(srv.submit(x, y, z, stages(), 16))
+    for t in good:
+        cols, status, times = srv.result(t)
+        assert np.all(np.isfinite(cols[1][0][(status[1] & 0x3C) == 0]))
+    srv.close()
+
+
+def test_wait_timeout_is_distinct():
+    """A wait that times out reports GF_ERR_TIMEOUT (not a failure), and the
+    request can still be waited for afterwards."""
+    srv = g.Server(workers=1)
+    x, y, z = small_cloud(2_000_000, seed=611)
+    t = srv.submit(x, y, z, stages(), 16)
+    rc = srv.L.gf_server_wait(srv.h, t, 0.0)
+    assert rc in (g.GF_OK, g.GF_ERR_TIMEOUT)
+    cols, status, times = srv.result(t)
+    assert times[2] >= times[0]
+    srv.close()
