@@ -56,13 +56,23 @@ HBM_PEAK_GBS = 8000.0
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 
 
+def new_context(args):
+    """A library context on GPU 0 with the --tune settings applied."""
+    import gpupdal_amd as g
+    ctx = g.Context(0)
+    for kv in args.tune or []:
+        key, val = kv.split("=", 1)
+        ctx.set_tuning(key, float(val))
+    return ctx
+
+
 def gpu_leg(args):
     import numpy as np
     import gpupdal_amd as g
 
     n, knn = args.points, args.knn
     x, y, z = g.synth_cloud(n, tile=0, side=500.0)
-    ctx = g.Context(0)
+    ctx = new_context(args)
     cloud = ctx.cloud(x, y, z)
     stages = [g.stage(g.STAGE_NORMAL, knn), g.stage(g.STAGE_COVFEAT, knn)]
     ncols = sum(g.stage_columns(s) for s in stages)
@@ -201,7 +211,7 @@ def batched_leg(args):
     specs, knns = c4_specs(args.tiles)
     tiles = gen_tiles(specs)
     n_total = sum(len(t[0]) for t in tiles)
-    ctx = g.Context(0)
+    ctx = new_context(args)
     ctx.set_tuning("timing", 0)
     stages = [g.stage(g.STAGE_NORMAL, 16), g.stage(g.STAGE_COVFEAT, 16)]
     ncols = 8
@@ -261,7 +271,7 @@ def las_leg(args, xyz):
     x, y, z = xyz
     img, _ = las_image(x, y, z, offset=(float(np.floor(x.min())), float(np.floor(y.min())),
                                         0.0), point_format=1)
-    ctx = g.Context(0)
+    ctx = new_context(args)
     for _ in range(2):
         ctx.cloud_from_las(img).close()
     dec, wall = [], []
@@ -293,7 +303,7 @@ def optimal_leg(args, xyz):
     per-prefix eigenentropy; the index and tree are rebuilt every step and
     the two output columns copied to the host."""
     import gpupdal_amd as g
-    ctx = g.Context(0)
+    ctx = new_context(args)
     cl = ctx.cloud(*xyz)
     cl.invalidate()
     cl.optimal_neighborhood(10, 14)
@@ -381,7 +391,7 @@ def precision_leg(args, sample_rows=0):
     n, knn = args.precision_points, 32
     side = float(np.sqrt(n / 80.0))
     x, y, z = g.synth_cloud(n, tile=3, side=side, roof_frac=0.05, wire_frac=0.02)
-    ctx = g.Context(0)
+    ctx = new_context(args)
     cl = ctx.cloud(x, y, z)
     stages = [g.stage(g.STAGE_EIGENVALUES, knn), g.stage(g.STAGE_COVFEAT, knn)]
     t0 = time.perf_counter()
@@ -519,6 +529,8 @@ def main():
                          "the search (0: after it)")
     ap.add_argument("--serial", action="store_true",
                     help="also time the step with the exact tree built after the search")
+    ap.add_argument("--tune", action="append", metavar="KEY=VALUE",
+                    help="a library tuning setting for every context (repeatable)")
     ap.add_argument("--only-batched", action="store_true",
                     help="only the batched-vs-plain leg (for profiling it)")
     args = ap.parse_args()

@@ -1628,11 +1628,14 @@ __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, Fuse
             if (mine) out.ids[static_cast<uint64_t>(qid) * k + lane] = kept_id;
             continue;
         }
+        // Coordinates relative to the query point (exact differences), so the
+        // mean carries no rounding of the cloud's absolute offset.
         const uint32_t id = mine ? kept_id : 0u;
-        const double px = mine ? x[id] : 0.0, py = mine ? y[id] : 0.0, pz = mine ? z[id] : 0.0;
-        const double inv_k = 1.0 / static_cast<double>(k);
-        const double mx = wave_sum(px) * inv_k, my = wave_sum(py) * inv_k,
-                     mz = wave_sum(pz) * inv_k;
+        const double px = mine ? x[id] - q0 : 0.0, py = mine ? y[id] - q1 : 0.0,
+                     pz = mine ? z[id] - q2 : 0.0;
+        const double mx = wave_sum(px) / static_cast<double>(k),
+                     my = wave_sum(py) / static_cast<double>(k),
+                     mz = wave_sum(pz) / static_cast<double>(k);
         const double dx = mine ? px - mx : 0.0, dy = mine ? py - my : 0.0,
                      dz = mine ? pz - mz : 0.0;
         const double den = static_cast<double>(k - 1);

@@ -756,21 +756,26 @@ __global__ __launch_bounds__(knn_waves<kTiled>() * kWave) __attribute__((amdgpu_
     if (lane >= nq) return;
     const uint64_t s = q0 + lane;
     const uint32_t* list = nbr + lane * a.lds_stride;
+    // Coordinates relative to the query point (exact differences at any
+    // offset), so the mean carries no rounding of the cloud's absolute
+    // position: at UTM scale that rounding (~1e-9 m) would add a rank-one
+    // term to the covariance of flat and collinear neighborhoods.
+    const double ox = cv.sx[s], oy = cv.sy[s], oz = cv.sz[s];
     double mx = 0.0, my = 0.0, mz = 0.0;
     for (int l = 0; l < k; ++l) {
         const uint32_t j = list[l];
-        mx += cv.sx[j];
-        my += cv.sy[j];
-        mz += cv.sz[j];
+        mx += cv.sx[j] - ox;
+        my += cv.sy[j] - oy;
+        mz += cv.sz[j] - oz;
     }
-    const double inv_k = 1.0 / static_cast<double>(k);
-    mx *= inv_k;
-    my *= inv_k;
-    mz *= inv_k;
+    mx /= static_cast<double>(k);
+    my /= static_cast<double>(k);
+    mz /= static_cast<double>(k);
     double c[6] = {0, 0, 0, 0, 0, 0};
     for (int l = 0; l < k; ++l) {
         const uint32_t j = list[l];
-        const double dx = cv.sx[j] - mx, dy = cv.sy[j] - my, dz = cv.sz[j] - mz;
+        const double dx = (cv.sx[j] - ox) - mx, dy = (cv.sy[j] - oy) - my,
+                     dz = (cv.sz[j] - oz) - mz;
         c[0] += dx * dx;
         c[1] += dx * dy;
         c[2] += dx * dz;
