@@ -87,8 +87,9 @@ EXPORTED = [
     "gf_server_release", "gf_server_destroy",
     "gf_autotune", "gf_tuning_save", "gf_tuning_load",
     "gf_las_parse", "gf_cloud_create_las", "gf_las_pack_extra", "gf_debug_cloud_xyz",
-    "gf_optimal_neighborhood", "gf_debug_grid_params",
+    "gf_optimal_neighborhood", "gf_debug_grid_params", "gf_run_noalloc",
 ]
+NOALLOC_MAX_POINTS = 65536
 
 
 def load_library(path=LIB_PATH):
@@ -114,6 +115,8 @@ def load_library(path=LIB_PATH):
     L.gf_run.argtypes = [vp, vp, ctypes.POINTER(GfStage), ctypes.c_int,
                          ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int]
     L.gf_last_stats.argtypes = [vp, ctypes.POINTER(GfStats)]
+    L.gf_run_noalloc.argtypes = [vp, vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(GfStage),
+                                 ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.gf_knn.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.c_int]
     L.gf_debug_eigen.argtypes = [vp, _dp, ctypes.c_uint64, _dp, _dp, _i32p]
     L.gf_debug_grid.argtypes = [vp, vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p, _u32p]
@@ -318,6 +321,16 @@ class Context:
         """A batch of tiles [(x, y, z), ...] resident on the device."""
         return Batch(self, tiles)
 
+    def run_noalloc(self, x, y, z, n, stages, cols, status):
+        """gf_run_noalloc: the fused stage run on device arrays (x, y, z:
+        n doubles each; cols, status as Cloud.run_device) with no device
+        allocation; n <= NOALLOC_MAX_POINTS."""
+        stages = list(stages)
+        arr = (GfStage * len(stages))(*stages)
+        cp = (ctypes.c_void_p * max(1, len(cols)))(*cols)
+        sp = (ctypes.c_void_p * len(status))(*status)
+        return self.L.gf_run_noalloc(self.h, x, y, z, n, arr, len(stages), cp, sp)
+
     def alloc(self, nbytes):
         p = ctypes.c_void_p()
         self._check(self.L.gf_dev_alloc(self.h, nbytes, ctypes.byref(p)), "gf_dev_alloc")
@@ -329,6 +342,13 @@ class Context:
     def copy(self, dst, src, count):
         """Plain device copy of count doubles (HBM counter calibration)."""
         self._check(self.L.gf_debug_copy(self.h, dst, src, count), "gf_debug_copy")
+
+    def to_device(self, arr):
+        """A new device copy of a host array (free it with free())."""
+        arr = np.ascontiguousarray(arr)
+        p = self.alloc(max(arr.nbytes, 8))
+        self._check(self.L.gf_memcpy(self.h, p, arr.ctypes.data, arr.nbytes, 1), "gf_memcpy")
+        return p
 
     def to_host(self, ptr, arr):
         self._check(self.L.gf_memcpy(self.h, arr.ctypes.data, ptr, arr.nbytes, 2), "gf_memcpy")

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 using namespace gf;
@@ -453,6 +454,233 @@ void Context::reset_spans()
     evnext = 0;
 }
 
+// ------------------------------------------------- no-allocation path ----
+//
+// gf_run_noalloc runs the same fused pass as gf_run on caller-owned device
+// arrays, with every working buffer (grid index, covariance records, flagged
+// rows, exact tree) carved from one fixed array per GPU, `scratch`, so that it
+// calls no hipMalloc, hipFree or other allocating routine.  Each GPU holds its
+// own copy of the array until the process ends; its address on a device is
+// taken once, with that device current, when the first context on it is
+// created.  Calls that overlap on one GPU share the array and race.
+
+constexpr uint64_t kStaticScratchBytes = 24ull << 20;
+constexpr uint64_t kStaticMaxPoints = GF_NOALLOC_MAX_POINTS;
+constexpr int kStaticMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kStaticMaxDevices] = {};
+}  // namespace
+
+// The device address of `scratch` on `dev` (the current device), looked up
+// once per device.
+void* bind_static_scratch(int dev)
+{
+    if (dev < 0 || dev >= kStaticMaxDevices)
+        throw Error(GF_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(-2, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+namespace {
+
+// Consecutive 256-byte-aligned pieces of the scratch array.
+struct Arena {
+    unsigned char* base;
+    uint64_t off = 0;
+    uint64_t cap;
+    template <class T> T* take(uint64_t bytes)
+    {
+        off = (off + 255) & ~uint64_t(255);
+        T* p = reinterpret_cast<T*>(base + off);
+        off += bytes;
+        if (off > cap)
+            throw Error(GF_ERR_ARG, "gf_run_noalloc: the working set of " + std::to_string(off) +
+                                        " bytes exceeds the fixed scratch of " +
+                                        std::to_string(cap) + " bytes");
+        return p;
+    }
+};
+
+}  // namespace
+
+void Context::run_group_static(const double* x, const double* y, const double* z, uint64_t n,
+                               const gf_stage* st, int ns, double* const* cols,
+                               uint8_t* const* status)
+{
+    if (n > kStaticMaxPoints)
+        throw Error(GF_ERR_ARG, "gf_run_noalloc takes at most " + std::to_string(kStaticMaxPoints) +
+                                    " points (the fixed scratch size)");
+    if (!static_scratch) throw Error(GF_ERR_STATE, "no scratch array bound on this device");
+    const int stride = st[0].stage == GF_STAGE_NORMAL ? 1 : st[0].stride;
+    const uint64_t K = static_cast<uint64_t>(st[0].knn) + 1;
+    const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(n, K));
+    const uint32_t fetch = static_cast<uint32_t>(stride) * k;
+    check_neighborhood(n, st[0].knn, stride);
+    FusedProgram prog{};
+    prog.nstages = ns;
+    prog.k = static_cast<int32_t>(k);
+    prog.stride = stride;
+    prog.fetch = static_cast<int32_t>(fetch);
+    int ncol_total = 0;
+    for (int i = 0; i < ns; ++i) {
+        StageProgram& sp = prog.st[i];
+        sp.stage = st[i].stage;
+        sp.mode = st[i].mode;
+        sp.features = st[i].features;
+        sp.always_up = st[i].always_up;
+        sp.normalize = st[i].normalize;
+        sp.ncols = stage_ncols(st[i]);
+        sp.col_offset = ncol_total;
+        ncol_total += sp.ncols;
+    }
+    if (ncol_total > kMaxOutColumns) throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
+
+    // Every piece is laid out before the first launch, so a call that does
+    // not fit fails before touching the device.
+    Arena ar{static_cast<unsigned char*>(static_scratch), 0, kStaticScratchBytes};
+    double* part = ar.take<double>(kBoundsMaxParts * 6 * sizeof(double));
+    double* dbounds = ar.take<double>(6 * sizeof(double));
+    uint64_t* dstarts = ar.take<uint64_t>(2 * sizeof(uint64_t));
+    uint32_t* keys_a = ar.take<uint32_t>(n * 4 + 4);
+    uint32_t* vals_a = ar.take<uint32_t>(n * 4 + 4);
+    uint32_t* keys_b = ar.take<uint32_t>(n * 4 + 4);
+    uint32_t* vals_b = ar.take<uint32_t>(n * 4 + 4);
+    const uint64_t col_cap = std::max<uint64_t>(4 * n, 4096);  // make_grid's bound on columns
+    uint32_t* colstart = ar.take<uint32_t>((col_cap + 1) * 4);
+    double* sx = ar.take<double>(n * 8 + 8);
+    double* sy = ar.take<double>(n * 8 + 8);
+    double* sz = ar.take<double>(n * 8 + 8);
+    uint32_t* inv = ar.take<uint32_t>(n * 4 + 8);
+    double* xyz = ar.take<double>(n * 24 + 8);
+    uint32_t* hist = ar.take<uint32_t>(radix_hist_words(n) * 4);
+    uint32_t* scan = ar.take<uint32_t>(
+        scan_scratch_words(std::max<uint64_t>(radix_hist_words(n), col_cap)) * 4);
+    uint32_t* flg = ar.take<uint32_t>(n * 4 + 4);
+    uint32_t* cnt = ar.take<uint32_t>(64);
+    double* cov = ar.take<double>(n * 8 * kCovRecord + 8);
+    const uint32_t cap = kd_node_capacity(n) + 1;
+    const uint32_t lcap = kd_list_capacity(n) + 1;
+    KdView t{};
+    t.vind = ar.take<uint32_t>(n * 4 + 8);
+    t.px = ar.take<double>(n * 8 + 8);
+    t.py = ar.take<double>(n * 8 + 8);
+    t.pz = ar.take<double>(n * 8 + 8);
+    t.nl = ar.take<uint32_t>(static_cast<uint64_t>(cap) * 16);
+    t.nr = t.nl + cap;
+    t.split = t.nr + cap;
+    t.aux = t.split + cap;
+    t.c1 = ar.take<int32_t>(static_cast<uint64_t>(cap) * 16);
+    t.c2 = t.c1 + cap;
+    t.feat = t.c2 + cap;
+    t.parent = t.feat + cap;
+    t.which = ar.take<uint8_t>(cap + 8);
+    t.divlow = ar.take<double>(static_cast<uint64_t>(cap) * 8 * 9);
+    t.divhigh = t.divlow + cap;
+    t.cutval = t.divhigh + cap;
+    t.box = t.cutval + cap;
+    t.mn = ar.take<unsigned long long>(static_cast<uint64_t>(cap) * 8 * 6);
+    t.mx = t.mn + 3 * static_cast<uint64_t>(cap);
+    t.cap = cap;
+    KdScratch w{};
+    w.s1 = ar.take<uint32_t>((n + 1) * 4);
+    w.s2 = ar.take<uint32_t>((n + 1) * 4);
+    w.tmp = ar.take<uint32_t>((n + 1) * 4);
+    w.sums = ar.take<uint32_t>(kd_sums_words(n) * 4);
+    uint32_t* L = ar.take<uint32_t>((static_cast<uint64_t>(lcap) * 5 + 2 + kKdHeaderWords) * 4);
+    w.hdr = L;
+    w.act[0] = L + kKdHeaderWords;
+    w.act[1] = w.act[0] + lcap;
+    w.off[0] = w.act[1] + lcap;
+    w.off[1] = w.off[0] + lcap + 1;
+    w.small = w.off[1] + lcap + 1;
+    w.list_cap = lcap;
+
+    // Grid index (the same kernels as the resident-cloud path).
+    launch_bounds(x, y, z, n, part, dbounds, stream);
+    GF_HIP(hipMemcpyAsync(host_bounds, dbounds, 6 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipStreamSynchronize(stream));
+    const GridParams g = make_grid(host_bounds, n, fetch, tune);
+    launch_keys(x, y, z, n, g, keys_a, vals_a, xyz, stream);
+    uint32_t *kres, *vres;
+    launch_radix_sort(keys_a, vals_a, keys_b, vals_b, n, g.key_bits, hist, scan, &kres, &vres,
+                      stream);
+    launch_colstart(kres, n, g, colstart, stream);
+    launch_gather_sorted(vres, xyz, n, sx, sy, sz, inv, stream);
+    CloudView cv{};
+    cv.sx = sx;
+    cv.sy = sy;
+    cv.sz = sz;
+    cv.perm = vres;
+    cv.keys = kres;
+    cv.colstart = colstart;
+    cv.n = n;
+    cv.g = g;
+
+    KnnOutputs out{};
+    for (int c = 0; c < ncol_total; ++c) out.col[c] = cols[c];
+    for (int i = 0; i < ns; ++i) out.status[i] = status[i];
+    out.flagged = flg;
+    out.nflagged = cnt;
+    out.cov = cov;
+    launch_fill_u32(cnt, 16, 0u, stream);
+    if (fetch > n)
+        launch_flag_all(n, cv.perm, 0, out, stream);
+    else
+        launch_knn(cv, prog, 0, out, knn_tuning(), stream);
+    launch_features(cv, prog, out, inv, stream, nullptr, kFeaturesUnflagged);
+    uint32_t h[4] = {0, 0, 0, 0};
+    GF_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipStreamSynchronize(stream));
+    stats.tie_rows = h[1];
+    stats.incomplete_rows = h[2];
+    if (h[0]) {
+        // The exact tree for the flagged rows, in the scratch array: its
+        // tables are sized for the worst case and never grown.
+        launch_fill_u32(reinterpret_cast<uint32_t*>(dstarts), 4, 0u, stream);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(dstarts) + 2, 1, static_cast<uint32_t>(n),
+                        stream);
+        KdTuning kt{};
+        kt.subtree_points = static_cast<uint32_t>(tune.kd_subtree);
+        kt.wave_node = static_cast<uint32_t>(tune.kd_wave_node);
+        kt.small_blocks = tune.kd_small_blocks < 0 ? 0u : static_cast<uint32_t>(tune.kd_small_blocks);
+        kt.minmax_lds_nodes = static_cast<uint32_t>(tune.kd_minmax_lds);
+        uint32_t nodes = 0;
+        const uint32_t rc = launch_kd_build(x, y, z, n, dstarts, 1, dbounds, t, w, kt, &nodes, stream);
+        if (rc != 0)
+            throw Error(GF_ERR_UNSUPPORTED, "gf_run_noalloc: exact KD3 tree tables overflowed (code " +
+                                                std::to_string(rc) + ")");
+        uint32_t* failed = cnt + 5;
+        launch_fill_u32(failed, 1, 0u, stream);
+        launch_kd_resolve(t, cv, prog, 0, out, x, y, z, h[0], failed, stream);
+        launch_features(cv, prog, out, inv, stream, nullptr, kFeaturesFlagged, h[0]);
+        uint32_t nfail = 0;
+        GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, stream));
+        GF_HIP(hipStreamSynchronize(stream));
+        if (nfail)
+            throw Error(GF_ERR_UNSUPPORTED, "exact KD3 search exceeded its depth bound on " +
+                                                std::to_string(nfail) + " rows");
+    }
+    GF_HIP(hipStreamSynchronize(stream));
+    stats.points = n;
+    stats.cell = g.c;
+    stats.cell_z = g.cz;
+    stats.grid_x = g.gx;
+    stats.grid_y = g.gy;
+    stats.grid_z = g.gz;
+    stats.key_bits = g.key_bits;
+}
+
 } // namespace gf
 
 // ------------------------------------------------------------------ ABI ----
@@ -490,6 +718,12 @@ int gf_ctx_create(int device, gf_ctx** out)
     *out = nullptr;
     try {
         Context* c = new Context(device);
+        try {
+            c->static_scratch = bind_static_scratch(device);
+        } catch (...) {
+            delete c;
+            throw;
+        }
         *out = reinterpret_cast<gf_ctx*>(c);
         return GF_OK;
     } catch (const gf::Error& e) {
@@ -617,6 +851,42 @@ int gf_run(gf_ctx* ctx, gf_cloud* cloud, const gf_stage* stages, int nstages,
                 ++j;
             }
             c->run_group(*cl, stages + i, j - i, cols + col0, status + i, where);
+            col0 += ncol;
+            i = j;
+        }
+    });
+}
+
+int gf_run_noalloc(gf_ctx* ctx, const double* x, const double* y, const double* z, uint64_t n,
+                   const gf_stage* stages, int nstages, double* const* cols,
+                   uint8_t* const* status)
+{
+    if (!ctx || !stages || nstages < 1 || nstages > kMaxStages || !status) return GF_ERR_ARG;
+    if (n && (!x || !y || !z)) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        c->stats = gf_stats{};
+        int total_cols = 0;
+        for (int i = 0; i < nstages; ++i) {
+            validate_stage(stages[i]);
+            total_cols += stage_ncols(stages[i]);
+        }
+        if (total_cols && !cols) throw Error(GF_ERR_ARG, "null column array");
+        if (n == 0) return;
+        int col0 = 0;
+        int i = 0;
+        while (i < nstages) {
+            int j = i + 1, ncol = stage_ncols(stages[i]);
+            auto same = [&](const gf_stage& a, const gf_stage& b) {
+                int sa = a.stage == GF_STAGE_NORMAL ? 1 : a.stride;
+                int sb = b.stage == GF_STAGE_NORMAL ? 1 : b.stride;
+                return a.knn == b.knn && sa == sb;
+            };
+            while (j < nstages && same(stages[i], stages[j])) {
+                ncol += stage_ncols(stages[j]);
+                ++j;
+            }
+            c->run_group_static(x, y, z, n, stages + i, j - i, cols + col0, status + i);
             col0 += ncol;
             i = j;
         }

@@ -291,6 +291,14 @@ struct Context {
     // Returns the number of flagged rows.
     uint32_t resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog,
                              int mode, const KnnOutputs& out, hipEvent_t from);
+    // The device address of this GPU's fixed scratch array (the no-allocation
+    // entry point's only working memory), bound when the context is created.
+    void* static_scratch = nullptr;
+    // gf_run_noalloc: one group of stages sharing a neighborhood, every
+    // working buffer carved from static_scratch.
+    void run_group_static(const double* x, const double* y, const double* z, uint64_t n,
+                          const gf_stage* st, int ns, double* const* cols,
+                          uint8_t* const* status);
     void build_batch_index(Batch& b, const std::vector<uint32_t>& fetch);
     void run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* tile_knn,
                    double* const* cols, uint8_t* const* status, int where);

@@ -160,6 +160,21 @@ int gf_run(gf_ctx *ctx, gf_cloud *cloud, const gf_stage *stages, int nstages,
            double *const *cols, uint8_t *const *status, int where);
 int gf_last_stats(const gf_ctx *ctx, gf_stats *out);
 
+/* ---- the same fused run with no device allocation.  No counterpart in the
+ *      reference: for callers that cannot allocate device memory (code under
+ *      HIP graph capture, tight serving loops, harnesses that forbid
+ *      hipMalloc).  x, y, z: n FP64 device arrays (original order); cols and
+ *      status: device arrays as gf_run's with where = GF_WHERE_DEVICE.  Every
+ *      working buffer comes from one fixed array per GPU (24 MiB, held until
+ *      the process ends), which limits n to GF_NOALLOC_MAX_POINTS; larger n
+ *      returns GF_ERR_ARG.  The call synchronises the context's stream, as
+ *      gf_run does.  Calls that overlap on one GPU (from any context) share
+ *      the array and race: callers serialise them. */
+#define GF_NOALLOC_MAX_POINTS 65536u
+int gf_run_noalloc(gf_ctx *ctx, const double *x, const double *y, const double *z,
+                   uint64_t n, const gf_stage *stages, int nstages, double *const *cols,
+                   uint8_t *const *status);
+
 /* ---- neighbor lists in KD3Index order semantics: replaces
  *      pdg::knnGather (include/pdg/index/SpatialIndex.hpp:480-490) and
  *      KD3Index::neighbors(id, k, stride) (pdal/KDIndex.hpp:103-106).
