@@ -35,7 +35,8 @@ class GfStage(ctypes.Structure):
     _fields_ = [("stage", ctypes.c_int32), ("knn", ctypes.c_int32),
                 ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("features", ctypes.c_uint32), ("always_up", ctypes.c_int32),
-                ("normalize", ctypes.c_int32)]
+                ("normalize", ctypes.c_int32), ("min_k", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("radius", ctypes.c_double)]
 
 
 class GfLasInfo(ctypes.Structure):
@@ -88,6 +89,7 @@ EXPORTED = [
     "gf_autotune", "gf_tuning_save", "gf_tuning_load",
     "gf_las_parse", "gf_cloud_create_las", "gf_las_pack_extra", "gf_debug_cloud_xyz",
     "gf_optimal_neighborhood", "gf_debug_grid_params", "gf_run_noalloc",
+    "gf_radius_counts",
 ]
 NOALLOC_MAX_POINTS = 65536
 
@@ -115,6 +117,7 @@ def load_library(path=LIB_PATH):
     L.gf_run.argtypes = [vp, vp, ctypes.POINTER(GfStage), ctypes.c_int,
                          ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int]
     L.gf_last_stats.argtypes = [vp, ctypes.POINTER(GfStats)]
+    L.gf_radius_counts.argtypes = [vp, vp, ctypes.c_double, vp, ctypes.c_int]
     L.gf_run_noalloc.argtypes = [vp, vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(GfStage),
                                  ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.gf_knn.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.c_int]
@@ -231,8 +234,8 @@ def synth_cloud(n, **kw):
 
 
 def stage(kind, knn, stride=1, mode=MODE_SQRT, features=F_DIMENSIONALITY, always_up=1,
-          normalize=0):
-    return GfStage(kind, knn, stride, mode, features, always_up, normalize)
+          normalize=0, radius=0.0, min_k=3):
+    return GfStage(kind, knn, stride, mode, features, always_up, normalize, min_k, 0, radius)
 
 
 def stage_columns(st):
@@ -433,6 +436,14 @@ class Cloud:
         sp = (ctypes.c_void_p * len(status))(*status)
         self.ctx._check(self.ctx.L.gf_run(self.ctx.h, self.h, arr, len(stages), cp, sp,
                                           WHERE_DEVICE), "gf_run")
+
+    def radius_counts(self, radius):
+        """Points within `radius` of each point (itself included), strictly."""
+        out = np.zeros(self.n, np.uint32)
+        self.ctx._check(self.ctx.L.gf_radius_counts(self.ctx.h, self.h, radius,
+                                                    out.ctypes.data, WHERE_HOST),
+                        "gf_radius_counts")
+        return out
 
     def knn(self, k, stride=1):
         kk = min(k, self.n)

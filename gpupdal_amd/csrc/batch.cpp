@@ -188,7 +188,7 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         ncol_total += sp.ncols;
     }
     if (ncol_total > kMaxOutColumns)
-        throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
+        throw Error(GF_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxOutColumns) + " output columns");
     reset_spans();
     hipEvent_t t_all = mark();
 
@@ -458,6 +458,9 @@ int gf_batch_run(gf_ctx* ctx, gf_batch* batch, const gf_stage* stages, int nstag
         int total_cols = 0;
         for (int i = 0; i < nstages; ++i) {
             validate_stage(stages[i]);
+            if (stages[i].radius > 0.0)
+                throw Error(GF_ERR_UNSUPPORTED,
+                            "radius neighborhoods run through gf_run, not batches");
             total_cols += stage_ncols(stages[i]);
         }
         if (total_cols && !cols) throw Error(GF_ERR_ARG, "null column array");

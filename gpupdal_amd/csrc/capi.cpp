@@ -20,7 +20,7 @@ using namespace gf;
 namespace gf {
 
 GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch,
-                     const Tuning& t)
+                     const Tuning& t, double cell)
 {
     GridParams g{};
     g.ox = b[0];
@@ -31,9 +31,10 @@ GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch,
     const double tiny = std::max(big * 1e-9, 1e-9);
     const double ax = std::max(sx, tiny), ay = std::max(sy, tiny);
     const double rho = static_cast<double>(std::max<uint64_t>(n, 1)) / (ax * ay);
-    double c = t.factor_for(rho, fetch) *
-               std::sqrt(static_cast<double>(std::max<uint32_t>(fetch, 1)) /
-                         (3.141592653589793 * rho));
+    double c = cell > 0.0 ? cell
+                          : t.factor_for(rho, fetch) *
+                                std::sqrt(static_cast<double>(std::max<uint32_t>(fetch, 1)) /
+                                          (3.141592653589793 * rho));
     if (!(c > tiny)) c = tiny;
     const uint64_t col_cap = std::max<uint64_t>(4 * n, 4096);
     for (;;) {
@@ -74,10 +75,11 @@ GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch,
     return g;
 }
 
-void Context::build_index(Cloud& cl, uint32_t fetch)
+void Context::build_index(Cloud& cl, uint32_t fetch, double cell)
 {
     const uint64_t n = cl.n;
-    if (cl.idx_valid && cl.idx_fetch == fetch && cl.idx_tune == tune.signature())
+    if (cl.idx_valid && cl.idx_fetch == fetch && cl.idx_cell == cell &&
+        cl.idx_tune == tune.signature())
         return;
     cl.idx_valid = false;
     hipEvent_t e0 = mark();
@@ -88,7 +90,7 @@ void Context::build_index(Cloud& cl, uint32_t fetch)
     GF_HIP(hipMemcpyAsync(host_bounds, bounds.ptr, 6 * sizeof(double),
                           hipMemcpyDeviceToHost, stream));
     GF_HIP(hipStreamSynchronize(stream));
-    GridParams g = make_grid(host_bounds, n, fetch, tune);
+    GridParams g = make_grid(host_bounds, n, fetch, tune, cell);
     for (int i = 0; i < 6; ++i) cl.bounds[i] = host_bounds[i];
     cl.dbounds.ensure(64);
     GF_HIP(hipMemcpyAsync(cl.dbounds.ptr, bounds.ptr, 6 * sizeof(double),
@@ -125,6 +127,7 @@ void Context::build_index(Cloud& cl, uint32_t fetch)
     cl.g = g;
     cl.idx_valid = true;
     cl.idx_fetch = fetch;
+    cl.idx_cell = cell;
     cl.idx_tune = tune.signature();
     span(e0, mark(), &stats.ms_index);
     stats.cell = g.c;
@@ -162,6 +165,9 @@ int stage_ncols(const gf_stage& s)
 void validate_stage(const gf_stage& s)
 {
     if (stage_ncols(s) < 0) throw Error(GF_ERR_ARG, "unknown stage kind");
+    if (!(s.radius >= 0.0) || s.radius == INFINITY)
+        throw Error(GF_ERR_ARG, "radius must be a finite number, 0 for knn neighborhoods");
+    if (s.radius > 0.0) return;  // knn and stride are not used
     if (s.knn < 1) throw Error(GF_ERR_ARG, "knn must be at least 1");
     if (s.stage != GF_STAGE_NORMAL && s.stride < 1)
         throw Error(GF_ERR_ARG, "stride must be at least 1");
@@ -185,6 +191,10 @@ void check_neighborhood(uint64_t n, int64_t knn, int stride)
 void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
                         uint8_t* const* status, int where)
 {
+    if (st[0].radius > 0.0) {
+        run_group_radius(cl, st, ns, cols, status, where);
+        return;
+    }
     const uint64_t n = cl.n;
     const int stride = st[0].stage == GF_STAGE_NORMAL ? 1 : st[0].stride;
     const uint64_t K = static_cast<uint64_t>(st[0].knn) + 1;
@@ -209,7 +219,7 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
         ncol_total += sp.ncols;
     }
     if (ncol_total > kMaxOutColumns)
-        throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
+        throw Error(GF_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxOutColumns) + " output columns");
 
     reset_spans();
     hipEvent_t t_all = mark();
@@ -454,6 +464,74 @@ void Context::reset_spans()
     evnext = 0;
 }
 
+// Stages with a radius neighborhood (their `radius` option): one radius
+// search writes every row's covariance record and neighbor count, then the
+// stage columns.  Every row is exact; there is nothing to re-solve.
+void Context::run_group_radius(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
+                               uint8_t* const* status, int where)
+{
+    const uint64_t n = cl.n;
+    const double r = st[0].radius;
+    FusedProgram prog{};
+    prog.nstages = ns;
+    prog.k = 0;
+    prog.stride = 1;
+    prog.fetch = 0;
+    prog.radius = r;
+    prog.min_k = st[0].min_k;
+    int ncol_total = 0;
+    for (int i = 0; i < ns; ++i) {
+        StageProgram& sp = prog.st[i];
+        sp.stage = st[i].stage;
+        sp.mode = st[i].mode;
+        sp.features = st[i].features;
+        sp.always_up = st[i].always_up;
+        sp.normalize = st[i].normalize;
+        sp.ncols = stage_ncols(st[i]);
+        sp.col_offset = ncol_total;
+        ncol_total += sp.ncols;
+    }
+    if (ncol_total > kMaxOutColumns)
+        throw Error(GF_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxOutColumns) + " output columns");
+    reset_spans();
+    hipEvent_t t_all = mark();
+    points_mark = false;
+    build_index(cl, 0, r * tune.radius_cell);
+    KnnOutputs out{};
+    if (where == GF_WHERE_DEVICE) {
+        for (int c = 0; c < ncol_total; ++c) out.col[c] = cols[c];
+        for (int i = 0; i < ns; ++i) out.status[i] = status[i];
+    } else {
+        outbuf.ensure(static_cast<uint64_t>(ncol_total) * n * 8 + 8);
+        statbuf.ensure(static_cast<uint64_t>(ns) * n + 8);
+        for (int c = 0; c < ncol_total; ++c) {
+            out.col[c] = outbuf.as<double>() + static_cast<uint64_t>(c) * n;
+            GF_HIP(hipMemcpyAsync(out.col[c], cols[c], n * 8, hipMemcpyHostToDevice, stream));
+        }
+        for (int i = 0; i < ns; ++i) out.status[i] = statbuf.as<uint8_t>() + i * n;
+    }
+    covbuf.ensure(n * 8 * kCovRecord + 8);
+    out.cov = covbuf.as<double>();
+    const CloudView cv = view(cl);
+    hipEvent_t e0 = mark();
+    launch_radius(cv, r, 0, out.cov, nullptr, tune.group_z, stream);
+    hipEvent_t e1 = mark();
+    span(e0, e1, &stats.ms_search);
+    launch_features(cv, prog, out, cl.inv.as<uint32_t>(), stream, nullptr, kFeaturesAll);
+    hipEvent_t e2 = mark();
+    span(e1, e2, &stats.ms_features);
+    span(t_all, e2, &stats.ms_total);
+    if (where == GF_WHERE_HOST) {
+        for (int c = 0; c < ncol_total; ++c)
+            GF_HIP(hipMemcpyAsync(cols[c], out.col[c], n * 8, hipMemcpyDeviceToHost, stream));
+        for (int i = 0; i < ns; ++i)
+            GF_HIP(hipMemcpyAsync(status[i], out.status[i], n, hipMemcpyDeviceToHost, stream));
+    }
+    GF_HIP(hipStreamSynchronize(stream));
+    collect_spans();
+    stats.points = n;
+}
+
 // ------------------------------------------------- no-allocation path ----
 //
 // gf_run_noalloc runs the same fused pass as gf_run on caller-owned device
@@ -521,6 +599,9 @@ void Context::run_group_static(const double* x, const double* y, const double* z
     if (n > kStaticMaxPoints)
         throw Error(GF_ERR_ARG, "gf_run_noalloc takes at most " + std::to_string(kStaticMaxPoints) +
                                     " points (the fixed scratch size)");
+    if (st[0].radius > 0.0)
+        throw Error(GF_ERR_UNSUPPORTED, "gf_run_noalloc takes knn neighborhoods; radius "
+                                        "neighborhoods run through gf_run");
     if (!static_scratch) throw Error(GF_ERR_STATE, "no scratch array bound on this device");
     const int stride = st[0].stage == GF_STAGE_NORMAL ? 1 : st[0].stride;
     const uint64_t K = static_cast<uint64_t>(st[0].knn) + 1;
@@ -544,7 +625,7 @@ void Context::run_group_static(const double* x, const double* y, const double* z
         sp.col_offset = ncol_total;
         ncol_total += sp.ncols;
     }
-    if (ncol_total > kMaxOutColumns) throw Error(GF_ERR_UNSUPPORTED, "more than 16 output columns");
+    if (ncol_total > kMaxOutColumns) throw Error(GF_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxOutColumns) + " output columns");
 
     // Every piece is laid out before the first launch, so a call that does
     // not fit fails before touching the device.
@@ -842,6 +923,8 @@ int gf_run(gf_ctx* ctx, gf_cloud* cloud, const gf_stage* stages, int nstages,
         while (i < nstages) {
             int j = i + 1, ncol = stage_ncols(stages[i]);
             auto same = [&](const gf_stage& a, const gf_stage& b) {
+                if (a.radius > 0.0 || b.radius > 0.0)
+                    return a.radius == b.radius && a.min_k == b.min_k;
                 int sa = a.stage == GF_STAGE_NORMAL ? 1 : a.stride;
                 int sb = b.stage == GF_STAGE_NORMAL ? 1 : b.stride;
                 return a.knn == b.knn && sa == sb;
@@ -878,6 +961,8 @@ int gf_run_noalloc(gf_ctx* ctx, const double* x, const double* y, const double* 
         while (i < nstages) {
             int j = i + 1, ncol = stage_ncols(stages[i]);
             auto same = [&](const gf_stage& a, const gf_stage& b) {
+                if (a.radius > 0.0 || b.radius > 0.0)
+                    return a.radius == b.radius && a.min_k == b.min_k;
                 int sa = a.stage == GF_STAGE_NORMAL ? 1 : a.stride;
                 int sb = b.stage == GF_STAGE_NORMAL ? 1 : b.stride;
                 return a.knn == b.knn && sa == sb;
@@ -890,6 +975,29 @@ int gf_run_noalloc(gf_ctx* ctx, const double* x, const double* y, const double* 
             col0 += ncol;
             i = j;
         }
+    });
+}
+
+int gf_radius_counts(gf_ctx* ctx, gf_cloud* cloud, double radius, uint32_t* counts, int where)
+{
+    if (!ctx || !cloud || !counts) return GF_ERR_ARG;
+    if (!(radius > 0.0) || radius == INFINITY) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        const uint64_t n = cl->n;
+        if (!n) return;
+        c->points_mark = false;
+        c->build_index(*cl, 0, radius * c->tune.radius_cell);
+        uint32_t* dst = counts;
+        if (where == GF_WHERE_HOST) {
+            c->outbuf.ensure(n * 4 + 8);
+            dst = c->outbuf.as<uint32_t>();
+        }
+        launch_radius(c->view(*cl), radius, 1, nullptr, dst, c->tune.group_z, c->stream);
+        if (where == GF_WHERE_HOST)
+            GF_HIP(hipMemcpyAsync(counts, dst, n * 4, hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipStreamSynchronize(c->stream));
     });
 }
 

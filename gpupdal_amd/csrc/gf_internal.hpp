@@ -34,7 +34,8 @@ enum : int32_t { MODE_RAW = 0, MODE_SQRT = 1, MODE_NORMALIZED = 2 };
 constexpr int kWave = 64;
 constexpr int kMaxStages = 4;
 constexpr int kMaxFetch = 64;     // K*stride held by the wave-wide top list
-constexpr int kMaxOutColumns = 16; // output columns of one fused run
+// Output columns of one fused run: four stages write at most 10 + 10 + 4 + 3.
+constexpr int kMaxOutColumns = 32;
 
 // Uniform column grid over one cloud (or one tile of a batch).  Columns are
 // (cx, cy) cells of edge `c`, stored in 8x8 tiles (row-major tiles, Morton
@@ -70,6 +71,12 @@ struct FusedProgram {
     int32_t stride;     // fetch stride*k, keep every stride-th
     int32_t fetch;      // stride*k
     StageProgram st[kMaxStages];
+    // Radius neighborhoods (radius > 0): every point within `radius`; each
+    // row's neighbor count is in its covariance record, and covfeat /
+    // eigenvalues skip rows with fewer than min_k.
+    double radius;
+    int32_t min_k;
+    int32_t pad;
 };
 
 struct Error : std::runtime_error {

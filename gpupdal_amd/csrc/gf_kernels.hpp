@@ -68,7 +68,7 @@ void launch_copy_f64(const double* src, double* dst, uint64_t n, hipStream_t s);
 
 // knn.hip
 struct KnnOutputs {
-    double* col[16];         // device output columns (original point order)
+    double* col[kMaxOutColumns];  // device output columns (original point order)
     uint8_t* status[kMaxStages];
     uint32_t* ids;           // neighbor-list mode: n x k (original order)
     uint32_t* flagged;       // sorted indices of tie/incomplete rows
@@ -153,6 +153,11 @@ inline int knn_queries_per_wave(int max_k) { return max_k <= 32 ? 64 : 32; }
 void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
                 const KnnOutputs& out, const KnnTuning& tune, hipStream_t s,
                 const TileTables* tt = nullptr);
+// Radius neighborhoods: mode 0 writes each sorted row's covariance record
+// (FP64 two-pass over the points with d2 < radius^2, neighbor count in word
+// 7); mode 1 writes the counts (original order) to `counts`.
+void launch_radius(const CloudView& cv, double radius, int mode, double* cov, uint32_t* counts,
+                   int group_z, hipStream_t s);
 // Eigensystem + stage columns from out.cov, in original point order (inv:
 // original id -> sorted position), so outputs are written contiguously.
 // rows: kFeaturesAll every row; kFeaturesUnflagged every row the search did

@@ -66,6 +66,7 @@ struct Tuning {
     int ring_max = 8;
     int small_col = 0;
     int group_z = 3;             // measured: flat for knn 16, 4% faster search for knn 32
+    double radius_cell = 0.5;    // radius grids: column edge = radius_cell * radius
     // Exact tree: subtree-per-workgroup threshold (points; 2048-point
     // workgroups, two per CU, measured fastest alone and beside the search).
     int kd_subtree = 2048;
@@ -88,6 +89,7 @@ struct Tuning {
         else if (!std::strcmp(key, "ring_max") && v >= 1) ring_max = static_cast<int>(v);
         else if (!std::strcmp(key, "small_col") && v >= 0) small_col = static_cast<int>(v);
         else if (!std::strcmp(key, "group_z") && v >= 0) group_z = static_cast<int>(v);
+        else if (!std::strcmp(key, "radius_cell") && v >= 0.2 && v <= 4.0) radius_cell = v;
         else if (!std::strcmp(key, "kd_subtree") && v >= 101 && v <= 4096)
             kd_subtree = static_cast<int>(v);
         else if (!std::strcmp(key, "kd_wave_node") && v >= 101 && v <= 1024)
@@ -156,6 +158,7 @@ struct Cloud {
     // index (valid for idx_fetch / idx_tune)
     bool idx_valid = false;
     uint32_t idx_fetch = 0;
+    double idx_cell = 0.0;       // forced column edge of a radius grid (0: density rule)
     uint64_t idx_tune = 0;
     GridParams g{};
     DevBuf keys_a, vals_a, keys_b, vals_b, colstart, sx, sy, sz;
@@ -199,7 +202,9 @@ struct Batch {
     void reset(uint32_t ntiles, const uint64_t* counts, hipStream_t s);
 };
 
-GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch, const Tuning& t);
+// cell > 0: that column edge instead of the density rule (radius grids).
+GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch, const Tuning& t,
+                     double cell = 0.0);
 int stage_ncols(const gf_stage& s);
 void validate_stage(const gf_stage& s);
 // Throws GF_ERR_UNSUPPORTED when a cloud (or tile) of n points with this
@@ -273,12 +278,14 @@ struct Context {
         return k;
     }
 
-    void build_index(Cloud& cl, uint32_t fetch);
+    void build_index(Cloud& cl, uint32_t fetch, double cell = 0.0);
     // The grid of the cloud's current index (host copy).
     GridParams grid_of(const Cloud& cl) const { return cl.g; }
     CloudView view(const Cloud& cl) const;
     void run_group(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
                    uint8_t* const* status, int where);
+    void run_group_radius(Cloud& cl, const gf_stage* st, int ns, double* const* cols,
+                          uint8_t* const* status, int where);
     // beside: built on the side stream while the search runs.
     void build_kdtree(Cloud& cl, hipStream_t s, bool beside);
     void build_kdtree(Cloud& cl) { build_kdtree(cl, stream, false); }

@@ -76,6 +76,13 @@ __device__ __forceinline__ void merge_sorted(Ent& best, const Ent& cand, int lan
     }
 }
 
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane)
 {
 #pragma unroll

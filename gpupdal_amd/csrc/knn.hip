@@ -794,6 +794,224 @@ __global__ __launch_bounds__(knn_waves<kTiled>() * kWave) __attribute__((amdgpu_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Radius neighborhoods (the stages' `radius` option; KD3Index::radius,
+// pdal/private/KDImpl.hpp:398-415, nanoflann radiusSearch): every point whose
+// FP64 squared distance (KD3Index's operation order) is strictly below r*r.
+// The set is decided by that comparison alone -- no boundary tie, no exact
+// re-solve.  A wavefront takes a group of consecutive sorted queries (one
+// 2x2 block of columns with nearby z cells, as the kNN search groups them)
+// and the box of every cell that can hold a point within r of any of them:
+// the block grown by R = ceil(r / c) columns, the group's z cells by
+// ceil(r / cz) (cell edges chosen so that R is 2).  Per query the
+// wavefront masks the box's candidates by d2 < r*r and forms the count and
+// the FP64 two-pass covariance (coordinates relative to the query point)
+// from masked wavefront sums.  Boxes of up to 256 candidates and 64 columns
+// are held in registers for the whole group; larger boxes are streamed from
+// memory per query.
+struct RadiusArgs {
+    CloudView cv;
+    double r2;          // r * r, the strict bound on d2
+    int32_t rx, rz;     // box growth in columns and z cells
+    int32_t group_z;
+    int32_t mode;       // 0: covariance records; 1: counts only
+    double* cov;        // mode 0: records in sorted order (count in word 7)
+    uint32_t* counts;   // mode 1: counts in original order
+};
+
+// Candidates of one chunk: d2 bits per register batch (kInfD outside r).
+__device__ __forceinline__ void radius_pass1(const double (&px)[kRegBatches],
+                                             const double (&py)[kRegBatches],
+                                             const double (&pz)[kRegBatches],
+                                             const uint32_t (&pj)[kRegBatches], uint32_t n1,
+                                             double qx, double qy, double qz, double r2,
+                                             uint32_t& cnt, double& sx, double& sy, double& sz)
+{
+#pragma unroll
+    for (int b = 0; b < kRegBatches; ++b) {
+        if (static_cast<uint32_t>(b) * 64u >= n1) break;
+        const bool valid = pj[b] != kInfU;
+        const double d2 = bits_d2(dist_bits(qx, qy, qz, px[b], py[b], pz[b]));
+        const bool in = valid && d2 < r2;
+        cnt += static_cast<uint32_t>(__popcll(__ballot(in)));
+        if (in) {
+            sx += px[b] - qx;
+            sy += py[b] - qy;
+            sz += pz[b] - qz;
+        }
+    }
+}
+
+__device__ __forceinline__ void radius_pass2(const double (&px)[kRegBatches],
+                                             const double (&py)[kRegBatches],
+                                             const double (&pz)[kRegBatches],
+                                             const uint32_t (&pj)[kRegBatches], uint32_t n1,
+                                             double qx, double qy, double qz, double r2,
+                                             double mx, double my, double mz, double (&c)[6])
+{
+#pragma unroll
+    for (int b = 0; b < kRegBatches; ++b) {
+        if (static_cast<uint32_t>(b) * 64u >= n1) break;
+        const bool valid = pj[b] != kInfU;
+        const double d2 = bits_d2(dist_bits(qx, qy, qz, px[b], py[b], pz[b]));
+        if (valid && d2 < r2) {
+            const double dx = (px[b] - qx) - mx, dy = (py[b] - qy) - my, dz = (pz[b] - qz) - mz;
+            c[0] += dx * dx;
+            c[1] += dx * dy;
+            c[2] += dx * dz;
+            c[3] += dy * dy;
+            c[4] += dy * dz;
+            c[5] += dz * dz;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_radius(RadiusArgs a)
+{
+    const int lane = __lane_id();
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kWave));
+    const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const CloudView& cv = a.cv;
+    const GridParams& g = cv.g;
+    const uint64_t q0 = (static_cast<uint64_t>(lb) * 4 + w) * kWave;
+    if (q0 >= cv.n) return;
+    const int nq = static_cast<int>(cv.n - q0 < 64 ? cv.n - q0 : 64);
+    const uint32_t zmask = (1u << g.bz) - 1u;
+    // box_ranges reads only these fields of the kNN arguments.
+    KnnArgs ka{};
+    ka.cv = cv;
+    ka.tune.small_col = 0;
+    WaveGrid wg{};
+    wg.g = g;
+    wg.colstart = cv.colstart;
+    int qi = 0;
+    while (qi < nq) {
+        const uint64_t s0 = q0 + qi;
+        const bool in_range = qi + lane < nq;
+        const uint32_t key_l = in_range ? cv.keys[s0 + lane] : 0u;
+        const uint32_t key0 = __shfl(key_l, 0);
+        const int cz0 = static_cast<int>(key0 & zmask);
+        const int cz_l = static_cast<int>(key_l & zmask);
+        const bool zok = in_range && abs(cz_l - cz0) <= a.group_z;
+        const uint32_t shift = g.bz + 2u;
+        const uint64_t mm = __ballot(zok && (key_l >> shift) == (key0 >> shift));
+        const int gm = (~mm == 0ull) ? 64 : __builtin_ctzll(~mm);
+        int zl = lane < gm ? cz_l : cz0, zh = zl;
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) {
+            zl = min(zl, __shfl_xor(zl, m));
+            zh = max(zh, __shfl_xor(zh, m));
+        }
+        const double fx = cv.sx[s0], fy = cv.sy[s0];
+        const int qcx = static_cast<int>(cell_coord(fx, g.ox, g.inv_c, g.gx));
+        const int qcy = static_cast<int>(cell_coord(fy, g.oy, g.inv_c, g.gy));
+        const int bx0 = qcx & ~1, by0 = qcy & ~1;
+        const Box box{bx0 - a.rx, bx0 + 1 + a.rx, by0 - a.rx, by0 + 1 + a.rx, zl - a.rz, zh + a.rz};
+        const int ncb = (2 + 2 * a.rx) * (2 + 2 * a.rx);
+        uint32_t tot = 0;
+        for (int cb = 0; cb < ncb; cb += kWave) {
+            uint32_t lo, len;
+            box_ranges(ka, wg, box, cb, lane, lo, len);
+            tot += __shfl(wave_incl_scan_u32(len, lane), 63);
+        }
+        const bool cached = ncb <= kWave && tot <= kRegCand;
+        double px[kRegBatches], py[kRegBatches], pz[kRegBatches];
+        uint32_t pj[kRegBatches];
+        uint32_t lo0 = 0, excl0 = 0;
+        if (cached) {
+            uint32_t len;
+            box_ranges(ka, wg, box, 0, lane, lo0, len);
+            excl0 = wave_incl_scan_u32(len, lane) - len;
+#pragma unroll
+            for (int b = 0; b < kRegBatches; ++b) {
+                px[b] = py[b] = pz[b] = 0.0;
+                pj[b] = kInfU;
+                if (static_cast<uint32_t>(b * 64) < tot) {
+                    const uint32_t t = b * 64 + lane;
+                    const uint32_t j = candidate_pos(lo0, excl0, t);
+                    if (t < tot) {
+                        px[b] = cv.sx[j];
+                        py[b] = cv.sy[j];
+                        pz[b] = cv.sz[j];
+                        pj[b] = j;
+                    }
+                }
+            }
+        }
+        for (int gi = 0; gi < gm; ++gi) {
+            const uint64_t s = s0 + gi;
+            const double qx = cv.sx[s], qy = cv.sy[s], qz = cv.sz[s];
+            uint32_t cnt = 0;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            double c[6] = {0, 0, 0, 0, 0, 0};
+            for (int pass = 0; pass < (a.mode == 0 ? 2 : 1); ++pass) {
+                double mx = 0.0, my = 0.0, mz = 0.0;
+                if (pass == 1) {
+                    const double nn = static_cast<double>(cnt);
+                    mx = wave_sum_f64(sx) / nn;
+                    my = wave_sum_f64(sy) / nn;
+                    mz = wave_sum_f64(sz) / nn;
+                }
+                if (cached) {
+                    if (pass == 0)
+                        radius_pass1(px, py, pz, pj, tot, qx, qy, qz, a.r2, cnt, sx, sy, sz);
+                    else
+                        radius_pass2(px, py, pz, pj, tot, qx, qy, qz, a.r2, mx, my, mz, c);
+                    continue;
+                }
+                // Streamed: chunks of 64 columns and 256 candidates.
+                for (int cb = 0; cb < ncb; cb += kWave) {
+                    uint32_t lo, len;
+                    box_ranges(ka, wg, box, cb, lane, lo, len);
+                    const uint32_t incl = wave_incl_scan_u32(len, lane);
+                    const uint32_t ctot = __shfl(incl, 63);
+                    const uint32_t excl = incl - len;
+                    for (uint32_t t0 = 0; t0 < ctot; t0 += kRegCand) {
+                        double cx[kRegBatches], cy[kRegBatches], czv[kRegBatches];
+                        uint32_t cj[kRegBatches];
+                        const uint32_t n1 = min(ctot - t0, kRegCand);
+#pragma unroll
+                        for (int b = 0; b < kRegBatches; ++b) {
+                            cx[b] = cy[b] = czv[b] = 0.0;
+                            cj[b] = kInfU;
+                            if (static_cast<uint32_t>(b * 64) < n1) {
+                                const uint32_t t = t0 + b * 64 + lane;
+                                const uint32_t j = candidate_pos(lo, excl, t);
+                                if (t < ctot) {
+                                    cx[b] = cv.sx[j];
+                                    cy[b] = cv.sy[j];
+                                    czv[b] = cv.sz[j];
+                                    cj[b] = j;
+                                }
+                            }
+                        }
+                        if (pass == 0)
+                            radius_pass1(cx, cy, czv, cj, n1, qx, qy, qz, a.r2, cnt, sx, sy, sz);
+                        else
+                            radius_pass2(cx, cy, czv, cj, n1, qx, qy, qz, a.r2, mx, my, mz, c);
+                    }
+                }
+            }
+            if (a.mode == 1) {
+                if (lane == 0) a.counts[cv.perm[s]] = cnt;
+                continue;
+            }
+            const double den = static_cast<double>(cnt) - 1.0;
+            double v[6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) v[i] = wave_sum_f64(c[i]) / den;
+            if (lane == 0) {
+                double* rec = a.cov + kCovRecord * s;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) rec[i] = v[i];
+                reinterpret_cast<uint64_t*>(rec)[6] = 0;
+                reinterpret_cast<uint64_t*>(rec)[7] = cnt;
+            }
+        }
+        qi += gm;
+    }
+}
+
 // Per sorted row: eigensystem of the stored covariance and every requested
 // stage column, written at the row's original index.
 __device__ __forceinline__ uint32_t tile_of(const uint64_t* __restrict__ starts, uint32_t T,
@@ -831,9 +1049,12 @@ __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov
     } else {
         s = inv[qid];  // the row's sorted position: one record read
     }
-    // The neighborhood size of the row's tile (stages are the same in every tile).
-    const int k = tiles ? tiles[tile_of(starts, ntiles, s)].prog.k : prog.k;
+    // The neighborhood size: the row's own count in radius mode, else its
+    // tile's k (stages are the same in every tile).
     const double* rec = cov + kCovRecord * s;
+    const int k = prog.radius > 0.0
+                      ? static_cast<int>(reinterpret_cast<const uint64_t*>(rec)[7])
+                      : (tiles ? tiles[tile_of(starts, ntiles, s)].prog.k : prog.k);
     double c[6];
     for (int i = 0; i < 6; ++i) c[i] = rec[i];
     const uint8_t f = static_cast<uint8_t>(reinterpret_cast<const uint64_t*>(rec)[6]);
@@ -845,7 +1066,12 @@ __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov
     for (int si = 0; si < prog.nstages; ++si) {
         const StageProgram& sp = prog.st[si];
         double vals[10];
-        uint8_t st = stage_outputs(sp, k, c, zero, info, ev, vec, vals);
+        // Radius mode: covariancefeatures and eigenvalues skip rows with
+        // fewer than min_k neighbors (CovarianceFeaturesFilter.cpp:216-223,
+        // EigenvaluesFilter.cpp:125-135); normal skips fewer than three.
+        uint8_t st = (prog.radius > 0.0 && sp.stage != STAGE_NORMAL && k < prog.min_k)
+                         ? static_cast<uint8_t>(ST_FEW_NEIGHBORS)
+                         : stage_outputs(sp, k, c, zero, info, ev, vec, vals);
         if (st == 0)
             for (int cc = 0; cc < sp.ncols; ++cc) out.col[sp.col_offset + cc][qid] = vals[cc];
         out.status[si][qid] = st | (f & kReportedSearchFlags);
@@ -906,6 +1132,30 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
     const uint64_t per_block = static_cast<uint64_t>(waves) * a.q_per_wave;
     const uint32_t blocks = div_up(cv.n, per_block);
     hipLaunchKernelGGL(k_knn<false>, dim3(blocks), dim3(waves * kWave), lds, s, a);
+}
+
+void launch_radius(const CloudView& cv, double radius, int mode, double* cov, uint32_t* counts,
+                   int group_z, hipStream_t s)
+{
+    if (cv.n == 0) return;
+    RadiusArgs a{};
+    a.cv = cv;
+    a.r2 = radius * radius;
+    // Every point within r of a query lies within these cells: the cell
+    // coordinates' own rounding (relative 4 eps of the grid's span) is
+    // covered.
+    const GridParams& g = cv.g;
+    const double span = fmax(fmax(static_cast<double>(g.gx) * g.c, static_cast<double>(g.gy) * g.c),
+                             static_cast<double>(g.gz) * g.cz);
+    const double slack = 1e-15 * (span + fabs(g.ox) + fabs(g.oy) + fabs(g.oz));
+    a.rx = static_cast<int32_t>(ceil((radius + slack) / g.c));
+    a.rz = static_cast<int32_t>(ceil((radius + slack) / g.cz));
+    a.group_z = group_z;
+    a.mode = mode;
+    a.cov = cov;
+    a.counts = counts;
+    const uint32_t blocks = div_up(cv.n, 256);
+    hipLaunchKernelGGL(k_radius, dim3(blocks), dim3(256), 0, s, a);
 }
 
 void launch_features(const CloudView& cv, const FusedProgram& prog,

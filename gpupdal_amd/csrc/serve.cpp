@@ -301,6 +301,7 @@ int gf_server_submit(gf_server* server, const double* x, const double* y, const 
         } catch (const Error& e) {
             return e.code;
         }
+        if (r->st[i].radius > 0.0) return GF_ERR_UNSUPPORTED;  // served stages use knn
         if (i && (r->st[i].stage == GF_STAGE_NORMAL ? 1 : r->st[i].stride) !=
                      (r->st[0].stage == GF_STAGE_NORMAL ? 1 : r->st[0].stride))
             return GF_ERR_ARG;  // the stages of one request share one neighborhood

@@ -402,6 +402,23 @@ StageResult run_on_gpu(PointView& view, const gf_stage& st, const std::vector<Di
 constexpr uint8_t kSkipOrFail = GF_ST_ZERO_COV | GF_ST_EIGEN_FAIL | GF_ST_FEATURE_INVALID |
                                 GF_ST_FEW_NEIGHBORS;
 
+// Neighbor counts within r of every point (for PDAL's "Found n neighbors but
+// required min_k" messages), fetched only when some row was skipped for it.
+std::vector<uint32_t> radius_counts_if_skipped(PointView& view, double r,
+                                               const std::vector<uint8_t>& status,
+                                               const Stage& stage)
+{
+    std::vector<uint32_t> counts;
+    bool any = false;
+    for (uint8_t s : status) any |= (s & GF_ST_FEW_NEIGHBORS) != 0;
+    if (!any) return counts;
+    counts.assign(view.size(), 0);
+    gf_ctx* ctx = device_context();
+    const int rc = gf_radius_counts(ctx, view.resident()->cloud, r, counts.data(), GF_WHERE_HOST);
+    if (rc != GF_OK) throw pdal_error(stage.getName() + ": " + gf_last_error(ctx));
+    return counts;
+}
+
 } // namespace
 
 // --------------------------------------------------- covariancefeatures --
@@ -463,8 +480,6 @@ void CovarianceFeaturesFilter::prepared(PointTable& table)
         throwError("optimized=true (a per-point neighbor count) is not supported by the GPU "
                    "stage.");
     }
-    if (m_radiusArg->set())
-        throwError("radius searches are not supported by the GPU stage; use knn.");
 }
 
 void CovarianceFeaturesFilter::filter(PointView& view)
@@ -492,7 +507,14 @@ void CovarianceFeaturesFilter::filter(PointView& view)
                                   : (m_mode == Mode::SQRT ? GF_MODE_SQRT : GF_MODE_NORMALIZED);
     st.features = mask;
     st.always_up = 1;
+    const bool radius = m_radiusArg->set();
+    if (radius) {
+        st.radius = m_radius;
+        st.min_k = m_minK;
+    }
     StageResult r = run_on_gpu(view, st, outdims, *this);
+    const std::vector<uint32_t> counts =
+        radius ? radius_counts_if_skipped(view, m_radius, r.status, *this) : std::vector<uint32_t>{};
 
     // Rows are applied in order; the first row PDAL would throw on ends the
     // stage after the rows before it were written.
@@ -513,6 +535,13 @@ void CovarianceFeaturesFilter::filter(PointView& view)
     }
     const bool density = std::count(m_extraDims.begin(), m_extraDims.end(), Id::Density) > 0;
     for (uint64_t i = 0; i < end; ++i) {
+        if (r.status[i] & GF_ST_FEW_NEIGHBORS) {
+            log().write(LogLevel::Info, "Skipping point " + std::to_string(i) + ". Found " +
+                                            std::to_string(counts[i]) +
+                                            " neighbors but required " + std::to_string(m_minK) +
+                                            ".\n");
+            continue;
+        }
         if (r.status[i] & kSkipOrFail) {
             log().write(LogLevel::Info,
                         "Skipping point " + std::to_string(i) +
@@ -589,10 +618,7 @@ void NormalFilter::prepared(PointTable&)
         log().write(LogLevel::Warning, "Viewpoint provided. Ignoring always_up = TRUE.\n");
         m_up = false;
     }
-    if (m_radiusArg->set()) {
-        if (m_knnArg->set()) throwError("Cannot set both knn and radius.");
-        throwError("radius searches are not supported by the GPU stage; use knn.");
-    }
+    if (m_radiusArg->set() && m_knnArg->set()) throwError("Cannot set both knn and radius.");
     if (m_refine)
         throwError("refine (minimum spanning tree propagation) is not supported by the GPU "
                    "stage.");
@@ -611,6 +637,7 @@ void NormalFilter::filter(PointView& view)
     st.knn = m_knn;
     st.stride = 1;
     st.always_up = m_up ? 1 : 0;
+    if (m_radiusArg->set()) st.radius = m_radius;
     const std::vector<Id> outdims{Id::NormalX, Id::NormalY, Id::NormalZ, Id::Curvature};
     StageResult r = run_on_gpu(view, st, outdims, *this);
     double vp[3] = {0, 0, 0};
@@ -671,7 +698,7 @@ void EigenvaluesFilter::prepared(PointTable&)
     if (m_radiusArg->set()) {
         log().write(LogLevel::Warning, "Radius has been set. Ignoring knn and stride values.\n");
         if (m_radius <= 0.0) log().write(LogLevel::Error, "Radius must be greater than 0.\n");
-        throwError("radius searches are not supported by the GPU stage; use knn.");
+        return;
     }
     log().write(LogLevel::Warning,
                 "No radius specified. Proceeding with knn and stride, but ignoring min_k.\n");
@@ -687,10 +714,24 @@ void EigenvaluesFilter::filter(PointView& view)
     st.knn = m_knn;
     st.stride = static_cast<int32_t>(m_stride);
     st.normalize = m_normalize ? 1 : 0;
+    const bool radius = m_radiusArg->set();
+    if (radius) {
+        st.radius = m_radius;
+        st.min_k = m_minK;
+    }
     const std::vector<Id> outdims{Id::Eigenvalue0, Id::Eigenvalue1, Id::Eigenvalue2};
     StageResult r = run_on_gpu(view, st, outdims, *this);
+    const std::vector<uint32_t> counts =
+        radius ? radius_counts_if_skipped(view, m_radius, r.status, *this) : std::vector<uint32_t>{};
     for (uint64_t i = 0; i < n; ++i) {
         const uint8_t s = r.status[i];
+        if (s & GF_ST_FEW_NEIGHBORS) {
+            log().write(LogLevel::Info, "Skipping point " + std::to_string(i) + ". Found " +
+                                            std::to_string(counts[i]) +
+                                            " neighbors but required " + std::to_string(m_minK) +
+                                            ".\n");
+            continue;
+        }
         if (s & GF_ST_EIGEN_FAIL) throwError("Cannot perform eigen decomposition.");
         if (s & kSkipOrFail) {
             log().write(LogLevel::Info,

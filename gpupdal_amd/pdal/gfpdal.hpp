@@ -15,8 +15,7 @@
 // the filter classes and swaps this API layer for libpdal's.
 //
 // Not supported on the GPU (each raises pdal_error naming the option):
-// radius searches (radius/min_k), optimized=true, and filters.normal's
-// viewpoint and refine.
+// optimized=true and filters.normal's refine.
 #pragma once
 
 #include <cstdint>

@@ -85,6 +85,13 @@ typedef struct gf_stage {
     uint32_t features; /* covfeat: GF_F_* mask */
     int32_t always_up; /* normal */
     int32_t normalize; /* eigenvalues */
+    int32_t min_k;     /* radius mode, covfeat / eigenvalues: rows with fewer
+                          neighbors are skipped (GF_ST_FEW_NEIGHBORS); PDAL's
+                          default is 3 */
+    int32_t reserved;  /* 0 */
+    double radius;     /* > 0: the neighborhood is every point within this
+                          distance (KD3Index::radius, strict), knn and stride
+                          are ignored -- the stages' `radius` option */
 } gf_stage;
 
 typedef struct gf_stats {
@@ -182,6 +189,14 @@ int gf_run_noalloc(gf_ctx *ctx, const double *x, const double *y, const double *
 int gf_knn(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, uint32_t stride,
            uint32_t *ids, uint8_t *status, int where);
 
+/* ---- radius neighbor counts: replaces pdg::detail::radiusCountsDevice
+ *      (include/pdg/index/SpatialIndex.hpp:53; src/index/SpatialIndexKernels.cu:
+ *      127-228) and the size of KD3Index::radius(id, r)
+ *      (pdal/private/KDImpl.hpp:497-516).  counts: n uint32, row i = the number
+ *      of points (point i included) within r of point i, strictly. */
+int gf_radius_counts(gf_ctx *ctx, gf_cloud *cloud, double radius, uint32_t *counts,
+                     int where);
+
 /* ---- batches of independent tiles: replaces running the stages tile by
  *      tile (the reference processes one PointView / one SpatialTile at a time,
  *      include/pdg/index/SpatialTile.hpp:29-40; src/pdal/PdgNeighborhood.cpp
@@ -223,7 +238,7 @@ int gf_server_submit(gf_server *server, const double *x, const double *y, const 
  * result code, or GF_ERR_TIMEOUT if the timeout expired first (the request
  * is still queued or running).  Requests are checked when submitted: a stage
  * gf_run would refuse (bad kind, mode, knn or stride; knn and stride outside
- * the device envelope; more than 16 columns) is refused by gf_server_submit
+ * the device envelope; more than 32 columns) is refused by gf_server_submit
  * and never joins a batch. */
 int gf_server_wait(gf_server *server, uint64_t ticket, double timeout_s);
 /* times: submit, batch start, done (seconds, monotonic clock). */

@@ -382,6 +382,157 @@ uint64_t orc_kd3_neighbors(const orc_kd3 *t, double qx, double qy, double qz,
     return k;
 }
 
+/* KD3Index::radius (pdal/private/KDImpl.hpp:398-415): nanoflann's
+ * radiusSearch (nanoflann.hpp:1116-1138) with RadiusResultSet (:229-258):
+ * the same searchLevel walk with the worst distance fixed at r*r, a point
+ * kept when its squared distance is strictly below it, then sorted by
+ * distance (params.sorted).  The reference sorts with std::sort, whose order
+ * among exactly equal distances is the library's; here equal distances keep
+ * the walk's order (a stable sort). */
+typedef struct {
+    double r2;
+    uint64_t count, cap;
+    uint64_t *ids;
+    double *d2;
+} radset;
+
+static void kd_radius_walk(const struct orc_kd3 *t, radset *s, const double q[3], int32_t node,
+                           double mindistsq, double dists[3])
+{
+    if (t->c1[node] < 0 && t->c2[node] < 0) {
+        for (uint64_t i = t->lo[node]; i < t->hi[node]; ++i) {
+            uint64_t idx = t->vind[i];
+            double d = kd_dist(t, q, idx);
+            if (d < s->r2) {
+                if (s->count < s->cap) {
+                    s->ids[s->count] = idx;
+                    s->d2[s->count] = d;
+                }
+                s->count++;
+            }
+        }
+        return;
+    }
+    int f = t->feat[node];
+    double val = q[f];
+    double diff1 = val - t->divlow[node];
+    double diff2 = val - t->divhigh[node];
+    int32_t best, other;
+    double cut;
+    if ((diff1 + diff2) < 0) {
+        best = t->c1[node];
+        other = t->c2[node];
+        cut = (val - t->divhigh[node]) * (val - t->divhigh[node]);
+    } else {
+        best = t->c2[node];
+        other = t->c1[node];
+        cut = (val - t->divlow[node]) * (val - t->divlow[node]);
+    }
+    kd_radius_walk(t, s, q, best, mindistsq, dists);
+    double dst = dists[f];
+    mindistsq = mindistsq + cut - dst;
+    dists[f] = cut;
+    if (mindistsq * (double)1.0f <= s->r2) kd_radius_walk(t, s, q, other, mindistsq, dists);
+    dists[f] = dst;
+}
+
+uint64_t orc_kd3_radius(const orc_kd3 *t, double qx, double qy, double qz, double r,
+                        uint64_t cap, uint64_t *ids, double *d2)
+{
+    radset s;
+    s.r2 = r * r;
+    s.count = 0;
+    s.cap = cap;
+    s.ids = ids;
+    s.d2 = d2;
+    if (t->n == 0) return 0;
+    double q[3] = {qx, qy, qz};
+    double dists[3] = {0.0, 0.0, 0.0};
+    double distsq = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        if (q[i] < t->root_box[i][0]) {
+            dists[i] = (q[i] - t->root_box[i][0]) * (q[i] - t->root_box[i][0]);
+            distsq += dists[i];
+        }
+        if (q[i] > t->root_box[i][1]) {
+            dists[i] = (q[i] - t->root_box[i][1]) * (q[i] - t->root_box[i][1]);
+            distsq += dists[i];
+        }
+    }
+    kd_radius_walk(t, &s, q, 0, distsq, dists);
+    /* Stable insertion sort by distance of the entries held. */
+    uint64_t m = s.count < cap ? s.count : cap;
+    for (uint64_t i = 1; i < m; ++i) {
+        double d = d2[i];
+        uint64_t id = ids[i];
+        uint64_t j = i;
+        while (j > 0 && d2[j - 1] > d) {
+            d2[j] = d2[j - 1];
+            ids[j] = ids[j - 1];
+            --j;
+        }
+        d2[j] = d;
+        ids[j] = id;
+    }
+    return s.count;
+}
+
+typedef struct {
+    const orc_kd3 *t;
+    double r;
+    uint64_t begin, end;
+    uint32_t *counts;
+} rad_job;
+
+static void *rad_worker(void *arg)
+{
+    rad_job *j = arg;
+    for (uint64_t i = j->begin; i < j->end; ++i)
+        j->counts[i] = (uint32_t)orc_kd3_radius(j->t, j->t->pt[0][i], j->t->pt[1][i],
+                                                j->t->pt[2][i], j->r, 0, NULL, NULL);
+    return NULL;
+}
+
+void orc_kd3_radius_counts(const orc_kd3 *t, double r, int threads, uint32_t *counts)
+{
+    if (threads < 1) threads = 1;
+    uint64_t n = t->n, chunk = n / (uint64_t)threads + 1;
+    pthread_t th[64];
+    rad_job jobs[64];
+    if (threads > 64) threads = 64;
+    for (int i = 0; i < threads; ++i) {
+        jobs[i].t = t;
+        jobs[i].r = r;
+        jobs[i].begin = (uint64_t)i * chunk < n ? (uint64_t)i * chunk : n;
+        jobs[i].end = (uint64_t)(i + 1) * chunk < n ? (uint64_t)(i + 1) * chunk : n;
+        jobs[i].counts = counts;
+        pthread_create(&th[i], NULL, rad_worker, &jobs[i]);
+    }
+    for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+}
+
+/* Brute-force radius counts (d2 < r*r, KD3Index's distance arithmetic). */
+void orc_brute_radius_counts(const double *x, const double *y, const double *z, uint64_t n,
+                             double r, const uint64_t *queries, uint64_t nq, uint32_t *counts)
+{
+    const double r2 = r * r;
+    for (uint64_t a = 0; a < nq; ++a) {
+        uint64_t q = queries ? queries[a] : a;
+        uint32_t c = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            double d = 0.0, diff;
+            diff = x[q] - x[i];
+            d += diff * diff;
+            diff = y[q] - y[i];
+            d += diff * diff;
+            diff = z[q] - z[i];
+            d += diff * diff;
+            if (d < r2) ++c;
+        }
+        counts[a] = c;
+    }
+}
+
 typedef struct {
     const orc_kd3 *t;
     uint64_t k, stride, begin, end;

@@ -138,6 +138,9 @@ typedef struct {
     uint32_t features; /* covfeat: ORC_F_* mask */
     int32_t always_up; /* normal */
     int32_t normalize; /* eigenvalues */
+    int32_t min_k;     /* radius mode, covfeat/eigenvalues: fewer neighbors -> skip */
+    int32_t pad;
+    double radius;     /* > 0: neighbors within this radius (knn, stride ignored) */
 } orc_stage;
 
 /* Number of output columns a stage writes, in PDAL's registration order. */
@@ -164,6 +167,25 @@ void orc_stage_from_lists(const double *x, const double *y, const double *z,
                           uint64_t n, const uint32_t *nbr, uint64_t kk,
                           const orc_stage *st, int threads,
                           double *const *cols, uint8_t *status);
+
+/* KD3Index::radius (pdal/private/KDImpl.hpp:398-415): the points with
+ * squared distance strictly below r*r, sorted by distance (equal distances in
+ * walk order); fills up to cap ids/d2, returns the full count. */
+uint64_t orc_kd3_radius(const orc_kd3 *t, double qx, double qy, double qz, double r,
+                        uint64_t cap, uint64_t *ids, double *d2);
+/* Neighbor counts within r of every point (the point itself included). */
+void orc_kd3_radius_counts(const orc_kd3 *t, double r, int threads, uint32_t *counts);
+/* The same counts by brute force over all points, for nq query rows
+ * (queries NULL: rows 0..nq-1). */
+void orc_brute_radius_counts(const double *x, const double *y, const double *z, uint64_t n,
+                             double r, const uint64_t *queries, uint64_t nq, uint32_t *counts);
+
+/* The truth (below) from variable-length neighbor lists: row i's neighbors
+ * are ids[offsets[i] .. offsets[i+1]).  Radius stages apply min_k. */
+void orc_truth_from_csr(const double *x, const double *y, const double *z, uint64_t n,
+                        const uint64_t *offsets, const uint32_t *ids, const orc_stage *st,
+                        int threads, double *const *cols, uint8_t *status, double *gap,
+                        double *ratio);
 
 /* Extended-precision truth for the same stage from neighbor lists: long
  * double two-pass covariance without float narrowing, cyclic Jacobi to

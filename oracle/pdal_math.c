@@ -383,6 +383,7 @@ typedef struct {
     uint64_t n;
     const orc_stage *st;
     const uint32_t *lists;
+    const uint64_t *offsets; /* with lists: CSR rows of any length */
     uint64_t kk;
     int truth;
     uint64_t begin, end;
@@ -409,9 +410,43 @@ static void *stage_worker(void *arg)
     double *d2 = malloc((cap ? cap : 1) * sizeof(double));
     int ncol = orc_stage_columns(st);
     double vals[16];
+    /* Radius stages: growable id/distance buffers. */
+    uint64_t rcap = 0;
+    uint64_t *rids = NULL;
+    double *rd2 = NULL;
     for (uint64_t i = j->begin; i < j->end; ++i) {
         uint8_t s;
-        if (j->lists) {
+        if (j->lists && j->offsets) {
+            const uint32_t *row = j->lists + j->offsets[i];
+            uint64_t cnt = j->offsets[i + 1] - j->offsets[i];
+            if (st->radius > 0 && st->stage != ORC_STAGE_NORMAL && cnt < (uint64_t)st->min_k)
+                s = ORC_ST_FEW_NEIGHBORS;
+            else if (j->truth)
+                s = truth_row(st, j->x, j->y, j->z, row, cnt, vals,
+                              j->gap ? j->gap + i : NULL, j->ratio ? j->ratio + i : NULL);
+            else
+                s = stage_row(st, j->x, j->y, j->z, row, cnt, cov_u32, vals);
+        } else if (!j->lists && st->radius > 0) {
+            /* KD3Index::radius, then the stage (CovarianceFeaturesFilter.cpp:
+             * 216-223, EigenvaluesFilter.cpp:123-136: fewer than min_k
+             * neighbors skips the row; NormalFilter.cpp:149-150: findNormal
+             * skips fewer than three). */
+            uint64_t cnt = orc_kd3_radius(j->t, j->x[i], j->y[i], j->z[i], st->radius, rcap,
+                                          rids, rd2);
+            if (cnt > rcap) {
+                rcap = cnt + cnt / 2 + 16;
+                free(rids);
+                free(rd2);
+                rids = malloc(rcap * sizeof(uint64_t));
+                rd2 = malloc(rcap * sizeof(double));
+                cnt = orc_kd3_radius(j->t, j->x[i], j->y[i], j->z[i], st->radius, rcap, rids,
+                                     rd2);
+            }
+            if (st->stage != ORC_STAGE_NORMAL && cnt < (uint64_t)st->min_k)
+                s = ORC_ST_FEW_NEIGHBORS;
+            else
+                s = stage_row(st, j->x, j->y, j->z, rids, cnt, cov_u64, vals);
+        } else if (j->lists) {
             const uint32_t *row = j->lists + i * j->kk;
             if (j->truth)
                 s = truth_row(st, j->x, j->y, j->z, row, j->kk, vals,
@@ -429,6 +464,8 @@ static void *stage_worker(void *arg)
     }
     free(ids);
     free(d2);
+    free(rids);
+    free(rd2);
     return NULL;
 }
 
@@ -709,6 +746,28 @@ static uint8_t truth_row(const orc_stage *st, const double *x, const double *y,
         vals[k++] = (double)v;
     }
     return 0;
+}
+
+void orc_truth_from_csr(const double *x, const double *y, const double *z, uint64_t n,
+                        const uint64_t *offsets, const uint32_t *ids, const orc_stage *st,
+                        int threads, double *const *cols, uint8_t *status, double *gap,
+                        double *ratio)
+{
+    stage_job p;
+    memset(&p, 0, sizeof(p));
+    p.x = x;
+    p.y = y;
+    p.z = z;
+    p.n = n;
+    p.st = st;
+    p.lists = ids;
+    p.offsets = offsets;
+    p.truth = 1;
+    p.cols = cols;
+    p.status = status;
+    p.gap = gap;
+    p.ratio = ratio;
+    run_jobs(p, n, threads);
 }
 
 void orc_truth_from_lists(const double *x, const double *y, const double *z,

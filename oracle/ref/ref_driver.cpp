@@ -108,6 +108,25 @@ uint64_t ref_kd3_neighbors(void* hv, double qx, double qy, double qz,
     return k;
 }
 
+// KD3Index::radius (pdal/private/KDImpl.hpp:398-415): radiusSearch with
+// r*r and sorted results; returns the count, fills up to cap ids and squared
+// distances.
+uint64_t ref_kd3_radius(void* hv, double qx, double qy, double qz, double r, uint64_t cap,
+                        uint64_t* ids, double* d2)
+{
+    auto* h = static_cast<Handle*>(hv);
+    std::vector<std::pair<std::size_t, double>> matches;
+    nanoflann::SearchParams params;
+    params.sorted = true;
+    const double q[3] = {qx, qy, qz};
+    const std::size_t count = h->tree.radiusSearch(q, r * r, matches, params);
+    for (std::size_t i = 0; i < count && i < cap; ++i) {
+        ids[i] = matches[i].first;
+        d2[i] = matches[i].second;
+    }
+    return count;
+}
+
 uint64_t ref_kd3_node_count(void* hv)
 {
     Handle* h = static_cast<Handle*>(hv);

@@ -179,16 +179,20 @@ TEST(CpuPrepared, NormalKnnAndRadius)
                      "filters.normal: Cannot set both knn and radius.");
 }
 
-TEST(CpuPrepared, RadiusIsNotOnTheGpu)
+TEST(CpuPrepared, EigenvaluesRadiusWarns)
 {
+    // EigenvaluesFilter.cpp:96-112: radius set -> a warning that knn and
+    // stride are ignored, no error.
     PointTable table;
     EigenvaluesFilter f;
     Options o;
     o.add("radius", 2.0);
     f.setOptions(o);
-    EXPECT_THROW_MSG(f.prepare(table), pdal_error,
-                     "filters.eigenvalues: radius searches are not supported by the GPU "
-                     "stage; use knn.");
+    std::ostringstream log;
+    f.setLog(&log, LogLevel::Warning);
+    f.prepare(table);
+    ASSERT_TRUE(log.str().find("Radius has been set. Ignoring knn and stride values.") !=
+                std::string::npos);
 }
 
 TEST(CpuFactory, StageNames)
@@ -437,6 +441,56 @@ TEST(GpuSkip, StridePaddingSkipsRowZero)
     PointViewPtr out = *filter.execute(table).begin();
     ASSERT_EQ(out->getFieldAs<double>(Id::Linearity, 0), 0.0);  // skipped, prior value
     ASSERT_FLOAT_EQ(out->getFieldAs<double>(Id::Linearity, 1), 1.0);
+}
+
+TEST(GpuRadius, LineSkipsEndsBelowMinK)
+{
+    // Five points on a line, spacing 1, radius 1.5: the ends have two
+    // neighbors (themselves included), the others three.  min_k = 3 skips
+    // the ends with PDAL's message; the middle rows are a line.
+    PointTable table;
+    BufferReader reader;
+    CovarianceFeaturesFilter filter;
+    Options ops;
+    ops.add("radius", 1.5);
+    filter.setInput(reader);
+    filter.setOptions(ops);
+    std::ostringstream log;
+    filter.setLog(&log, LogLevel::Info);
+    filter.prepare(table);
+    reader.addView(make_view(table, {{0, 0, 0}, {1, 0, 0}, {2, 0, 0}, {3, 0, 0}, {4, 0, 0}}));
+    PointViewPtr out = *filter.execute(table).begin();
+    ASSERT_EQ(out->getFieldAs<double>(Id::Linearity, 0), 0.0);  // skipped, prior value
+    ASSERT_EQ(out->getFieldAs<double>(Id::Linearity, 4), 0.0);
+    for (point_count_t i = 1; i < 4; ++i) {
+        ASSERT_FLOAT_EQ(out->getFieldAs<double>(Id::Linearity, i), 1.0);
+        ASSERT_FLOAT_EQ(out->getFieldAs<double>(Id::Planarity, i), 0.0);
+    }
+    ASSERT_TRUE(log.str().find("Skipping point 0. Found 2 neighbors but required 3.") !=
+                std::string::npos);
+    ASSERT_TRUE(log.str().find("Skipping point 4. Found 2 neighbors but required 3.") !=
+                std::string::npos);
+}
+
+TEST(GpuRadius, NormalOfAPlane)
+{
+    // A 4 x 4 lattice in z = 0 with radius 1.1: every row has at least three
+    // neighbors and the normal is +z.
+    PointTable table;
+    BufferReader reader;
+    NormalFilter filter;
+    Options ops;
+    ops.add("radius", 1.1);
+    filter.setInput(reader);
+    filter.setOptions(ops);
+    filter.prepare(table);
+    reader.addView(make_view(table, faux_grid(0, 4, 0, 4, 0, 0, 16)));
+    PointViewPtr out = *filter.execute(table).begin();
+    for (point_count_t i = 0; i < 16; ++i) {
+        ASSERT_NEAR(out->getFieldAs<double>(Id::NormalX, i), 0.0, 1e-12);
+        ASSERT_NEAR(out->getFieldAs<double>(Id::NormalY, i), 0.0, 1e-12);
+        ASSERT_NEAR(out->getFieldAs<double>(Id::NormalZ, i), 1.0, 1e-12);
+    }
 }
 
 TEST(GpuChain, OptimalNeighborhoodThenDensity)

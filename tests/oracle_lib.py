@@ -45,12 +45,13 @@ class Stage(ctypes.Structure):
     _fields_ = [("stage", ctypes.c_int32), ("knn", ctypes.c_int32),
                 ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("features", ctypes.c_uint32), ("always_up", ctypes.c_int32),
-                ("normalize", ctypes.c_int32)]
+                ("normalize", ctypes.c_int32), ("min_k", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("radius", ctypes.c_double)]
 
 
 def stage(kind, knn, stride=1, mode=MODE_SQRT, features=F_DIMENSIONALITY,
-          always_up=1, normalize=0):
-    return Stage(kind, knn, stride, mode, features, always_up, normalize)
+          always_up=1, normalize=0, radius=0.0, min_k=3):
+    return Stage(kind, knn, stride, mode, features, always_up, normalize, min_k, 0, radius)
 
 
 def stage_columns(st):
@@ -123,6 +124,16 @@ class Oracle:
                                            _u32p, ctypes.c_uint64, sp,
                                            ctypes.c_int, ctypes.POINTER(_dp),
                                            _u8p, _dp, _dp]
+        L.orc_kd3_radius.restype = ctypes.c_uint64
+        L.orc_kd3_radius.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                     _u64p, _dp]
+        L.orc_kd3_radius_counts.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                            _u32p]
+        L.orc_brute_radius_counts.argtypes = [_dp, _dp, _dp, ctypes.c_uint64, ctypes.c_double,
+                                              _u64p, ctypes.c_uint64, _u32p]
+        L.orc_truth_from_csr.argtypes = [_dp, _dp, _dp, ctypes.c_uint64, _u64p, _u32p, sp,
+                                         ctypes.c_int, ctypes.POINTER(_dp), _u8p, _dp, _dp]
         self.L = L
 
     # --------------------------------------------------------------- KD3
@@ -162,6 +173,30 @@ class Oracle:
             self.orc.L.orc_kd3_neighbors_rows(self.h, _ptr(rows, _u64p), len(rows), k, stride,
                                               threads, _ptr(out, _u32p))
             return out
+
+        def radius(self, qx, qy, qz, r):
+            """KD3Index::radius: (ids, d2) sorted by distance."""
+            cnt = self.orc.L.orc_kd3_radius(self.h, qx, qy, qz, r, 0, None, None)
+            ids = np.zeros(max(cnt, 1), np.uint64)
+            d2 = np.zeros(max(cnt, 1), np.float64)
+            self.orc.L.orc_kd3_radius(self.h, qx, qy, qz, r, cnt, _ptr(ids, _u64p), _ptr(d2, _dp))
+            return ids[:cnt], d2[:cnt]
+
+        def radius_counts(self, r, threads=8):
+            out = np.zeros(self.n, np.uint32)
+            self.orc.L.orc_kd3_radius_counts(self.h, r, threads, _ptr(out, _u32p))
+            return out
+
+        def radius_csr(self, rows, r):
+            """CSR (offsets, ids) of KD3Index::radius for each row."""
+            offs = [0]
+            parts = []
+            for q in rows:
+                ids, _ = self.radius(self.x[q], self.y[q], self.z[q], r)
+                parts.append(ids.astype(np.uint32))
+                offs.append(offs[-1] + len(ids))
+            ids = np.concatenate(parts) if parts else np.zeros(0, np.uint32)
+            return np.array(offs, np.uint64), ids
 
         def export(self):
             m = self.orc.L.orc_kd3_node_count(self.h)
@@ -295,6 +330,42 @@ class Oracle:
         return cols, status, gap
 
 
+def _brute_radius_counts(orc, x, y, z, r, rows=None):
+    x, y, z = (_c(v, np.float64) for v in (x, y, z))
+    n = len(x)
+    q = None if rows is None else _c(rows, np.uint64)
+    nq = n if rows is None else len(q)
+    out = np.zeros(nq, np.uint32)
+    orc.L.orc_brute_radius_counts(_ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp), n, r,
+                                  None if q is None else _ptr(q, _u64p), nq, _ptr(out, _u32p))
+    return out
+
+
+Oracle.brute_radius_counts = _brute_radius_counts
+
+
+def _truth_from_csr(orc, x, y, z, offsets, ids, st, threads=8, init=np.nan, with_ratio=False):
+    """Extended-precision truth over CSR neighbor lists (rows in order)."""
+    x, y, z = (_c(v, np.float64) for v in (x, y, z))
+    offsets = _c(offsets, np.uint64)
+    ids = _c(ids, np.uint32)
+    n = len(offsets) - 1
+    cols, arr = orc._cols(st, n, init)
+    status = np.zeros(n, np.uint8)
+    gap = np.zeros(n, np.float64)
+    ratio = np.zeros(n, np.float64)
+    orc.L.orc_truth_from_csr(_ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp), n,
+                             _ptr(offsets, _u64p), _ptr(ids if len(ids) else np.zeros(1, np.uint32), _u32p),
+                             ctypes.byref(st), threads, arr, _ptr(status, _u8p), _ptr(gap, _dp),
+                             _ptr(ratio, _dp))
+    if with_ratio:
+        return cols, status, gap, ratio
+    return cols, status, gap
+
+
+Oracle.truth_from_csr = _truth_from_csr
+
+
 class Ref:
     """The reference's own nanoflann + Eigen (oracle/_ref).  Present only
     where /root/reference was available when build() ran."""
@@ -317,6 +388,10 @@ class Ref:
         L.ref_eigen_sym3.restype = ctypes.c_int
         L.ref_eigen_sym3.argtypes = [_dp, _dp, _dp]
         L.ref_pdal_covariance.argtypes = [_dp, _dp, _dp, _u64p, ctypes.c_uint64, _dp]
+        L.ref_kd3_radius.restype = ctypes.c_uint64
+        L.ref_kd3_radius.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                     _u64p, _dp]
         self.L = L
 
     class Tree:
@@ -337,6 +412,14 @@ class Ref:
             c = self.ref.L.ref_kd3_neighbors(self.h, qx, qy, qz, k, stride,
                                              _ptr(ids, _u64p))
             return ids[:c].copy()
+
+        def radius(self, qx, qy, qz, r):
+            cnt = self.ref.L.ref_kd3_radius(self.h, qx, qy, qz, r, 0, None, None)
+            ids = np.zeros(max(cnt, 1), np.uint64)
+            d2 = np.zeros(max(cnt, 1), np.float64)
+            self.ref.L.ref_kd3_radius(self.h, qx, qy, qz, r, cnt, _ptr(ids, _u64p),
+                                      _ptr(d2, _dp))
+            return ids[:cnt], d2[:cnt]
 
         def export(self):
             m = self.ref.L.ref_kd3_node_count(self.h)
