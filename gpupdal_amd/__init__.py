@@ -18,7 +18,7 @@ GF_OK = 0
 GF_ERR_ARG, GF_ERR_HIP, GF_ERR_UNSUPPORTED, GF_ERR_OOM, GF_ERR_STATE = -1, -2, -3, -4, -5
 GF_ERR_TIMEOUT = -6
 WHERE_HOST, WHERE_DEVICE = 0, 1
-STAGE_COVFEAT, STAGE_NORMAL, STAGE_EIGENVALUES = 0, 1, 2
+STAGE_COVFEAT, STAGE_NORMAL, STAGE_EIGENVALUES, STAGE_COPLANAR, STAGE_RANK = 0, 1, 2, 3, 4
 MODE_RAW, MODE_SQRT, MODE_NORMALIZED = 0, 1, 2
 F_DIMENSIONALITY = 0xF
 F_ALL10 = 0x3FF
@@ -36,7 +36,8 @@ class GfStage(ctypes.Structure):
                 ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("features", ctypes.c_uint32), ("always_up", ctypes.c_int32),
                 ("normalize", ctypes.c_int32), ("min_k", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("radius", ctypes.c_double)]
+                ("reserved", ctypes.c_int32), ("radius", ctypes.c_double),
+                ("p1", ctypes.c_double), ("p2", ctypes.c_double)]
 
 
 class GfLasInfo(ctypes.Structure):
@@ -89,9 +90,11 @@ EXPORTED = [
     "gf_autotune", "gf_tuning_save", "gf_tuning_load",
     "gf_las_parse", "gf_cloud_create_las", "gf_las_pack_extra", "gf_debug_cloud_xyz",
     "gf_optimal_neighborhood", "gf_debug_grid_params", "gf_run_noalloc",
-    "gf_radius_counts",
+    "gf_radius_counts", "gf_nndistance", "gf_outlier",
 ]
 NOALLOC_MAX_POINTS = 65536
+NND_KTH, NND_AVG = 0, 1
+OUTLIER_STATISTICAL, OUTLIER_RADIUS = 0, 1
 
 
 def load_library(path=LIB_PATH):
@@ -121,6 +124,9 @@ def load_library(path=LIB_PATH):
     L.gf_run_noalloc.argtypes = [vp, vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(GfStage),
                                  ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.gf_knn.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.c_int]
+    L.gf_nndistance.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_int, vp, ctypes.c_int]
+    L.gf_outlier.argtypes = [vp, vp, ctypes.c_int, ctypes.c_uint32, ctypes.c_double,
+                             ctypes.c_double, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
     L.gf_debug_eigen.argtypes = [vp, _dp, ctypes.c_uint64, _dp, _dp, _i32p]
     L.gf_debug_grid.argtypes = [vp, vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p, _u32p]
     L.gf_debug_grid_params.argtypes = [vp, vp, ctypes.c_uint32, _dp, _u32p]
@@ -234,8 +240,15 @@ def synth_cloud(n, **kw):
 
 
 def stage(kind, knn, stride=1, mode=MODE_SQRT, features=F_DIMENSIONALITY, always_up=1,
-          normalize=0, radius=0.0, min_k=3):
-    return GfStage(kind, knn, stride, mode, features, always_up, normalize, min_k, 0, radius)
+          normalize=0, radius=0.0, min_k=3, p1=None, p2=None):
+    """A stage of a chain.  p1/p2 default to PDAL's: approximatecoplanar
+    thresh1 25, thresh2 6; estimaterank thresh 0.01."""
+    if p1 is None:
+        p1 = {STAGE_COPLANAR: 25.0, STAGE_RANK: 0.01}.get(kind, 0.0)
+    if p2 is None:
+        p2 = 6.0 if kind == STAGE_COPLANAR else 0.0
+    return GfStage(kind, knn, stride, mode, features, always_up, normalize, min_k, 0, radius,
+                   p1, p2)
 
 
 def stage_columns(st):
@@ -444,6 +457,33 @@ class Cloud:
                                                     out.ctypes.data, WHERE_HOST),
                         "gf_radius_counts")
         return out
+
+    def nndistance(self, k=10, mode="kth"):
+        """filters.nndistance (NNDistance): the distance to the k-th nearest
+        neighbor ("kth") or the mean distance to the k nearest ("avg")."""
+        m = {"kth": NND_KTH, "avg": NND_AVG}.get(str(mode).lower())
+        if m is None:
+            raise ValueError("nndistance mode must be 'kth' or 'avg'")
+        out = np.zeros(self.n, np.float64)
+        self.ctx._check(self.ctx.L.gf_nndistance(self.ctx.h, self.h, k, m, out.ctypes.data,
+                                                 WHERE_HOST), "gf_nndistance")
+        return out
+
+    def outlier(self, method="statistical", mean_k=8, multiplier=2.0, radius=1.0, min_k=2):
+        """filters.outlier: (uint8 flags, 1 = outlier; the statistical
+        threshold, or None for the radius method)."""
+        m = {"statistical": OUTLIER_STATISTICAL, "radius": OUTLIER_RADIUS}.get(
+            str(method).lower())
+        if m is None:
+            raise ValueError("outlier method must be 'statistical' or 'radius'")
+        if mean_k < 0:
+            raise ValueError("mean_k must be >= 0")
+        out = np.zeros(max(self.n, 1), np.uint8)
+        thr = ctypes.c_double(float("nan"))
+        self.ctx._check(self.ctx.L.gf_outlier(self.ctx.h, self.h, m, mean_k, multiplier, radius,
+                                              min_k, out.ctypes.data, ctypes.byref(thr)),
+                        "gf_outlier")
+        return out[:self.n], (thr.value if m == OUTLIER_STATISTICAL else None)
 
     def knn(self, k, stride=1):
         kk = min(k, self.n)

@@ -183,6 +183,8 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         sp.features = st[i].features;
         sp.always_up = st[i].always_up;
         sp.normalize = st[i].normalize;
+        sp.p1 = st[i].p1;
+        sp.p2 = st[i].p2;
         sp.ncols = stage_ncols(st[i]);
         sp.col_offset = ncol_total;
         ncol_total += sp.ncols;
@@ -203,7 +205,7 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         if (knn < 1) throw Error(GF_ERR_ARG, "knn must be at least 1");
         const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(nt, static_cast<uint64_t>(knn) + 1));
         const uint32_t f = static_cast<uint32_t>(stride) * k;
-        check_neighborhood(nt, knn, stride);
+        check_neighborhood(nt, static_cast<int64_t>(knn) + 1, stride);
         fetch[t] = std::max<uint32_t>(f, 1);
         TileInfo& ti = tiles[t];
         ti.prog = base;
@@ -461,6 +463,9 @@ int gf_batch_run(gf_ctx* ctx, gf_batch* batch, const gf_stage* stages, int nstag
             if (stages[i].radius > 0.0)
                 throw Error(GF_ERR_UNSUPPORTED,
                             "radius neighborhoods run through gf_run, not batches");
+            if (stages[i].stage == GF_STAGE_COPLANAR || stages[i].stage == GF_STAGE_RANK)
+                throw Error(GF_ERR_UNSUPPORTED,
+                            "approximatecoplanar and estimaterank run through gf_run");
             total_cols += stage_ncols(stages[i]);
         }
         if (total_cols && !cols) throw Error(GF_ERR_ARG, "null column array");

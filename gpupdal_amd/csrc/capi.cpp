@@ -157,6 +157,8 @@ int stage_ncols(const gf_stage& s)
     switch (s.stage) {
     case GF_STAGE_NORMAL: return 4;
     case GF_STAGE_EIGENVALUES: return 3;
+    case GF_STAGE_COPLANAR: return 1;
+    case GF_STAGE_RANK: return 1;
     case GF_STAGE_COVFEAT: return __builtin_popcount(s.features & 0x3FFu);
     default: return -1;
     }
@@ -179,9 +181,19 @@ void validate_stage(const gf_stage& s)
     }
 }
 
-void check_neighborhood(uint64_t n, int64_t knn, int stride)
+int64_t stage_neighbors(const gf_stage& s)
 {
-    const uint64_t k = std::min<uint64_t>(n, static_cast<uint64_t>(knn) + 1);
+    // approximatecoplanar and estimaterank call KD3Index::neighbors(p, knn)
+    // (ApproximateCoplanarFilter.cpp:96, EstimateRankFilter.cpp:80); the other
+    // stages ask knn + 1, the point itself included.
+    return s.stage == GF_STAGE_COPLANAR || s.stage == GF_STAGE_RANK
+               ? static_cast<int64_t>(s.knn)
+               : static_cast<int64_t>(s.knn) + 1;
+}
+
+void check_neighborhood(uint64_t n, int64_t K, int stride)
+{
+    const uint64_t k = std::min<uint64_t>(n, static_cast<uint64_t>(K));
     if (static_cast<uint64_t>(stride) * k > static_cast<uint64_t>(kMaxFetch - 1))
         throw Error(GF_ERR_UNSUPPORTED,
                     "(knn+1)*stride above 63 is outside the device kNN envelope");
@@ -197,10 +209,10 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     }
     const uint64_t n = cl.n;
     const int stride = st[0].stage == GF_STAGE_NORMAL ? 1 : st[0].stride;
-    const uint64_t K = static_cast<uint64_t>(st[0].knn) + 1;
+    const uint64_t K = static_cast<uint64_t>(stage_neighbors(st[0]));
     const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(n, K));
     const uint32_t fetch = static_cast<uint32_t>(stride) * k;
-    check_neighborhood(n, st[0].knn, stride);
+    check_neighborhood(n, stage_neighbors(st[0]), stride);
     FusedProgram prog{};
     prog.nstages = ns;
     prog.k = static_cast<int32_t>(k);
@@ -214,6 +226,8 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
         sp.features = st[i].features;
         sp.always_up = st[i].always_up;
         sp.normalize = st[i].normalize;
+        sp.p1 = st[i].p1;
+        sp.p2 = st[i].p2;
         sp.ncols = stage_ncols(st[i]);
         sp.col_offset = ncol_total;
         ncol_total += sp.ncols;
@@ -487,6 +501,8 @@ void Context::run_group_radius(Cloud& cl, const gf_stage* st, int ns, double* co
         sp.features = st[i].features;
         sp.always_up = st[i].always_up;
         sp.normalize = st[i].normalize;
+        sp.p1 = st[i].p1;
+        sp.p2 = st[i].p2;
         sp.ncols = stage_ncols(st[i]);
         sp.col_offset = ncol_total;
         ncol_total += sp.ncols;
@@ -530,6 +546,61 @@ void Context::run_group_radius(Cloud& cl, const gf_stage* st, int ns, double* co
     GF_HIP(hipStreamSynchronize(stream));
     collect_spans();
     stats.points = n;
+}
+
+void Context::knn_lists(Cloud& cl, uint32_t k, uint32_t stride, int mode, uint32_t* ids,
+                        uint8_t* status)
+{
+    const uint64_t n = cl.n;
+    const uint32_t kk = static_cast<uint32_t>(std::min<uint64_t>(n, k));
+    const uint32_t fetch = kk * stride;
+    if (fetch > static_cast<uint32_t>(kMaxFetch - 1))
+        throw Error(GF_ERR_UNSUPPORTED, "k*stride above 63 is outside the device kNN envelope");
+    if (mode == 2 && stride != 1) throw Error(GF_ERR_ARG, "sorted-position lists need stride 1");
+    points_mark = false;
+    build_index(cl, fetch);
+    FusedProgram prog{};
+    prog.k = static_cast<int32_t>(kk);
+    prog.stride = static_cast<int32_t>(stride);
+    prog.fetch = static_cast<int32_t>(fetch);
+    KnnOutputs out{};
+    out.ids = ids;
+    out.status[0] = status;
+    flagged.ensure(n * 4 + 4);
+    counters.ensure(128);
+    out.flagged = flagged.as<uint32_t>();
+    out.nflagged = counters.as<uint32_t>();
+    launch_fill_u32(out.nflagged, 4, 0u, stream);
+    const CloudView cv = view(cl);
+    const bool early = fetch <= n && !cl.kd_valid && tune.kd_overlap_min &&
+                       n >= tune.kd_overlap_min;
+    if (early) fork_side();
+    if (fetch > n)
+        launch_flag_all(n, cv.perm, mode, out, stream);
+    else
+        launch_knn(cv, prog, mode, out, knn_tuning(), stream);
+    if (early) build_kdtree(cl, side, true);
+    const uint32_t nflagged = resolve_flagged(cl, cv, prog, mode, out, nullptr);
+    if (mode == 2)
+        launch_lists_to_sorted(ids, static_cast<int>(kk), out.flagged, nflagged, cv.perm,
+                               cl.inv.as<uint32_t>(), stream);
+}
+
+// filters.nndistance (filters/NNDistanceFilter.cpp:103-133): the distance to
+// the k-th neighbor, or the mean distance to the k neighbors, from each
+// point's k + 1 nearest (itself first) in ascending distance order.
+void Context::nn_distance(Cloud& cl, uint32_t k, int mode, double* out)
+{
+    const uint64_t n = cl.n;
+    const uint32_t K = k + 1;
+    const uint32_t kk = static_cast<uint32_t>(std::min<uint64_t>(n, K));
+    listbuf.ensure(n * kk * 4 + 8);
+    liststat.ensure(n + 8);
+    knn_lists(cl, K, 1, 2, listbuf.as<uint32_t>(), liststat.as<uint8_t>());
+    const CloudView cv = view(cl);
+    launch_nn_distance(listbuf.as<uint32_t>(), static_cast<int>(kk), static_cast<int>(K), n,
+                       mode, cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), cv.sx,
+                       cv.sy, cv.sz, out, stream);
 }
 
 // ------------------------------------------------- no-allocation path ----
@@ -604,10 +675,10 @@ void Context::run_group_static(const double* x, const double* y, const double* z
                                         "neighborhoods run through gf_run");
     if (!static_scratch) throw Error(GF_ERR_STATE, "no scratch array bound on this device");
     const int stride = st[0].stage == GF_STAGE_NORMAL ? 1 : st[0].stride;
-    const uint64_t K = static_cast<uint64_t>(st[0].knn) + 1;
+    const uint64_t K = static_cast<uint64_t>(stage_neighbors(st[0]));
     const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(n, K));
     const uint32_t fetch = static_cast<uint32_t>(stride) * k;
-    check_neighborhood(n, st[0].knn, stride);
+    check_neighborhood(n, stage_neighbors(st[0]), stride);
     FusedProgram prog{};
     prog.nstages = ns;
     prog.k = static_cast<int32_t>(k);
@@ -621,6 +692,8 @@ void Context::run_group_static(const double* x, const double* y, const double* z
         sp.features = st[i].features;
         sp.always_up = st[i].always_up;
         sp.normalize = st[i].normalize;
+        sp.p1 = st[i].p1;
+        sp.p2 = st[i].p2;
         sp.ncols = stage_ncols(st[i]);
         sp.col_offset = ncol_total;
         ncol_total += sp.ncols;
@@ -927,7 +1000,7 @@ int gf_run(gf_ctx* ctx, gf_cloud* cloud, const gf_stage* stages, int nstages,
                     return a.radius == b.radius && a.min_k == b.min_k;
                 int sa = a.stage == GF_STAGE_NORMAL ? 1 : a.stride;
                 int sb = b.stage == GF_STAGE_NORMAL ? 1 : b.stride;
-                return a.knn == b.knn && sa == sb;
+                return stage_neighbors(a) == stage_neighbors(b) && sa == sb;
             };
             while (j < nstages && same(stages[i], stages[j])) {
                 ncol += stage_ncols(stages[j]);
@@ -965,7 +1038,7 @@ int gf_run_noalloc(gf_ctx* ctx, const double* x, const double* y, const double* 
                     return a.radius == b.radius && a.min_k == b.min_k;
                 int sa = a.stage == GF_STAGE_NORMAL ? 1 : a.stride;
                 int sb = b.stage == GF_STAGE_NORMAL ? 1 : b.stride;
-                return a.knn == b.knn && sa == sb;
+                return stage_neighbors(a) == stage_neighbors(b) && sa == sb;
             };
             while (j < nstages && same(stages[i], stages[j])) {
                 ncol += stage_ncols(stages[j]);
@@ -1001,6 +1074,82 @@ int gf_radius_counts(gf_ctx* ctx, gf_cloud* cloud, double radius, uint32_t* coun
     });
 }
 
+int gf_nndistance(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, int mode, double* out, int where)
+{
+    if (!ctx || !cloud || !out || (mode != GF_NND_KTH && mode != GF_NND_AVG)) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        const uint64_t n = cl->n;
+        if (!n) return;
+        double* dst = out;
+        if (where != GF_WHERE_DEVICE) {
+            c->outbuf.ensure(n * 8 + 8);
+            dst = c->outbuf.as<double>();
+        }
+        c->nn_distance(*cl, k, mode, dst);
+        if (where != GF_WHERE_DEVICE)
+            GF_HIP(hipMemcpyAsync(out, dst, n * 8, hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int gf_outlier(gf_ctx* ctx, gf_cloud* cloud, int method, uint32_t mean_k, double multiplier,
+               double radius, int32_t min_k, uint8_t* outlier, double* threshold)
+{
+    if (!ctx || !cloud || !outlier) return GF_ERR_ARG;
+    if (method != GF_OUTLIER_STATISTICAL && method != GF_OUTLIER_RADIUS) return GF_ERR_ARG;
+    if (method == GF_OUTLIER_RADIUS && (!(radius > 0.0) || radius == INFINITY)) return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        const uint64_t n = cl->n;
+        if (!n) return;
+        if (method == GF_OUTLIER_RADIUS) {
+            // processRadius (filters/OutlierFilter.cpp:80-96): an inlier has
+            // more than min_k points within radius (itself included).
+            c->points_mark = false;
+            c->build_index(*cl, 0, radius * c->tune.radius_cell);
+            c->outbuf.ensure(n * 4 + 8);
+            launch_radius(c->view(*cl), radius, 1, nullptr, c->outbuf.as<uint32_t>(),
+                          c->tune.group_z, c->stream);
+            std::vector<uint32_t> counts(n);
+            GF_HIP(hipMemcpyAsync(counts.data(), c->outbuf.ptr, n * 4, hipMemcpyDeviceToHost,
+                                  c->stream));
+            GF_HIP(hipStreamSynchronize(c->stream));
+            // ids.size() > size_t(min_k): a negative min_k converts to a
+            // count no row reaches.
+            const uint64_t need = static_cast<uint64_t>(static_cast<int64_t>(min_k));
+            for (uint64_t i = 0; i < n; ++i) outlier[i] = counts[i] > need ? 0 : 1;
+            return;
+        }
+        // processStatistical (filters/OutlierFilter.cpp:98-170): each
+        // point's mean distance to its mean_k neighbors (a running mean over
+        // them in ascending order, on the device), then the running moments
+        // over all points in point order (on the host, in that order, so the
+        // threshold is the reference's to the bit).
+        c->outbuf.ensure(n * 8 + 8);
+        c->nn_distance(*cl, mean_k, kNndRunningMean, c->outbuf.as<double>());
+        std::vector<double> d(n);
+        GF_HIP(hipMemcpyAsync(d.data(), c->outbuf.ptr, n * 8, hipMemcpyDeviceToHost, c->stream));
+        GF_HIP(hipStreamSynchronize(c->stream));
+        uint64_t cnt = 0;
+        double M1 = 0.0, M2 = 0.0;
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t n1 = cnt;
+            cnt++;
+            const double delta = d[i] - M1;
+            const double delta_n = delta / static_cast<double>(cnt);
+            M1 += delta_n;
+            M2 += delta * delta_n * static_cast<double>(n1);
+        }
+        const double variance = M2 / (static_cast<double>(cnt) - 1.0);
+        const double thr = M1 + multiplier * std::sqrt(variance);
+        if (threshold) *threshold = thr;
+        for (uint64_t i = 0; i < n; ++i) outlier[i] = d[i] < thr ? 0 : 1;
+    });
+}
+
 int gf_last_stats(const gf_ctx* ctx, gf_stats* out)
 {
     if (!ctx || !out) return GF_ERR_ARG;
@@ -1018,44 +1167,18 @@ int gf_knn(gf_ctx* ctx, gf_cloud* cloud, uint32_t k, uint32_t stride, uint32_t* 
         const uint64_t n = cl->n;
         if (!n) return;
         const uint32_t kk = static_cast<uint32_t>(std::min<uint64_t>(n, k));
-        const uint32_t fetch = kk * stride;
-        if (fetch > static_cast<uint32_t>(kMaxFetch - 1))
-            throw Error(GF_ERR_UNSUPPORTED, "k*stride above 63 is outside the device kNN envelope");
-        c->points_mark = false;
-        c->build_index(*cl, fetch);
-        FusedProgram prog{};
-        prog.k = static_cast<int32_t>(kk);
-        prog.stride = static_cast<int32_t>(stride);
-        prog.fetch = static_cast<int32_t>(fetch);
-        KnnOutputs out{};
-        DevBuf tmp_ids, tmp_st;
-        if (where == GF_WHERE_DEVICE) {
-            out.ids = ids;
-            out.status[0] = status;
-        } else {
-            tmp_ids.ensure(n * kk * 4);
-            tmp_st.ensure(n + 8);
-            out.ids = tmp_ids.as<uint32_t>();
-            out.status[0] = tmp_st.as<uint8_t>();
+        uint32_t* dids = ids;
+        uint8_t* dst = status;
+        if (where != GF_WHERE_DEVICE) {
+            c->listbuf.ensure(n * kk * 4 + 8);
+            c->liststat.ensure(n + 8);
+            dids = c->listbuf.as<uint32_t>();
+            dst = c->liststat.as<uint8_t>();
         }
-        c->flagged.ensure(n * 4 + 4);
-        c->counters.ensure(64);
-        out.flagged = c->flagged.as<uint32_t>();
-        out.nflagged = c->counters.as<uint32_t>();
-        launch_fill_u32(out.nflagged, 4, 0u, c->stream);
-        const CloudView cv = c->view(*cl);
-        const bool early = fetch <= n && !cl->kd_valid && c->tune.kd_overlap_min &&
-                           n >= c->tune.kd_overlap_min;
-        if (early) c->fork_side();
-        if (fetch > n)
-            launch_flag_all(n, cv.perm, 1, out, c->stream);
-        else
-            launch_knn(cv, prog, 1, out, c->knn_tuning(), c->stream);
-        if (early) c->build_kdtree(*cl, c->side, true);
-        c->resolve_flagged(*cl, cv, prog, 1, out, nullptr);
-        if (where == GF_WHERE_HOST) {
-            GF_HIP(hipMemcpyAsync(ids, out.ids, n * kk * 4, hipMemcpyDeviceToHost, c->stream));
-            GF_HIP(hipMemcpyAsync(status, out.status[0], n, hipMemcpyDeviceToHost, c->stream));
+        c->knn_lists(*cl, k, stride, 1, dids, dst);
+        if (where != GF_WHERE_DEVICE) {
+            GF_HIP(hipMemcpyAsync(ids, dids, n * kk * 4, hipMemcpyDeviceToHost, c->stream));
+            GF_HIP(hipMemcpyAsync(status, dst, n, hipMemcpyDeviceToHost, c->stream));
         }
         GF_HIP(hipStreamSynchronize(c->stream));
     });
@@ -1433,7 +1556,7 @@ int gf_autotune(gf_ctx* ctx, gf_cloud* cloud, const gf_stage* stages, int nstage
         const double rho = static_cast<double>(n) / (std::max(sx, tiny) * std::max(sy, tiny));
         const int stride = stages[0].stage == GF_STAGE_NORMAL ? 1 : stages[0].stride;
         const uint32_t fetch = static_cast<uint32_t>(stride) *
-                               static_cast<uint32_t>(std::min<uint64_t>(n, stages[0].knn + 1));
+                               static_cast<uint32_t>(std::min<uint64_t>(n, stage_neighbors(stages[0])));
         c->tune.remember(rho, fetch, bestf);
         cl->idx_valid = false;
         cl->kd_valid = false;

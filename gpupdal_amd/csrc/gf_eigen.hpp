@@ -222,8 +222,34 @@ __device__ inline uint8_t stage_outputs(const StageProgram& sp, int k,
         vals[3] = curv;
         return 0;
     }
+    if (sp.stage == STAGE_RANK) {
+        // math::computeRank (pdal/private/MathUtils.cpp:197-208): Eigen's
+        // JacobiSVD::rank() with the threshold narrowed to float -- the
+        // singular values of the symmetric covariance are its eigenvalues'
+        // magnitudes; count those (nonzero) at or above max(thresh * largest,
+        // DBL_MIN).  No isZero skip: a zero matrix has rank 0.
+        if (info) return ST_EIGEN_FAIL;
+        double a = fabs(ev_in[0]), b = fabs(ev_in[1]), c = fabs(ev_in[2]);
+        double t;
+        if (a < b) { t = a; a = b; b = t; }
+        if (b < c) { t = b; b = c; c = t; }
+        if (a < b) { t = a; a = b; b = t; }
+        const double thr = static_cast<double>(static_cast<float>(sp.p1));
+        const double pre = fmax(a * thr, 2.2250738585072014e-308);
+        int r = 0;
+        if (a != 0.0 && a >= pre) r = 1;
+        if (r == 1 && b != 0.0 && b >= pre) r = 2;
+        if (r == 2 && c != 0.0 && c >= pre) r = 3;
+        vals[0] = static_cast<double>(r);
+        return 0;
+    }
     if (zero) return ST_ZERO_COV;
     if (info) return ST_EIGEN_FAIL;
+    if (sp.stage == STAGE_COPLANAR) {
+        // ApproximateCoplanarFilter.cpp:117-121, eigenvalues ascending.
+        vals[0] = (ev_in[1] > sp.p1 * ev_in[0] && sp.p2 * ev_in[1] > ev_in[2]) ? 1.0 : 0.0;
+        return 0;
+    }
     if (sp.stage == STAGE_EIGENVALUES) {
         double e0 = ev_in[0], e1 = ev_in[1], e2 = ev_in[2];
         if (sp.normalize) {

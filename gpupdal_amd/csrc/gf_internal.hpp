@@ -28,7 +28,9 @@ enum : uint8_t {
 };
 constexpr uint8_t kReportedSearchFlags = ST_TIE | ST_INCOMPLETE;
 
-enum : int32_t { STAGE_COVFEAT = 0, STAGE_NORMAL = 1, STAGE_EIGENVALUES = 2 };
+enum : int32_t {
+    STAGE_COVFEAT = 0, STAGE_NORMAL = 1, STAGE_EIGENVALUES = 2, STAGE_COPLANAR = 3, STAGE_RANK = 4
+};
 enum : int32_t { MODE_RAW = 0, MODE_SQRT = 1, MODE_NORMALIZED = 2 };
 
 constexpr int kWave = 64;
@@ -36,6 +38,11 @@ constexpr int kMaxStages = 4;
 constexpr int kMaxFetch = 64;     // K*stride held by the wave-wide top list
 // Output columns of one fused run: four stages write at most 10 + 10 + 4 + 3.
 constexpr int kMaxOutColumns = 32;
+// Per-row distance statistics of a kNN list: the public GF_NND_KTH and
+// GF_NND_AVG, and the outlier filter's running mean.
+constexpr int kNndKth = 0;
+constexpr int kNndAvg = 1;
+constexpr int kNndRunningMean = 2;
 
 // Uniform column grid over one cloud (or one tile of a batch).  Columns are
 // (cx, cy) cells of edge `c`, stored in 8x8 tiles (row-major tiles, Morton
@@ -63,6 +70,7 @@ struct StageProgram {
     int32_t ncols;       // output columns of this stage
     int32_t col_offset;  // first output column
     int32_t pad;
+    double p1, p2;       // approximatecoplanar thresh1/thresh2; estimaterank thresh
 };
 
 struct FusedProgram {

@@ -236,6 +236,12 @@ void launch_optimal_nbhd(const uint32_t* ids, uint64_t n, int kmin, int kmax, co
                          const double* y, const double* z, const double* sx, const double* sy,
                          const double* sz, uint64_t* kopt, double* ropt, uint32_t* failed,
                          hipStream_t s);
+// nndistance (kNndKth, kNndAvg) and the outlier filter's running mean
+// distance (kNndRunningMean) per row from n x kk lists of sorted positions of
+// the K = k + 1 nearest points (kk = min(n, K)); rows in original order.
+void launch_nn_distance(const uint32_t* ids, int kk, int K, uint64_t n, int mode,
+                        const double* x, const double* y, const double* z, const double* sx,
+                        const double* sy, const double* sz, double* out, hipStream_t s);
 // The flagged rows' lists (point ids from the exact re-solve) to sorted
 // positions.
 void launch_lists_to_sorted(uint32_t* ids, int kmax, const uint32_t* flagged, uint32_t nflagged,

@@ -207,9 +207,12 @@ GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch, const Tuning
                      double cell = 0.0);
 int stage_ncols(const gf_stage& s);
 void validate_stage(const gf_stage& s);
-// Throws GF_ERR_UNSUPPORTED when a cloud (or tile) of n points with this
-// knn and stride is outside what the device search handles.
-void check_neighborhood(uint64_t n, int64_t knn, int stride);
+// Throws GF_ERR_UNSUPPORTED when a cloud (or tile) of n points whose search
+// asks K neighbors with this stride is outside what the device search handles.
+void check_neighborhood(uint64_t n, int64_t K, int stride);
+// Neighbors a stage's search asks (knn + 1, or knn for approximatecoplanar
+// and estimaterank).
+int64_t stage_neighbors(const gf_stage& s);
 
 // A device-time interval between two events, read after the call's final
 // synchronisation (timing adds no host wait between kernels).
@@ -239,6 +242,7 @@ struct Context {
     DevBuf part, bounds, hist, scan_tmp, outbuf, statbuf, flagged, counters, covbuf;
     DevBuf xyz_stage;  // interleaved copy of the points between the keys and the gather
     DevBuf lasbuf;     // LAS point records (decode input, extra-bytes packing)
+    DevBuf listbuf, liststat;  // kNN lists (gf_knn host copies, nndistance, outlier)
     // Exact-tree scratch, its own: the tree may be built on the side stream
     // while the index build uses scan_tmp on the main stream.
     DevBuf kd_s1, kd_s2, kd_tmp, kd_lists, kd_sums;
@@ -295,6 +299,14 @@ struct Context {
     // Makes the side stream wait for the points and bounds (points_mark), or
     // for all work queued on `stream` so far.
     void fork_side();
+    // KD3Index::neighbors(id, k, stride) lists of every point (n x min(n, k),
+    // rows in original order) and each row's status; mode 1 writes point
+    // ids, mode 2 sorted positions (stride 1 only).  Queued on the stream.
+    void knn_lists(Cloud& cl, uint32_t k, uint32_t stride, int mode, uint32_t* ids,
+                   uint8_t* status);
+    // Per-row distance statistic (kNndKth, kNndAvg, kNndRunningMean) of the
+    // k + 1 nearest points into out (device, original order).
+    void nn_distance(Cloud& cl, uint32_t k, int mode, double* out);
     // Returns the number of flagged rows.
     uint32_t resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog,
                              int mode, const KnnOutputs& out, hipEvent_t from);

@@ -1062,7 +1062,11 @@ __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov
     const bool zero = cov_is_zero(c);
     double ev[3] = {0, 0, 0}, vec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int info = 0;
-    if (!zero) info = eigen_sym3(c, ev, vec);
+    // estimaterank has no isZero skip: it takes the singular values of any
+    // covariance.
+    bool rank_stage = false;
+    for (int si = 0; si < prog.nstages; ++si) rank_stage |= prog.st[si].stage == STAGE_RANK;
+    if (!zero || rank_stage) info = eigen_sym3(c, ev, vec);
     for (int si = 0; si < prog.nstages; ++si) {
         const StageProgram& sp = prog.st[si];
         double vals[10];

@@ -1,3 +1,5 @@
+// Consumers of whole kNN lists: optimalneighborhood, nndistance and the
+// outlier filter's mean distances.
 // filters.optimalneighborhood: for each point, the k in [min_k, max_k] whose
 // neighborhood has the smallest eigenentropy (filters/OptimalNeighborhoodFilter.cpp:
 // 72-163).  Input: every row's max_k neighbors in KD3Index order (the
@@ -98,7 +100,61 @@ __global__ __launch_bounds__(256) void k_optimal_nbhd(const uint32_t* __restrict
     ropt_out[i] = sqrt(ropt);
 }
 
+// filters.nndistance (filters/NNDistanceFilter.cpp:113-132) and the
+// per-point mean distance of filters.outlier's statistical method
+// (filters/OutlierFilter.cpp:122-136), from each row's list of the K = k + 1
+// nearest points as sorted positions.  KD3Index::knnSearch returns the
+// squared distances in ascending order, which do not depend on how ties
+// between equal distances were ordered; with fewer than K points in the cloud
+// its tail entries stay 0 (:119-120, KDImpl.hpp:386).
+__global__ __launch_bounds__(256) void k_nn_distance(const uint32_t* __restrict__ ids, int kk,
+                                                     int K, uint64_t n, int mode,
+                                                     const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     const double* __restrict__ z,
+                                                     const double* __restrict__ sx,
+                                                     const double* __restrict__ sy,
+                                                     const double* __restrict__ sz,
+                                                     double* __restrict__ out)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* nb = ids + i * static_cast<uint64_t>(kk);
+    const double qx = x[i], qy = y[i], qz = z[i];
+    // nanoflann's squared distance: query minus point, per axis, summed in
+    // axis order.
+    auto dist = [&](int j) -> double {
+        if (j >= kk) return 0.0;
+        const uint32_t p = nb[j];
+        const double ex = qx - sx[p], ey = qy - sy[p], ez = qz - sz[p];
+        return std::sqrt(ex * ex + ey * ey + ez * ez);
+    };
+    double val = 0.0;
+    if (mode == kNndKth) {
+        val = dist(K - 1);
+    } else if (mode == kNndAvg) {
+        for (int j = 1; j < K; ++j) val += dist(j);
+        val /= static_cast<double>(K - 1);
+    } else {
+        // Running mean over the neighbors in ascending order.
+        for (int j = 1; j < K; ++j) {
+            const double delta = dist(j) - val;
+            val += delta / static_cast<double>(j);
+        }
+    }
+    out[i] = val;
+}
+
 } // namespace
+
+void launch_nn_distance(const uint32_t* ids, int kk, int K, uint64_t n, int mode,
+                        const double* x, const double* y, const double* z, const double* sx,
+                        const double* sy, const double* sz, double* out, hipStream_t s)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_nn_distance, dim3(static_cast<uint32_t>(div_up(n, 256))), dim3(256), 0,
+                       s, ids, kk, K, n, mode, x, y, z, sx, sy, sz, out);
+}
 
 void launch_lists_to_sorted(uint32_t* ids, int kmax, const uint32_t* flagged, uint32_t nflagged,
                             const uint32_t* perm, const uint32_t* inv, hipStream_t s)

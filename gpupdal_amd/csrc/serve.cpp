@@ -302,6 +302,8 @@ int gf_server_submit(gf_server* server, const double* x, const double* y, const 
             return e.code;
         }
         if (r->st[i].radius > 0.0) return GF_ERR_UNSUPPORTED;  // served stages use knn
+        if (r->st[i].stage == GF_STAGE_COPLANAR || r->st[i].stage == GF_STAGE_RANK)
+            return GF_ERR_UNSUPPORTED;
         if (i && (r->st[i].stage == GF_STAGE_NORMAL ? 1 : r->st[i].stride) !=
                      (r->st[0].stage == GF_STAGE_NORMAL ? 1 : r->st[0].stride))
             return GF_ERR_ARG;  // the stages of one request share one neighborhood
@@ -309,7 +311,8 @@ int gf_server_submit(gf_server* server, const double* x, const double* y, const 
     }
     if (ncols > kMaxOutColumns) return GF_ERR_UNSUPPORTED;
     try {
-        check_neighborhood(n, knn, r->st[0].stage == GF_STAGE_NORMAL ? 1 : r->st[0].stride);
+        check_neighborhood(n, static_cast<int64_t>(knn) + 1,
+                           r->st[0].stage == GF_STAGE_NORMAL ? 1 : r->st[0].stride);
     } catch (const Error& e) {
         return e.code;
     }

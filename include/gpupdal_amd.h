@@ -42,6 +42,16 @@ extern "C" {
 #define GF_STAGE_COVFEAT 0
 #define GF_STAGE_NORMAL 1
 #define GF_STAGE_EIGENVALUES 2
+/* filters.approximatecoplanar (filters/ApproximateCoplanarFilter.cpp:67-124):
+ * one column, Coplanar 1/0, from ev[1] > thresh1 * ev[0] && thresh2 * ev[1] >
+ * ev[2] with thresh1 = p1 (25), thresh2 = p2 (6); the search asks knn (the
+ * point itself included), not knn + 1. */
+#define GF_STAGE_COPLANAR 3
+/* filters.estimaterank (filters/EstimateRankFilter.cpp:62-84,
+ * pdal/private/MathUtils.cpp:197-208): one column, Rank, the number of the
+ * covariance's singular values at or above (float)thresh times the largest,
+ * thresh = p1 (0.01); the search asks knn. */
+#define GF_STAGE_RANK 4
 
 #define GF_MODE_RAW 0
 #define GF_MODE_SQRT 1
@@ -92,6 +102,8 @@ typedef struct gf_stage {
     double radius;     /* > 0: the neighborhood is every point within this
                           distance (KD3Index::radius, strict), knn and stride
                           are ignored -- the stages' `radius` option */
+    double p1, p2;     /* stage parameters: approximatecoplanar thresh1 and
+                          thresh2, estimaterank thresh (p1) */
 } gf_stage;
 
 typedef struct gf_stats {
@@ -266,6 +278,29 @@ int gf_memcpy(gf_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind)
  * does not converge: GF_ERR_UNSUPPORTED with PDAL's message. */
 int gf_optimal_neighborhood(gf_ctx *ctx, gf_cloud *cloud, uint32_t min_k, uint32_t max_k,
                             uint64_t *optimal_knn, double *optimal_radius, int where);
+
+/* filters.nndistance (filters/NNDistanceFilter.cpp:97-133): per point, from
+ * its k + 1 nearest points (itself included, KD3Index::knnSearch) in
+ * ascending distance: GF_NND_KTH the distance to the last of them, GF_NND_AVG
+ * the mean distance to all but the first.  out: n FP64 (NNDistance).  With
+ * fewer than k + 1 points the missing distances count as 0, as in PDAL. */
+#define GF_NND_KTH 0
+#define GF_NND_AVG 1
+int gf_nndistance(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, int mode, double *out, int where);
+
+/* filters.outlier (filters/OutlierFilter.cpp:78-172).  GF_OUTLIER_STATISTICAL
+ * (mean_k, multiplier): each point's running mean distance to its mean_k
+ * nearest neighbors; the mean and sample standard deviation of those over the
+ * cloud (online moments in point order) give threshold = mean +
+ * multiplier * stdev, and a point at or above it is an outlier.
+ * GF_OUTLIER_RADIUS (radius, min_k): a point with at most min_k points
+ * within radius (itself included) is an outlier.  outlier: n host bytes,
+ * 1 = outlier (PDAL gives these the noise class); threshold (host, may be
+ * NULL) receives the statistical threshold. */
+#define GF_OUTLIER_STATISTICAL 0
+#define GF_OUTLIER_RADIUS 1
+int gf_outlier(gf_ctx *ctx, gf_cloud *cloud, int method, uint32_t mean_k, double multiplier,
+               double radius, int32_t min_k, uint8_t *outlier, double *threshold);
 
 /* ---- LAS point records in and out (the on-disk format either side of the
  * path).  Replaces GPUPDAL's device LAS hydration, src/pdal/PdgNeighborhood.cpp:

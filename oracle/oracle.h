@@ -102,7 +102,9 @@ int orc_eigen_sym3(const double cov[9], double evals[3], double evecs[9]);
 enum {
     ORC_STAGE_COVFEAT = 0,
     ORC_STAGE_NORMAL = 1,
-    ORC_STAGE_EIGENVALUES = 2
+    ORC_STAGE_EIGENVALUES = 2,
+    ORC_STAGE_COPLANAR = 3, /* filters.approximatecoplanar (neighbors(p, knn)) */
+    ORC_STAGE_RANK = 4      /* filters.estimaterank (neighbors(p, knn))       */
 };
 enum { ORC_MODE_RAW = 0, ORC_MODE_SQRT = 1, ORC_MODE_NORMALIZED = 2 };
 
@@ -141,7 +143,13 @@ typedef struct {
     int32_t min_k;     /* radius mode, covfeat/eigenvalues: fewer neighbors -> skip */
     int32_t pad;
     double radius;     /* > 0: neighbors within this radius (knn, stride ignored) */
+    double p1, p2;     /* approximatecoplanar thresh1/thresh2; estimaterank thresh */
 } orc_stage;
+
+/* Eigen's JacobiSVD::rank() of a symmetric PSD 3x3 from its eigenvalues (the
+ * singular values are their magnitudes), threshold narrowed to float as
+ * math::computeRank does (pdal/private/MathUtils.cpp:197-208). */
+int orc_rank_from_eigenvalues(const double ev[3], double thresh);
 
 /* Number of output columns a stage writes, in PDAL's registration order. */
 int orc_stage_columns(const orc_stage *st);
@@ -209,6 +217,19 @@ void orc_truth_from_lists(const double *x, const double *y, const double *z,
 uint64_t orc_optimal_neighborhood(const orc_kd3 *t, const double *x, const double *y,
                                   const double *z, uint64_t n, uint64_t kmin, uint64_t kmax,
                                   int threads, uint64_t *kopt, double *ropt, double *gap);
+
+/* filters.nndistance per point from its k + 1 nearest (mode 0 kth, 1 avg),
+ * or mode 2: filters.outlier's statistical running mean distance. */
+void orc_nn_distance(const orc_kd3 *t, const double *x, const double *y, const double *z,
+                     uint64_t n, uint64_t k, int mode, int threads, double *out);
+/* filters.outlier, statistical and radius: outlier[i] = 1 for an outlier;
+ * returns the number of outliers.  threshold and dist (n) may be NULL. */
+uint64_t orc_outlier_statistical(const orc_kd3 *t, const double *x, const double *y,
+                                 const double *z, uint64_t n, uint64_t mean_k,
+                                 double multiplier, int threads, uint8_t *outlier,
+                                 double *threshold, double *dist);
+uint64_t orc_outlier_radius(const orc_kd3 *t, uint64_t n, double r, int64_t min_k,
+                            int threads, uint8_t *outlier);
 
 /* Extended-precision symmetric 3x3 eigensolve (cyclic Jacobi): ascending
  * values, column-major vectors.  Returns the sweep count. */

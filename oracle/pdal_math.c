@@ -251,6 +251,8 @@ int orc_stage_columns(const orc_stage *st)
     switch (st->stage) {
     case ORC_STAGE_NORMAL: return 4;
     case ORC_STAGE_EIGENVALUES: return 3;
+    case ORC_STAGE_COPLANAR: return 1;
+    case ORC_STAGE_RANK: return 1;
     default: {
         int c = 0;
         for (int b = 0; b < 10; ++b)
@@ -336,11 +338,46 @@ static void cov_u32(const double *x, const double *y, const double *z,
     orc_pdal_covariance_u32(x, y, z, (const uint32_t *)ids, n, cov);
 }
 
+int orc_rank_from_eigenvalues(const double ev[3], double thresh)
+{
+    /* SVDBase::rank (Eigen/src/SVD/SVDBase.h:148-157): singular values in
+     * decreasing order; the leading ones at or above max(thresh * largest,
+     * DBL_MIN), among the nonzero ones. */
+    double s[3] = {fabs(ev[0]), fabs(ev[1]), fabs(ev[2])};
+    for (int i = 0; i < 3; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (s[j] > s[i]) {
+                double t = s[i];
+                s[i] = s[j];
+                s[j] = t;
+            }
+    const double thr = (double)(float)thresh;
+    const double pre = fmax(s[0] * thr, DBL_MIN);
+    int r = 0;
+    while (r < 3 && s[r] != 0.0 && s[r] >= pre) ++r;
+    return r;
+}
+
 static uint8_t stage_row(const orc_stage *st, const double *x, const double *y,
                          const double *z, const void *ids, uint64_t cnt,
                          cov_fn covf, double *vals)
 {
     double cov[9], ev[3], vec[9];
+    if (st->stage == ORC_STAGE_RANK) {
+        /* EstimateRankFilter.cpp:77-83: computeRank, no isZero skip. */
+        covf(x, y, z, ids, cnt, cov);
+        if (orc_eigen_sym3(cov, ev, vec) != 0) return ORC_ST_EIGEN_FAIL;
+        vals[0] = (double)orc_rank_from_eigenvalues(ev, st->p1);
+        return 0;
+    }
+    if (st->stage == ORC_STAGE_COPLANAR) {
+        /* ApproximateCoplanarFilter.cpp:90-121. */
+        covf(x, y, z, ids, cnt, cov);
+        if (orc_is_zero(cov)) return ORC_ST_ZERO_COV;
+        if (orc_eigen_sym3(cov, ev, vec) != 0) return ORC_ST_EIGEN_FAIL;
+        vals[0] = (ev[1] > st->p1 * ev[0] && st->p2 * ev[1] > ev[2]) ? 1.0 : 0.0;
+        return 0;
+    }
     if (st->stage == ORC_STAGE_NORMAL) {
         /* math::findNormal (MathUtils.cpp:572-611). */
         if (cnt < 3) return ORC_ST_FEW_NEIGHBORS;
@@ -401,7 +438,10 @@ static void *stage_worker(void *arg)
 {
     stage_job *j = arg;
     const orc_stage *st = j->st;
-    uint64_t K = (uint64_t)st->knn + 1;
+    /* approximatecoplanar and estimaterank ask neighbors(p, knn); the other
+     * stages knn + 1. */
+    uint64_t K = (uint64_t)st->knn +
+                 ((st->stage == ORC_STAGE_COPLANAR || st->stage == ORC_STAGE_RANK) ? 0 : 1);
     uint64_t stride = st->stage == ORC_STAGE_NORMAL ? 1 : (uint64_t)st->stride;
     if (stride < 1) stride = 1;
     uint64_t kc = K < j->n ? K : j->n;
@@ -679,9 +719,18 @@ static uint8_t truth_row(const orc_stage *st, const double *x, const double *y,
             c[a][b] /= (ld)(cnt - 1);
             if (fabsl(c[a][b]) > 1e-12L) allzero = 0;
         }
-    if (allzero) return ORC_ST_ZERO_COV;
+    if (allzero && st->stage != ORC_STAGE_RANK) return ORC_ST_ZERO_COV;
     ld ev[3], vec[3][3];
     jacobi_sorted(c, ev, vec);
+    if (st->stage == ORC_STAGE_RANK) {
+        double evd[3] = {(double)ev[0], (double)ev[1], (double)ev[2]};
+        vals[0] = (double)orc_rank_from_eigenvalues(evd, st->p1);
+        return 0;
+    }
+    if (st->stage == ORC_STAGE_COPLANAR) {
+        vals[0] = (ev[1] > (ld)st->p1 * ev[0] && (ld)st->p2 * ev[1] > ev[2]) ? 1.0 : 0.0;
+        return 0;
+    }
     if (gap) {
         ld big = fabsl(ev[2]) > 0 ? fabsl(ev[2]) : 1.0L;
         ld g1 = ev[1] - ev[0], g2 = ev[2] - ev[1];
@@ -901,4 +950,120 @@ uint64_t orc_optimal_neighborhood(const orc_kd3 *t, const double *x, const doubl
     free(th);
     free(jobs);
     return failed;
+}
+
+/* ------------------------------------------- kNN distance consumers --- */
+/* filters.nndistance (NNDistanceFilter.cpp:113-132): K = k + 1 squared
+ * distances from KD3Index::knnSearch, zero-initialised so that entries past
+ * the cloud's size stay 0 (KDImpl.hpp:383-396 clamps k to the point count);
+ * kth = sqrt(d2[K-1]), avg = sum_{i=1}^{K-1} sqrt(d2[i]) / (K - 1).
+ * mode 2 is OutlierFilter::processStatistical's per-point running mean
+ * (OutlierFilter.cpp:122-136): dist += (sqrt(d2[j]) - dist) / j. */
+typedef struct {
+    const orc_kd3 *t;
+    const double *x, *y, *z;
+    uint64_t k;
+    int mode;
+    uint64_t b, e;
+    double *out;
+} nnd_job;
+
+static void *nnd_worker(void *arg)
+{
+    nnd_job *jb = (nnd_job *)arg;
+    const uint64_t K = jb->k + 1;
+    uint64_t *ids = malloc(sizeof(uint64_t) * (size_t)K);
+    double *d2 = malloc(sizeof(double) * (size_t)K);
+    for (uint64_t i = jb->b; i < jb->e; ++i) {
+        memset(d2, 0, sizeof(double) * (size_t)K);
+        orc_kd3_neighbors(jb->t, jb->x[i], jb->y[i], jb->z[i], K, 1, ids, d2);
+        double val = 0.0;
+        if (jb->mode == 0) {
+            val = sqrt(d2[K - 1]);
+        } else if (jb->mode == 1) {
+            for (uint64_t q = 1; q < K; ++q) val += sqrt(d2[q]);
+            val /= (double)(K - 1);
+        } else {
+            for (uint64_t q = 1; q < K; ++q) {
+                double delta = sqrt(d2[q]) - val;
+                val += delta / (double)q;
+            }
+        }
+        jb->out[i] = val;
+    }
+    free(ids);
+    free(d2);
+    return NULL;
+}
+
+void orc_nn_distance(const orc_kd3 *t, const double *x, const double *y, const double *z,
+                     uint64_t n, uint64_t k, int mode, int threads, double *out)
+{
+    if (threads < 1) threads = 1;
+    uint64_t chunk = n / (uint64_t)threads + ((n % (uint64_t)threads) ? 1 : 0);
+    pthread_t *th = malloc(sizeof(pthread_t) * (size_t)threads);
+    nnd_job *jobs = malloc(sizeof(nnd_job) * (size_t)threads);
+    for (int i = 0; i < threads; ++i) {
+        uint64_t b = (uint64_t)i * chunk, e = (i + 1 == threads) ? n : (uint64_t)(i + 1) * chunk;
+        if (b > n) b = n;
+        if (e > n) e = n;
+        jobs[i] = (nnd_job){t, x, y, z, k, mode, b, e, out};
+        if (threads == 1)
+            nnd_worker(&jobs[i]);
+        else
+            pthread_create(&th[i], NULL, nnd_worker, &jobs[i]);
+    }
+    if (threads > 1)
+        for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    free(th);
+    free(jobs);
+}
+
+/* OutlierFilter::processStatistical (OutlierFilter.cpp:98-170): the online
+ * moments over the per-point distances in point order, variance M2 / (n - 1),
+ * threshold = mean + multiplier * stdev; outlier iff distance >= threshold
+ * (the inlier test is distance < threshold). */
+uint64_t orc_outlier_statistical(const orc_kd3 *t, const double *x, const double *y,
+                                 const double *z, uint64_t n, uint64_t mean_k,
+                                 double multiplier, int threads, uint8_t *outlier,
+                                 double *threshold, double *dist)
+{
+    double *d = dist ? dist : malloc(sizeof(double) * (size_t)(n ? n : 1));
+    orc_nn_distance(t, x, y, z, n, mean_k, 2, threads, d);
+    uint64_t cnt = 0;
+    double M1 = 0.0, M2 = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+        uint64_t n1 = cnt;
+        cnt++;
+        double delta = d[i] - M1;
+        double delta_n = delta / (double)cnt;
+        M1 += delta_n;
+        M2 += delta * delta_n * (double)n1;
+    }
+    double variance = M2 / ((double)cnt - 1.0);
+    double thr = M1 + multiplier * sqrt(variance);
+    uint64_t nout = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        outlier[i] = d[i] < thr ? 0 : 1;
+        nout += outlier[i];
+    }
+    if (threshold) *threshold = thr;
+    if (!dist) free(d);
+    return nout;
+}
+
+/* OutlierFilter::processRadius (OutlierFilter.cpp:78-96): inlier iff
+ * |KD3Index::radius(i, r)| > size_t(min_k). */
+uint64_t orc_outlier_radius(const orc_kd3 *t, uint64_t n, double r, int64_t min_k,
+                            int threads, uint8_t *outlier)
+{
+    uint32_t *counts = malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+    orc_kd3_radius_counts(t, r, threads, counts);
+    uint64_t nout = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        outlier[i] = (uint64_t)counts[i] > (uint64_t)min_k ? 0 : 1;
+        nout += outlier[i];
+    }
+    free(counts);
+    return nout;
 }

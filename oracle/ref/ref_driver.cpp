@@ -127,6 +127,18 @@ uint64_t ref_kd3_radius(void* hv, double qx, double qy, double qz, double r, uin
     return count;
 }
 
+// math::computeRank (pdal/private/MathUtils.cpp:197-208) on a given 3x3:
+// Eigen's JacobiSVD with the threshold narrowed to float.
+int ref_jacobi_rank(const double* cov, double threshold)
+{
+    Eigen::Matrix3d B;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) B(i, j) = cov[3 * i + j];
+    Eigen::JacobiSVD<Eigen::Matrix3d> svd(B);
+    svd.setThreshold((float)threshold);
+    return static_cast<int>(svd.rank());
+}
+
 uint64_t ref_kd3_node_count(void* hv)
 {
     Handle* h = static_cast<Handle*>(hv);

@@ -19,7 +19,7 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 
-STAGE_COVFEAT, STAGE_NORMAL, STAGE_EIGENVALUES = 0, 1, 2
+STAGE_COVFEAT, STAGE_NORMAL, STAGE_EIGENVALUES, STAGE_COPLANAR, STAGE_RANK = 0, 1, 2, 3, 4
 MODE_RAW, MODE_SQRT, MODE_NORMALIZED = 0, 1, 2
 F_LINEARITY = 1 << 0
 F_PLANARITY = 1 << 1
@@ -46,15 +46,23 @@ class Stage(ctypes.Structure):
                 ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("features", ctypes.c_uint32), ("always_up", ctypes.c_int32),
                 ("normalize", ctypes.c_int32), ("min_k", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("radius", ctypes.c_double)]
+                ("pad", ctypes.c_int32), ("radius", ctypes.c_double),
+                ("p1", ctypes.c_double), ("p2", ctypes.c_double)]
 
 
 def stage(kind, knn, stride=1, mode=MODE_SQRT, features=F_DIMENSIONALITY,
-          always_up=1, normalize=0, radius=0.0, min_k=3):
-    return Stage(kind, knn, stride, mode, features, always_up, normalize, min_k, 0, radius)
+          always_up=1, normalize=0, radius=0.0, min_k=3, p1=None, p2=None):
+    if p1 is None:
+        p1 = {STAGE_COPLANAR: 25.0, STAGE_RANK: 0.01}.get(kind, 0.0)
+    if p2 is None:
+        p2 = 6.0 if kind == STAGE_COPLANAR else 0.0
+    return Stage(kind, knn, stride, mode, features, always_up, normalize, min_k, 0, radius,
+                 p1, p2)
 
 
 def stage_columns(st):
+    if st.stage in (STAGE_COPLANAR, STAGE_RANK):
+        return 1
     if st.stage == STAGE_NORMAL:
         return 4
     if st.stage == STAGE_EIGENVALUES:
@@ -124,6 +132,8 @@ class Oracle:
                                            _u32p, ctypes.c_uint64, sp,
                                            ctypes.c_int, ctypes.POINTER(_dp),
                                            _u8p, _dp, _dp]
+        L.orc_rank_from_eigenvalues.restype = ctypes.c_int
+        L.orc_rank_from_eigenvalues.argtypes = [_dp, ctypes.c_double]
         L.orc_kd3_radius.restype = ctypes.c_uint64
         L.orc_kd3_radius.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
@@ -134,6 +144,15 @@ class Oracle:
                                               _u64p, ctypes.c_uint64, _u32p]
         L.orc_truth_from_csr.argtypes = [_dp, _dp, _dp, ctypes.c_uint64, _u64p, _u32p, sp,
                                          ctypes.c_int, ctypes.POINTER(_dp), _u8p, _dp, _dp]
+        L.orc_nn_distance.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_uint64,
+                                      ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_outlier_statistical.restype = ctypes.c_uint64
+        L.orc_outlier_statistical.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_uint64,
+                                              ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
+                                              _u8p, _dp, _dp]
+        L.orc_outlier_radius.restype = ctypes.c_uint64
+        L.orc_outlier_radius.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double,
+                                         ctypes.c_int64, ctypes.c_int, _u8p]
         self.L = L
 
     # --------------------------------------------------------------- KD3
@@ -237,6 +256,10 @@ class Oracle:
         self.L.orc_pdal_covariance(_ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp),
                                    _ptr(ids, _u64p), len(ids), _ptr(cov, _dp))
         return cov.reshape(3, 3)
+
+    def rank_from_eigenvalues(self, ev, thresh):
+        e = _c(np.asarray(ev, np.float64).reshape(3), np.float64)
+        return self.L.orc_rank_from_eigenvalues(_ptr(e, _dp), thresh)
 
     def is_zero(self, cov):
         c = _c(np.asarray(cov).reshape(9), np.float64)
@@ -365,6 +388,41 @@ def _truth_from_csr(orc, x, y, z, offsets, ids, st, threads=8, init=np.nan, with
 
 Oracle.truth_from_csr = _truth_from_csr
 
+NND_KTH, NND_AVG, NND_RUNNING_MEAN = 0, 1, 2
+
+
+def _nn_distance(orc, tree, k, mode=NND_KTH, threads=8):
+    """filters.nndistance per point (mode 2: the outlier running mean)."""
+    n = len(tree.x)
+    out = np.zeros(n)
+    orc.L.orc_nn_distance(tree.h, _ptr(tree.x, _dp), _ptr(tree.y, _dp), _ptr(tree.z, _dp), n,
+                          k, mode, threads, _ptr(out, _dp))
+    return out
+
+
+def _outlier_statistical(orc, tree, mean_k=8, multiplier=2.0, threads=8):
+    """(outlier flags, threshold, per-point mean distances)."""
+    n = len(tree.x)
+    out = np.zeros(max(n, 1), np.uint8)
+    dist = np.zeros(max(n, 1))
+    thr = ctypes.c_double(0.0)
+    orc.L.orc_outlier_statistical(tree.h, _ptr(tree.x, _dp), _ptr(tree.y, _dp),
+                                  _ptr(tree.z, _dp), n, mean_k, multiplier, threads,
+                                  _ptr(out, _u8p), ctypes.byref(thr), _ptr(dist, _dp))
+    return out[:n], thr.value, dist[:n]
+
+
+def _outlier_radius(orc, tree, radius=1.0, min_k=2, threads=8):
+    n = len(tree.x)
+    out = np.zeros(max(n, 1), np.uint8)
+    orc.L.orc_outlier_radius(tree.h, n, radius, min_k, threads, _ptr(out, _u8p))
+    return out[:n]
+
+
+Oracle.nn_distance = _nn_distance
+Oracle.outlier_statistical = _outlier_statistical
+Oracle.outlier_radius = _outlier_radius
+
 
 class Ref:
     """The reference's own nanoflann + Eigen (oracle/_ref).  Present only
@@ -388,6 +446,8 @@ class Ref:
         L.ref_eigen_sym3.restype = ctypes.c_int
         L.ref_eigen_sym3.argtypes = [_dp, _dp, _dp]
         L.ref_pdal_covariance.argtypes = [_dp, _dp, _dp, _u64p, ctypes.c_uint64, _dp]
+        L.ref_jacobi_rank.restype = ctypes.c_int
+        L.ref_jacobi_rank.argtypes = [_dp, ctypes.c_double]
         L.ref_kd3_radius.restype = ctypes.c_uint64
         L.ref_kd3_radius.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
@@ -439,6 +499,10 @@ class Ref:
 
     def kd3(self, x, y, z, threads=1):
         return Ref.Tree(self, x, y, z, threads)
+
+    def jacobi_rank(self, cov, threshold):
+        c = _c(np.asarray(cov).reshape(9), np.float64)
+        return self.L.ref_jacobi_rank(_ptr(c, _dp), threshold)
 
     def eigen_sym3(self, cov):
         c = _c(np.asarray(cov).reshape(9), np.float64)
