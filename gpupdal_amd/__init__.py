@@ -90,7 +90,7 @@ EXPORTED = [
     "gf_autotune", "gf_tuning_save", "gf_tuning_load",
     "gf_las_parse", "gf_cloud_create_las", "gf_las_pack_extra", "gf_debug_cloud_xyz",
     "gf_optimal_neighborhood", "gf_debug_grid_params", "gf_run_noalloc",
-    "gf_radius_counts", "gf_nndistance", "gf_outlier",
+    "gf_radius_counts", "gf_nndistance", "gf_outlier", "gf_run_rowk",
 ]
 NOALLOC_MAX_POINTS = 65536
 NND_KTH, NND_AVG = 0, 1
@@ -124,6 +124,8 @@ def load_library(path=LIB_PATH):
     L.gf_run_noalloc.argtypes = [vp, vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(GfStage),
                                  ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.gf_knn.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.c_int]
+    L.gf_run_rowk.argtypes = [vp, vp, ctypes.POINTER(GfStage), ctypes.c_int, vp,
+                              ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int]
     L.gf_nndistance.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_int, vp, ctypes.c_int]
     L.gf_outlier.argtypes = [vp, vp, ctypes.c_int, ctypes.c_uint32, ctypes.c_double,
                              ctypes.c_double, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
@@ -434,6 +436,27 @@ class Cloud:
         sp = (ctypes.c_void_p * len(status))(*[s.ctypes.data for s in status])
         self.ctx._check(self.ctx.L.gf_run(self.ctx.h, self.h, arr, len(stages), cp, sp,
                                           WHERE_HOST), "gf_run")
+        return per_stage, status
+
+    def run_rowk(self, stages, row_k, init=0.0):
+        """Runs stages over each row's own neighbor count (row_k[i] nearest
+        points in KD3Index order; covariancefeatures' optimized=true)."""
+        stages = list(stages)
+        row_k = np.ascontiguousarray(row_k, dtype=np.uint32)
+        if row_k.shape != (self.n,):
+            raise ValueError("row_k needs one count per point")
+        arr = (GfStage * len(stages))(*stages)
+        cols, per_stage = [], []
+        for st in stages:
+            c = [np.full(self.n, init, np.float64) for _ in range(stage_columns(st))]
+            per_stage.append(c)
+            cols.extend(c)
+        status = [np.zeros(self.n, np.uint8) for _ in stages]
+        cp = (ctypes.c_void_p * max(1, len(cols)))(*[c.ctypes.data for c in cols])
+        sp = (ctypes.c_void_p * len(status))(*[s.ctypes.data for s in status])
+        self.ctx._check(self.ctx.L.gf_run_rowk(self.ctx.h, self.h, arr, len(stages),
+                                               row_k.ctypes.data, cp, sp, WHERE_HOST),
+                        "gf_run_rowk")
         return per_stage, status
 
     def invalidate(self):

@@ -175,22 +175,7 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     FusedProgram base{};
     base.nstages = ns;
     base.stride = stride;
-    int ncol_total = 0;
-    for (int i = 0; i < ns; ++i) {
-        StageProgram& sp = base.st[i];
-        sp.stage = st[i].stage;
-        sp.mode = st[i].mode;
-        sp.features = st[i].features;
-        sp.always_up = st[i].always_up;
-        sp.normalize = st[i].normalize;
-        sp.p1 = st[i].p1;
-        sp.p2 = st[i].p2;
-        sp.ncols = stage_ncols(st[i]);
-        sp.col_offset = ncol_total;
-        ncol_total += sp.ncols;
-    }
-    if (ncol_total > kMaxOutColumns)
-        throw Error(GF_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxOutColumns) + " output columns");
+    const int ncol_total = fill_stage_programs(base, st, ns);
     reset_spans();
     hipEvent_t t_all = mark();
 

@@ -164,6 +164,29 @@ int stage_ncols(const gf_stage& s)
     }
 }
 
+int fill_stage_programs(FusedProgram& prog, const gf_stage* st, int ns)
+{
+    int ncol_total = 0;
+    prog.nstages = ns;
+    for (int i = 0; i < ns; ++i) {
+        StageProgram& sp = prog.st[i];
+        sp.stage = st[i].stage;
+        sp.mode = st[i].mode;
+        sp.features = st[i].features;
+        sp.always_up = st[i].always_up;
+        sp.normalize = st[i].normalize;
+        sp.p1 = st[i].p1;
+        sp.p2 = st[i].p2;
+        sp.ncols = stage_ncols(st[i]);
+        sp.col_offset = ncol_total;
+        ncol_total += sp.ncols;
+    }
+    if (ncol_total > kMaxOutColumns)
+        throw Error(GF_ERR_UNSUPPORTED,
+                    "more than " + std::to_string(kMaxOutColumns) + " output columns");
+    return ncol_total;
+}
+
 void validate_stage(const gf_stage& s)
 {
     if (stage_ncols(s) < 0) throw Error(GF_ERR_ARG, "unknown stage kind");
@@ -218,22 +241,7 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     prog.k = static_cast<int32_t>(k);
     prog.stride = stride;
     prog.fetch = static_cast<int32_t>(fetch);
-    int ncol_total = 0;
-    for (int i = 0; i < ns; ++i) {
-        StageProgram& sp = prog.st[i];
-        sp.stage = st[i].stage;
-        sp.mode = st[i].mode;
-        sp.features = st[i].features;
-        sp.always_up = st[i].always_up;
-        sp.normalize = st[i].normalize;
-        sp.p1 = st[i].p1;
-        sp.p2 = st[i].p2;
-        sp.ncols = stage_ncols(st[i]);
-        sp.col_offset = ncol_total;
-        ncol_total += sp.ncols;
-    }
-    if (ncol_total > kMaxOutColumns)
-        throw Error(GF_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxOutColumns) + " output columns");
+    const int ncol_total = fill_stage_programs(prog, st, ns);
 
     reset_spans();
     hipEvent_t t_all = mark();
@@ -493,22 +501,7 @@ void Context::run_group_radius(Cloud& cl, const gf_stage* st, int ns, double* co
     prog.fetch = 0;
     prog.radius = r;
     prog.min_k = st[0].min_k;
-    int ncol_total = 0;
-    for (int i = 0; i < ns; ++i) {
-        StageProgram& sp = prog.st[i];
-        sp.stage = st[i].stage;
-        sp.mode = st[i].mode;
-        sp.features = st[i].features;
-        sp.always_up = st[i].always_up;
-        sp.normalize = st[i].normalize;
-        sp.p1 = st[i].p1;
-        sp.p2 = st[i].p2;
-        sp.ncols = stage_ncols(st[i]);
-        sp.col_offset = ncol_total;
-        ncol_total += sp.ncols;
-    }
-    if (ncol_total > kMaxOutColumns)
-        throw Error(GF_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxOutColumns) + " output columns");
+    const int ncol_total = fill_stage_programs(prog, st, ns);
     reset_spans();
     hipEvent_t t_all = mark();
     points_mark = false;
@@ -603,6 +596,60 @@ void Context::nn_distance(Cloud& cl, uint32_t k, int mode, double* out)
                        cv.sy, cv.sz, out, stream);
 }
 
+// Stages whose neighborhood size is each row's own (row_k, device array):
+// one KD3Index-ordered list of the largest size, each row's prefix.
+void Context::run_rowk(Cloud& cl, const gf_stage* st, int ns, const uint32_t* row_k,
+                       double* const* cols, uint8_t* const* status, int where)
+{
+    const uint64_t n = cl.n;
+    FusedProgram prog{};
+    const int ncol_total = fill_stage_programs(prog, st, ns);
+    counters.ensure(128);
+    uint32_t* dmax = counters.as<uint32_t>() + 16;
+    launch_max_u32(row_k, n, dmax, stream);
+    uint32_t K = 0;
+    GF_HIP(hipMemcpyAsync(&K, dmax, 4, hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipStreamSynchronize(stream));
+    const uint32_t kk = static_cast<uint32_t>(std::min<uint64_t>(n, K));
+    if (kk > static_cast<uint32_t>(kMaxFetch - 1))
+        throw Error(GF_ERR_UNSUPPORTED,
+                    "a per-row neighbor count above 63 is outside the device kNN envelope");
+    prog.k = static_cast<int32_t>(kk);
+    prog.stride = 1;
+    prog.fetch = static_cast<int32_t>(kk);
+    listbuf.ensure(n * std::max<uint32_t>(kk, 1) * 4 + 8);
+    liststat.ensure(n + 8);
+    if (kk) knn_lists(cl, kk, 1, 2, listbuf.as<uint32_t>(), liststat.as<uint8_t>());
+    else {
+        points_mark = false;
+        build_index(cl, 1);  // sorted coordinates only: no row reads a neighbor
+    }
+    KnnOutputs out{};
+    if (where == GF_WHERE_DEVICE) {
+        for (int c = 0; c < ncol_total; ++c) out.col[c] = cols[c];
+        for (int i = 0; i < ns; ++i) out.status[i] = status[i];
+    } else {
+        outbuf.ensure(static_cast<uint64_t>(ncol_total) * n * 8 + 8);
+        statbuf.ensure(static_cast<uint64_t>(ns) * n + 8);
+        for (int c = 0; c < ncol_total; ++c) {
+            out.col[c] = outbuf.as<double>() + static_cast<uint64_t>(c) * n;
+            GF_HIP(hipMemcpyAsync(out.col[c], cols[c], n * 8, hipMemcpyHostToDevice, stream));
+        }
+        for (int i = 0; i < ns; ++i) out.status[i] = statbuf.as<uint8_t>() + i * n;
+    }
+    const CloudView cv = view(cl);
+    launch_prefix_stages(listbuf.as<uint32_t>(), static_cast<int>(kk), row_k, n,
+                         cl.x.as<double>(), cl.y.as<double>(), cl.z.as<double>(), cv.sx, cv.sy,
+                         cv.sz, prog, out, stream);
+    if (where != GF_WHERE_DEVICE) {
+        for (int c = 0; c < ncol_total; ++c)
+            GF_HIP(hipMemcpyAsync(cols[c], out.col[c], n * 8, hipMemcpyDeviceToHost, stream));
+        for (int i = 0; i < ns; ++i)
+            GF_HIP(hipMemcpyAsync(status[i], out.status[i], n, hipMemcpyDeviceToHost, stream));
+    }
+    GF_HIP(hipStreamSynchronize(stream));
+}
+
 // ------------------------------------------------- no-allocation path ----
 //
 // gf_run_noalloc runs the same fused pass as gf_run on caller-owned device
@@ -684,21 +731,7 @@ void Context::run_group_static(const double* x, const double* y, const double* z
     prog.k = static_cast<int32_t>(k);
     prog.stride = stride;
     prog.fetch = static_cast<int32_t>(fetch);
-    int ncol_total = 0;
-    for (int i = 0; i < ns; ++i) {
-        StageProgram& sp = prog.st[i];
-        sp.stage = st[i].stage;
-        sp.mode = st[i].mode;
-        sp.features = st[i].features;
-        sp.always_up = st[i].always_up;
-        sp.normalize = st[i].normalize;
-        sp.p1 = st[i].p1;
-        sp.p2 = st[i].p2;
-        sp.ncols = stage_ncols(st[i]);
-        sp.col_offset = ncol_total;
-        ncol_total += sp.ncols;
-    }
-    if (ncol_total > kMaxOutColumns) throw Error(GF_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxOutColumns) + " output columns");
+    const int ncol_total = fill_stage_programs(prog, st, ns);
 
     // Every piece is laid out before the first launch, so a call that does
     // not fit fails before touching the device.
@@ -1071,6 +1104,42 @@ int gf_radius_counts(gf_ctx* ctx, gf_cloud* cloud, double radius, uint32_t* coun
         if (where == GF_WHERE_HOST)
             GF_HIP(hipMemcpyAsync(counts, dst, n * 4, hipMemcpyDeviceToHost, c->stream));
         GF_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int gf_run_rowk(gf_ctx* ctx, gf_cloud* cloud, const gf_stage* stages, int nstages,
+                const uint32_t* row_k, double* const* cols, uint8_t* const* status, int where)
+{
+    if (!ctx || !cloud || !stages || !row_k || !status || nstages < 1 ||
+        nstages > kMaxStages)
+        return GF_ERR_ARG;
+    return guarded(ctx, [&] {
+        for (int i = 0; i < nstages; ++i) {
+            gf_stage s = stages[i];
+            if (s.radius != 0.0)
+                throw Error(GF_ERR_ARG, "per-row neighbor counts take knn neighborhoods only");
+            s.knn = 1;  // knn and stride are not used: each row has its own count
+            s.stride = 1;
+            validate_stage(s);
+            if (!status[i]) throw Error(GF_ERR_ARG, "null status array");
+        }
+        int ncols = 0;
+        for (int i = 0; i < nstages; ++i) ncols += stage_ncols(stages[i]);
+        if (ncols && !cols) throw Error(GF_ERR_ARG, "null column array");
+        for (int c = 0; c < ncols; ++c)
+            if (!cols[c]) throw Error(GF_ERR_ARG, "null column array");
+        Context* c = C(ctx);
+        Cloud* cl = CL(cloud);
+        const uint64_t n = cl->n;
+        if (!n) return;
+        const uint32_t* drow = row_k;
+        DevBuf tmp;
+        if (where != GF_WHERE_DEVICE) {
+            tmp.ensure(n * 4 + 8);
+            GF_HIP(hipMemcpyAsync(tmp.ptr, row_k, n * 4, hipMemcpyHostToDevice, c->stream));
+            drow = tmp.as<uint32_t>();
+        }
+        c->run_rowk(*cl, stages, nstages, drow, cols, status, where);
     });
 }
 

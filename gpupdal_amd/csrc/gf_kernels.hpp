@@ -242,6 +242,14 @@ void launch_optimal_nbhd(const uint32_t* ids, uint64_t n, int kmin, int kmax, co
 void launch_nn_distance(const uint32_t* ids, int kk, int K, uint64_t n, int mode,
                         const double* x, const double* y, const double* z, const double* sx,
                         const double* sy, const double* sz, double* out, hipStream_t s);
+// Stages over row i's first min(row_k[i], kk) neighbors of an n x kk list of
+// sorted positions (KD3Index order); rows in original order, no search flags.
+void launch_prefix_stages(const uint32_t* ids, int kk, const uint32_t* row_k, uint64_t n,
+                          const double* x, const double* y, const double* z, const double* sx,
+                          const double* sy, const double* sz, const FusedProgram& prog,
+                          const KnnOutputs& out, hipStream_t s);
+// out[0] = max of v[0..n) (out cleared first).
+void launch_max_u32(const uint32_t* v, uint64_t n, uint32_t* out, hipStream_t s);
 // The flagged rows' lists (point ids from the exact re-solve) to sorted
 // positions.
 void launch_lists_to_sorted(uint32_t* ids, int kmax, const uint32_t* flagged, uint32_t nflagged,

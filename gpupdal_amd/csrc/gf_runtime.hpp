@@ -206,6 +206,9 @@ struct Batch {
 GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch, const Tuning& t,
                      double cell = 0.0);
 int stage_ncols(const gf_stage& s);
+// The stages' device programs (columns laid out in stage order); returns the
+// column count, GF_ERR_UNSUPPORTED above kMaxOutColumns.
+int fill_stage_programs(FusedProgram& prog, const gf_stage* st, int ns);
 void validate_stage(const gf_stage& s);
 // Throws GF_ERR_UNSUPPORTED when a cloud (or tile) of n points whose search
 // asks K neighbors with this stride is outside what the device search handles.
@@ -304,6 +307,10 @@ struct Context {
     // ids, mode 2 sorted positions (stride 1 only).  Queued on the stream.
     void knn_lists(Cloud& cl, uint32_t k, uint32_t stride, int mode, uint32_t* ids,
                    uint8_t* status);
+    // gf_run_rowk: the stages over each row's first row_k[i] KD3Index
+    // neighbors (row_k: device array).
+    void run_rowk(Cloud& cl, const gf_stage* st, int ns, const uint32_t* row_k,
+                  double* const* cols, uint8_t* const* status, int where);
     // Per-row distance statistic (kNndKth, kNndAvg, kNndRunningMean) of the
     // k + 1 nearest points into out (device, original order).
     void nn_distance(Cloud& cl, uint32_t k, int mode, double* out);

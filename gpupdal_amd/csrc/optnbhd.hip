@@ -145,7 +145,98 @@ __global__ __launch_bounds__(256) void k_nn_distance(const uint32_t* __restrict_
     out[i] = val;
 }
 
+// Stages over a per-row neighbor count (covariancefeatures' optimized=true:
+// KD3Index::neighbors(p, OptimalKNN, 1), filters/CovarianceFeaturesFilter.cpp
+// :204-210): row i's neighborhood is the first row_k[i] entries (at most the
+// cloud's size) of its KD3Index-ordered list, whose prefixes are exactly the
+// smaller queries' lists.  The covariance is formed as the fused search's
+// (relative to the query point, mean first, divided by k - 1), then the same
+// eigen solve and stage formulas as the feature kernel.
+__global__ __launch_bounds__(256) void k_prefix_stages(const uint32_t* __restrict__ ids, int kk,
+                                                       const uint32_t* __restrict__ row_k,
+                                                       uint64_t n, const double* __restrict__ x,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ z,
+                                                       const double* __restrict__ sx,
+                                                       const double* __restrict__ sy,
+                                                       const double* __restrict__ sz,
+                                                       FusedProgram prog, KnnOutputs out)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const int k = static_cast<int>(min(row_k[i], static_cast<uint32_t>(kk)));
+    const uint32_t* nb = ids + i * static_cast<uint64_t>(kk);
+    const double ox = x[i], oy = y[i], oz = z[i];
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    for (int l = 0; l < k; ++l) {
+        const uint32_t j = nb[l];
+        mx += sx[j] - ox;
+        my += sy[j] - oy;
+        mz += sz[j] - oz;
+    }
+    mx /= static_cast<double>(k);
+    my /= static_cast<double>(k);
+    mz /= static_cast<double>(k);
+    double c[6] = {0, 0, 0, 0, 0, 0};
+    for (int l = 0; l < k; ++l) {
+        const uint32_t j = nb[l];
+        const double dx = (sx[j] - ox) - mx, dy = (sy[j] - oy) - my, dz = (sz[j] - oz) - mz;
+        c[0] += dx * dx;
+        c[1] += dx * dy;
+        c[2] += dx * dz;
+        c[3] += dy * dy;
+        c[4] += dy * dz;
+        c[5] += dz * dz;
+    }
+    const double den = static_cast<double>(k - 1);
+    for (int t = 0; t < 6; ++t) c[t] /= den;
+    const bool zero = cov_is_zero(c);
+    double ev[3] = {0, 0, 0}, vec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int info = 0;
+    bool rank_stage = false;
+    for (int si = 0; si < prog.nstages; ++si) rank_stage |= prog.st[si].stage == STAGE_RANK;
+    if (!zero || rank_stage) info = eigen_sym3(c, ev, vec);
+    for (int si = 0; si < prog.nstages; ++si) {
+        const StageProgram& sp = prog.st[si];
+        double vals[10];
+        const uint8_t st = stage_outputs(sp, k, c, zero, info, ev, vec, vals);
+        if (st == 0)
+            for (int cc = 0; cc < sp.ncols; ++cc) out.col[sp.col_offset + cc][i] = vals[cc];
+        out.status[si][i] = st;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_max_u32(const uint32_t* __restrict__ v, uint64_t n,
+                                                 uint32_t* __restrict__ out)
+{
+    uint32_t m = 0;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        m = max(m, v[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(m, o)));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
 } // namespace
+
+void launch_prefix_stages(const uint32_t* ids, int kk, const uint32_t* row_k, uint64_t n,
+                          const double* x, const double* y, const double* z, const double* sx,
+                          const double* sy, const double* sz, const FusedProgram& prog,
+                          const KnnOutputs& out, hipStream_t s)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_prefix_stages, dim3(static_cast<uint32_t>(div_up(n, 256))), dim3(256),
+                       0, s, ids, kk, row_k, n, x, y, z, sx, sy, sz, prog, out);
+}
+
+void launch_max_u32(const uint32_t* v, uint64_t n, uint32_t* out, hipStream_t s)
+{
+    launch_fill_u32(out, 1, 0u, s);
+    if (!n) return;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(div_up(n, 256), 1024));
+    hipLaunchKernelGGL(k_max_u32, dim3(blocks), dim3(256), 0, s, v, n, out);
+}
 
 void launch_nn_distance(const uint32_t* ids, int kk, int K, uint64_t n, int mode,
                         const double* x, const double* y, const double* z, const double* sx,

@@ -40,6 +40,10 @@ const char* Dimension::name(Id id)
     case Id::Eigenvalue0: return "Eigenvalue0";
     case Id::Eigenvalue1: return "Eigenvalue1";
     case Id::Eigenvalue2: return "Eigenvalue2";
+    case Id::Coplanar: return "Coplanar";
+    case Id::Rank: return "Rank";
+    case Id::NNDistance: return "NNDistance";
+    case Id::Classification: return "Classification";
     }
     return "";
 }
@@ -124,6 +128,25 @@ bool convert(const std::string& s, CovarianceFeaturesFilter::Mode& m)
     if (t == "raw") m = CovarianceFeaturesFilter::Mode::Raw;
     else if (t == "sqrt") m = CovarianceFeaturesFilter::Mode::SQRT;
     else if (t == "normalized") m = CovarianceFeaturesFilter::Mode::Normalized;
+    else return false;
+    return true;
+}
+
+bool convert(const std::string& s, uint8_t& v)
+{
+    int r = 0;
+    if (!convert(trim(s), r) || r < 0 || r > 255) return false;
+    v = static_cast<uint8_t>(r);
+    return true;
+}
+
+// NNDistanceFilter's mode: case-insensitive kth | avg
+// (filters/NNDistanceFilter.cpp:64-78).
+bool convert(const std::string& s, NNDistanceFilter::Mode& m)
+{
+    std::string t = lower(trim(s));
+    if (t == "kth") m = NNDistanceFilter::Mode::Kth;
+    else if (t == "avg") m = NNDistanceFilter::Mode::Average;
     else return false;
     return true;
 }
@@ -223,6 +246,11 @@ template Arg& ProgramArgs::add<size_t>(const std::string&, const std::string&, s
 template Arg& ProgramArgs::add<bool>(const std::string&, const std::string&, bool&, bool);
 template Arg& ProgramArgs::add<std::string>(const std::string&, const std::string&,
                                             std::string&, std::string);
+template Arg& ProgramArgs::add<uint8_t>(const std::string&, const std::string&, uint8_t&,
+                                        uint8_t);
+template Arg& ProgramArgs::add<NNDistanceFilter::Mode>(const std::string&, const std::string&,
+                                                       NNDistanceFilter::Mode&,
+                                                       NNDistanceFilter::Mode);
 template Arg& ProgramArgs::add<CovarianceFeaturesFilter::Mode>(
     const std::string&, const std::string&, CovarianceFeaturesFilter::Mode&,
     CovarianceFeaturesFilter::Mode);
@@ -381,8 +409,11 @@ struct StageResult {
     std::vector<uint8_t> status;
 };
 
-StageResult run_on_gpu(PointView& view, const gf_stage& st, const std::vector<Dimension::Id>& outdims,
-                  const Stage& stage)
+// row_k (optional): each row's own neighbor count instead of the stage's
+// knn (covariancefeatures' optimized=true).
+StageResult run_on_gpu(PointView& view, const gf_stage& st,
+                       const std::vector<Dimension::Id>& outdims, const Stage& stage,
+                       const std::vector<uint32_t>* row_k = nullptr)
 {
     StageResult r;
     const uint64_t n = view.size();
@@ -393,8 +424,9 @@ StageResult run_on_gpu(PointView& view, const gf_stage& st, const std::vector<Di
     auto res = view.resident();
     uint8_t* sp = r.status.data();
     gf_ctx* ctx = device_context();
-    int rc = gf_run(ctx, res->cloud, &st, 1, ptrs.empty() ? nullptr : ptrs.data(), &sp,
-                    GF_WHERE_HOST);
+    double* const* cp = ptrs.empty() ? nullptr : ptrs.data();
+    int rc = row_k ? gf_run_rowk(ctx, res->cloud, &st, 1, row_k->data(), cp, &sp, GF_WHERE_HOST)
+                   : gf_run(ctx, res->cloud, &st, 1, cp, &sp, GF_WHERE_HOST);
     if (rc != GF_OK) throw pdal_error(stage.getName() + ": " + gf_last_error(ctx));
     return r;
 }
@@ -477,8 +509,6 @@ void CovarianceFeaturesFilter::prepared(PointTable& table)
     if (m_optimal) {
         if (!layout->hasDim(Id::OptimalKNN))
             throwError("Missing OptimalKNN dimension in input PointView.");
-        throwError("optimized=true (a per-point neighbor count) is not supported by the GPU "
-                   "stage.");
     }
 }
 
@@ -507,12 +537,23 @@ void CovarianceFeaturesFilter::filter(PointView& view)
                                   : (m_mode == Mode::SQRT ? GF_MODE_SQRT : GF_MODE_NORMALIZED);
     st.features = mask;
     st.always_up = 1;
-    const bool radius = m_radiusArg->set();
+    // optimized=true takes each point's OptimalKNN nearest (stride 1) and
+    // wins over radius (CovarianceFeaturesFilter.cpp:204-224).
+    const bool radius = !m_optimal && m_radiusArg->set();
     if (radius) {
         st.radius = m_radius;
         st.min_k = m_minK;
     }
-    StageResult r = run_on_gpu(view, st, outdims, *this);
+    std::vector<uint32_t> row_k;
+    if (m_optimal) {
+        row_k.resize(n);
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t k = view.getFieldAs<uint64_t>(Id::OptimalKNN, i);
+            if (k > 63) throwError("OptimalKNN above 63 is outside the device kNN envelope");
+            row_k[i] = static_cast<uint32_t>(k);
+        }
+    }
+    StageResult r = run_on_gpu(view, st, outdims, *this, m_optimal ? &row_k : nullptr);
     const std::vector<uint32_t> counts =
         radius ? radius_counts_if_skipped(view, m_radius, r.status, *this) : std::vector<uint32_t>{};
 
@@ -776,8 +817,170 @@ void OptimalNeighborhoodFilter::filter(PointView& view)
     }
 }
 
+// --------------------------------------------------- approximatecoplanar --
+void ApproximateCoplanarFilter::addArgs(ProgramArgs& args)
+{
+    args.add("knn", "k-Nearest Neighbors", m_knn, 8);
+    args.add("thresh1", "Threshold 1", m_thresh1, 25.0);
+    args.add("thresh2", "Threshold 2", m_thresh2, 6.0);
+}
+
+void ApproximateCoplanarFilter::addDimensions(PointLayout* layout)
+{
+    layout->registerDim(Dimension::Id::Coplanar);
+}
+
+void ApproximateCoplanarFilter::filter(PointView& view)
+{
+    const uint64_t n = view.size();
+    if (!n) return;
+    gf_stage st{};
+    st.stage = GF_STAGE_COPLANAR;
+    st.knn = m_knn;
+    st.stride = 1;
+    st.p1 = m_thresh1;
+    st.p2 = m_thresh2;
+    StageResult r = run_on_gpu(view, st, {Dimension::Id::Coplanar}, *this);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint8_t s = r.status[i];
+        if (s & GF_ST_EIGEN_FAIL) throwError("Cannot perform eigen decomposition.");
+        if (s & kSkipOrFail) {
+            log().write(LogLevel::Info,
+                        "Skipping point " + std::to_string(i) +
+                            ". Covariance matrix is all zeros. This suggests a large number "
+                            "of redundant points. Consider using filters.sample with a "
+                            "small radius to remove redundant points.\n");
+            continue;
+        }
+        view.setField(Dimension::Id::Coplanar, i, r.cols[0][i]);
+    }
+}
+
+// ---------------------------------------------------------- estimaterank --
+void EstimateRankFilter::addArgs(ProgramArgs& args)
+{
+    args.add("knn", "k-Nearest Neighbors", m_knn, 8);
+    args.add("thresh", "Threshold", m_thresh, 0.01);
+}
+
+void EstimateRankFilter::addDimensions(PointLayout* layout)
+{
+    layout->registerDim(Dimension::Id::Rank);
+}
+
+void EstimateRankFilter::filter(PointView& view)
+{
+    const uint64_t n = view.size();
+    if (!n) return;
+    gf_stage st{};
+    st.stage = GF_STAGE_RANK;
+    st.knn = m_knn;
+    st.stride = 1;
+    st.p1 = m_thresh;
+    StageResult r = run_on_gpu(view, st, {Dimension::Id::Rank}, *this);
+    for (uint64_t i = 0; i < n; ++i) {
+        // The device takes the singular values of the symmetric covariance
+        // from its eigen solve; one that does not converge has no rank.
+        if (r.status[i] & kSkipOrFail) throwError("Cannot perform eigen decomposition.");
+        view.setField(Dimension::Id::Rank, i, r.cols[0][i]);
+    }
+}
+
+// ------------------------------------------------------------ nndistance --
+void NNDistanceFilter::addArgs(ProgramArgs& args)
+{
+    args.add("mode", "Distance computation mode (kth, avg)", m_mode, Mode::Kth);
+    args.add("k", "k neighbors", m_k, size_t(10));
+}
+
+void NNDistanceFilter::addDimensions(PointLayout* layout)
+{
+    layout->registerDim(Dimension::Id::NNDistance);
+}
+
+void NNDistanceFilter::filter(PointView& view)
+{
+    const uint64_t n = view.size();
+    if (!n) return;
+    if (m_k > UINT32_MAX - 1) throwError("k is outside the device kNN envelope");
+    std::vector<double>& out = view.column(Dimension::Id::NNDistance);
+    gf_ctx* ctx = device_context();
+    const int rc = gf_nndistance(ctx, view.resident()->cloud, static_cast<uint32_t>(m_k),
+                                 m_mode == Mode::Kth ? GF_NND_KTH : GF_NND_AVG, out.data(),
+                                 GF_WHERE_HOST);
+    if (rc != GF_OK) throwError(gf_last_error(ctx));
+}
+
+// --------------------------------------------------------------- outlier --
+void OutlierFilter::addArgs(ProgramArgs& args)
+{
+    args.add("method", "Method [default: statistical]", m_method, std::string("statistical"));
+    args.add("min_k", "Minimum number of neighbors in radius", m_minK, 2);
+    args.add("radius", "Radius", m_radius, 1.0);
+    args.add("mean_k", "Mean number of neighbors", m_meanK, 8);
+    args.add("multiplier", "Standard deviation threshold", m_multiplier, 2.0);
+    args.add("class", "Class to use for noise points", m_class, uint8_t(7));
+}
+
+void OutlierFilter::addDimensions(PointLayout* layout)
+{
+    layout->registerDim(Dimension::Id::Classification);
+}
+
+PointViewSet OutlierFilter::run(PointTable& table)
+{
+    PointViewSet out;
+    for (const auto& v : Stage::run(table))
+        if (v->size()) out.insert(v);
+    return out;
+}
+
+void OutlierFilter::filter(PointView& view)
+{
+    const uint64_t n = view.size();
+    if (!n) return;
+    const std::string m = lower(m_method);
+    int method;
+    if (m == "statistical") {
+        method = GF_OUTLIER_STATISTICAL;
+    } else if (m == "radius") {
+        method = GF_OUTLIER_RADIUS;
+    } else {
+        log().write(LogLevel::Warning, "Requested method is unrecognized. Please choose from "
+                                       "\"statistical\" or \"radius\".\n");
+        return;
+    }
+    if (method == GF_OUTLIER_STATISTICAL && m_meanK < 0)
+        throwError("mean_k must not be negative");
+    std::vector<uint8_t> flags(n, 0);
+    gf_ctx* ctx = device_context();
+    const int rc = gf_outlier(ctx, view.resident()->cloud, method,
+                              static_cast<uint32_t>(std::max(m_meanK, 0)), m_multiplier,
+                              m_radius, m_minK, flags.data(), nullptr);
+    if (rc != GF_OK) throwError(gf_last_error(ctx));
+    uint64_t nout = 0;
+    for (uint8_t f : flags) nout += f;
+    if (nout == n) {
+        log().write(LogLevel::Warning, "Requested filter would remove all points. Try a larger "
+                                       "radius/smaller minimum neighbors.\n");
+        return;
+    }
+    if (!nout) {
+        log().write(LogLevel::Warning, "Filtered cloud has no outliers!\n");
+        return;
+    }
+    log().write(LogLevel::Debug, "Labeled " + std::to_string(nout) + " outliers as noise!\n");
+    for (uint64_t i = 0; i < n; ++i)
+        if (flags[i]) view.setField(Dimension::Id::Classification, i, m_class);
+}
+
 std::unique_ptr<Stage> createStage(const std::string& name)
 {
+    if (name == "filters.approximatecoplanar")
+        return std::unique_ptr<Stage>(new ApproximateCoplanarFilter);
+    if (name == "filters.estimaterank") return std::unique_ptr<Stage>(new EstimateRankFilter);
+    if (name == "filters.nndistance") return std::unique_ptr<Stage>(new NNDistanceFilter);
+    if (name == "filters.outlier") return std::unique_ptr<Stage>(new OutlierFilter);
     if (name == "filters.optimalneighborhood")
         return std::unique_ptr<Stage>(new OptimalNeighborhoodFilter);
     if (name == "filters.covariancefeatures")

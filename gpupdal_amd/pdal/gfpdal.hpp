@@ -1,6 +1,6 @@
 // gfpdal.hpp -- PDAL-compatible stage layer over the gpupdal_amd C ABI.
 //
-// The three neighborhood stages with PDAL's names, options, defaults,
+// The neighborhood stages with PDAL's names, options, defaults,
 // dimension registration order, skip and error behaviour, running on one
 // AMD GPU through include/gpupdal_amd.h:
 //
@@ -8,14 +8,18 @@
 //   filters.normal              (filters/NormalFilter.cpp:85-182)
 //   filters.eigenvalues         (filters/EigenvaluesFilter.cpp:77-172)
 //
+// and the other consumers of KD3Index neighborhoods beside them:
+// filters.optimalneighborhood, filters.approximatecoplanar,
+// filters.estimaterank, filters.nndistance and filters.outlier.
+//
 // libpdal itself is not available to link against, so the small part of its
 // Stage API these stages use is provided here under the same names (Options,
 // PointTable/PointLayout, PointView, BufferReader, Stage::prepare/execute,
 // pdal_error).  A maintainer adding the GPU stages to a real PDAL build keeps
 // the filter classes and swaps this API layer for libpdal's.
 //
-// Not supported on the GPU (each raises pdal_error naming the option):
-// optimized=true and filters.normal's refine.
+// Not supported on the GPU (raises pdal_error naming the option):
+// filters.normal's refine.
 #pragma once
 
 #include <cstdint>
@@ -49,6 +53,7 @@ enum class Id {
     OptimalKNN, OptimalRadius,
     NormalX, NormalY, NormalZ, Curvature,
     Eigenvalue0, Eigenvalue1, Eigenvalue2,
+    Coplanar, Rank, NNDistance, Classification,
 };
 const char* name(Id id);
 } // namespace Dimension
@@ -287,8 +292,81 @@ private:
     uint64_t m_kMax = 14;
 };
 
+// filters.approximatecoplanar (filters/ApproximateCoplanarFilter.cpp:66-121):
+// Coplanar = 1 where the neighborhood's eigenvalues satisfy
+// ev1 > thresh1 * ev0 and thresh2 * ev1 > ev2.
+class ApproximateCoplanarFilter : public Filter {
+public:
+    std::string getName() const override { return "filters.approximatecoplanar"; }
+
+protected:
+    void addArgs(ProgramArgs& args) override;
+    void addDimensions(PointLayout* layout) override;
+    void filter(PointView& view) override;
+
+private:
+    int m_knn = 8;
+    double m_thresh1 = 25.0;
+    double m_thresh2 = 6.0;
+};
+
+// filters.estimaterank (filters/EstimateRankFilter.cpp:64-83): Rank, the
+// numerical rank of each neighborhood's covariance.
+class EstimateRankFilter : public Filter {
+public:
+    std::string getName() const override { return "filters.estimaterank"; }
+
+protected:
+    void addArgs(ProgramArgs& args) override;
+    void addDimensions(PointLayout* layout) override;
+    void filter(PointView& view) override;
+
+private:
+    int m_knn = 8;
+    double m_thresh = 0.01;
+};
+
+// filters.nndistance (filters/NNDistanceFilter.cpp:64-133): NNDistance, the
+// distance to the k-th neighbor or the mean distance to the k nearest.
+class NNDistanceFilter : public Filter {
+public:
+    enum class Mode { Kth, Average };
+    std::string getName() const override { return "filters.nndistance"; }
+
+protected:
+    void addArgs(ProgramArgs& args) override;
+    void addDimensions(PointLayout* layout) override;
+    void filter(PointView& view) override;
+
+private:
+    Mode m_mode = Mode::Kth;
+    size_t m_k = 10;
+};
+
+// filters.outlier (filters/OutlierFilter.cpp:62-230): outliers by the
+// statistical or radius method get Classification = class (default 7, low
+// point); an empty view is dropped, as PDAL's run does.
+class OutlierFilter : public Filter {
+public:
+    std::string getName() const override { return "filters.outlier"; }
+
+protected:
+    void addArgs(ProgramArgs& args) override;
+    void addDimensions(PointLayout* layout) override;
+    PointViewSet run(PointTable& table) override;
+    void filter(PointView& view) override;
+
+private:
+    std::string m_method = "statistical";
+    int m_minK = 2;
+    double m_radius = 1.0;
+    int m_meanK = 8;
+    double m_multiplier = 2.0;
+    uint8_t m_class = 7;
+};
+
 // Creates a stage by its PDAL name (nullptr for other names), as
-// StageFactory::createStage does for these three.
+// StageFactory::createStage does for these stages.
 std::unique_ptr<Stage> createStage(const std::string& name);
 
 // The GPU the stages run on (default: device 0, or GFPDAL_DEVICE).

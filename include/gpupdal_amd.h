@@ -279,6 +279,17 @@ int gf_memcpy(gf_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind)
 int gf_optimal_neighborhood(gf_ctx *ctx, gf_cloud *cloud, uint32_t min_k, uint32_t max_k,
                             uint64_t *optimal_knn, double *optimal_radius, int where);
 
+/* covariancefeatures' optimized=true (filters/CovarianceFeaturesFilter.cpp:
+ * 204-210: KD3Index::neighbors(p, OptimalKNN, 1)), for any stage kinds:
+ * row i's neighborhood is its row_k[i] nearest points in KD3Index order
+ * (clamped to the cloud's size).  stages' knn and stride are not used
+ * (radius must be 0); row_k holds n counts (host or device, as `where`), at
+ * most 63.  cols/status as gf_run's; no search flags are reported (the lists
+ * are exact).  A row with k = 0 has a zero covariance (skipped); k = 1 gives
+ * 0/0 covariances, as PDAL's division by k - 1 does. */
+int gf_run_rowk(gf_ctx *ctx, gf_cloud *cloud, const gf_stage *stages, int nstages,
+                const uint32_t *row_k, double *const *cols, uint8_t *const *status, int where);
+
 /* filters.nndistance (filters/NNDistanceFilter.cpp:97-133): per point, from
  * its k + 1 nearest points (itself included, KD3Index::knnSearch) in
  * ascending distance: GF_NND_KTH the distance to the last of them, GF_NND_AVG

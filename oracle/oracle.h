@@ -218,6 +218,13 @@ uint64_t orc_optimal_neighborhood(const orc_kd3 *t, const double *x, const doubl
                                   const double *z, uint64_t n, uint64_t kmin, uint64_t kmax,
                                   int threads, uint64_t *kopt, double *ropt, double *gap);
 
+/* The stage with each row's own neighbor count: KD3Index::neighbors(p,
+ * row_k[i], 1) (covariancefeatures' optimized=true); the stage's knn and
+ * stride are not used. */
+void orc_stage_run_rowk(const orc_kd3 *t, const double *x, const double *y, const double *z,
+                        uint64_t n, const orc_stage *st, const uint32_t *row_k, int threads,
+                        double *const *cols, uint8_t *status);
+
 /* filters.nndistance per point from its k + 1 nearest (mode 0 kth, 1 avg),
  * or mode 2: filters.outlier's statistical running mean distance. */
 void orc_nn_distance(const orc_kd3 *t, const double *x, const double *y, const double *z,

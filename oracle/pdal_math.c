@@ -428,6 +428,7 @@ typedef struct {
     uint8_t *status;
     double *gap;
     double *ratio;
+    const uint32_t *row_k; /* per-row neighbor counts (optimized=true) */
 } stage_job;
 
 static uint8_t truth_row(const orc_stage *st, const double *x, const double *y,
@@ -486,6 +487,21 @@ static void *stage_worker(void *arg)
                 s = ORC_ST_FEW_NEIGHBORS;
             else
                 s = stage_row(st, j->x, j->y, j->z, rids, cnt, cov_u64, vals);
+        } else if (j->row_k) {
+            /* CovarianceFeaturesFilter.cpp:204-210 (optimized=true):
+             * KD3Index::neighbors(p, OptimalKNN, 1). */
+            uint64_t kr = j->row_k[i] < j->n ? j->row_k[i] : j->n;
+            if (kr > rcap) {
+                rcap = kr;
+                free(rids);
+                free(rd2);
+                rids = malloc(rcap * sizeof(uint64_t));
+                rd2 = malloc(rcap * sizeof(double));
+            }
+            uint64_t cnt = kr ? orc_kd3_neighbors(j->t, j->x[i], j->y[i], j->z[i], kr, 1, rids,
+                                                  rd2)
+                              : 0;
+            s = stage_row(st, j->x, j->y, j->z, rids, cnt, cov_u64, vals);
         } else if (j->lists) {
             const uint32_t *row = j->lists + i * j->kk;
             if (j->truth)
@@ -554,6 +570,24 @@ static void run_jobs(stage_job proto, uint64_t n, int threads)
         for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
     free(th);
     free(jobs);
+}
+
+void orc_stage_run_rowk(const orc_kd3 *t, const double *x, const double *y, const double *z,
+                        uint64_t n, const orc_stage *st, const uint32_t *row_k, int threads,
+                        double *const *cols, uint8_t *status)
+{
+    stage_job p;
+    memset(&p, 0, sizeof(p));
+    p.t = t;
+    p.x = x;
+    p.y = y;
+    p.z = z;
+    p.n = n;
+    p.st = st;
+    p.cols = cols;
+    p.status = status;
+    p.row_k = row_k;
+    run_jobs(p, n, threads);
 }
 
 void orc_stage_run(const orc_kd3 *t, const double *x, const double *y,

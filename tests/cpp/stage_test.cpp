@@ -6,7 +6,8 @@
 //   tests/cpp/stage_test Cpu      or      tests/cpp/stage_test Gpu
 //
 // The known answers are those of test/unit/filters/CovarianceFeaturesTest.cpp
-// :46-188 and test/unit/filters/NormalFilterTest.cpp:47-253.
+// :46-188, test/unit/filters/NormalFilterTest.cpp:47-253 and
+// test/unit/filters/NNDistanceTest.cpp:43-125.
 #include "gfpdal.hpp"
 #include "mini_test.hpp"
 
@@ -201,7 +202,51 @@ TEST(CpuFactory, StageNames)
               std::string("filters.covariancefeatures"));
     ASSERT_EQ(createStage("filters.normal")->getName(), std::string("filters.normal"));
     ASSERT_EQ(createStage("filters.eigenvalues")->getName(), std::string("filters.eigenvalues"));
-    ASSERT_TRUE(createStage("filters.outlier") == nullptr);
+    ASSERT_EQ(createStage("filters.approximatecoplanar")->getName(),
+              std::string("filters.approximatecoplanar"));
+    ASSERT_EQ(createStage("filters.estimaterank")->getName(), std::string("filters.estimaterank"));
+    ASSERT_EQ(createStage("filters.nndistance")->getName(), std::string("filters.nndistance"));
+    ASSERT_EQ(createStage("filters.outlier")->getName(), std::string("filters.outlier"));
+    ASSERT_TRUE(createStage("filters.smrf") == nullptr);
+}
+
+TEST(CpuOptions, NNDistanceMode)
+{
+    PointTable table;
+    NNDistanceFilter f;
+    Options o;
+    o.add("mode", "median");
+    f.setOptions(o);
+    EXPECT_THROW_MSG(f.prepare(table), pdal_error,
+                     "filters.nndistance: Invalid value 'median' for argument 'mode'.");
+    PointTable t2;
+    NNDistanceFilter f2;
+    Options o2;
+    o2.add("mode", "AVG");
+    f2.setOptions(o2);
+    f2.prepare(t2);
+    ASSERT_TRUE(t2.layout()->hasDim(Id::NNDistance));
+}
+
+TEST(CpuDims, ConsumerDims)
+{
+    PointTable table;
+    ApproximateCoplanarFilter c;
+    EstimateRankFilter r;
+    OutlierFilter o;
+    c.prepare(table);
+    r.prepare(table);
+    o.prepare(table);
+    ASSERT_TRUE(table.layout()->hasDim(Id::Coplanar));
+    ASSERT_TRUE(table.layout()->hasDim(Id::Rank));
+    ASSERT_TRUE(table.layout()->hasDim(Id::Classification));
+    PointTable t2;
+    OutlierFilter bad;
+    Options ob;
+    ob.add("class", 300);
+    bad.setOptions(ob);
+    EXPECT_THROW_MSG(bad.prepare(t2), pdal_error,
+                     "filters.outlier: Invalid value '300' for argument 'class'.");
 }
 
 // ------------------------------------------------------------------ GPU --
@@ -529,6 +574,212 @@ TEST(GpuChain, OptimalNeighborhoodThenDensity)
                     (k + 1) / (4.0 / 3.0 * 3.14159265 * r * r * r), 1e-9);
     }
     ASSERT_TRUE(chosen > 100);
+}
+
+TEST(GpuChain, OptimalNeighborhoodThenOptimized)
+{
+    // optimized=true: each row's neighborhood is its OptimalKNN nearest
+    // (CovarianceFeaturesFilter.cpp:204-210).  Checked against the plain
+    // stage run with knn = OptimalKNN - 1 for the rows of each k (the same
+    // neighbor set; the sums may differ in the last bits).
+    std::vector<std::array<double, 3>> p;
+    for (int y = 0; y < 20; ++y)
+        for (int x = 0; x < 20; ++x)
+            p.push_back({x * 0.5 + 0.03 * (y % 4), y * 0.5 + 0.01 * (x % 3),
+                         0.2 * ((x * 7 + y * 3) % 5)});
+    PointTable table;
+    BufferReader reader;
+    OptimalNeighborhoodFilter opt;
+    CovarianceFeaturesFilter feats;
+    Options b;
+    b.add("optimized", true);
+    b.add("feature_set", "Linearity,Planarity,Omnivariance");
+    feats.setOptions(b);
+    opt.setInput(reader);
+    feats.setInput(opt);
+    feats.prepare(table);
+    reader.addView(make_view(table, p));
+    PointViewPtr out = *feats.execute(table).begin();
+    int checked = 0;
+    for (int k = 10; k <= 14; ++k) {
+        PointTable t2;
+        BufferReader r2;
+        CovarianceFeaturesFilter plain;
+        Options o2;
+        o2.add("knn", k - 1);
+        o2.add("feature_set", "Linearity,Planarity,Omnivariance");
+        plain.setOptions(o2);
+        plain.setInput(r2);
+        plain.prepare(t2);
+        r2.addView(make_view(t2, p));
+        PointViewPtr ref = *plain.execute(t2).begin();
+        for (PointId i = 0; i < out->size(); ++i) {
+            if (out->getFieldAs<double>(Id::OptimalKNN, i) != k) continue;
+            for (Id d : {Id::Linearity, Id::Planarity, Id::Omnivariance}) {
+                const double a = out->getFieldAs<double>(d, i), e = ref->getFieldAs<double>(d, i);
+                ASSERT_NEAR(a, e, 1e-9 * std::max(1.0, std::fabs(e)));
+            }
+            ++checked;
+        }
+    }
+    ASSERT_TRUE(checked > 200);
+}
+
+TEST(GpuNNDistance, ReferenceGrid)
+{
+    // test/unit/filters/NNDistanceTest.cpp:43-125: a 10 x 10 x 10 lattice.
+    auto run = [](const char* mode, int k) {
+        PointTable table;
+        BufferReader reader;
+        NNDistanceFilter filter;
+        Options o;
+        o.add("mode", mode);
+        o.add("k", k);
+        filter.setInput(reader);
+        filter.setOptions(o);
+        filter.prepare(table);
+        reader.addView(make_view(table, faux_grid(0, 10, 0, 10, 0, 10, 1000)));
+        return *filter.execute(table).begin();
+    };
+    PointViewPtr v = run("kth", 3);
+    for (PointId i = 0; i < v->size(); ++i)
+        ASSERT_EQ(v->getFieldAs<double>(Id::NNDistance, i), 1.0);
+    v = run("kth", 4);
+    int corners = 0;
+    for (PointId i = 0; i < v->size(); ++i) {
+        const double x = v->getFieldAs<double>(Id::X, i), y = v->getFieldAs<double>(Id::Y, i),
+                     z = v->getFieldAs<double>(Id::Z, i);
+        if ((x == 9 || x == 0) && (y == 9 || y == 0) && (z == 9 || z == 0)) {
+            ASSERT_EQ(v->getFieldAs<double>(Id::NNDistance, i), std::sqrt(2));
+            ++corners;
+        }
+    }
+    ASSERT_EQ(corners, 8);
+    v = run("avg", 6);
+    for (PointId i = 0; i < v->size(); ++i) {
+        const double d = v->getFieldAs<double>(Id::NNDistance, i);
+        const double x = v->getFieldAs<double>(Id::X, i), y = v->getFieldAs<double>(Id::Y, i),
+                     z = v->getFieldAs<double>(Id::Z, i);
+        const int e = (x == 9 || x == 0) + (y == 9 || y == 0) + (z == 9 || z == 0);
+        if (e == 3) ASSERT_EQ(d, (1 + std::sqrt(2)) / 2.0);
+        else if (e == 2) ASSERT_EQ(d, (2 + std::sqrt(2)) / 3.0);
+        else if (e == 1) ASSERT_EQ(d, (5 + std::sqrt(2)) / 6.0);
+        else ASSERT_EQ(d, 1.0);
+    }
+}
+
+TEST(GpuCoplanarRank, PlaneLineBox)
+{
+    auto run = [](Stage& f, const std::vector<std::array<double, 3>>& pts) {
+        PointTable table;
+        BufferReader reader;
+        f.setInput(reader);
+        f.prepare(table);
+        reader.addView(make_view(table, pts));
+        return *f.execute(table).begin();
+    };
+    std::vector<std::array<double, 3>> line;
+    for (int i = 0; i < 20; ++i) line.push_back({0.5 * i, 0.25 * i, 0.0});
+    const auto plane = faux_grid(0, 10, 0, 10, 0, 0, 100);
+    const auto box = faux_grid(0, 6, 0, 6, 0, 6, 216);
+    {
+        // Interior lattice rows: the point, its 4 axis neighbors and 3 of
+        // its 4 diagonal ones (ties), eigenvalues 0 and about 0.55, 0.84.
+        ApproximateCoplanarFilter f;
+        PointViewPtr v = run(f, plane);
+        int interior = 0;
+        for (PointId i = 0; i < v->size(); ++i) {
+            const double x = v->getFieldAs<double>(Id::X, i), y = v->getFieldAs<double>(Id::Y, i);
+            if (x < 2 || x > 7 || y < 2 || y > 7) continue;
+            ASSERT_EQ(v->getFieldAs<double>(Id::Coplanar, i), 1.0);
+            ++interior;
+        }
+        ASSERT_EQ(interior, 36);
+    }
+    {
+        ApproximateCoplanarFilter f;
+        PointViewPtr v = run(f, line);
+        for (PointId i = 0; i < v->size(); ++i) ASSERT_EQ(v->getFieldAs<double>(Id::Coplanar, i), 0.0);
+    }
+    const std::pair<const std::vector<std::array<double, 3>>*, double> cases[] = {
+        {&line, 1.0}, {&plane, 2.0}, {&box, 3.0}};
+    for (const auto& c : cases) {
+        EstimateRankFilter f;
+        Options o;
+        o.add("knn", 12);
+        f.setOptions(o);
+        PointViewPtr v = run(f, *c.first);
+        for (PointId i = 0; i < v->size(); ++i) ASSERT_EQ(v->getFieldAs<double>(Id::Rank, i), c.second);
+    }
+    {
+        EstimateRankFilter f;  // duplicates: a zero covariance has rank 0
+        PointViewPtr v = run(f, {{1, 2, 3}, {1, 2, 3}, {1, 2, 3}, {1, 2, 3}});
+        for (PointId i = 0; i < v->size(); ++i) ASSERT_EQ(v->getFieldAs<double>(Id::Rank, i), 0.0);
+    }
+}
+
+TEST(GpuOutlier, FarPointsAreNoise)
+{
+    // A 10 x 10 lattice plus two far points: both methods label exactly the
+    // far points (class 18 here), and leave the others' classification.
+    auto pts = faux_grid(0, 10, 0, 10, 0, 0, 100);
+    pts.push_back({100, 100, 0});
+    pts.push_back({-80, 50, 3});
+    for (const char* method : {"statistical", "radius"}) {
+        PointTable table;
+        BufferReader reader;
+        OutlierFilter f;
+        Options o;
+        o.add("method", method);
+        o.add("class", 18);
+        o.add("radius", 2.0);
+        f.setInput(reader);
+        f.setOptions(o);
+        f.prepare(table);
+        PointViewPtr view = make_view(table, pts);
+        for (PointId i = 0; i < view->size(); ++i) view->setField(Id::Classification, i, 2.0);
+        reader.addView(view);
+        PointViewPtr v = *f.execute(table).begin();
+        for (PointId i = 0; i < v->size(); ++i)
+            ASSERT_EQ(v->getFieldAs<double>(Id::Classification, i), i >= 100 ? 18.0 : 2.0);
+    }
+}
+
+TEST(GpuOutlier, WarningsAndEmptyView)
+{
+    PointTable table;
+    BufferReader reader;
+    OutlierFilter f;
+    Options o;
+    o.add("method", "radius");
+    o.add("radius", 0.5);  // no lattice point has a neighbor within 0.5
+    f.setInput(reader);
+    f.setOptions(o);
+    std::ostringstream log;
+    f.setLog(&log, LogLevel::Warning);
+    f.prepare(table);
+    reader.addView(make_view(table, faux_grid(0, 5, 0, 5, 0, 0, 25)));
+    PointViewSet out = f.execute(table);
+    ASSERT_EQ(out.size(), size_t(1));
+    PointViewPtr v = *out.begin();
+    for (PointId i = 0; i < v->size(); ++i)
+        ASSERT_EQ(v->getFieldAs<double>(Id::Classification, i), 0.0);
+    ASSERT_TRUE(log.str().find("Requested filter would remove all points.") != std::string::npos);
+
+    PointTable t2;
+    BufferReader r2;
+    OutlierFilter f2;
+    Options o2;
+    o2.add("method", "median");
+    f2.setInput(r2);
+    f2.setOptions(o2);
+    std::ostringstream log2;
+    f2.setLog(&log2, LogLevel::Warning);
+    f2.prepare(t2);
+    r2.addView(make_view(t2, {{0, 0, 0}, {1, 0, 0}}));
+    r2.addView(PointViewPtr(new PointView(t2)));  // empty: dropped
+    ASSERT_EQ(f2.execute(t2).size(), size_t(1));
+    ASSERT_TRUE(log2.str().find("Requested method is unrecognized.") != std::string::npos);
 }
 
 int main(int argc, char** argv) { return mini::run_all(argc, argv); }

@@ -63,7 +63,12 @@ def truth_errors(st, cols, tcols, ok, ratio):
     arithmetic); other columns must be within max(RTOL*|truth|, atol), with
     atol = 1e-7 for the unit-scale features (sqrt of a 1e-16 relative
     eigenvalue is 1.5e-8) and 1e-12 of the row's largest eigenvalue for the
-    Eigenvalue columns."""
+    Eigenvalue columns.  In sqrt mode SurfaceVariation is sqrt(lambda_min)
+    over the sum of the raw eigenvalues, so that noise scales as
+    1 / sqrt(sum): its atol is 1e-7 / sqrt(EigenvalueSum) where the truth
+    has that column (tiny neighborhoods, k = 3 on a 1 cm lattice, have sums
+    near 1e-4 m^2).  Normals whose truth is horizontal are compared
+    unsigned: always_up does not fix their sign."""
     out = {}
     names = column_names(st)
     deg = ok & (ratio <= DEGENERATE_RATIO)
@@ -71,9 +76,21 @@ def truth_errors(st, cols, tcols, ok, ratio):
     lam_max = None
     if st.stage == ol.STAGE_EIGENVALUES:
         lam_max = np.abs(tcols[2])
+    eigsum = None
+    if st.stage == ol.STAGE_COVFEAT and "EigenvalueSum" in names:
+        eigsum = np.abs(tcols[names.index("EigenvalueSum")])
+    # A horizontal normal (nz = 0, a vertical neighborhood) has no "up":
+    # always_up leaves its sign to the solver.
+    flat_z = None
+    if st.stage == ol.STAGE_NORMAL:
+        flat_z = np.abs(tcols[2]) < 1e-9
     for name, c, t in zip(names, cols, tcols):
-        if st.stage == ol.STAGE_NORMAL and name.startswith("Normal") and not st.always_up:
-            c, t = np.abs(c), np.abs(t)  # eigenvector sign is the solver's choice
+        if st.stage == ol.STAGE_NORMAL and name.startswith("Normal"):
+            if not st.always_up:
+                c, t = np.abs(c), np.abs(t)  # eigenvector sign is the solver's choice
+            else:
+                c = np.where(flat_z, np.abs(c), c)
+                t = np.where(flat_z, np.abs(t), t)
         e = np.zeros(len(c))
         e[good] = np.abs(c[good] - t[good]) / (RTOL * np.maximum(np.abs(t[good]), 1e-6))
         if name != "Omnivariance":
@@ -81,6 +98,9 @@ def truth_errors(st, cols, tcols, ok, ratio):
                 atol = 1e-12 * lam_max[deg]
             elif name == "Curvature":
                 atol = np.full(np.count_nonzero(deg), 1e-12)
+            elif (name == "SurfaceVariation" and st.mode == ol.MODE_SQRT
+                  and eigsum is not None):
+                atol = 1e-7 / np.sqrt(np.minimum(np.maximum(eigsum[deg], 1e-300), 1.0))
             else:
                 atol = np.full(np.count_nonzero(deg), 1e-7)
             allow = np.maximum(RTOL * np.abs(t[deg]), atol)

@@ -144,6 +144,8 @@ class Oracle:
                                               _u64p, ctypes.c_uint64, _u32p]
         L.orc_truth_from_csr.argtypes = [_dp, _dp, _dp, ctypes.c_uint64, _u64p, _u32p, sp,
                                          ctypes.c_int, ctypes.POINTER(_dp), _u8p, _dp, _dp]
+        L.orc_stage_run_rowk.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_uint64, sp,
+                                         _u32p, ctypes.c_int, ctypes.POINTER(_dp), _u8p]
         L.orc_nn_distance.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_uint64,
                                       ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_outlier_statistical.restype = ctypes.c_uint64
@@ -292,6 +294,17 @@ class Oracle:
         self.L.orc_stage_run(tree.h, _ptr(tree.x, _dp), _ptr(tree.y, _dp),
                              _ptr(tree.z, _dp), n, ctypes.byref(st), threads,
                              arr, _ptr(status, _u8p))
+        return cols, status
+
+    def stage_run_rowk(self, tree, st, row_k, threads=8, init=0.0):
+        """The stage over each row's own neighbor count (optimized=true)."""
+        n = tree.n
+        row_k = _c(row_k, np.uint32)
+        cols, arr = self._cols(st, n, init)
+        status = np.zeros(n, np.uint8)
+        self.L.orc_stage_run_rowk(tree.h, _ptr(tree.x, _dp), _ptr(tree.y, _dp),
+                                  _ptr(tree.z, _dp), n, ctypes.byref(st), _ptr(row_k, _u32p),
+                                  threads, arr, _ptr(status, _u8p))
         return cols, status
 
     def optimal_neighborhood(self, tree, kmin=10, kmax=14, threads=8):

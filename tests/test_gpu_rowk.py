@@ -1,0 +1,96 @@
+"""Per-row neighbor counts on the GPU (gf_run_rowk; covariancefeatures'
+optimized=true): each row's neighborhood is the first row_k[i] entries of its
+KD3Index-ordered list.  Statuses equal PDAL's CPU path (the oracle's
+KD3Index::neighbors(p, k, 1) per row); columns are within the 1e-5 bar of
+the extended-precision truth over those lists.  Counts of 1 (a 0/0
+covariance in PDAL) are not exercised: their result is parity unpinned."""
+import numpy as np
+import pytest
+
+import gpupdal_amd as g
+import oracle_lib as ol
+from helpers import small_cloud, truth_errors
+
+pytestmark = pytest.mark.gpu
+
+SKIP_BITS = np.uint8(g.ST_ZERO_COV | g.ST_EIGEN_FAIL | g.ST_FEATURE_INVALID | g.ST_FEW_NEIGHBORS)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = g.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return ol.Oracle()
+
+
+def _check(ctx, oracle, x, y, z, row_k, stages):
+    cl = ctx.cloud(x, y, z)
+    cols, status = cl.run_rowk(stages, row_k, init=np.nan)
+    cl.close()
+    t = oracle.kd3(x, y, z)
+    kmax = int(row_k.max())
+    nbr = t.neighbors_all(kmax, 1, threads=8)
+    kk = np.minimum(row_k, len(x)).astype(np.uint64)
+    offs = np.concatenate([[0], np.cumsum(kk)]).astype(np.uint64)
+    ids = np.concatenate([nbr[i, :kk[i]] for i in range(len(x))]).astype(np.uint32)
+    for st_, c_, s_ in zip(stages, cols, status):
+        ost = ol.stage(st_.stage, st_.knn, st_.stride, st_.mode, st_.features, st_.always_up,
+                       st_.normalize)
+        _, ps = oracle.stage_run_rowk(t, ost, row_k, init=np.nan)
+        assert np.array_equal(s_, ps)
+        assert np.all(np.isnan(c_[0][(s_ & SKIP_BITS) != 0]))
+        tc, ts, gap, ratio = oracle.truth_from_csr(x, y, z, offs, ids, ost, with_ratio=True)
+        ok = (s_ == 0) & (ts == 0) & (gap > 1e-3)
+        assert np.count_nonzero(ok) > 0.5 * len(x)
+        errs = truth_errors(ost, c_, tc, ok, ratio)
+        if max(errs.values()) > 1.0:
+            for j, (a, b) in enumerate(zip(c_, tc)):
+                d = np.where(ok, np.abs(a - b) / np.maximum(np.abs(b), 1e-6), 0)
+                w = int(np.argmax(d))
+                print(f"col {j}: row {w} k={kk[w]} dev={a[w]!r} truth={b[w]!r} "
+                      f"ratio={ratio[w]!r} gap={gap[w]!r}", flush=True)
+        assert max(errs.values()) <= 1.0, errs
+
+
+@pytest.mark.parametrize("seed", [41, 42])
+def test_rowk_random_counts(ctx, oracle, seed):
+    x, y, z = small_cloud(20000, seed=seed, dup_frac=0.02)
+    rng = np.random.default_rng(seed)
+    row_k = rng.integers(2, 40, len(x)).astype(np.uint32)
+    row_k[::53] = 0
+    print(f"rowk seed={seed} kmax={row_k.max()}", flush=True)
+    stages = [g.stage(g.STAGE_COVFEAT, 8, features=g.F_ALL10),
+              g.stage(g.STAGE_EIGENVALUES, 8), g.stage(g.STAGE_NORMAL, 8)]
+    _check(ctx, oracle, x, y, z, row_k, stages)
+
+
+def test_rowk_from_optimal_neighborhood(ctx, oracle):
+    x, y, z = small_cloud(30000, seed=43, dup_frac=0.01)
+    cl = ctx.cloud(x, y, z)
+    kopt, _ = cl.optimal_neighborhood(10, 14)
+    cl.close()
+    row_k = kopt.astype(np.uint32)
+    assert set(np.unique(row_k)) <= {0, 10, 11, 12, 13, 14}
+    _check(ctx, oracle, x, y, z, row_k,
+           [g.stage(g.STAGE_COVFEAT, 10, mode=g.MODE_NORMALIZED, features=g.F_ALL10)])
+
+
+def test_rowk_small_cloud_and_errors(ctx, oracle):
+    x, y, z = small_cloud(12, seed=44)
+    row_k = np.array([0, 2, 3, 5, 12, 40, 40, 3, 2, 0, 12, 7], np.uint32)  # 40 > n: clamped
+    cl = ctx.cloud(x, y, z)
+    cols, status = cl.run_rowk([g.stage(g.STAGE_EIGENVALUES, 8)], row_k, init=np.nan)
+    t = oracle.kd3(x, y, z)
+    _, ps = oracle.stage_run_rowk(t, ol.stage(ol.STAGE_EIGENVALUES, 8), row_k, init=np.nan)
+    assert np.array_equal(status[0], ps)
+    cl.close()
+    big = small_cloud(200, seed=45)
+    cl = ctx.cloud(*big)
+    with pytest.raises(RuntimeError, match="63"):
+        cl.run_rowk([g.stage(g.STAGE_EIGENVALUES, 8)], np.full(200, 64, np.uint32))
+    cl.close()
