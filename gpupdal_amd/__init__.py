@@ -66,7 +66,7 @@ class GfStats(ctypes.Structure):
                 ("grid_z", ctypes.c_uint32), ("key_bits", ctypes.c_uint32),
                 ("search_passes", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
                 ("merges", ctypes.c_uint64), ("ms_tree", ctypes.c_double),
-                ("ms_total", ctypes.c_double)]
+                ("ms_total", ctypes.c_double), ("deep_rows", ctypes.c_uint64)]
 
 
 class GfSynth(ctypes.Structure):

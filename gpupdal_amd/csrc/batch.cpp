@@ -304,14 +304,15 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         forest(stream, false);
         uint32_t* failed = counters.as<uint32_t>() + 5;
         launch_fill_u32(failed, 1, 0u, stream);
+        deep_rows.ensure(static_cast<uint64_t>(h[0]) * 4 + 4);
         launch_kd_resolve(b.kd.view, cv, prog0, 0, out, b.x.as<double>(), b.y.as<double>(),
-                          b.z.as<double>(), h[0], failed, stream, &tt);
+                          b.z.as<double>(), h[0], failed, deep_rows.as<uint32_t>(), stream, &tt);
         uint32_t nfail = 0;
         GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, stream));
         GF_HIP(hipStreamSynchronize(stream));
         if (nfail)
-            throw Error(GF_ERR_UNSUPPORTED, "exact KD3 search exceeded its depth bound on " +
-                                                std::to_string(nfail) + " rows");
+            resolve_deep(b.kd, T, cv, prog0, 0, out, b.x.as<double>(), b.y.as<double>(),
+                         b.z.as<double>(), nfail, failed, &tt);
         launch_features(cv, prog0, out, b.inv.as<uint32_t>(), stream, &tt, kFeaturesFlagged,
                         h[0]);
     }

@@ -40,8 +40,15 @@ GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch,
     for (;;) {
         g.c = c;
         g.inv_c = 1.0 / c;
-        g.gx = static_cast<uint32_t>(std::floor(sx * g.inv_c)) + 1;
-        g.gy = static_cast<uint32_t>(std::floor(sy * g.inv_c)) + 1;
+        // Column counts beyond 2^30 per axis are refused before the cast (the
+        // column cap below refuses them anyway).
+        const double fx = std::floor(sx * g.inv_c) + 1.0, fy = std::floor(sy * g.inv_c) + 1.0;
+        if (!(fx <= 1073741824.0 && fy <= 1073741824.0)) {
+            c *= 2.0;
+            continue;
+        }
+        g.gx = static_cast<uint32_t>(fx);
+        g.gy = static_cast<uint32_t>(fy);
         g.tiles_x = (g.gx + 7) / 8;
         const uint64_t tiles_y = (g.gy + 7) / 8;
         const uint64_t ncols = static_cast<uint64_t>(g.tiles_x) * tiles_y * 64;
@@ -438,17 +445,54 @@ uint32_t Context::resolve_flagged(Cloud& cl, const CloudView& cv, const FusedPro
     build_kdtree(cl);
     uint32_t* failed = counters.as<uint32_t>() + 5;
     launch_fill_u32(failed, 1, 0u, stream);
+    deep_rows.ensure(static_cast<uint64_t>(h[0]) * 4 + 4);
     launch_kd_resolve(cl.kd.view, cv, prog, mode, out, cl.x.as<double>(), cl.y.as<double>(),
-                      cl.z.as<double>(), h[0], failed, stream);
+                      cl.z.as<double>(), h[0], failed, deep_rows.as<uint32_t>(), stream);
     uint32_t nfail = 0;
     GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, stream));
     GF_HIP(hipStreamSynchronize(stream));
-    span(e0, mark(), &stats.ms_exact);
     if (nfail)
-        throw Error(GF_ERR_UNSUPPORTED,
-                    "exact KD3 search exceeded its depth bound on " + std::to_string(nfail) +
-                        " rows");
+        resolve_deep(cl.kd, 1, cv, prog, mode, out, cl.x.as<double>(), cl.y.as<double>(),
+                     cl.z.as<double>(), nfail, failed, nullptr);
+    span(e0, mark(), &stats.ms_exact);
     return h[0];
+}
+
+// Rows whose exact search outgrew the 64 register frames (a tree deeper
+// than 64 levels, as very unevenly spread points build): again, with frames
+// for the tree's whole depth in device memory.  The depth comes from the
+// tree's child arrays; at most kDeepFrameBudget bytes of frames are held.
+void Context::resolve_deep(KdBuffers& kd, uint32_t nroots, const CloudView& cv,
+                           const FusedProgram& prog, int mode, const KnnOutputs& out,
+                           const double* x, const double* y, const double* z, uint32_t nfail,
+                           const uint32_t* dcount, const TileTables* tt)
+{
+    constexpr uint64_t kDeepFrameBudget = 1ull << 30;
+    const uint32_t nodes = kd.nodes;
+    std::vector<int32_t> c1(nodes), c2(nodes);
+    GF_HIP(hipMemcpyAsync(c1.data(), kd.view.c1, nodes * 4ull, hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipMemcpyAsync(c2.data(), kd.view.c2, nodes * 4ull, hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipStreamSynchronize(stream));
+    const int depth = kd_tree_depth(c1, c2, nroots) + 1;
+    const uint64_t per_wave = static_cast<uint64_t>(depth) * kd_deep_frame_bytes();
+    if (per_wave > kDeepFrameBudget)
+        throw Error(GF_ERR_UNSUPPORTED, "exact KD3 tree of " + std::to_string(depth) +
+                                            " levels is deeper than the re-solve supports");
+    const uint32_t waves = static_cast<uint32_t>(
+        std::min<uint64_t>({static_cast<uint64_t>(nfail), kDeepFrameBudget / per_wave, 1024}));
+    deep_frames.ensure(waves * per_wave + 8);
+    counters.ensure(128);
+    uint32_t* failed2 = counters.as<uint32_t>() + 20;
+    launch_fill_u32(failed2, 1, 0u, stream);
+    launch_kd_resolve_deep(kd.view, cv, prog, mode, out, x, y, z, deep_rows.as<uint32_t>(), dcount,
+                           deep_frames.ptr, depth, waves, failed2, stream, tt);
+    uint32_t nfail2 = 0;
+    GF_HIP(hipMemcpyAsync(&nfail2, failed2, 4, hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipStreamSynchronize(stream));
+    stats.deep_rows = nfail;
+    if (nfail2)
+        throw Error(GF_ERR_UNSUPPORTED, "exact KD3 search exceeded the tree's depth on " +
+                                            std::to_string(nfail2) + " rows");
 }
 
 hipEvent_t Context::mark(hipStream_t s)
@@ -849,7 +893,7 @@ void Context::run_group_static(const double* x, const double* y, const double* z
                                                 std::to_string(rc) + ")");
         uint32_t* failed = cnt + 5;
         launch_fill_u32(failed, 1, 0u, stream);
-        launch_kd_resolve(t, cv, prog, 0, out, x, y, z, h[0], failed, stream);
+        launch_kd_resolve(t, cv, prog, 0, out, x, y, z, h[0], failed, nullptr, stream);
         launch_features(cv, prog, out, inv, stream, nullptr, kFeaturesFlagged, h[0]);
         uint32_t nfail = 0;
         GF_HIP(hipMemcpyAsync(&nfail, failed, 4, hipMemcpyDeviceToHost, stream));

@@ -2,6 +2,8 @@
 #pragma once
 
 #include "gf_internal.hpp"
+
+#include <vector>
 #include "gpupdal_amd.h"
 
 namespace gf {
@@ -224,10 +226,25 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
                          const uint64_t* starts, uint32_t ntiles, const double* bounds,
                          KdView t, KdScratch w, const KdTuning& kt, uint32_t* nodes,
                          hipStream_t s);
+// failed[0] counts rows whose search needs more than 64 recursion frames (a
+// tree deeper than 64 levels); deep_rows (nflagged entries, may be null)
+// receives their flagged indices for launch_kd_resolve_deep.
 void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram& prog, int mode,
                        const KnnOutputs& out, const double* x, const double* y, const double* z,
-                       uint32_t nflagged, uint32_t* failed, hipStream_t s,
+                       uint32_t nflagged, uint32_t* failed, uint32_t* deep_rows, hipStream_t s,
                        const TileTables* tt = nullptr);
+// The listed rows (ndeep: device count) with `depth` frames per wave in
+// global memory (frames: waves * depth * kd_deep_frame_bytes() bytes);
+// failed2 counts rows that still overflow.
+size_t kd_deep_frame_bytes();
+void launch_kd_resolve_deep(const KdView& t, const CloudView& cv, const FusedProgram& prog,
+                            int mode, const KnnOutputs& out, const double* x, const double* y,
+                            const double* z, const uint32_t* deep_rows, const uint32_t* ndeep,
+                            void* frames, int depth, uint32_t waves, uint32_t* failed2,
+                            hipStream_t s, const TileTables* tt = nullptr);
+// Levels of the deepest of the trees rooted at nodes [0, nroots).
+int kd_tree_depth(const std::vector<int32_t>& c1, const std::vector<int32_t>& c2,
+                  uint32_t nroots);
 
 // optnbhd.hip -- filters.optimalneighborhood from n x kmax KD3-ordered lists
 // of sorted positions (rows in original order; x, y, z original order, sx,

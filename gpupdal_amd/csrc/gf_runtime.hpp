@@ -246,6 +246,7 @@ struct Context {
     DevBuf xyz_stage;  // interleaved copy of the points between the keys and the gather
     DevBuf lasbuf;     // LAS point records (decode input, extra-bytes packing)
     DevBuf listbuf, liststat;  // kNN lists (gf_knn host copies, nndistance, outlier)
+    DevBuf deep_rows, deep_frames;  // exact re-solve of trees deeper than 64 levels
     // Exact-tree scratch, its own: the tree may be built on the side stream
     // while the index build uses scan_tmp on the main stream.
     DevBuf kd_s1, kd_s2, kd_tmp, kd_lists, kd_sums;
@@ -314,6 +315,13 @@ struct Context {
     // Per-row distance statistic (kNndKth, kNndAvg, kNndRunningMean) of the
     // k + 1 nearest points into out (device, original order).
     void nn_distance(Cloud& cl, uint32_t k, int mode, double* out);
+    // Re-solves the rows the first exact pass listed in deep_rows (count
+    // nfail on the host, dcount on the device) with frames for the whole
+    // tree depth; trees rooted at nodes [0, nroots).
+    void resolve_deep(KdBuffers& kd, uint32_t nroots, const CloudView& cv,
+                      const FusedProgram& prog, int mode, const KnnOutputs& out,
+                      const double* x, const double* y, const double* z, uint32_t nfail,
+                      const uint32_t* dcount, const TileTables* tt);
     // Returns the number of flagged rows.
     uint32_t resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog,
                              int mode, const KnnOutputs& out, hipEvent_t from);

@@ -27,6 +27,8 @@
 #include "gf_wave.hpp"
 
 #include <algorithm>
+#include <utility>
+#include <vector>
 
 namespace gf {
 
@@ -1466,12 +1468,278 @@ __device__ __forceinline__ uint32_t kd_tile_of(const uint64_t* __restrict__ star
     return lo;
 }
 
+// Recursion frames of the exact search: the node, the child not taken, the
+// cut axis and its two incremental distances, the node's box distance and
+// the frame's stage (0 enter, 1 other child pending, 2 restore).
+// RegFrames: frame i in lane i's registers, 64 frames (the tree's depth).
+struct RegFrames {
+    int f_node, f_other, f_idx, f_stage;
+    double f_mind, f_cut, f_dst;
+    int lane;
+    __device__ int cap() const { return 64; }
+    __device__ void init(int root, double mind)
+    {
+        f_node = root;
+        f_other = -1;
+        f_idx = 0;
+        f_stage = 0;
+        f_mind = mind;
+        f_cut = 0.0;
+        f_dst = 0.0;
+    }
+    __device__ int node(int i) const { return lane_get(f_node, i); }
+    __device__ int stage(int i) const { return lane_get(f_stage, i); }
+    __device__ int other(int i) const { return lane_get(f_other, i); }
+    __device__ int idx(int i) const { return lane_get(f_idx, i); }
+    __device__ double mind(int i) const { return lane_get(f_mind, i); }
+    __device__ double cut(int i) const { return lane_get(f_cut, i); }
+    __device__ double dst(int i) const { return lane_get(f_dst, i); }
+    __device__ void push(int i, int nd, double m)
+    {
+        if (lane == i) {
+            f_node = nd;
+            f_stage = 0;
+            f_mind = m;
+        }
+    }
+    __device__ void set_split(int i, int oth, double c, int ix)
+    {
+        if (lane == i) {
+            f_other = oth;
+            f_cut = c;
+            f_idx = ix;
+            f_stage = 1;
+        }
+    }
+    __device__ void set_dst(int i, double d)
+    {
+        if (lane == i) {
+            f_dst = d;
+            f_stage = 2;
+        }
+    }
+};
+
+// MemFrames: the frames in global memory, each lane keeping its own copy of
+// every frame (the values are uniform across the wave, so every lane reads
+// back what it wrote itself), for trees deeper than 64 levels.
+struct MemFrame {
+    int node, other, idx, stage;
+    double mind, cut, dst;
+};
+
+struct MemFrames {
+    MemFrame* base;  // depth x 64 frames of this wave
+    int depth;
+    int lane;
+    __device__ int cap() const { return depth; }
+    __device__ MemFrame& at(int i) const { return base[static_cast<size_t>(i) * 64 + lane]; }
+    __device__ void init(int root, double mind)
+    {
+        MemFrame& f = at(0);
+        f.node = root;
+        f.other = -1;
+        f.idx = 0;
+        f.stage = 0;
+        f.mind = mind;
+        f.cut = 0.0;
+        f.dst = 0.0;
+    }
+    __device__ int node(int i) const { return at(i).node; }
+    __device__ int stage(int i) const { return at(i).stage; }
+    __device__ int other(int i) const { return at(i).other; }
+    __device__ int idx(int i) const { return at(i).idx; }
+    __device__ double mind(int i) const { return at(i).mind; }
+    __device__ double cut(int i) const { return at(i).cut; }
+    __device__ double dst(int i) const { return at(i).dst; }
+    __device__ void push(int i, int nd, double m)
+    {
+        MemFrame& f = at(i);
+        f.node = nd;
+        f.stage = 0;
+        f.mind = m;
+    }
+    __device__ void set_split(int i, int oth, double c, int ix)
+    {
+        MemFrame& f = at(i);
+        f.other = oth;
+        f.cut = c;
+        f.idx = ix;
+        f.stage = 1;
+    }
+    __device__ void set_dst(int i, double d)
+    {
+        MemFrame& f = at(i);
+        f.dst = d;
+        f.stage = 2;
+    }
+};
+
+// One flagged row (flagged[f]) re-solved by one wave; returns true when the
+// tree is deeper than the frames hold (nothing written for the row).
+template <class Frames>
+__device__ bool kd_resolve_row(const KdView& t, const CloudView& cv, const FusedProgram& prog0,
+                               int mode, const KnnOutputs& out, const double* __restrict__ x,
+                               const double* __restrict__ y, const double* __restrict__ z,
+                               const TileInfo* __restrict__ tiles,
+                               const uint64_t* __restrict__ starts, uint32_t ntiles,
+                               uint32_t f, Frames& fr)
+{
+    const int lane = __lane_id();
+    const uint32_t s = out.flagged[f];
+    // The row's tile: its root node (tile r's tree is rooted at node r)
+    // and its neighborhood size.
+    uint32_t root = 0;
+    uint32_t pad_id = 0;  // KD3Index pads short strided lists with the view's first point
+    uint32_t cap = static_cast<uint32_t>(prog0.fetch);
+    uint32_t k = static_cast<uint32_t>(prog0.k);
+    uint32_t stride = static_cast<uint32_t>(prog0.stride);
+    if (tiles) {
+        root = kd_tile_of(starts, ntiles, s);
+        pad_id = static_cast<uint32_t>(starts[root]);
+        const FusedProgram& pr = tiles[root].prog;
+        cap = static_cast<uint32_t>(pr.fetch);
+        k = static_cast<uint32_t>(pr.k);
+        stride = static_cast<uint32_t>(pr.stride);
+    }
+    const double* rb = t.box + 6 * static_cast<size_t>(root);  // the tile's bounds
+    const uint32_t qid = cv.perm[s];
+    const double q0 = x[qid], q1 = y[qid], q2 = z[qid];
+    const double q[3] = {q0, q1, q2};
+    Ent best = ent_inf();
+    uint32_t seq = 0;
+    double dists[3] = {0.0, 0.0, 0.0};
+    double mind = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        if (q[i] < rb[i]) {
+            dists[i] = (q[i] - rb[i]) * (q[i] - rb[i]);
+            mind += dists[i];
+        }
+        if (q[i] > rb[3 + i]) {
+            dists[i] = (q[i] - rb[3 + i]) * (q[i] - rb[3 + i]);
+            mind += dists[i];
+        }
+    }
+    fr.init(static_cast<int>(root), mind);
+    const int depth_cap = fr.cap();
+    int sp = 1;
+    while (sp > 0) {
+        const int top = sp - 1;
+        const int nd = fr.node(top);
+        const int stage = fr.stage(top);
+        if (stage == 0) {
+            if (t.c1[nd] < 0) {
+                const Ent wst = ent_shfl(best, static_cast<int>(cap) - 1);
+                const double worst = wst.d == kInfD ? INFINITY : ent_d2(wst);
+                const uint32_t l0 = t.nl[nd], l1 = t.nr[nd];
+                for (uint32_t i0 = l0; i0 < l1; i0 += 64) {
+                    const uint32_t i = i0 + lane;
+                    Ent c = ent_inf();
+                    if (i < l1) {
+                        double d = 0.0;
+                        const double d0 = q0 - t.px[i];
+                        d += d0 * d0;
+                        const double d1 = q1 - t.py[i];
+                        d += d1 * d1;
+                        const double d2 = q2 - t.pz[i];
+                        d += d2 * d2;
+                        if (d < worst)
+                            c = Ent{static_cast<uint64_t>(__double_as_longlong(d)),
+                                    seq + (i - l0), t.vind[i]};
+                    }
+                    if (__ballot(c.d != kInfD)) {
+                        bitonic_sort64(c, lane);
+                        merge_sorted(best, c, lane);
+                    }
+                }
+                seq += l1 - l0;
+                --sp;
+                continue;
+            }
+            const int idx = t.feat[nd];
+            const double val = q[idx];
+            const double dl = t.divlow[nd], dh = t.divhigh[nd];
+            const double diff1 = val - dl;
+            const double diff2 = val - dh;
+            int bestc, other;
+            double cut;
+            if ((diff1 + diff2) < 0) {
+                bestc = t.c1[nd];
+                other = t.c2[nd];
+                cut = (val - dh) * (val - dh);
+            } else {
+                bestc = t.c2[nd];
+                other = t.c1[nd];
+                cut = (val - dl) * (val - dl);
+            }
+            const double fm = fr.mind(top);
+            fr.set_split(top, other, cut, idx);
+            if (sp >= depth_cap) return true;
+            fr.push(sp, bestc, fm);
+            ++sp;
+        } else if (stage == 1) {
+            const int idx = fr.idx(top);
+            const double cut = fr.cut(top);
+            const double dst = dists[idx];
+            const double m2 = fr.mind(top) + cut - dst;
+            dists[idx] = cut;
+            fr.set_dst(top, dst);
+            const Ent wst = ent_shfl(best, static_cast<int>(cap) - 1);
+            const double worst = wst.d == kInfD ? INFINITY : ent_d2(wst);
+            if (m2 * 1.0 <= worst) {
+                if (sp >= depth_cap) return true;
+                fr.push(sp, fr.other(top), m2);
+                ++sp;
+            }
+        } else {
+            dists[fr.idx(top)] = fr.dst(top);
+            --sp;
+        }
+    }
+    // KD3ImplT::neighbors: every stride-th entry of the result set; slots
+    // past the points found keep the output's zero initialisation.
+    const Ent kept = ent_shfl(best, static_cast<int>((static_cast<uint32_t>(lane) * stride) & 63u));
+    const bool mine = static_cast<uint32_t>(lane) < k;
+    const uint32_t kept_id = kept.d == kInfD ? pad_id : kept.j;
+    if (mode != 0) {  // neighbor lists
+        if (mine) out.ids[static_cast<uint64_t>(qid) * k + lane] = kept_id;
+        return false;
+    }
+    // Coordinates relative to the query point (exact differences), so the
+    // mean carries no rounding of the cloud's absolute offset.
+    const uint32_t id = mine ? kept_id : 0u;
+    const double px = mine ? x[id] - q0 : 0.0, py = mine ? y[id] - q1 : 0.0,
+                 pz = mine ? z[id] - q2 : 0.0;
+    const double mx = wave_sum(px) / static_cast<double>(k),
+                 my = wave_sum(py) / static_cast<double>(k),
+                 mz = wave_sum(pz) / static_cast<double>(k);
+    const double dx = mine ? px - mx : 0.0, dy = mine ? py - my : 0.0,
+                 dz = mine ? pz - mz : 0.0;
+    const double den = static_cast<double>(k - 1);
+    const double c0 = wave_sum(dx * dx) / den, c1 = wave_sum(dx * dy) / den,
+                 c2 = wave_sum(dx * dz) / den, c3 = wave_sum(dy * dy) / den,
+                 c4 = wave_sum(dy * dz) / den, c5 = wave_sum(dz * dz) / den;
+    if (lane == 0) {
+        double* rec = out.cov + kCovRecord * static_cast<uint64_t>(s);
+        rec[0] = c0;
+        rec[1] = c1;
+        rec[2] = c2;
+        rec[3] = c3;
+        rec[4] = c4;
+        rec[5] = c5;
+    }
+    return false;
+}
+
+// Rows whose search needs more than 64 frames are counted in failed[0] and,
+// with deep_rows, listed there for k_kd_resolve_deep.
 __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, FusedProgram prog0,
                                                     int mode, KnnOutputs out,
                                                     const double* __restrict__ x,
                                                     const double* __restrict__ y,
                                                     const double* __restrict__ z,
                                                     uint32_t* __restrict__ failed,
+                                                    uint32_t* __restrict__ deep_rows,
                                                     const TileInfo* __restrict__ tiles,
                                                     const uint64_t* __restrict__ starts,
                                                     uint32_t ntiles)
@@ -1480,177 +1748,44 @@ __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, Fuse
     const uint32_t waves = gridDim.x * (blockDim.x / 64);
     const uint32_t nflag = *out.nflagged;
     for (uint32_t f = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; f < nflag; f += waves) {
-        const uint32_t s = out.flagged[f];
-        // The row's tile: its root node (tile r's tree is rooted at node r)
-        // and its neighborhood size.
-        uint32_t root = 0;
-        uint32_t pad_id = 0;  // KD3Index pads short strided lists with the view's first point
-        uint32_t cap = static_cast<uint32_t>(prog0.fetch);
-        uint32_t k = static_cast<uint32_t>(prog0.k);
-        uint32_t stride = static_cast<uint32_t>(prog0.stride);
-        if (tiles) {
-            root = kd_tile_of(starts, ntiles, s);
-            pad_id = static_cast<uint32_t>(starts[root]);
-            const FusedProgram& pr = tiles[root].prog;
-            cap = static_cast<uint32_t>(pr.fetch);
-            k = static_cast<uint32_t>(pr.k);
-            stride = static_cast<uint32_t>(pr.stride);
+        RegFrames fr;
+        fr.lane = lane;
+        if (kd_resolve_row(t, cv, prog0, mode, out, x, y, z, tiles, starts, ntiles, f, fr) &&
+            lane == 0) {
+            const uint32_t slot = atomicAdd(failed, 1u);
+            if (deep_rows) deep_rows[slot] = f;
         }
-        const double* rb = t.box + 6 * static_cast<size_t>(root);  // the tile's bounds
-        const uint32_t qid = cv.perm[s];
-        const double q0 = x[qid], q1 = y[qid], q2 = z[qid];
-        const double q[3] = {q0, q1, q2};
-        Ent best = ent_inf();
-        uint32_t seq = 0;
-        double dists[3] = {0.0, 0.0, 0.0};
-        double mind = 0.0;
-        for (int i = 0; i < 3; ++i) {
-            if (q[i] < rb[i]) {
-                dists[i] = (q[i] - rb[i]) * (q[i] - rb[i]);
-                mind += dists[i];
-            }
-            if (q[i] > rb[3 + i]) {
-                dists[i] = (q[i] - rb[3 + i]) * (q[i] - rb[3 + i]);
-                mind += dists[i];
-            }
-        }
-        // Frame registers: lane i holds stack frame i.
-        int f_node = static_cast<int>(root), f_other = -1, f_idx = 0, f_stage = 0;
-        double f_mind = mind, f_cut = 0.0, f_dst = 0.0;
-        int sp = 1;
-        bool overflow = false;
-        while (sp > 0) {
-            const int top = sp - 1;
-            const int nd = lane_get(f_node, top);
-            const int stage = lane_get(f_stage, top);
-            if (stage == 0) {
-                if (t.c1[nd] < 0) {
-                    const Ent wst = ent_shfl(best, static_cast<int>(cap) - 1);
-                    const double worst = wst.d == kInfD ? INFINITY : ent_d2(wst);
-                    const uint32_t l0 = t.nl[nd], l1 = t.nr[nd];
-                    for (uint32_t i0 = l0; i0 < l1; i0 += 64) {
-                        const uint32_t i = i0 + lane;
-                        Ent c = ent_inf();
-                        if (i < l1) {
-                            double d = 0.0;
-                            const double d0 = q0 - t.px[i];
-                            d += d0 * d0;
-                            const double d1 = q1 - t.py[i];
-                            d += d1 * d1;
-                            const double d2 = q2 - t.pz[i];
-                            d += d2 * d2;
-                            if (d < worst)
-                                c = Ent{static_cast<uint64_t>(__double_as_longlong(d)),
-                                        seq + (i - l0), t.vind[i]};
-                        }
-                        if (__ballot(c.d != kInfD)) {
-                            bitonic_sort64(c, lane);
-                            merge_sorted(best, c, lane);
-                        }
-                    }
-                    seq += l1 - l0;
-                    --sp;
-                    continue;
-                }
-                const int idx = t.feat[nd];
-                const double val = q[idx];
-                const double dl = t.divlow[nd], dh = t.divhigh[nd];
-                const double diff1 = val - dl;
-                const double diff2 = val - dh;
-                int bestc, other;
-                double cut;
-                if ((diff1 + diff2) < 0) {
-                    bestc = t.c1[nd];
-                    other = t.c2[nd];
-                    cut = (val - dh) * (val - dh);
-                } else {
-                    bestc = t.c2[nd];
-                    other = t.c1[nd];
-                    cut = (val - dl) * (val - dl);
-                }
-                const double fm = lane_get(f_mind, top);
-                if (lane == top) {
-                    f_other = other;
-                    f_cut = cut;
-                    f_idx = idx;
-                    f_stage = 1;
-                }
-                if (sp >= 64) {
-                    overflow = true;
-                    break;
-                }
-                if (lane == sp) {
-                    f_node = bestc;
-                    f_stage = 0;
-                    f_mind = fm;
-                }
-                ++sp;
-            } else if (stage == 1) {
-                const int idx = lane_get(f_idx, top);
-                const double cut = lane_get(f_cut, top);
-                const double dst = dists[idx];
-                const double m2 = lane_get(f_mind, top) + cut - dst;
-                dists[idx] = cut;
-                if (lane == top) {
-                    f_dst = dst;
-                    f_stage = 2;
-                }
-                const Ent wst = ent_shfl(best, static_cast<int>(cap) - 1);
-                const double worst = wst.d == kInfD ? INFINITY : ent_d2(wst);
-                if (m2 * 1.0 <= worst) {
-                    if (sp >= 64) {
-                        overflow = true;
-                        break;
-                    }
-                    const int other = lane_get(f_other, top);
-                    if (lane == sp) {
-                        f_node = other;
-                        f_stage = 0;
-                        f_mind = m2;
-                    }
-                    ++sp;
-                }
-            } else {
-                dists[lane_get(f_idx, top)] = lane_get(f_dst, top);
-                --sp;
-            }
-        }
-        if (overflow) {
-            if (lane == 0) atomicAdd(failed, 1u);
-            continue;
-        }
-        // KD3ImplT::neighbors: every stride-th entry of the result set; slots
-        // past the points found keep the output's zero initialisation.
-        const Ent kept = ent_shfl(best, static_cast<int>((static_cast<uint32_t>(lane) * stride) & 63u));
-        const bool mine = static_cast<uint32_t>(lane) < k;
-        const uint32_t kept_id = kept.d == kInfD ? pad_id : kept.j;
-        if (mode != 0) {  // neighbor lists
-            if (mine) out.ids[static_cast<uint64_t>(qid) * k + lane] = kept_id;
-            continue;
-        }
-        // Coordinates relative to the query point (exact differences), so the
-        // mean carries no rounding of the cloud's absolute offset.
-        const uint32_t id = mine ? kept_id : 0u;
-        const double px = mine ? x[id] - q0 : 0.0, py = mine ? y[id] - q1 : 0.0,
-                     pz = mine ? z[id] - q2 : 0.0;
-        const double mx = wave_sum(px) / static_cast<double>(k),
-                     my = wave_sum(py) / static_cast<double>(k),
-                     mz = wave_sum(pz) / static_cast<double>(k);
-        const double dx = mine ? px - mx : 0.0, dy = mine ? py - my : 0.0,
-                     dz = mine ? pz - mz : 0.0;
-        const double den = static_cast<double>(k - 1);
-        const double c0 = wave_sum(dx * dx) / den, c1 = wave_sum(dx * dy) / den,
-                     c2 = wave_sum(dx * dz) / den, c3 = wave_sum(dy * dy) / den,
-                     c4 = wave_sum(dy * dz) / den, c5 = wave_sum(dz * dz) / den;
-        if (lane == 0) {
-            double* rec = out.cov + kCovRecord * static_cast<uint64_t>(s);
-            rec[0] = c0;
-            rec[1] = c1;
-            rec[2] = c2;
-            rec[3] = c3;
-            rec[4] = c4;
-            rec[5] = c5;
-        }
+    }
+}
+
+// The rows k_kd_resolve listed, with frames for the tree's whole depth in
+// global memory (frames: waves x depth x 64); rows that still overflow are
+// counted in failed2.
+__global__ __launch_bounds__(64) void k_kd_resolve_deep(KdView t, CloudView cv,
+                                                        FusedProgram prog0, int mode,
+                                                        KnnOutputs out,
+                                                        const double* __restrict__ x,
+                                                        const double* __restrict__ y,
+                                                        const double* __restrict__ z,
+                                                        const uint32_t* __restrict__ deep_rows,
+                                                        const uint32_t* __restrict__ ndeep,
+                                                        MemFrame* __restrict__ frames, int depth,
+                                                        uint32_t* __restrict__ failed2,
+                                                        const TileInfo* __restrict__ tiles,
+                                                        const uint64_t* __restrict__ starts,
+                                                        uint32_t ntiles)
+{
+    const int lane = __lane_id();
+    const uint32_t nd = *ndeep;
+    MemFrames fr;
+    fr.base = frames + static_cast<size_t>(blockIdx.x) * static_cast<size_t>(depth) * 64;
+    fr.depth = depth;
+    fr.lane = lane;
+    for (uint32_t i = blockIdx.x; i < nd; i += gridDim.x) {
+        if (kd_resolve_row(t, cv, prog0, mode, out, x, y, z, tiles, starts, ntiles,
+                           deep_rows[i], fr) &&
+            lane == 0)
+            atomicAdd(failed2, 1u);
     }
 }
 
@@ -1801,14 +1936,50 @@ uint32_t kd_list_capacity(uint64_t n) { return static_cast<uint32_t>(n / kSubSma
 
 void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram& prog, int mode,
                        const KnnOutputs& out, const double* x, const double* y, const double* z,
-                       uint32_t nflagged, uint32_t* failed, hipStream_t s,
+                       uint32_t nflagged, uint32_t* failed, uint32_t* deep_rows, hipStream_t s,
                        const TileTables* tt)
 {
     if (!nflagged) return;
     const uint32_t blocks = std::min<uint32_t>(div_up(nflagged, 4), 4096);
     hipLaunchKernelGGL(k_kd_resolve, dim3(blocks), dim3(256), 0, s, t, cv, prog, mode, out, x, y,
-                       z, failed, tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
+                       z, failed, deep_rows, tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
                        tt ? tt->ntiles : 0u);
+}
+
+size_t kd_deep_frame_bytes() { return sizeof(MemFrame) * 64; }
+
+void launch_kd_resolve_deep(const KdView& t, const CloudView& cv, const FusedProgram& prog,
+                            int mode, const KnnOutputs& out, const double* x, const double* y,
+                            const double* z, const uint32_t* deep_rows, const uint32_t* ndeep,
+                            void* frames, int depth, uint32_t waves, uint32_t* failed2,
+                            hipStream_t s, const TileTables* tt)
+{
+    if (!waves) return;
+    hipLaunchKernelGGL(k_kd_resolve_deep, dim3(waves), dim3(64), 0, s, t, cv, prog, mode, out, x,
+                       y, z, deep_rows, ndeep, static_cast<MemFrame*>(frames), depth, failed2,
+                       tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
+                       tt ? tt->ntiles : 0u);
+}
+
+// Depth of the deepest node (root 0) of a tree given its child arrays, or of
+// the deepest of several roots [0, nroots).
+int kd_tree_depth(const std::vector<int32_t>& c1, const std::vector<int32_t>& c2,
+                  uint32_t nroots)
+{
+    int best = 0;
+    std::vector<std::pair<int32_t, int>> st;
+    for (uint32_t r = 0; r < nroots; ++r) {
+        st.push_back({static_cast<int32_t>(r), 1});
+        while (!st.empty()) {
+            const auto [nd, d] = st.back();
+            st.pop_back();
+            best = std::max(best, d);
+            if (nd < 0 || static_cast<size_t>(nd) >= c1.size()) continue;
+            if (c1[nd] >= 0) st.push_back({c1[nd], d + 1});
+            if (c2[nd] >= 0) st.push_back({c2[nd], d + 1});
+        }
+    }
+    return best;
 }
 
 } // namespace gf

@@ -129,6 +129,9 @@ typedef struct gf_stats {
     double ms_tree;           /* exact KD3 tree build on its own stream (it overlaps
                                  the search; 0 when the tree was cached) */
     double ms_total;          /* the call's device time, index build to features */
+    uint64_t deep_rows;       /* flagged rows whose exact search needed more than 64
+                                 recursion frames (a tree deeper than 64 levels) and
+                                 ran again with frames in device memory */
 } gf_stats;
 
 /* ---- context: replaces pdg::CudaMemoryResource + stream ownership
