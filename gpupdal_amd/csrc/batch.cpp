@@ -190,7 +190,7 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         if (knn < 1) throw Error(GF_ERR_ARG, "knn must be at least 1");
         const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(nt, static_cast<uint64_t>(knn) + 1));
         const uint32_t f = static_cast<uint32_t>(stride) * k;
-        check_neighborhood(nt, static_cast<int64_t>(knn) + 1, stride);
+        check_neighborhood(nt, static_cast<int64_t>(knn) + 1, stride, kMaxFetch - 1);
         fetch[t] = std::max<uint32_t>(f, 1);
         TileInfo& ti = tiles[t];
         ti.prog = base;

@@ -221,12 +221,12 @@ int64_t stage_neighbors(const gf_stage& s)
                : static_cast<int64_t>(s.knn) + 1;
 }
 
-void check_neighborhood(uint64_t n, int64_t K, int stride)
+void check_neighborhood(uint64_t n, int64_t K, int stride, uint32_t limit)
 {
     const uint64_t k = std::min<uint64_t>(n, static_cast<uint64_t>(K));
-    if (static_cast<uint64_t>(stride) * k > static_cast<uint64_t>(kMaxFetch - 1))
-        throw Error(GF_ERR_UNSUPPORTED,
-                    "(knn+1)*stride above 63 is outside the device kNN envelope");
+    if (static_cast<uint64_t>(stride) * k > limit)
+        throw Error(GF_ERR_UNSUPPORTED, "(knn+1)*stride above " + std::to_string(limit) +
+                                            " is outside the device kNN envelope");
 }
 
 // Runs one group of stages that share the neighborhood (knn, stride).
@@ -242,13 +242,17 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     const uint64_t K = static_cast<uint64_t>(stage_neighbors(st[0]));
     const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(n, K));
     const uint32_t fetch = static_cast<uint32_t>(stride) * k;
-    check_neighborhood(n, stage_neighbors(st[0]), stride);
+    // Neighborhoods of up to 63 entries (knn + 1) * stride run the grid
+    // search; larger ones (up to the exact search's capacity) run every row
+    // through the exact search.
+    check_neighborhood(n, stage_neighbors(st[0]), stride, kd_max_capacity());
     FusedProgram prog{};
     prog.nstages = ns;
     prog.k = static_cast<int32_t>(k);
     prog.stride = stride;
     prog.fetch = static_cast<int32_t>(fetch);
     const int ncol_total = fill_stage_programs(prog, st, ns);
+    const bool all_exact = fetch > n || fetch > static_cast<uint32_t>(kMaxFetch - 1);
 
     reset_spans();
     hipEvent_t t_all = mark();
@@ -286,11 +290,11 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     // The exact tree depends only on the points: on large clouds, which
     // nearly always have a flagged row, it is built on the side stream while
     // the search runs.
-    const bool early = fetch <= n && !cl.kd_valid && tune.kd_overlap_min &&
+    const bool early = !all_exact && !cl.kd_valid && tune.kd_overlap_min &&
                        n >= tune.kd_overlap_min;
     if (early) fork_side();
     hipEvent_t e0 = mark();
-    if (fetch > n)  // fewer points than the result capacity: all rows exact
+    if (all_exact)  // fewer points than the result capacity, or a large one
         launch_flag_all(n, cv.perm, 0, out, stream);
     else
         launch_knn(cv, prog, 0, out, knn_tuning(), stream);
@@ -590,10 +594,13 @@ void Context::knn_lists(Cloud& cl, uint32_t k, uint32_t stride, int mode, uint32
 {
     const uint64_t n = cl.n;
     const uint32_t kk = static_cast<uint32_t>(std::min<uint64_t>(n, k));
-    const uint32_t fetch = kk * stride;
-    if (fetch > static_cast<uint32_t>(kMaxFetch - 1))
-        throw Error(GF_ERR_UNSUPPORTED, "k*stride above 63 is outside the device kNN envelope");
+    const uint64_t fetch64 = static_cast<uint64_t>(kk) * stride;
+    if (fetch64 > kd_max_capacity())
+        throw Error(GF_ERR_UNSUPPORTED, "k*stride above " + std::to_string(kd_max_capacity()) +
+                                            " is outside the device kNN envelope");
+    const uint32_t fetch = static_cast<uint32_t>(fetch64);
     if (mode == 2 && stride != 1) throw Error(GF_ERR_ARG, "sorted-position lists need stride 1");
+    const bool all_exact = fetch > n || fetch > static_cast<uint32_t>(kMaxFetch - 1);
     points_mark = false;
     build_index(cl, fetch);
     FusedProgram prog{};
@@ -609,10 +616,10 @@ void Context::knn_lists(Cloud& cl, uint32_t k, uint32_t stride, int mode, uint32
     out.nflagged = counters.as<uint32_t>();
     launch_fill_u32(out.nflagged, 4, 0u, stream);
     const CloudView cv = view(cl);
-    const bool early = fetch <= n && !cl.kd_valid && tune.kd_overlap_min &&
+    const bool early = !all_exact && !cl.kd_valid && tune.kd_overlap_min &&
                        n >= tune.kd_overlap_min;
     if (early) fork_side();
-    if (fetch > n)
+    if (all_exact)
         launch_flag_all(n, cv.perm, mode, out, stream);
     else
         launch_knn(cv, prog, mode, out, knn_tuning(), stream);
@@ -655,9 +662,10 @@ void Context::run_rowk(Cloud& cl, const gf_stage* st, int ns, const uint32_t* ro
     GF_HIP(hipMemcpyAsync(&K, dmax, 4, hipMemcpyDeviceToHost, stream));
     GF_HIP(hipStreamSynchronize(stream));
     const uint32_t kk = static_cast<uint32_t>(std::min<uint64_t>(n, K));
-    if (kk > static_cast<uint32_t>(kMaxFetch - 1))
-        throw Error(GF_ERR_UNSUPPORTED,
-                    "a per-row neighbor count above 63 is outside the device kNN envelope");
+    if (kk > kd_max_capacity())
+        throw Error(GF_ERR_UNSUPPORTED, "a per-row neighbor count above " +
+                                            std::to_string(kd_max_capacity()) +
+                                            " is outside the device kNN envelope");
     prog.k = static_cast<int32_t>(kk);
     prog.stride = 1;
     prog.fetch = static_cast<int32_t>(kk);
@@ -769,7 +777,7 @@ void Context::run_group_static(const double* x, const double* y, const double* z
     const uint64_t K = static_cast<uint64_t>(stage_neighbors(st[0]));
     const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(n, K));
     const uint32_t fetch = static_cast<uint32_t>(stride) * k;
-    check_neighborhood(n, stage_neighbors(st[0]), stride);
+    check_neighborhood(n, stage_neighbors(st[0]), stride, kMaxFetch - 1);
     FusedProgram prog{};
     prog.nstages = ns;
     prog.k = static_cast<int32_t>(k);

@@ -237,6 +237,9 @@ void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram&
 // global memory (frames: waves * depth * kd_deep_frame_bytes() bytes);
 // failed2 counts rows that still overflow.
 size_t kd_deep_frame_bytes();
+// Largest result capacity ((knn + 1) * stride) of the exact search; above 64
+// its result set lives in LDS.
+uint32_t kd_max_capacity();
 void launch_kd_resolve_deep(const KdView& t, const CloudView& cv, const FusedProgram& prog,
                             int mode, const KnnOutputs& out, const double* x, const double* y,
                             const double* z, const uint32_t* deep_rows, const uint32_t* ndeep,

@@ -211,8 +211,10 @@ int stage_ncols(const gf_stage& s);
 int fill_stage_programs(FusedProgram& prog, const gf_stage* st, int ns);
 void validate_stage(const gf_stage& s);
 // Throws GF_ERR_UNSUPPORTED when a cloud (or tile) of n points whose search
-// asks K neighbors with this stride is outside what the device search handles.
-void check_neighborhood(uint64_t n, int64_t K, int stride);
+// asks K neighbors with this stride holds more than `limit` entries: 63 for
+// the grid search (batches, serving, the no-allocation path), the exact
+// search's capacity for gf_run.
+void check_neighborhood(uint64_t n, int64_t K, int stride, uint32_t limit);
 // Neighbors a stage's search asks (knn + 1, or knn for approximatecoplanar
 // and estimaterank).
 int64_t stage_neighbors(const gf_stage& s);

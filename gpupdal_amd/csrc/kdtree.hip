@@ -1575,38 +1575,112 @@ struct MemFrames {
     }
 };
 
-// One flagged row (flagged[f]) re-solved by one wave; returns true when the
-// tree is deeper than the frames hold (nothing written for the row).
-template <class Frames>
-__device__ bool kd_resolve_row(const KdView& t, const CloudView& cv, const FusedProgram& prog0,
-                               int mode, const KnnOutputs& out, const double* __restrict__ x,
-                               const double* __restrict__ y, const double* __restrict__ z,
-                               const TileInfo* __restrict__ tiles,
-                               const uint64_t* __restrict__ starts, uint32_t ntiles,
-                               uint32_t f, Frames& fr)
+// Result sets of the exact search, both keyed by (d2, visit order): the
+// capacity smallest by that key is exactly what KNNResultSet::addPoint keeps
+// when the points arrive one by one with strict < (earlier points win ties).
+// RegResults: one entry per lane, capacity <= 64.
+struct RegResults {
+    Ent best;
+    uint32_t cap;
+    int lane;
+    __device__ void init(uint32_t c, int l)
+    {
+        best = ent_inf();
+        cap = c;
+        lane = l;
+    }
+    __device__ double worst() const
+    {
+        const Ent w = ent_shfl(best, static_cast<int>(cap) - 1);
+        return w.d == kInfD ? INFINITY : ent_d2(w);
+    }
+    // c: at most one candidate per lane (ent_inf() for none).
+    __device__ void add(Ent c)
+    {
+        bitonic_sort64(c, lane);
+        merge_sorted(best, c, lane);
+    }
+};
+
+// LdsResults: a sorted array of up to cap (<= kBigCap) entries in LDS, merged
+// out of place (buf -> other) with every entry's final rank: an existing
+// entry moves up by the candidates below it, a candidate lands after the
+// entries below it.  The workgroup is one wave.
+constexpr uint32_t kBigCap = 1024;
+
+struct LdsResults {
+    Ent* buf;
+    Ent* other;
+    uint32_t cap;
+    uint32_t count;
+    int lane;
+    __device__ void init(Ent* lds, uint32_t c, int l)
+    {
+        buf = lds;
+        other = lds + c;
+        cap = c;
+        count = 0;
+        lane = l;
+    }
+    __device__ double worst() const
+    {
+        return count < cap ? INFINITY : ent_d2(buf[cap - 1]);
+    }
+    __device__ void add(Ent c)
+    {
+        bitonic_sort64(c, lane);
+        const uint32_t nc = static_cast<uint32_t>(__popcll(__ballot(c.d != kInfD)));
+        // Existing entries: rank among the sorted candidates by binary search
+        // over the lanes.
+        for (uint32_t j0 = 0; j0 < count; j0 += 64) {
+            const uint32_t j = j0 + static_cast<uint32_t>(lane);
+            const Ent e = j < count ? buf[j] : ent_inf();
+            uint32_t lo = 0, hi = nc;  // candidates [0, lo) are below e
+            for (int step = 0; step < 7; ++step) {  // 2^7 > 64: converged
+                const uint32_t mid = (lo + hi) >> 1;
+                const Ent cm = ent_shfl(c, static_cast<int>(mid < 64 ? mid : 63));
+                const bool below = ent_less(cm, e);
+                if (lo < hi) {
+                    if (below)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+            }
+            const uint32_t pos = j + lo;
+            if (j < count && pos < cap) other[pos] = e;
+        }
+        // Candidates: rank among the existing entries.
+        if (static_cast<uint32_t>(lane) < nc) {
+            uint32_t lo = 0, hi = count;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (ent_less(buf[mid], c))
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            const uint32_t pos = static_cast<uint32_t>(lane) + lo;
+            if (pos < cap) other[pos] = c;
+        }
+        __syncthreads();
+        Ent* t = buf;
+        buf = other;
+        other = t;
+        count = min(cap, count + nc);
+    }
+};
+
+// The exact KD3Index search of one query (a uniform walk of the tree rooted
+// at `root`, closer child first, the other child when the incremental box
+// distance is <= the worst kept distance); returns true when the tree is
+// deeper than the frames hold.
+template <class Frames, class Results>
+__device__ bool kd_traverse(const KdView& t, uint32_t root, const double q[3], Frames& fr,
+                            Results& res)
 {
     const int lane = __lane_id();
-    const uint32_t s = out.flagged[f];
-    // The row's tile: its root node (tile r's tree is rooted at node r)
-    // and its neighborhood size.
-    uint32_t root = 0;
-    uint32_t pad_id = 0;  // KD3Index pads short strided lists with the view's first point
-    uint32_t cap = static_cast<uint32_t>(prog0.fetch);
-    uint32_t k = static_cast<uint32_t>(prog0.k);
-    uint32_t stride = static_cast<uint32_t>(prog0.stride);
-    if (tiles) {
-        root = kd_tile_of(starts, ntiles, s);
-        pad_id = static_cast<uint32_t>(starts[root]);
-        const FusedProgram& pr = tiles[root].prog;
-        cap = static_cast<uint32_t>(pr.fetch);
-        k = static_cast<uint32_t>(pr.k);
-        stride = static_cast<uint32_t>(pr.stride);
-    }
     const double* rb = t.box + 6 * static_cast<size_t>(root);  // the tile's bounds
-    const uint32_t qid = cv.perm[s];
-    const double q0 = x[qid], q1 = y[qid], q2 = z[qid];
-    const double q[3] = {q0, q1, q2};
-    Ent best = ent_inf();
     uint32_t seq = 0;
     double dists[3] = {0.0, 0.0, 0.0};
     double mind = 0.0;
@@ -1629,28 +1703,24 @@ __device__ bool kd_resolve_row(const KdView& t, const CloudView& cv, const Fused
         const int stage = fr.stage(top);
         if (stage == 0) {
             if (t.c1[nd] < 0) {
-                const Ent wst = ent_shfl(best, static_cast<int>(cap) - 1);
-                const double worst = wst.d == kInfD ? INFINITY : ent_d2(wst);
+                const double worst = res.worst();
                 const uint32_t l0 = t.nl[nd], l1 = t.nr[nd];
                 for (uint32_t i0 = l0; i0 < l1; i0 += 64) {
                     const uint32_t i = i0 + lane;
                     Ent c = ent_inf();
                     if (i < l1) {
                         double d = 0.0;
-                        const double d0 = q0 - t.px[i];
+                        const double d0 = q[0] - t.px[i];
                         d += d0 * d0;
-                        const double d1 = q1 - t.py[i];
+                        const double d1 = q[1] - t.py[i];
                         d += d1 * d1;
-                        const double d2 = q2 - t.pz[i];
+                        const double d2 = q[2] - t.pz[i];
                         d += d2 * d2;
                         if (d < worst)
                             c = Ent{static_cast<uint64_t>(__double_as_longlong(d)),
                                     seq + (i - l0), t.vind[i]};
                     }
-                    if (__ballot(c.d != kInfD)) {
-                        bitonic_sort64(c, lane);
-                        merge_sorted(best, c, lane);
-                    }
+                    if (__ballot(c.d != kInfD)) res.add(c);
                 }
                 seq += l1 - l0;
                 --sp;
@@ -1684,9 +1754,7 @@ __device__ bool kd_resolve_row(const KdView& t, const CloudView& cv, const Fused
             const double m2 = fr.mind(top) + cut - dst;
             dists[idx] = cut;
             fr.set_dst(top, dst);
-            const Ent wst = ent_shfl(best, static_cast<int>(cap) - 1);
-            const double worst = wst.d == kInfD ? INFINITY : ent_d2(wst);
-            if (m2 * 1.0 <= worst) {
+            if (m2 * 1.0 <= res.worst()) {
                 if (sp >= depth_cap) return true;
                 fr.push(sp, fr.other(top), m2);
                 ++sp;
@@ -1696,29 +1764,73 @@ __device__ bool kd_resolve_row(const KdView& t, const CloudView& cv, const Fused
             --sp;
         }
     }
-    // KD3ImplT::neighbors: every stride-th entry of the result set; slots
-    // past the points found keep the output's zero initialisation.
-    const Ent kept = ent_shfl(best, static_cast<int>((static_cast<uint32_t>(lane) * stride) & 63u));
-    const bool mine = static_cast<uint32_t>(lane) < k;
-    const uint32_t kept_id = kept.d == kInfD ? pad_id : kept.j;
-    if (mode != 0) {  // neighbor lists
-        if (mine) out.ids[static_cast<uint64_t>(qid) * k + lane] = kept_id;
-        return false;
+    return false;
+}
+
+// A flagged row's tile: its root node (tile r's tree is rooted at node r),
+// the id short strided lists are padded with, and its neighborhood size.
+struct RowShape {
+    uint32_t root, pad_id, cap, k, stride;
+};
+
+__device__ __forceinline__ RowShape row_shape(const FusedProgram& prog0,
+                                              const TileInfo* __restrict__ tiles,
+                                              const uint64_t* __restrict__ starts,
+                                              uint32_t ntiles, uint32_t s)
+{
+    RowShape r{0, 0, static_cast<uint32_t>(prog0.fetch), static_cast<uint32_t>(prog0.k),
+               static_cast<uint32_t>(prog0.stride)};
+    if (tiles) {
+        r.root = kd_tile_of(starts, ntiles, s);
+        r.pad_id = static_cast<uint32_t>(starts[r.root]);  // KD3Index pads with the view's first point
+        const FusedProgram& pr = tiles[r.root].prog;
+        r.cap = static_cast<uint32_t>(pr.fetch);
+        r.k = static_cast<uint32_t>(pr.k);
+        r.stride = static_cast<uint32_t>(pr.stride);
     }
-    // Coordinates relative to the query point (exact differences), so the
-    // mean carries no rounding of the cloud's absolute offset.
-    const uint32_t id = mine ? kept_id : 0u;
-    const double px = mine ? x[id] - q0 : 0.0, py = mine ? y[id] - q1 : 0.0,
-                 pz = mine ? z[id] - q2 : 0.0;
-    const double mx = wave_sum(px) / static_cast<double>(k),
-                 my = wave_sum(py) / static_cast<double>(k),
-                 mz = wave_sum(pz) / static_cast<double>(k);
-    const double dx = mine ? px - mx : 0.0, dy = mine ? py - my : 0.0,
-                 dz = mine ? pz - mz : 0.0;
+    return r;
+}
+
+// The covariance record of a row from its neighbors' point ids (one per
+// lane and chunk of 64), relative to the query point (exact differences),
+// so the mean carries no rounding of the cloud's absolute offset.
+template <class IdOf>
+__device__ void write_cov(const KnnOutputs& out, uint32_t s, uint32_t k,
+                          const double* __restrict__ x, const double* __restrict__ y,
+                          const double* __restrict__ z, const double q[3], IdOf id_of)
+{
+    const int lane = __lane_id();
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (uint32_t o0 = 0; o0 < k; o0 += 64) {
+        const uint32_t o = o0 + static_cast<uint32_t>(lane);
+        if (o < k) {
+            const uint32_t id = id_of(o);
+            sx += x[id] - q[0];
+            sy += y[id] - q[1];
+            sz += z[id] - q[2];
+        }
+    }
+    const double mx = wave_sum(sx) / static_cast<double>(k),
+                 my = wave_sum(sy) / static_cast<double>(k),
+                 mz = wave_sum(sz) / static_cast<double>(k);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0;
+    for (uint32_t o0 = 0; o0 < k; o0 += 64) {
+        const uint32_t o = o0 + static_cast<uint32_t>(lane);
+        if (o < k) {
+            const uint32_t id = id_of(o);
+            const double dx = (x[id] - q[0]) - mx, dy = (y[id] - q[1]) - my,
+                         dz = (z[id] - q[2]) - mz;
+            a0 += dx * dx;
+            a1 += dx * dy;
+            a2 += dx * dz;
+            a3 += dy * dy;
+            a4 += dy * dz;
+            a5 += dz * dz;
+        }
+    }
     const double den = static_cast<double>(k - 1);
-    const double c0 = wave_sum(dx * dx) / den, c1 = wave_sum(dx * dy) / den,
-                 c2 = wave_sum(dx * dz) / den, c3 = wave_sum(dy * dy) / den,
-                 c4 = wave_sum(dy * dz) / den, c5 = wave_sum(dz * dz) / den;
+    const double c0 = wave_sum(a0) / den, c1 = wave_sum(a1) / den, c2 = wave_sum(a2) / den,
+                 c3 = wave_sum(a3) / den, c4 = wave_sum(a4) / den, c5 = wave_sum(a5) / den;
     if (lane == 0) {
         double* rec = out.cov + kCovRecord * static_cast<uint64_t>(s);
         rec[0] = c0;
@@ -1728,11 +1840,83 @@ __device__ bool kd_resolve_row(const KdView& t, const CloudView& cv, const Fused
         rec[4] = c4;
         rec[5] = c5;
     }
+}
+
+// One flagged row (flagged[f]) re-solved by one wave with a register result
+// set (capacity <= 64); returns true when the tree is deeper than the frames
+// hold (nothing written for the row).
+template <class Frames>
+__device__ bool kd_resolve_row(const KdView& t, const CloudView& cv, const FusedProgram& prog0,
+                               int mode, const KnnOutputs& out, const double* __restrict__ x,
+                               const double* __restrict__ y, const double* __restrict__ z,
+                               const TileInfo* __restrict__ tiles,
+                               const uint64_t* __restrict__ starts, uint32_t ntiles,
+                               uint32_t f, Frames& fr, Ent*)
+{
+    const int lane = __lane_id();
+    const uint32_t s = out.flagged[f];
+    const RowShape sh = row_shape(prog0, tiles, starts, ntiles, s);
+    const uint32_t qid = cv.perm[s];
+    const double q[3] = {x[qid], y[qid], z[qid]};
+    RegResults res;
+    res.init(sh.cap, lane);
+    if (kd_traverse(t, sh.root, q, fr, res)) return true;
+    // KD3ImplT::neighbors: every stride-th entry of the result set; slots
+    // past the points found keep the output's zero initialisation.
+    const Ent kept = ent_shfl(res.best, static_cast<int>((static_cast<uint32_t>(lane) * sh.stride) & 63u));
+    const uint32_t kept_id = kept.d == kInfD ? sh.pad_id : kept.j;
+    if (mode != 0) {  // neighbor lists
+        if (static_cast<uint32_t>(lane) < sh.k)
+            out.ids[static_cast<uint64_t>(qid) * sh.k + lane] = kept_id;
+        return false;
+    }
+    // k <= 64: neighbor o is lane o's.
+    write_cov(out, s, sh.k, x, y, z, q, [&](uint32_t) { return kept_id; });
+    return false;
+}
+
+// The same with the result set in LDS (capacity up to kBigCap): the
+// neighborhoods of more than 63 neighbors (knn + 1) * stride.
+template <class Frames>
+__device__ bool kd_resolve_row_big(const KdView& t, const CloudView& cv,
+                                   const FusedProgram& prog0, int mode, const KnnOutputs& out,
+                                   const double* __restrict__ x, const double* __restrict__ y,
+                                   const double* __restrict__ z,
+                                   const TileInfo* __restrict__ tiles,
+                                   const uint64_t* __restrict__ starts, uint32_t ntiles,
+                                   uint32_t f, Frames& fr, Ent* lds)
+{
+    const int lane = __lane_id();
+    const uint32_t s = out.flagged[f];
+    const RowShape sh = row_shape(prog0, tiles, starts, ntiles, s);
+    const uint32_t qid = cv.perm[s];
+    const double q[3] = {x[qid], y[qid], z[qid]};
+    LdsResults res;
+    res.init(lds, sh.cap, lane);
+    if (kd_traverse(t, sh.root, q, fr, res)) {
+        __syncthreads();
+        return true;
+    }
+    const Ent* kept = res.buf;
+    const uint32_t found = res.count;
+    auto id_of = [&](uint32_t o) {
+        const uint32_t e = o * sh.stride;
+        return e < found ? kept[e].j : sh.pad_id;
+    };
+    if (mode != 0) {
+        for (uint32_t o = static_cast<uint32_t>(lane); o < sh.k; o += 64)
+            out.ids[static_cast<uint64_t>(qid) * sh.k + o] = id_of(o);
+    } else {
+        write_cov(out, s, sh.k, x, y, z, q, id_of);
+    }
+    __syncthreads();  // the next row reuses the LDS
     return false;
 }
 
 // Rows whose search needs more than 64 frames are counted in failed[0] and,
-// with deep_rows, listed there for k_kd_resolve_deep.
+// with deep_rows, listed there for k_kd_resolve_deep.  BIG: result sets in
+// LDS (one-wave workgroups, 2 * cap entries of dynamic LDS).
+template <bool BIG>
 __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, FusedProgram prog0,
                                                     int mode, KnnOutputs out,
                                                     const double* __restrict__ x,
@@ -1744,14 +1928,19 @@ __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, Fuse
                                                     const uint64_t* __restrict__ starts,
                                                     uint32_t ntiles)
 {
+    extern __shared__ Ent lds_ents[];
     const int lane = __lane_id();
     const uint32_t waves = gridDim.x * (blockDim.x / 64);
     const uint32_t nflag = *out.nflagged;
     for (uint32_t f = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; f < nflag; f += waves) {
         RegFrames fr;
         fr.lane = lane;
-        if (kd_resolve_row(t, cv, prog0, mode, out, x, y, z, tiles, starts, ntiles, f, fr) &&
-            lane == 0) {
+        const bool over =
+            BIG ? kd_resolve_row_big(t, cv, prog0, mode, out, x, y, z, tiles, starts, ntiles, f,
+                                     fr, lds_ents)
+                : kd_resolve_row(t, cv, prog0, mode, out, x, y, z, tiles, starts, ntiles, f, fr,
+                                 lds_ents);
+        if (over && lane == 0) {
             const uint32_t slot = atomicAdd(failed, 1u);
             if (deep_rows) deep_rows[slot] = f;
         }
@@ -1761,6 +1950,7 @@ __global__ __launch_bounds__(256) void k_kd_resolve(KdView t, CloudView cv, Fuse
 // The rows k_kd_resolve listed, with frames for the tree's whole depth in
 // global memory (frames: waves x depth x 64); rows that still overflow are
 // counted in failed2.
+template <bool BIG>
 __global__ __launch_bounds__(64) void k_kd_resolve_deep(KdView t, CloudView cv,
                                                         FusedProgram prog0, int mode,
                                                         KnnOutputs out,
@@ -1775,6 +1965,7 @@ __global__ __launch_bounds__(64) void k_kd_resolve_deep(KdView t, CloudView cv,
                                                         const uint64_t* __restrict__ starts,
                                                         uint32_t ntiles)
 {
+    extern __shared__ Ent lds_ents[];
     const int lane = __lane_id();
     const uint32_t nd = *ndeep;
     MemFrames fr;
@@ -1782,10 +1973,13 @@ __global__ __launch_bounds__(64) void k_kd_resolve_deep(KdView t, CloudView cv,
     fr.depth = depth;
     fr.lane = lane;
     for (uint32_t i = blockIdx.x; i < nd; i += gridDim.x) {
-        if (kd_resolve_row(t, cv, prog0, mode, out, x, y, z, tiles, starts, ntiles,
-                           deep_rows[i], fr) &&
-            lane == 0)
-            atomicAdd(failed2, 1u);
+        const uint32_t f = deep_rows[i];
+        const bool over =
+            BIG ? kd_resolve_row_big(t, cv, prog0, mode, out, x, y, z, tiles, starts, ntiles, f,
+                                     fr, lds_ents)
+                : kd_resolve_row(t, cv, prog0, mode, out, x, y, z, tiles, starts, ntiles, f, fr,
+                                 lds_ents);
+        if (over && lane == 0) atomicAdd(failed2, 1u);
     }
 }
 
@@ -1934,17 +2128,35 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
 
 uint32_t kd_list_capacity(uint64_t n) { return static_cast<uint32_t>(n / kSubSmall * 4 + 4096); }
 
+// The rows' result capacity decides between register and LDS result sets.
+static uint32_t max_capacity(const FusedProgram& prog, const TileTables* tt)
+{
+    return tt ? 64u : static_cast<uint32_t>(prog.fetch);  // batch tiles: fetch <= 63
+}
+
 void launch_kd_resolve(const KdView& t, const CloudView& cv, const FusedProgram& prog, int mode,
                        const KnnOutputs& out, const double* x, const double* y, const double* z,
                        uint32_t nflagged, uint32_t* failed, uint32_t* deep_rows, hipStream_t s,
                        const TileTables* tt)
 {
     if (!nflagged) return;
-    const uint32_t blocks = std::min<uint32_t>(div_up(nflagged, 4), 4096);
-    hipLaunchKernelGGL(k_kd_resolve, dim3(blocks), dim3(256), 0, s, t, cv, prog, mode, out, x, y,
-                       z, failed, deep_rows, tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
-                       tt ? tt->ntiles : 0u);
+    const uint32_t cap = max_capacity(prog, tt);
+    const TileInfo* tiles = tt ? tt->tiles : nullptr;
+    const uint64_t* starts = tt ? tt->starts : nullptr;
+    const uint32_t ntiles = tt ? tt->ntiles : 0u;
+    if (cap <= 64) {
+        const uint32_t blocks = std::min<uint32_t>(div_up(nflagged, 4), 4096);
+        hipLaunchKernelGGL(k_kd_resolve<false>, dim3(blocks), dim3(256), 0, s, t, cv, prog, mode,
+                           out, x, y, z, failed, deep_rows, tiles, starts, ntiles);
+    } else {
+        const uint32_t blocks = std::min<uint32_t>(nflagged, 16384);
+        const size_t lds = sizeof(Ent) * 2 * cap;
+        hipLaunchKernelGGL(k_kd_resolve<true>, dim3(blocks), dim3(64), lds, s, t, cv, prog, mode,
+                           out, x, y, z, failed, deep_rows, tiles, starts, ntiles);
+    }
 }
+
+uint32_t kd_max_capacity() { return kBigCap; }
 
 size_t kd_deep_frame_bytes() { return sizeof(MemFrame) * 64; }
 
@@ -1955,10 +2167,19 @@ void launch_kd_resolve_deep(const KdView& t, const CloudView& cv, const FusedPro
                             hipStream_t s, const TileTables* tt)
 {
     if (!waves) return;
-    hipLaunchKernelGGL(k_kd_resolve_deep, dim3(waves), dim3(64), 0, s, t, cv, prog, mode, out, x,
-                       y, z, deep_rows, ndeep, static_cast<MemFrame*>(frames), depth, failed2,
-                       tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
-                       tt ? tt->ntiles : 0u);
+    const uint32_t cap = max_capacity(prog, tt);
+    const TileInfo* tiles = tt ? tt->tiles : nullptr;
+    const uint64_t* starts = tt ? tt->starts : nullptr;
+    const uint32_t ntiles = tt ? tt->ntiles : 0u;
+    MemFrame* fp = static_cast<MemFrame*>(frames);
+    if (cap <= 64)
+        hipLaunchKernelGGL(k_kd_resolve_deep<false>, dim3(waves), dim3(64), 0, s, t, cv, prog,
+                           mode, out, x, y, z, deep_rows, ndeep, fp, depth, failed2, tiles,
+                           starts, ntiles);
+    else
+        hipLaunchKernelGGL(k_kd_resolve_deep<true>, dim3(waves), dim3(64), sizeof(Ent) * 2 * cap,
+                           s, t, cv, prog, mode, out, x, y, z, deep_rows, ndeep, fp, depth,
+                           failed2, tiles, starts, ntiles);
 }
 
 // Depth of the deepest node (root 0) of a tree given its child arrays, or of

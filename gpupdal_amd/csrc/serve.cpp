@@ -312,7 +312,8 @@ int gf_server_submit(gf_server* server, const double* x, const double* y, const 
     if (ncols > kMaxOutColumns) return GF_ERR_UNSUPPORTED;
     try {
         check_neighborhood(n, static_cast<int64_t>(knn) + 1,
-                           r->st[0].stage == GF_STAGE_NORMAL ? 1 : r->st[0].stride);
+                           r->st[0].stage == GF_STAGE_NORMAL ? 1 : r->st[0].stride,
+                           kMaxFetch - 1);
     } catch (const Error& e) {
         return e.code;
     }

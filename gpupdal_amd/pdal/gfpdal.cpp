@@ -549,7 +549,7 @@ void CovarianceFeaturesFilter::filter(PointView& view)
         row_k.resize(n);
         for (uint64_t i = 0; i < n; ++i) {
             const uint64_t k = view.getFieldAs<uint64_t>(Id::OptimalKNN, i);
-            if (k > 63) throwError("OptimalKNN above 63 is outside the device kNN envelope");
+            if (k > 1024) throwError("OptimalKNN above 1024 is outside the device kNN envelope");
             row_k[i] = static_cast<uint32_t>(k);
         }
     }

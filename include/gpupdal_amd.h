@@ -176,7 +176,10 @@ int gf_cloud_invalidate(gf_cloud *cloud);
  * cols: the stages' output columns in order (gf_stage_columns each), n FP64
  * values per column, original point order.  Rows a stage skips keep their
  * prior values.  status: one n-byte array per stage.  where says whether
- * cols/status are host or device pointers. */
+ * cols/status are host or device pointers.  A neighborhood of (knn + 1) *
+ * stride up to 63 entries runs the grid search (rows it cannot certify go to
+ * the exact KD3 search); larger ones, up to 1024, run every row through the
+ * exact search; beyond that GF_ERR_UNSUPPORTED. */
 int gf_stage_columns(const gf_stage *st);
 int gf_run(gf_ctx *ctx, gf_cloud *cloud, const gf_stage *stages, int nstages,
            double *const *cols, uint8_t *const *status, int where);
@@ -197,10 +200,16 @@ int gf_run_noalloc(gf_ctx *ctx, const double *x, const double *y, const double *
                    uint64_t n, const gf_stage *stages, int nstages, double *const *cols,
                    uint8_t *const *status);
 
-/* ---- neighbor lists in KD3Index order semantics: replaces
- *      pdg::knnGather (include/pdg/index/SpatialIndex.hpp:480-490) and
- *      KD3Index::neighbors(id, k, stride) (pdal/KDIndex.hpp:103-106).
- * ids: n x min(n,k) uint32 (row i = neighbors of point i). */
+/* ---- neighbor lists: replaces pdg::knnGather
+ *      (include/pdg/index/SpatialIndex.hpp:480-490) and KD3Index::neighbors(id,
+ *      k, stride) (pdal/KDIndex.hpp:103-106).
+ * ids: n x min(n,k) uint32 (row i = neighbors of point i).  Every row holds
+ * exactly KD3Index::neighbors' set.  The order: rows the exact search
+ * re-solved are in KD3Index's order -- those with status != 0 (a tie at the
+ * boundary or within a strided list, or an uncertified grid search), and
+ * every row when k * stride is above 63 or above the cloud's size; the other
+ * rows are ascending by (squared distance, point id), which is KD3Index's
+ * order wherever no two kept distances are equal.  k * stride up to 1024. */
 int gf_knn(gf_ctx *ctx, gf_cloud *cloud, uint32_t k, uint32_t stride,
            uint32_t *ids, uint8_t *status, int where);
 
@@ -287,7 +296,7 @@ int gf_optimal_neighborhood(gf_ctx *ctx, gf_cloud *cloud, uint32_t min_k, uint32
  * row i's neighborhood is its row_k[i] nearest points in KD3Index order
  * (clamped to the cloud's size).  stages' knn and stride are not used
  * (radius must be 0); row_k holds n counts (host or device, as `where`), at
- * most 63.  cols/status as gf_run's; no search flags are reported (the lists
+ * most 1024.  cols/status as gf_run's; no search flags are reported (the lists
  * are exact).  A row with k = 0 has a zero covariance (skipped); k = 1 gives
  * 0/0 covariances, as PDAL's division by k - 1 does. */
 int gf_run_rowk(gf_ctx *ctx, gf_cloud *cloud, const gf_stage *stages, int nstages,

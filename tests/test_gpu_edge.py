@@ -183,13 +183,13 @@ def test_argument_errors(ctx):
     x, y, z = small_cloud(1000, seed=60)
     cl = ctx.cloud(x, y, z)
     with pytest.raises(g.GfError) as e:
-        cl.run([g.stage(g.STAGE_COVFEAT, 40, stride=2)])  # 41*2 > 63
+        cl.run([g.stage(g.STAGE_COVFEAT, 400, stride=3)])  # 401*3 > 1024
     assert e.value.code == -3
     with pytest.raises(g.GfError) as e:
         cl.run([g.stage(7, 8)])
     assert e.value.code == -1
     with pytest.raises(g.GfError):
-        cl.knn(64)
+        cl.knn(600, stride=2)  # 1200 > 1024
     # The context stays usable after an error.
     cols, status = gpu_stage(ctx, x, y, z, g.stage(g.STAGE_NORMAL, 8))
     assert np.count_nonzero(status & 0x3C) < len(x)
