@@ -88,7 +88,8 @@ EXPORTED = [
     "gf_server_create", "gf_server_submit", "gf_server_wait", "gf_server_request_info",
     "gf_server_release", "gf_server_destroy",
     "gf_autotune", "gf_tuning_save", "gf_tuning_load",
-    "gf_las_parse", "gf_cloud_create_las", "gf_las_pack_extra", "gf_debug_cloud_xyz",
+    "gf_las_parse", "gf_cloud_create_las", "gf_las_pack_extra", "gf_las_write_extra",
+    "gf_debug_cloud_xyz",
     "gf_optimal_neighborhood", "gf_debug_grid_params", "gf_run_noalloc",
     "gf_radius_counts", "gf_nndistance", "gf_outlier", "gf_run_rowk",
 ]
@@ -138,6 +139,11 @@ def load_library(path=LIB_PATH):
                                ctypes.c_uint64]
     L.gf_cloud_create_las.argtypes = [vp, vp, ctypes.c_uint64, ctypes.POINTER(GfLasInfo),
                                       ctypes.POINTER(vp)]
+    L.gf_las_write_extra.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_char_p),
+                                     ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(vp),
+                                     ctypes.c_int, vp, ctypes.c_uint64,
+                                     ctypes.POINTER(ctypes.c_uint64)]
     L.gf_las_pack_extra.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_int, ctypes.c_uint64, vp,
                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
     L.gf_debug_cloud_xyz.argtypes = [vp, vp, vp, vp, vp]
@@ -316,6 +322,30 @@ class Context:
         self._check(self.L.gf_las_pack_extra(self.h, cp, len(cols), n, records.ctypes.data,
                                              stride, offset, WHERE_HOST), "gf_las_pack_extra")
         return records
+
+    def las_write_extra(self, data, names, cols, descriptions=None):
+        """writers.las extra_dims: a new LAS file image (bytes) from the
+        uncompressed image `data` with the double columns (host arrays, one
+        value per point) appended to every record and described in the Extra
+        Bytes VLR."""
+        src = np.frombuffer(data, dtype=np.uint8)
+        cols = [_c64(c) for c in cols]
+        k = len(cols)
+        if k != len(names):
+            raise ValueError("one name per column")
+        descriptions = list(descriptions) if descriptions is not None else [""] * k
+        nm = (ctypes.c_char_p * k)(*[str(v).encode("ascii") for v in names])
+        ds = (ctypes.c_char_p * k)(*[str(v).encode("ascii") for v in descriptions])
+        cp = (ctypes.c_void_p * max(1, k))(*[c.ctypes.data for c in cols])
+        size = ctypes.c_uint64()
+        self._check(self.L.gf_las_write_extra(self.h, src.ctypes.data, len(src), k, nm, ds, cp,
+                                              WHERE_HOST, None, 0, ctypes.byref(size)),
+                    "gf_las_write_extra")
+        out = np.empty(size.value, np.uint8)
+        self._check(self.L.gf_las_write_extra(self.h, src.ctypes.data, len(src), k, nm, ds, cp,
+                                              WHERE_HOST, out.ctypes.data, len(out),
+                                              ctypes.byref(size)), "gf_las_write_extra")
+        return out.tobytes()
 
     def autotune(self, cloud, stages, factors=(0.4, 0.5, 0.6, 0.75, 0.9)):
         """Times the run for each column-size factor; the fastest is kept for

@@ -126,6 +126,8 @@ struct LasColumns {
     const double* col[16];
 };
 bool las_parse(const uint8_t* b, uint64_t nbytes, gf_las_info& h, std::string& err);
+// Base record size of a supported point format (0, 1, 2, 3, 6, 7, 8), else 0.
+uint32_t las_base_size(int format);
 // rec: n records of L bytes in device memory, padded by at least 4 bytes.
 void launch_las_xyz(const uint8_t* rec, uint64_t n, uint32_t L, const LasXform& xf, double* x,
                     double* y, double* z, hipStream_t s);

@@ -324,6 +324,12 @@ struct Context {
                       const FusedProgram& prog, int mode, const KnnOutputs& out,
                       const double* x, const double* y, const double* z, uint32_t nfail,
                       const uint32_t* dcount, const TileTables* tt);
+    // gf_las_write_extra: the widened file image into dst (or its size only,
+    // dst null); cols host or device as `where`.
+    void las_write_extra(const uint8_t* src, uint64_t src_nbytes, int ncols,
+                         const char* const* names, const char* const* descriptions,
+                         const double* const* cols, int where, uint8_t* dst,
+                         uint64_t dst_capacity, uint64_t* dst_nbytes);
     // Returns the number of flagged rows.
     uint32_t resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog,
                              int mode, const KnnOutputs& out, hipEvent_t from);

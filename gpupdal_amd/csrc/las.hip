@@ -104,6 +104,8 @@ double rdf64(const uint8_t* p)
     return v;
 }
 
+} // namespace
+
 // Base record size of a supported point format, 0 otherwise
 // (Header.cpp:15-36 with Utils.hpp:93-107).
 uint32_t las_base_size(int format)
@@ -119,8 +121,6 @@ uint32_t las_base_size(int format)
     default: return 0;
     }
 }
-
-} // namespace
 
 // Header fields (Header.cpp:38-75); the first of PDAL's validation errors
 // (Header.cpp:118-144) goes to `err` and the return value is false.

@@ -367,6 +367,26 @@ int gf_cloud_create_las(gf_ctx *ctx, const void *bytes, uint64_t nbytes, gf_las_
 int gf_las_pack_extra(gf_ctx *ctx, const double *const *cols, int ncols, uint64_t n,
                       uint8_t *records, uint32_t stride, uint32_t offset, int where);
 
+/* writers.las extra_dims for FP64 columns (io/LasWriter.cpp:583-596 with
+ * io/private/las/Utils.cpp:244-267): a new LAS file image from an
+ * uncompressed source image (host memory), every point record followed by
+ * the ncols (1..16) columns as little-endian doubles, described by Extra
+ * Bytes descriptors (data type 10, names and descriptions zero-padded to 32
+ * bytes, longer ones truncated) appended to the source's Extra Bytes VLR, or
+ * in a new one ("LASF_Spec", record 4) after the last VLR.  Source record
+ * bytes that no descriptor covers are declared as undocumented extra bytes
+ * first, so each column is described at its offset.  The header's point
+ * offset, VLR count and record length are updated, and the waveform and
+ * EVLR offsets (LAS 1.3/1.4) moved past the wider records; everything else
+ * is copied.  The records are widened and packed on the device.  cols: n
+ * (the source's point count) values each, host or device as `where`.
+ * *dst_nbytes receives the image's size; dst NULL asks for the size only,
+ * and a dst_capacity below it is GF_ERR_ARG. */
+int gf_las_write_extra(gf_ctx *ctx, const void *src, uint64_t src_nbytes, int ncols,
+                       const char *const *names, const char *const *descriptions,
+                       const double *const *cols, int where, void *dst, uint64_t dst_capacity,
+                       uint64_t *dst_nbytes);
+
 /* ---- parity hooks (no reference counterpart; used by the test suite). */
 /* The device eigensolver on m row-major 3x3 matrices. */
 int gf_debug_eigen(gf_ctx *ctx, const double *cov, uint64_t m, double *evals,

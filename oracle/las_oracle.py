@@ -115,3 +115,153 @@ def pack_extra(records, cols, stride, offset):
         raw = np.ascontiguousarray(np.asarray(col, dtype="<f8")).view(np.uint8).reshape(n, 8)
         view[:, offset + 8 * c: offset + 8 * c + 8] = raw
     return records
+
+
+# ------------------------------------------------------------ extra bytes --
+# LAS 1.4 Extra Bytes VLR (user "LASF_Spec", record 4): 192-byte descriptors
+# (io/private/las/Utils.hpp:195-208), data types 1-10 = u8, i8, u16, i16,
+# u32, i32, u64, i64, f32, f64 and 11-30 two or three of them
+# (io/private/las/Utils.cpp:54-58, 209-227), 0 = `options` undocumented
+# bytes.  PDAL names the fields of a multi-field descriptor name0, name1, ...
+# and lays the described bytes out after the base record in descriptor order.
+VLR_HEADER = 54
+EB_SIZE = 192
+EB_TYPES = [None, "u1", "i1", "<u2", "<i2", "<u4", "<i4", "<u8", "<i8", "<f4", "<f8"]
+
+
+def vlrs(buf):
+    """The VLRs of a LAS file image: dicts with offset (of the VLR header),
+    user_id, record_id, description and data (bytes)."""
+    h = parse_header(buf)
+    b = memoryview(buf)
+    out, at = [], h["header_size"]
+    for _ in range(h["vlr_count"]):
+        user = bytes(b[at + 2:at + 18]).split(b"\0")[0].decode("ascii", "replace")
+        rid, length = struct.unpack_from("<HH", b, at + 18)
+        desc = bytes(b[at + 22:at + 54]).split(b"\0")[0].decode("ascii", "replace")
+        out.append(dict(offset=at, user_id=user, record_id=rid, description=desc,
+                        data=bytes(b[at + VLR_HEADER:at + VLR_HEADER + length])))
+        at += VLR_HEADER + length
+    return out
+
+
+def _descriptor_size(dtype, options):
+    if dtype == 0:
+        return options
+    fields = (dtype - 1) // 10 + 1
+    return fields * np.dtype(EB_TYPES[(dtype - 1) % 10 + 1]).itemsize
+
+
+def extra_dims(buf):
+    """The extra dimensions an Extra Bytes VLR declares: (name, numpy dtype,
+    byte offset in the record, description) per field, undocumented bytes
+    skipped."""
+    h = parse_header(buf)
+    offset = BASE_SIZE[h["point_format"]]
+    out = []
+    for v in vlrs(buf):
+        if v["user_id"] != "LASF_Spec" or v["record_id"] != 4:
+            continue
+        d = v["data"]
+        for i in range(len(d) // EB_SIZE):
+            e = d[i * EB_SIZE:(i + 1) * EB_SIZE]
+            dtype, options = e[2], e[3]
+            name = e[4:36].split(b"\0")[0].decode("ascii", "replace")
+            desc = e[160:192].split(b"\0")[0].decode("ascii", "replace")
+            if dtype != 0:
+                fields = (dtype - 1) // 10 + 1
+                t = np.dtype(EB_TYPES[(dtype - 1) % 10 + 1])
+                for f in range(fields):
+                    out.append((name + (str(f) if fields > 1 else ""), t,
+                                offset + f * t.itemsize, desc))
+            offset += _descriptor_size(dtype, options)
+        break
+    return out
+
+
+def read_extra(buf):
+    """{name: values} of the declared extra dimensions."""
+    h = parse_header(buf)
+    n, size = h["point_count"], h["point_size"]
+    rec = np.frombuffer(buf, dtype=np.uint8, count=n * size,
+                        offset=h["point_offset"]).reshape(n, size)
+    return {name: np.ascontiguousarray(rec[:, off:off + t.itemsize]).view(t).reshape(n)
+            for name, t, off, _ in extra_dims(buf)}
+
+
+def write_extra(src, names, cols, descriptions=None):
+    """writers.las extra_dims for double columns (io/LasWriter.cpp:583-596,
+    io/private/las/Utils.cpp:244-267) as a new file image: the columns after
+    each source record, descriptors (type 10, zero-padded 32-byte name and
+    description) appended to the source's Extra Bytes VLR or in a new one
+    after the last VLR; source bytes no descriptor covers declared first as
+    undocumented ones; header offsets, VLR count and record length updated,
+    the 1.3 waveform and 1.4 EVLR offsets moved."""
+    h = parse_header(src)
+    descriptions = descriptions or [""] * len(names)
+    vl = vlrs(src)
+    eb = next((i for i, v in enumerate(vl)
+               if v["user_id"] == "LASF_Spec" and v["record_id"] == 4), None)
+    S, n = h["point_size"], h["point_count"]
+    described = BASE_SIZE[h["point_format"]]
+    if eb is not None:
+        d = vl[eb]["data"]
+        for i in range(len(d) // EB_SIZE):
+            described += _descriptor_size(d[i * EB_SIZE + 2], d[i * EB_SIZE + 3])
+
+    def descriptor(dtype, options, name, desc):
+        e = bytearray(EB_SIZE)
+        e[2], e[3] = dtype, options
+        nb, db = name.encode("ascii")[:32], desc.encode("ascii")[:32]
+        e[4:4 + len(nb)] = nb
+        e[160:160 + len(db)] = db
+        return bytes(e)
+
+    add = b""
+    gap = S - described
+    while gap > 0:
+        g = min(gap, 255)
+        add += descriptor(0, g, "", "")
+        gap -= g
+    for name, desc in zip(names, descriptions):
+        add += descriptor(10, 0, name, desc)
+    grow = len(add) + (0 if eb is not None else VLR_HEADER)
+    S2 = S + 8 * len(cols)
+    new_offset = h["point_offset"] + grow
+    src_points_end = h["point_offset"] + n * S
+    after = new_offset + n * S2 - src_points_end
+    hdr = bytearray(src[:h["header_size"]])
+    struct.pack_into("<I", hdr, 96, new_offset)
+    struct.pack_into("<I", hdr, 100, h["vlr_count"] + (0 if eb is not None else 1))
+    struct.pack_into("<H", hdr, 105, S2)
+    if h["version_minor"] >= 3 and h["header_size"] >= 235:
+        (wave,) = struct.unpack_from("<Q", src, 227)
+        if wave:
+            struct.pack_into("<Q", hdr, 227, wave + after)
+    if h["version_minor"] >= 4 and h["header_size"] >= 243:
+        (ev,) = struct.unpack_from("<Q", src, 235)
+        if ev:
+            struct.pack_into("<Q", hdr, 235, ev + after)
+    out = bytearray(hdr)
+    for i, v in enumerate(vl):
+        head = bytearray(src[v["offset"]:v["offset"] + VLR_HEADER])
+        data = v["data"]
+        if i == eb:
+            data = data + add
+            struct.pack_into("<H", head, 20, len(data))
+        out += head + data
+    vlr_end = vl[-1]["offset"] + VLR_HEADER + len(vl[-1]["data"]) if vl else h["header_size"]
+    if eb is None:
+        head = bytearray(VLR_HEADER)
+        head[2:11] = b"LASF_Spec"
+        struct.pack_into("<HH", head, 18, 4, len(add))
+        head[22:40] = b"Extra Bytes Record"
+        out += head + add
+    out += src[vlr_end:h["point_offset"]]
+    rec = np.frombuffer(src, dtype=np.uint8, count=n * S, offset=h["point_offset"]).reshape(n, S)
+    wide = np.zeros((n, S2), np.uint8)
+    wide[:, :S] = rec
+    pack_extra(wide.reshape(-1), list(cols), S2, S)
+    out += wide.tobytes()
+    out += src[src_points_end:]
+    return bytes(out)
