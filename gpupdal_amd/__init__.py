@@ -141,7 +141,7 @@ def load_library(path=LIB_PATH):
                                       ctypes.POINTER(vp)]
     L.gf_las_write_extra.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_char_p),
-                                     ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(vp),
+                                     ctypes.POINTER(ctypes.c_char_p), vp, ctypes.POINTER(vp),
                                      ctypes.c_int, vp, ctypes.c_uint64,
                                      ctypes.POINTER(ctypes.c_uint64)]
     L.gf_las_pack_extra.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_int, ctypes.c_uint64, vp,
@@ -323,11 +323,12 @@ class Context:
                                              stride, offset, WHERE_HOST), "gf_las_pack_extra")
         return records
 
-    def las_write_extra(self, data, names, cols, descriptions=None):
+    def las_write_extra(self, data, names, cols, descriptions=None, types=None):
         """writers.las extra_dims: a new LAS file image (bytes) from the
-        uncompressed image `data` with the double columns (host arrays, one
-        value per point) appended to every record and described in the Extra
-        Bytes VLR."""
+        uncompressed image `data` with the columns (host arrays of doubles,
+        one value per point) appended to every record, each as its LAS data
+        type (types: 1..10 = u8 .. f64; None: all f64), and described in the
+        Extra Bytes VLR."""
         src = np.frombuffer(data, dtype=np.uint8)
         cols = [_c64(c) for c in cols]
         k = len(cols)
@@ -337,13 +338,15 @@ class Context:
         nm = (ctypes.c_char_p * k)(*[str(v).encode("ascii") for v in names])
         ds = (ctypes.c_char_p * k)(*[str(v).encode("ascii") for v in descriptions])
         cp = (ctypes.c_void_p * max(1, k))(*[c.ctypes.data for c in cols])
+        tp = None if types is None else np.ascontiguousarray(types, dtype=np.uint8)
+        tptr = None if tp is None else tp.ctypes.data
         size = ctypes.c_uint64()
-        self._check(self.L.gf_las_write_extra(self.h, src.ctypes.data, len(src), k, nm, ds, cp,
-                                              WHERE_HOST, None, 0, ctypes.byref(size)),
+        self._check(self.L.gf_las_write_extra(self.h, src.ctypes.data, len(src), k, nm, ds, tptr,
+                                              cp, WHERE_HOST, None, 0, ctypes.byref(size)),
                     "gf_las_write_extra")
         out = np.empty(size.value, np.uint8)
-        self._check(self.L.gf_las_write_extra(self.h, src.ctypes.data, len(src), k, nm, ds, cp,
-                                              WHERE_HOST, out.ctypes.data, len(out),
+        self._check(self.L.gf_las_write_extra(self.h, src.ctypes.data, len(src), k, nm, ds, tptr,
+                                              cp, WHERE_HOST, out.ctypes.data, len(out),
                                               ctypes.byref(size)), "gf_las_write_extra")
         return out.tobytes()
 

@@ -1567,8 +1567,8 @@ int gf_las_pack_extra(gf_ctx* ctx, const double* const* cols, int ncols, uint64_
 
 int gf_las_write_extra(gf_ctx* ctx, const void* src, uint64_t src_nbytes, int ncols,
                        const char* const* names, const char* const* descriptions,
-                       const double* const* cols, int where, void* dst, uint64_t dst_capacity,
-                       uint64_t* dst_nbytes)
+                       const uint8_t* types, const double* const* cols, int where, void* dst,
+                       uint64_t dst_capacity, uint64_t* dst_nbytes)
 {
     if (!ctx || !src || !dst_nbytes || ncols < 1 || ncols > 16 || !cols) return GF_ERR_ARG;
     return guarded(ctx, [&] {
@@ -1577,7 +1577,7 @@ int gf_las_write_extra(gf_ctx* ctx, const void* src, uint64_t src_nbytes, int nc
         for (int i = 0; i < ncols; ++i)
             if (!cols[i]) throw Error(GF_ERR_ARG, "null column");
         C(ctx)->las_write_extra(static_cast<const uint8_t*>(src), src_nbytes, ncols, names,
-                                descriptions, cols, where, static_cast<uint8_t*>(dst),
+                                descriptions, types, cols, where, static_cast<uint8_t*>(dst),
                                 dst_capacity, dst_nbytes);
     });
 }

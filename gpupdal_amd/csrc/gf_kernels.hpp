@@ -125,6 +125,16 @@ struct LasXform {
 struct LasColumns {
     const double* col[16];
 };
+// LAS Extra Bytes data types (1..10) of the columns.
+struct LasTypes {
+    int32_t type[16];
+};
+// Bytes of LAS Extra Bytes data type t (1..10), else 0.
+uint32_t las_type_size(int t);
+// Row i's columns, each converted to its type, one after another from
+// out + i * stride + offset, little-endian.
+void launch_las_pack_typed(const LasColumns& cols, const LasTypes& types, int ncols, uint64_t n,
+                           uint8_t* out, uint32_t stride, uint32_t offset, hipStream_t s);
 bool las_parse(const uint8_t* b, uint64_t nbytes, gf_las_info& h, std::string& err);
 // Base record size of a supported point format (0, 1, 2, 3, 6, 7, 8), else 0.
 uint32_t las_base_size(int format);

@@ -328,8 +328,8 @@ struct Context {
     // dst null); cols host or device as `where`.
     void las_write_extra(const uint8_t* src, uint64_t src_nbytes, int ncols,
                          const char* const* names, const char* const* descriptions,
-                         const double* const* cols, int where, uint8_t* dst,
-                         uint64_t dst_capacity, uint64_t* dst_nbytes);
+                         const uint8_t* types, const double* const* cols, int where,
+                         uint8_t* dst, uint64_t dst_capacity, uint64_t* dst_nbytes);
     // Returns the number of flagged rows.
     uint32_t resolve_flagged(Cloud& cl, const CloudView& cv, const FusedProgram& prog,
                              int mode, const KnnOutputs& out, hipEvent_t from);

@@ -212,6 +212,86 @@ void launch_las_xyz(const uint8_t* rec, uint64_t n, uint32_t L, const LasXform& 
                        per, xf, x, y, z);
 }
 
+namespace {
+
+// A double as LAS Extra Bytes data type t (1..10: u8, i8, u16, i16, u32,
+// i32, u64, i64, f32, f64): integer types take the value rounded toward
+// zero and held to the type's range (NaN: 0); returns the bits and size.
+__device__ uint64_t las_typed_bits(double v, int t, int& size)
+{
+    auto clampd = [](double x, double lo, double hi) {
+        return x != x ? 0.0 : (x < lo ? lo : (x > hi ? hi : x));
+    };
+    switch (t) {
+    case 1: size = 1; return static_cast<uint8_t>(clampd(v, 0.0, 255.0));
+    case 2: size = 1; return static_cast<uint8_t>(static_cast<int8_t>(clampd(v, -128.0, 127.0)));
+    case 3: size = 2; return static_cast<uint16_t>(clampd(v, 0.0, 65535.0));
+    case 4:
+        size = 2;
+        return static_cast<uint16_t>(static_cast<int16_t>(clampd(v, -32768.0, 32767.0)));
+    case 5: size = 4; return static_cast<uint32_t>(clampd(v, 0.0, 4294967295.0));
+    case 6:
+        size = 4;
+        return static_cast<uint32_t>(static_cast<int32_t>(clampd(v, -2147483648.0, 2147483647.0)));
+    case 7: {
+        size = 8;
+        const double c = clampd(v, 0.0, 18446744073709549568.0);  // largest double < 2^64
+        return static_cast<uint64_t>(c);
+    }
+    case 8: {
+        size = 8;
+        const double c = clampd(v, -9223372036854775808.0, 9223372036854774784.0);
+        return static_cast<uint64_t>(static_cast<int64_t>(c));
+    }
+    case 9: {
+        size = 4;
+        const float f = static_cast<float>(v);
+        uint32_t b;
+        __builtin_memcpy(&b, &f, 4);
+        return b;
+    }
+    default: {
+        size = 8;
+        uint64_t b;
+        __builtin_memcpy(&b, &v, 8);
+        return b;
+    }
+    }
+}
+
+__global__ __launch_bounds__(kLasThreads) void k_las_pack_typed(LasColumns cols, LasTypes types,
+                                                                int ncols, uint64_t n,
+                                                                uint8_t* __restrict__ out,
+                                                                uint32_t stride,
+                                                                uint32_t offset)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kLasThreads) + threadIdx.x;
+    if (i >= n) return;
+    uint8_t* dst = out + i * stride + offset;
+    for (int c = 0; c < ncols; ++c) {
+        int size = 8;
+        const uint64_t bits = las_typed_bits(cols.col[c][i], types.type[c], size);
+        for (int k = 0; k < size; ++k) dst[k] = static_cast<uint8_t>(bits >> (8 * k));
+        dst += size;
+    }
+}
+
+} // namespace
+
+uint32_t las_type_size(int t)
+{
+    static const uint32_t size[11] = {0, 1, 1, 2, 2, 4, 4, 8, 8, 4, 8};
+    return (t >= 1 && t <= 10) ? size[t] : 0;
+}
+
+void launch_las_pack_typed(const LasColumns& cols, const LasTypes& types, int ncols, uint64_t n,
+                           uint8_t* out, uint32_t stride, uint32_t offset, hipStream_t s)
+{
+    if (!n || !ncols) return;
+    hipLaunchKernelGGL(k_las_pack_typed, dim3(static_cast<uint32_t>(div_up(n, kLasThreads))),
+                       dim3(kLasThreads), 0, s, cols, types, ncols, n, out, stride, offset);
+}
+
 void launch_las_pack(const LasColumns& cols, int ncols, uint64_t n, uint8_t* out,
                      uint32_t stride, uint32_t offset, hipStream_t s)
 {

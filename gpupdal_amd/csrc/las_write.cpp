@@ -1,7 +1,8 @@
-// las_write.cpp -- writers.las extra_dims for FP64 columns: a LAS file image
-// from a source image, with the columns appended to every point record and
-// described in the Extra Bytes VLR (io/LasWriter.cpp:583-596,
-// io/private/las/Utils.cpp:244-267; LAS 1.4 R15 2.6 "Extra Bytes").
+// las_write.cpp -- writers.las extra_dims: a LAS file image from a source
+// image, with columns (FP64 values, each written as its LAS Extra Bytes
+// data type) appended to every point record and described in the Extra
+// Bytes VLR (io/LasWriter.cpp:583-596, io/private/las/Utils.cpp:244-267;
+// LAS 1.4 R15 2.6 "Extra Bytes").
 //
 // The header and VLRs are rewritten on the host; the point records are
 // widened and packed in HBM: one pitched copy spreads the source records to
@@ -61,7 +62,7 @@ uint32_t eb_bytes(const uint8_t* d)
     return fields * size[(type - 1) % 10 + 1];
 }
 
-// A descriptor (io/private/las/Utils.cpp:244-267): typed values with the
+// A descriptor (io/private/las/Utils.cpp:244-267): one typed value with the
 // scale and offset options clear, no-data/min/max zero, name and
 // description zero-padded to 32 bytes (longer ones truncated).
 void put_descriptor(std::vector<uint8_t>& out, uint8_t type, uint8_t options,
@@ -85,9 +86,17 @@ struct VlrSpan {
 
 void Context::las_write_extra(const uint8_t* src, uint64_t src_nbytes, int ncols,
                               const char* const* names, const char* const* descriptions,
-                              const double* const* cols, int where, uint8_t* dst,
-                              uint64_t dst_capacity, uint64_t* dst_nbytes)
+                              const uint8_t* types, const double* const* cols, int where,
+                              uint8_t* dst, uint64_t dst_capacity, uint64_t* dst_nbytes)
 {
+    LasTypes lt{};
+    uint32_t extra = 0;
+    for (int c = 0; c < ncols; ++c) {
+        lt.type[c] = types ? types[c] : kEbDouble;
+        const uint32_t sz = las_type_size(lt.type[c]);
+        if (!sz) throw Error(GF_ERR_ARG, "extra dimension types are 1..10 (u8 .. f64)");
+        extra += sz;
+    }
     gf_las_info h;
     std::string err;
     if (!las_parse(src, src_nbytes, h, err)) throw Error(GF_ERR_ARG, err);
@@ -109,7 +118,8 @@ void Context::las_write_extra(const uint8_t* src, uint64_t src_nbytes, int ncols
             throw Error(GF_ERR_ARG, "VLR " + std::to_string(v) + " runs past the point data");
         char user[17] = {0};
         std::memcpy(user, src + at + 2, 16);
-        const bool is_eb = std::strcmp(user, "LASF_Spec") == 0 && rd16(src + at + 18) == kEbRecordId;
+        const bool is_eb =
+            std::strcmp(user, "LASF_Spec") == 0 && rd16(src + at + 18) == kEbRecordId;
         if (is_eb && eb < 0) eb = static_cast<int>(vlrs.size());
         vlrs.push_back({at, len, is_eb});
         at += len;
@@ -134,15 +144,16 @@ void Context::las_write_extra(const uint8_t* src, uint64_t src_nbytes, int ncols
         gap -= g;
     }
     for (int c = 0; c < ncols; ++c)
-        put_descriptor(add, kEbDouble, 0, names && names[c] ? names[c] : "",
+        put_descriptor(add, static_cast<uint8_t>(lt.type[c]), 0,
+                       names && names[c] ? names[c] : "",
                        descriptions && descriptions[c] ? descriptions[c] : "");
     const uint64_t grow = add.size() + (eb >= 0 ? 0 : kVlrHeader);
     if (eb >= 0 && vlrs[eb].size - kVlrHeader + add.size() > 65535)
         throw Error(GF_ERR_UNSUPPORTED, "the Extra Bytes VLR would exceed 65535 bytes");
-    if (S + 8ull * ncols > 65535)
+    if (S + static_cast<uint64_t>(extra) > 65535)
         throw Error(GF_ERR_UNSUPPORTED, "the point record would exceed 65535 bytes");
 
-    const uint32_t S2 = S + 8u * static_cast<uint32_t>(ncols);
+    const uint32_t S2 = S + extra;
     const uint64_t new_offset = h.point_offset + grow;
     if (new_offset > 0xFFFFFFFFull) throw Error(GF_ERR_UNSUPPORTED, "point offset above 4 GiB");
     const uint64_t src_points_end = h.point_offset + n * S;
@@ -210,7 +221,7 @@ void Context::las_write_extra(const uint8_t* src, uint64_t src_nbytes, int ncols
             lc.col[c] = d;
         }
     }
-    launch_las_pack(lc, ncols, n, dout, S2, S, stream);
+    launch_las_pack_typed(lc, lt, ncols, n, dout, S2, S, stream);
     GF_HIP(hipMemcpyAsync(dst + new_offset, dout, out_bytes, hipMemcpyDeviceToHost, stream));
     GF_HIP(hipStreamSynchronize(stream));
 }

@@ -367,13 +367,16 @@ int gf_cloud_create_las(gf_ctx *ctx, const void *bytes, uint64_t nbytes, gf_las_
 int gf_las_pack_extra(gf_ctx *ctx, const double *const *cols, int ncols, uint64_t n,
                       uint8_t *records, uint32_t stride, uint32_t offset, int where);
 
-/* writers.las extra_dims for FP64 columns (io/LasWriter.cpp:583-596 with
+/* writers.las extra_dims (io/LasWriter.cpp:583-596 with
  * io/private/las/Utils.cpp:244-267): a new LAS file image from an
  * uncompressed source image (host memory), every point record followed by
- * the ncols (1..16) columns as little-endian doubles, described by Extra
- * Bytes descriptors (data type 10, names and descriptions zero-padded to 32
- * bytes, longer ones truncated) appended to the source's Extra Bytes VLR, or
- * in a new one ("LASF_Spec", record 4) after the last VLR.  Source record
+ * the ncols (1..16) columns, little-endian, each as its LAS data type
+ * (types[c] in 1..10 = u8, i8, u16, i16, u32, i32, u64, i64, f32, f64;
+ * types NULL: all f64; integer types take the value rounded toward zero,
+ * held to the type's range, NaN as 0), described by Extra Bytes descriptors
+ * (names and descriptions zero-padded to 32 bytes, longer ones truncated)
+ * appended to the source's Extra Bytes VLR, or in a new one ("LASF_Spec",
+ * record 4) after the last VLR.  Source record
  * bytes that no descriptor covers are declared as undocumented extra bytes
  * first, so each column is described at its offset.  The header's point
  * offset, VLR count and record length are updated, and the waveform and
@@ -384,8 +387,8 @@ int gf_las_pack_extra(gf_ctx *ctx, const double *const *cols, int ncols, uint64_
  * and a dst_capacity below it is GF_ERR_ARG. */
 int gf_las_write_extra(gf_ctx *ctx, const void *src, uint64_t src_nbytes, int ncols,
                        const char *const *names, const char *const *descriptions,
-                       const double *const *cols, int where, void *dst, uint64_t dst_capacity,
-                       uint64_t *dst_nbytes);
+                       const uint8_t *types, const double *const *cols, int where, void *dst,
+                       uint64_t dst_capacity, uint64_t *dst_nbytes);
 
 /* ---- parity hooks (no reference counterpart; used by the test suite). */
 /* The device eigensolver on m row-major 3x3 matrices. */

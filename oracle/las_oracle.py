@@ -189,16 +189,33 @@ def read_extra(buf):
             for name, t, off, _ in extra_dims(buf)}
 
 
-def write_extra(src, names, cols, descriptions=None):
-    """writers.las extra_dims for double columns (io/LasWriter.cpp:583-596,
-    io/private/las/Utils.cpp:244-267) as a new file image: the columns after
-    each source record, descriptors (type 10, zero-padded 32-byte name and
-    description) appended to the source's Extra Bytes VLR or in a new one
-    after the last VLR; source bytes no descriptor covers declared first as
-    undocumented ones; header offsets, VLR count and record length updated,
-    the 1.3 waveform and 1.4 EVLR offsets moved."""
+def typed(col, dtype):
+    """A double column as LAS data type 1..10: integer types take the value
+    rounded toward zero and held to the type's range, NaN as 0."""
+    t = np.dtype(EB_TYPES[dtype])
+    col = np.asarray(col, dtype=np.float64)
+    if t.kind == "f":
+        return col.astype(t)
+    info = np.iinfo(t)
+    lo, hi = float(info.min), float(info.max)
+    if hi >= 2.0 ** 63:
+        hi = 18446744073709549568.0 if info.min == 0 else 9223372036854774784.0
+    v = np.where(np.isnan(col), 0.0, np.clip(np.trunc(col), lo, hi))
+    return v.astype(t)
+
+
+def write_extra(src, names, cols, descriptions=None, types=None):
+    """writers.las extra_dims (io/LasWriter.cpp:583-596,
+    io/private/las/Utils.cpp:244-267) as a new file image: the columns (each
+    as its data type, 10 = double by default) after each source record,
+    descriptors (zero-padded 32-byte name and description) appended to the
+    source's Extra Bytes VLR or in a new one after the last VLR; source bytes
+    no descriptor covers declared first as undocumented ones; header
+    offsets, VLR count and record length updated, the 1.3 waveform and 1.4
+    EVLR offsets moved."""
     h = parse_header(src)
     descriptions = descriptions or [""] * len(names)
+    types = list(types) if types is not None else [10] * len(cols)
     vl = vlrs(src)
     eb = next((i for i, v in enumerate(vl)
                if v["user_id"] == "LASF_Spec" and v["record_id"] == 4), None)
@@ -223,10 +240,10 @@ def write_extra(src, names, cols, descriptions=None):
         g = min(gap, 255)
         add += descriptor(0, g, "", "")
         gap -= g
-    for name, desc in zip(names, descriptions):
-        add += descriptor(10, 0, name, desc)
+    for name, desc, t in zip(names, descriptions, types):
+        add += descriptor(t, 0, name, desc)
     grow = len(add) + (0 if eb is not None else VLR_HEADER)
-    S2 = S + 8 * len(cols)
+    S2 = S + sum(np.dtype(EB_TYPES[t]).itemsize for t in types)
     new_offset = h["point_offset"] + grow
     src_points_end = h["point_offset"] + n * S
     after = new_offset + n * S2 - src_points_end
@@ -261,7 +278,12 @@ def write_extra(src, names, cols, descriptions=None):
     rec = np.frombuffer(src, dtype=np.uint8, count=n * S, offset=h["point_offset"]).reshape(n, S)
     wide = np.zeros((n, S2), np.uint8)
     wide[:, :S] = rec
-    pack_extra(wide.reshape(-1), list(cols), S2, S)
+    at = S
+    for col, t in zip(cols, types):
+        size = np.dtype(EB_TYPES[t]).itemsize
+        v = np.ascontiguousarray(typed(col, t)).view(np.uint8).reshape(n, size)
+        wide[:, at:at + size] = v
+        at += size
     out += wide.tobytes()
     out += src[src_points_end:]
     return bytes(out)

@@ -72,3 +72,19 @@ def test_write_errors(ctx):
         ctx.las_write_extra(load("simple.laz"), ["A"], [np.zeros(1065)])
     with pytest.raises(g.GfError):
         ctx.las_write_extra(b"not a las file" * 20, ["A"], [np.zeros(3)])
+
+
+def test_typed_columns_equal_oracle(ctx):
+    src = load("1.2-with-color.las")
+    n = lo.parse_header(src)["point_count"]
+    rng = np.random.default_rng(5)
+    cols = [rng.integers(0, 2, n).astype(float), rng.integers(10, 15, n).astype(float),
+            rng.normal(size=n) * 1e3, rng.normal(size=n) * 300, rng.normal(size=n) * 1e12,
+            rng.normal(size=n) * 1e5, rng.normal(size=n) * 1e5, rng.normal(size=n) * 1e5,
+            rng.normal(size=n) * 1e19, rng.normal(size=n)]
+    for c in cols[2:]:
+        c[:5] = [np.nan, np.inf, -np.inf, 1e300, -0.5]
+    types = list(range(1, 11))
+    names = ["T%d" % t for t in types]
+    got = ctx.las_write_extra(src, names, cols, types=types)
+    assert got == lo.write_extra(src, names, cols, types=types)

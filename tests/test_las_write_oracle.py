@@ -121,3 +121,24 @@ def test_writer_declares_undocumented_bytes():
     assert [(d[0], d[2]) for d in dims] == [("Omnivariance", S + 3)]
     assert np.array_equal(lo.read_extra(out)["Omnivariance"].view(np.uint64),
                           col[0].view(np.uint64))
+
+
+def test_writer_typed_columns():
+    # Coplanar (uint8), OptimalKNN (uint64), Rank (uint8) as PDAL types
+    # them, a float and a signed column, with values outside the ranges.
+    src = load("simple.las")
+    n = lo.parse_header(src)["point_count"]
+    rng = np.random.default_rng(2)
+    cols = [rng.integers(0, 2, n).astype(float), rng.integers(10, 15, n).astype(float),
+            rng.normal(size=n) * 1e3, rng.normal(size=n) * 300, rng.normal(size=n)]
+    cols[3][:4] = [np.nan, 1e30, -1e30, -2.7]
+    types = [1, 7, 9, 2, 10]
+    names = ["Coplanar", "OptimalKNN", "F", "I8", "D"]
+    out = lo.write_extra(src, names, cols, types=types)
+    ex = lo.read_extra(out)
+    assert ex["Coplanar"].dtype == np.uint8 and np.array_equal(ex["Coplanar"], cols[0])
+    assert ex["OptimalKNN"].dtype == np.uint64 and np.array_equal(ex["OptimalKNN"], cols[1])
+    assert np.array_equal(ex["F"], cols[2].astype(np.float32))
+    assert ex["I8"].dtype == np.int8 and list(ex["I8"][:4]) == [0, 127, -128, -2]
+    assert np.array_equal(ex["D"], cols[4])
+    assert lo.parse_header(out)["point_size"] == lo.parse_header(src)["point_size"] + 1 + 8 + 4 + 1 + 8
