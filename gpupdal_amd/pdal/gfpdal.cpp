@@ -8,6 +8,8 @@
 #include <cerrno>
 #include <cmath>
 #include <cstdlib>
+#include <fstream>
+#include <iterator>
 #include <mutex>
 #include <sstream>
 
@@ -319,6 +321,12 @@ void PointView::setField(Dimension::Id dim, PointId idx, double val)
     column(dim)[idx] = val;
     if (dim == Dimension::Id::X || dim == Dimension::Id::Y || dim == Dimension::Id::Z)
         m_resident.reset();
+}
+
+void PointView::resize(point_count_t n)
+{
+    m_size = n;
+    for (auto& kv : m_cols) kv.second.resize(m_size, 0.0);
 }
 
 template <class T> T PointView::getFieldAs(Dimension::Id dim, PointId idx) const
@@ -974,8 +982,221 @@ void OutlierFilter::filter(PointView& view)
         if (flags[i]) view.setField(Dimension::Id::Classification, i, m_class);
 }
 
+// ------------------------------------------------------------------- LAS --
+namespace {
+
+// LAS data type (1..10 = u8, i8, u16, i16, u32, i32, u64, i64, f32, f64) and
+// description of a dimension, from pdal/Dimension.json; descriptions are
+// what the Extra Bytes descriptor holds (32 bytes).
+struct DimInfo {
+    Dimension::Id id;
+    uint8_t las_type;
+    const char* description;
+};
+
+const DimInfo kDimInfo[] = {
+    {Dimension::Id::Linearity, 10, "Linearity of a point; larger val"},
+    {Dimension::Id::Planarity, 10, "Planarity of a point; larger val"},
+    {Dimension::Id::Scattering, 10, "Scattering of a point; larger va"},
+    {Dimension::Id::Verticality, 10, "Verticality of a point; larger v"},
+    {Dimension::Id::Omnivariance, 10, "Omnivariance of a point; cube ro"},
+    {Dimension::Id::Anisotropy, 10, "Anisotropy of a point; larger va"},
+    {Dimension::Id::Eigenentropy, 10, "Eigenentropy of a point; small v"},
+    {Dimension::Id::EigenvalueSum, 10, "Sum of computed eigenvalues."},
+    {Dimension::Id::SurfaceVariation, 10, "Surface variation of a point; la"},
+    {Dimension::Id::DemantkeVerticality, 10, "Verticality of a point; larger v"},
+    {Dimension::Id::Density, 10, "Estimate of point density"},
+    {Dimension::Id::OptimalKNN, 7, "Optimal number of k nearest neig"},
+    {Dimension::Id::OptimalRadius, 10, "Radius corresponding to optimal "},
+    {Dimension::Id::NormalX, 10, "X component of a vector normal t"},
+    {Dimension::Id::NormalY, 10, "Y component of a vector normal t"},
+    {Dimension::Id::NormalZ, 10, "Z component of a vector normal t"},
+    {Dimension::Id::Curvature, 10, "Curvature of surface at this poi"},
+    {Dimension::Id::Eigenvalue0, 10, "Smallest eigenvalue obtained for"},
+    {Dimension::Id::Eigenvalue1, 10, "Middle eigenvalue obtained form "},
+    {Dimension::Id::Eigenvalue2, 10, "Largest eigenvalue obtained form"},
+    {Dimension::Id::Coplanar, 1, "Indicator of whether or not a po"},
+    {Dimension::Id::Rank, 1, "Estimated rank of neighborhood o"},
+    {Dimension::Id::NNDistance, 10, "Distance metric related to a poi"},
+};
+
+const DimInfo* dim_info(Dimension::Id id)
+{
+    for (const DimInfo& d : kDimInfo)
+        if (d.id == id) return &d;
+    return nullptr;
+}
+
+// Dimension::type's names (pdal/DimUtil.hpp) as LAS data types.
+int las_type_named(const std::string& t)
+{
+    static const std::pair<const char*, int> names[] = {
+        {"uint8", 1},  {"uint8_t", 1},  {"int8", 2},    {"int8_t", 2},   {"uint16", 3},
+        {"uint16_t", 3}, {"int16", 4},  {"int16_t", 4}, {"uint32", 5},   {"uint32_t", 5},
+        {"int32", 6},  {"int32_t", 6},  {"uint64", 7},  {"uint64_t", 7}, {"int64", 8},
+        {"int64_t", 8}, {"float", 9},   {"float32", 9}, {"double", 10},  {"float64", 10}};
+    const std::string l = lower(trim(t));
+    for (const auto& nt : names)
+        if (l == nt.first) return nt.second;
+    return 0;
+}
+
+bool dim_named(const std::string& name, Dimension::Id& out)
+{
+    const std::string l = lower(trim(name));
+    for (const DimInfo& d : kDimInfo)
+        if (lower(Dimension::name(d.id)) == l) {
+            out = d.id;
+            return true;
+        }
+    return false;
+}
+
+} // namespace
+
+void LasReader::addArgs(ProgramArgs& args)
+{
+    args.add("filename", "Name of file to read", m_filename, std::string());
+}
+
+void LasReader::addDimensions(PointLayout* layout)
+{
+    layout->registerDim(Dimension::Id::Classification);
+}
+
+PointViewSet LasReader::run(PointTable& table)
+{
+    std::ifstream in(m_filename, std::ios::binary);
+    if (!in) throwError("Unable to open stream for '" + m_filename + "'.");
+    auto img = std::make_shared<std::vector<uint8_t>>(std::istreambuf_iterator<char>(in),
+                                                      std::istreambuf_iterator<char>());
+    gf_las_info info{};
+    char msg[256] = {0};
+    if (gf_las_parse(img->data(), img->size(), &info, msg, sizeof(msg)) != GF_OK)
+        throwError(msg);
+    gf_ctx* ctx = device_context();
+    auto r = std::make_shared<ResidentCloud>();
+    if (gf_cloud_create_las(ctx, img->data(), img->size(), &info, &r->cloud) != GF_OK)
+        throwError(gf_last_error(ctx));
+    PointViewPtr view(new PointView(table));
+    const uint64_t n = info.point_count;
+    view->resize(n);
+    std::vector<double>& x = view->column(Dimension::Id::X);
+    std::vector<double>& y = view->column(Dimension::Id::Y);
+    std::vector<double>& z = view->column(Dimension::Id::Z);
+    if (n && gf_debug_cloud_xyz(ctx, r->cloud, x.data(), y.data(), z.data()) != GF_OK)
+        throwError(gf_last_error(ctx));
+    // Classification: the low five bits of byte 15 (formats 0-5), byte 16
+    // (formats 6-10) (io/private/las/Utils.cpp point field layouts).
+    std::vector<double>& cls = view->column(Dimension::Id::Classification);
+    const uint8_t* rec = img->data() + info.point_offset;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint8_t* p = rec + i * info.point_size;
+        cls[i] = info.point_format < 6 ? (p[15] & 0x1F) : p[16];
+    }
+    view->adoptResident(r);
+    view->setLasSource(img);
+    return PointViewSet{view};
+}
+
+void LasWriter::addArgs(ProgramArgs& args)
+{
+    args.add("filename", "Output filename", m_filename, std::string());
+    args.add("extra_dims", "List of dimension names to write in addition to those of the "
+                           "point format or 'all' for all available dimensions",
+             m_extraDims, StringList{});
+}
+
+PointViewSet LasWriter::run(PointTable& table)
+{
+    PointViewSet views = m_input ? Stage::run(table) : PointViewSet{};
+    if (views.size() != 1) throwError("writes exactly one point view");
+    PointView& view = **views.begin();
+    const auto src = view.lasSource();
+    if (!src) throwError("writes the points of a view read by readers.las");
+    // extra_dims: "all" or names, each optionally "=type".
+    std::vector<Dimension::Id> ids;
+    std::vector<uint8_t> types;
+    for (const std::string& item : m_extraDims) {
+        const std::string t = trim(item);
+        if (lower(t) == "all") {
+            for (const DimInfo& d : kDimInfo)
+                if (view.findColumn(d.id) && table.layout()->hasDim(d.id)) {
+                    ids.push_back(d.id);
+                    types.push_back(d.las_type);
+                }
+            continue;
+        }
+        const size_t eq = t.find('=');
+        Dimension::Id id;
+        if (!dim_named(t.substr(0, eq), id))
+            throwError("Invalid extra dimension specified: '" + t + "'.  Need <dimension>=<type>.");
+        int lt = dim_info(id)->las_type;
+        if (eq != std::string::npos) {
+            lt = las_type_named(t.substr(eq + 1));
+            if (!lt) throwError("Invalid extra dimension type specified: '" + t + "'.");
+        }
+        ids.push_back(id);
+        types.push_back(static_cast<uint8_t>(lt));
+    }
+    gf_las_info info{};
+    char msg[256] = {0};
+    if (gf_las_parse(src->data(), src->size(), &info, msg, sizeof(msg)) != GF_OK)
+        throwError(msg);
+    const uint64_t n = info.point_count;
+    if (view.size() != n) throwError("the view no longer holds the file's points");
+    std::vector<uint8_t> img(*src);
+    // Classification back into the records.
+    if (const std::vector<double>* cls = view.findColumn(Dimension::Id::Classification)) {
+        uint8_t* rec = img.data() + info.point_offset;
+        for (uint64_t i = 0; i < n; ++i) {
+            uint8_t* p = rec + i * info.point_size;
+            const uint8_t c = static_cast<uint8_t>((*cls)[i]);
+            if (info.point_format < 6)
+                p[15] = static_cast<uint8_t>((p[15] & 0xE0) | (c & 0x1F));
+            else
+                p[16] = c;
+        }
+    }
+    std::vector<uint8_t> out;
+    if (ids.empty()) {
+        out.swap(img);
+    } else {
+        std::vector<std::string> names, descs;
+        std::vector<const double*> cols;
+        for (Dimension::Id id : ids) {
+            names.push_back(Dimension::name(id));
+            descs.push_back(dim_info(id)->description);
+            cols.push_back(view.column(id).data());
+        }
+        std::vector<const char*> np, dp;
+        for (size_t i = 0; i < names.size(); ++i) {
+            np.push_back(names[i].c_str());
+            dp.push_back(descs[i].c_str());
+        }
+        gf_ctx* ctx = device_context();
+        uint64_t size = 0;
+        const int k = static_cast<int>(ids.size());
+        if (gf_las_write_extra(ctx, img.data(), img.size(), k, np.data(), dp.data(), types.data(),
+                               cols.data(), GF_WHERE_HOST, nullptr, 0, &size) != GF_OK)
+            throwError(gf_last_error(ctx));
+        out.resize(size);
+        if (gf_las_write_extra(ctx, img.data(), img.size(), k, np.data(), dp.data(), types.data(),
+                               cols.data(), GF_WHERE_HOST, out.data(), out.size(), &size) !=
+            GF_OK)
+            throwError(gf_last_error(ctx));
+    }
+    std::ofstream f(m_filename, std::ios::binary | std::ios::trunc);
+    if (!f) throwError("Couldn't open file '" + m_filename + "' for output.");
+    f.write(reinterpret_cast<const char*>(out.data()), static_cast<std::streamsize>(out.size()));
+    if (!f) throwError("Couldn't write file '" + m_filename + "'.");
+    return views;
+}
+
 std::unique_ptr<Stage> createStage(const std::string& name)
 {
+    if (name == "readers.las") return std::unique_ptr<Stage>(new LasReader);
+    if (name == "writers.las") return std::unique_ptr<Stage>(new LasWriter);
     if (name == "filters.approximatecoplanar")
         return std::unique_ptr<Stage>(new ApproximateCoplanarFilter);
     if (name == "filters.estimaterank") return std::unique_ptr<Stage>(new EstimateRankFilter);

@@ -10,7 +10,8 @@
 //
 // and the other consumers of KD3Index neighborhoods beside them:
 // filters.optimalneighborhood, filters.approximatecoplanar,
-// filters.estimaterank, filters.nndistance and filters.outlier.
+// filters.estimaterank, filters.nndistance and filters.outlier; readers.las
+// and writers.las (uncompressed LAS, extra_dims) on either side.
 //
 // libpdal itself is not available to link against, so the small part of its
 // Stage API these stages use is provided here under the same names (Options,
@@ -153,12 +154,22 @@ public:
     const std::vector<double>* findColumn(Dimension::Id dim) const;
     PointTable& table() const { return *m_table; }
     std::shared_ptr<ResidentCloud> resident();
+    // n points, every column zero-filled to n.
+    void resize(point_count_t n);
+    // The cloud as already resident on the GPU (readers.las decodes there);
+    // X, Y, Z of the view must hold the same coordinates.
+    void adoptResident(std::shared_ptr<ResidentCloud> r) { m_resident = std::move(r); }
+    // The LAS file image the view was read from (readers.las), which
+    // writers.las writes the points of.
+    std::shared_ptr<const std::vector<uint8_t>> lasSource() const { return m_las; }
+    void setLasSource(std::shared_ptr<const std::vector<uint8_t>> img) { m_las = std::move(img); }
 
 private:
     PointTable* m_table;
     point_count_t m_size = 0;
     std::map<Dimension::Id, std::vector<double>> m_cols;
     std::shared_ptr<ResidentCloud> m_resident;
+    std::shared_ptr<const std::vector<uint8_t>> m_las;
 };
 using PointViewPtr = std::shared_ptr<PointView>;
 using PointViewSet = std::set<PointViewPtr>;
@@ -363,6 +374,41 @@ private:
     int m_meanK = 8;
     double m_multiplier = 2.0;
     uint8_t m_class = 7;
+};
+
+// readers.las (io/LasReader.cpp) for uncompressed LAS: the points are
+// decoded on the GPU (X, Y, Z stay resident for the stages that follow and
+// are copied to the view's columns) and Classification is read from the
+// records; the view keeps the file image for writers.las.
+class LasReader : public Stage {
+public:
+    std::string getName() const override { return "readers.las"; }
+
+protected:
+    void addArgs(ProgramArgs& args) override;
+    void addDimensions(PointLayout* layout) override;
+    PointViewSet run(PointTable& table) override;
+
+private:
+    std::string m_filename;
+};
+
+// writers.las (io/LasWriter.cpp) for the points of a readers.las view: the
+// source file's header, VLRs and records, with the view's Classification
+// written back into the records and the extra_dims appended as Extra Bytes
+// (each as its dimension's type, or the type the option names).  The
+// source's point format, version, scale and offset are kept.
+class LasWriter : public Stage {
+public:
+    std::string getName() const override { return "writers.las"; }
+
+protected:
+    void addArgs(ProgramArgs& args) override;
+    PointViewSet run(PointTable& table) override;
+
+private:
+    std::string m_filename;
+    StringList m_extraDims;
 };
 
 // Creates a stage by its PDAL name (nullptr for other names), as

@@ -9,10 +9,16 @@
 // :46-188, test/unit/filters/NormalFilterTest.cpp:47-253 and
 // test/unit/filters/NNDistanceTest.cpp:43-125.
 #include "gfpdal.hpp"
+#include "gpupdal_amd.h"
 #include "mini_test.hpp"
+
+#include <unistd.h>
 
 #include <array>
 #include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iterator>
 #include <sstream>
 
 using namespace gfpdal;
@@ -42,6 +48,28 @@ std::vector<std::array<double, 3>> faux_grid(int x0, int x1, int y0, int y1, int
             for (int x = x0; x < std::max(x1, x0 + 1); ++x)
                 if (p.size() < count) p.push_back({double(x), double(y), double(z)});
     return p;
+}
+
+// A LAS fixture of the test suite (tests/golden/las), found from this
+// program's own path (tests/cpp/stage_test).
+std::string fixture(const std::string& name)
+{
+    char buf[4096] = {0};
+    const ssize_t len = readlink("/proc/self/exe", buf, sizeof(buf) - 1);
+    std::string exe(buf, len > 0 ? static_cast<size_t>(len) : 0);
+    return exe.substr(0, exe.rfind('/')) + "/../golden/las/" + name;
+}
+
+std::vector<uint8_t> read_file(const std::string& path)
+{
+    std::ifstream in(path, std::ios::binary);
+    return std::vector<uint8_t>(std::istreambuf_iterator<char>(in),
+                                std::istreambuf_iterator<char>());
+}
+
+std::string temp_las(const char* tag)
+{
+    return std::string("/tmp/gfpdal_") + tag + "_" + std::to_string(getpid()) + ".las";
 }
 
 const char* kAllButDensity =
@@ -207,6 +235,8 @@ TEST(CpuFactory, StageNames)
     ASSERT_EQ(createStage("filters.estimaterank")->getName(), std::string("filters.estimaterank"));
     ASSERT_EQ(createStage("filters.nndistance")->getName(), std::string("filters.nndistance"));
     ASSERT_EQ(createStage("filters.outlier")->getName(), std::string("filters.outlier"));
+    ASSERT_EQ(createStage("readers.las")->getName(), std::string("readers.las"));
+    ASSERT_EQ(createStage("writers.las")->getName(), std::string("writers.las"));
     ASSERT_TRUE(createStage("filters.smrf") == nullptr);
 }
 
@@ -780,6 +810,99 @@ TEST(GpuOutlier, WarningsAndEmptyView)
     r2.addView(PointViewPtr(new PointView(t2)));  // empty: dropped
     ASSERT_EQ(f2.execute(t2).size(), size_t(1));
     ASSERT_TRUE(log2.str().find("Requested method is unrecognized.") != std::string::npos);
+}
+
+TEST(GpuLas, OutlierChainReferenceCount)
+{
+    // readers.las -> filters.outlier -> writers.las, read back: the points
+    // not classified 7 are the reference's 96 (test/unit/OldPCLBlockTest.cpp
+    // :43-80, statistical, mean_k 2, multiplier 1.5).
+    const std::string out = temp_las("outlier");
+    {
+        PointTable table;
+        LasReader reader;
+        OutlierFilter outlier;
+        LasWriter writer;
+        Options ro, fo, wo;
+        ro.add("filename", fixture("autzen-point-format-3.las"));
+        fo.add("method", "statistical");
+        fo.add("multiplier", 1.5);
+        fo.add("mean_k", 2);
+        wo.add("filename", out);
+        reader.setOptions(ro);
+        outlier.setOptions(fo);
+        writer.setOptions(wo);
+        outlier.setInput(reader);
+        writer.setInput(outlier);
+        writer.prepare(table);
+        writer.execute(table);
+    }
+    PointTable t2;
+    LasReader back;
+    Options bo;
+    bo.add("filename", out);
+    back.setOptions(bo);
+    back.prepare(t2);
+    PointViewPtr v = *back.execute(t2).begin();
+    ASSERT_EQ(v->size(), point_count_t(106));
+    int kept = 0;
+    for (PointId i = 0; i < v->size(); ++i)
+        kept += v->getFieldAs<double>(Id::Classification, i) != 7.0;
+    ASSERT_EQ(kept, 96);
+    std::remove(out.c_str());
+}
+
+TEST(GpuLas, FeaturesAsExtraBytes)
+{
+    // readers.las -> covariancefeatures -> writers.las extra_dims: the
+    // records grow by a double and a float, holding the view's values.
+    const std::string src = fixture("1.2-with-color.las");
+    const std::string out = temp_las("features");
+    PointTable table;
+    LasReader reader;
+    CovarianceFeaturesFilter feats;
+    LasWriter writer;
+    Options ro, wo;
+    ro.add("filename", src);
+    wo.add("filename", out);
+    wo.add("extra_dims", "Linearity, Planarity=float");
+    reader.setOptions(ro);
+    writer.setOptions(wo);
+    feats.setInput(reader);
+    writer.setInput(feats);
+    writer.prepare(table);
+    PointViewPtr v = *writer.execute(table).begin();
+    const std::vector<uint8_t> a = read_file(src), b = read_file(out);
+    gf_las_info ha{}, hb{};
+    ASSERT_EQ(gf_las_parse(a.data(), a.size(), &ha, nullptr, 0), GF_OK);
+    ASSERT_EQ(gf_las_parse(b.data(), b.size(), &hb, nullptr, 0), GF_OK);
+    ASSERT_EQ(hb.point_size, ha.point_size + 12);
+    ASSERT_EQ(hb.point_count, ha.point_count);
+    for (PointId i = 0; i < v->size(); ++i) {
+        const uint8_t* ra = a.data() + ha.point_offset + i * ha.point_size;
+        const uint8_t* rb = b.data() + hb.point_offset + i * hb.point_size;
+        ASSERT_TRUE(std::memcmp(ra, rb, ha.point_size) == 0);
+        double lin;
+        float pla;
+        std::memcpy(&lin, rb + ha.point_size, 8);
+        std::memcpy(&pla, rb + ha.point_size + 8, 4);
+        const double want = v->getFieldAs<double>(Id::Linearity, i);
+        ASSERT_TRUE(std::memcmp(&lin, &want, 8) == 0);
+        const float wantf = static_cast<float>(v->getFieldAs<double>(Id::Planarity, i));
+        ASSERT_TRUE(std::memcmp(&pla, &wantf, 4) == 0);
+    }
+    std::remove(out.c_str());
+}
+
+TEST(CpuLas, WriterOptions)
+{
+    PointTable table;
+    LasWriter w;
+    Options o;
+    o.add("dataformat_id", 3);
+    w.setOptions(o);
+    EXPECT_THROW_MSG(w.prepare(table), pdal_error,
+                     "writers.las: Unexpected argument 'dataformat_id'.");
 }
 
 int main(int argc, char** argv) { return mini::run_all(argc, argv); }
