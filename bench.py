@@ -213,6 +213,23 @@ def batched_leg(args):
     n_total = sum(len(t[0]) for t in tiles)
     ctx = new_context(args)
     ctx.set_tuning("timing", 0)
+    # Autotune each (density, knn) bucket the tiles fall in on one tile of
+    # at least 100k points: the column-size factor and the search shape,
+    # which both the batch and the tile-by-tile runs then use.
+    tuned = set()
+    if not args.no_autotune:
+        for (x, y, z), k in zip(tiles, knns):
+            if len(x) < 100_000:
+                continue
+            rho = len(x) / max((x.max() - x.min()) * (y.max() - y.min()), 1e-9)
+            key = (int(np.floor(np.log2(rho))), int(np.ceil(np.log2(k + 1))))
+            if key in tuned:
+                continue
+            tuned.add(key)
+            cl = ctx.cloud(x, y, z)
+            ctx.autotune(cl, [g.stage(g.STAGE_NORMAL, k), g.stage(g.STAGE_COVFEAT, k)],
+                         factors=(0.45, 0.6, 0.75, 0.9))
+            cl.close()
     stages = [g.stage(g.STAGE_NORMAL, 16), g.stage(g.STAGE_COVFEAT, 16)]
     ncols = 8
     b = ctx.batch(tiles)
@@ -257,7 +274,7 @@ def batched_leg(args):
             "value": n_total / (ms_batched / 1e3), "unit": "points/s",
             "ms_per_step": ms_batched,
             "plain_value": n_total / (ms_plain / 1e3), "plain_ms_per_step": ms_plain,
-            "batched_over_plain": ms_plain / ms_batched}
+            "batched_over_plain": ms_plain / ms_batched, "autotuned_buckets": len(tuned)}
 
 
 def las_leg(args, xyz):
@@ -512,6 +529,8 @@ def main():
     ap.add_argument("--tiles", type=int, default=256, help="tiles of the batched leg")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched leg: no autotuning of the tiles' buckets")
     ap.add_argument("--requests", type=int, default=64, help="requests of the serving leg")
     ap.add_argument("--rate", type=float, default=120.0, help="serving arrivals per second")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")

@@ -351,8 +351,10 @@ class Context:
         return out.tobytes()
 
     def autotune(self, cloud, stages, factors=(0.4, 0.5, 0.6, 0.75, 0.9)):
-        """Times the run for each column-size factor; the fastest is kept for
-        the cloud's (density, k) bucket.  Returns (factor, ms)."""
+        """Times the run for each column-size factor, then each search-shape
+        knob (z grouping, queries per wave, waves per workgroup) one at a
+        time; the fastest settings are kept for the cloud's (density, k)
+        bucket.  Returns (factor, ms of the fastest run)."""
         stages = list(stages)
         arr = (GfStage * len(stages))(*stages)
         f = np.ascontiguousarray(factors, dtype=np.float64)

@@ -205,7 +205,20 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     }
     points_mark = false;
     build_batch_index(b, fetch);
-    const int Q = knn_queries_per_wave(max_k);
+    // One search launch serves every tile: it takes the shape autotuned for
+    // the bucket of the batch's largest tile (the context's otherwise).
+    KnnTuning kt = knn_tuning();
+    uint32_t big = 0;
+    for (uint32_t t = 1; t < T; ++t)
+        if (b.starts[t + 1] - b.starts[t] > b.starts[big + 1] - b.starts[big]) big = t;
+    if (T) {
+        const uint64_t nb = b.starts[big + 1] - b.starts[big];
+        if (const TuneEntry* e = tune.entry_for(cloud_rho(&b.bounds[6 * big], nb), fetch[big])) {
+            if (e->group_z > 0) kt.group_z = e->group_z - 1;
+            if (e->q_per_wave > 0) kt.q_per_wave = e->q_per_wave;
+        }
+    }
+    const int Q = kt.q_per_wave > 0 ? kt.q_per_wave : knn_queries_per_wave(max_k);
     std::vector<uint32_t> task0(static_cast<size_t>(T) + 1, 0u);
     uint64_t ntasks = 0;
     for (uint32_t t = 0; t < T; ++t) {
@@ -277,7 +290,7 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     const bool early = !b.kd_valid && tune.kd_overlap_min && N >= tune.kd_overlap_min;
     if (early) fork_side();
     hipEvent_t e0 = mark();
-    launch_knn(cv, prog0, 0, out, knn_tuning(), stream, &tt);
+    launch_knn(cv, prog0, 0, out, kt, stream, &tt);
     if (any_exact_tile) launch_flag_tiles(N, cv.perm, 0, out, tt, stream);
     hipEvent_t e1 = mark();
     span(e0, e1, &stats.ms_search);

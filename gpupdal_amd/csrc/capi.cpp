@@ -19,6 +19,28 @@ using namespace gf;
 
 namespace gf {
 
+// Points per unit of the cloud's xy extent (the density rule's rho).
+double cloud_rho(const double b[6], uint64_t n)
+{
+    const double sx = b[3] - b[0], sy = b[4] - b[1];
+    const double big = std::max(std::max(sx, sy), std::max(b[5] - b[2], 0.0));
+    const double tiny = std::max(big * 1e-9, 1e-9);
+    return static_cast<double>(std::max<uint64_t>(n, 1)) /
+           (std::max(sx, tiny) * std::max(sy, tiny));
+}
+
+KnnTuning Context::knn_tuning(const Cloud& cl, uint32_t fetch) const
+{
+    KnnTuning k = knn_tuning();
+    if (tune.forced_factor > 0) return k;  // an autotuning run: the settings under test
+    if (const TuneEntry* e = tune.entry_for(cloud_rho(cl.bounds, cl.n), fetch)) {
+        if (e->group_z > 0) k.group_z = e->group_z - 1;
+        if (e->q_per_wave > 0) k.q_per_wave = e->q_per_wave;
+        if (e->knn_waves > 0) k.waves = e->knn_waves;
+    }
+    return k;
+}
+
 GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch,
                      const Tuning& t, double cell)
 {
@@ -297,7 +319,7 @@ void Context::run_group(Cloud& cl, const gf_stage* st, int ns, double* const* co
     if (all_exact)  // fewer points than the result capacity, or a large one
         launch_flag_all(n, cv.perm, 0, out, stream);
     else
-        launch_knn(cv, prog, 0, out, knn_tuning(), stream);
+        launch_knn(cv, prog, 0, out, knn_tuning(cl, fetch), stream);
     hipEvent_t e1 = mark();
     span(e0, e1, &stats.ms_search);
     // Rows the search did not flag are final: their features run now, beside
@@ -622,7 +644,7 @@ void Context::knn_lists(Cloud& cl, uint32_t k, uint32_t stride, int mode, uint32
     if (all_exact)
         launch_flag_all(n, cv.perm, mode, out, stream);
     else
-        launch_knn(cv, prog, mode, out, knn_tuning(), stream);
+        launch_knn(cv, prog, mode, out, knn_tuning(cl, fetch), stream);
     if (early) build_kdtree(cl, side, true);
     const uint32_t nflagged = resolve_flagged(cl, cv, prog, mode, out, nullptr);
     if (mode == 2)
@@ -1335,7 +1357,7 @@ int gf_optimal_neighborhood(gf_ctx* ctx, gf_cloud* cloud, uint32_t min_k, uint32
         const CloudView cv = c->view(*cl);
         const bool early = !cl->kd_valid && c->tune.kd_overlap_min && n >= c->tune.kd_overlap_min;
         if (early) c->fork_side();
-        launch_knn(cv, prog, 2, out, c->knn_tuning(), c->stream);
+        launch_knn(cv, prog, 2, out, c->knn_tuning(*cl, max_k), c->stream);
         if (early) c->build_kdtree(*cl, c->side, true);
         const uint32_t nflagged = c->resolve_flagged(*cl, cv, prog, 2, out, nullptr);
         launch_lists_to_sorted(out.ids, static_cast<int>(max_k), out.flagged, nflagged, cv.perm,
@@ -1655,47 +1677,89 @@ int gf_autotune(gf_ctx* ctx, gf_cloud* cloud, const gf_stage* stages, int nstage
         for (int k = 0; k < nstages; ++k) sp[k] = dstat.as<uint8_t>() + static_cast<uint64_t>(k) * n;
         const bool timing = c->timing;
         c->timing = false;
+        const Tuning saved = c->tune;
+        // One measurement: the whole run, index rebuilt, best of two.
+        auto measure = [&]() {
+            double t_min = 1e300;
+            for (int rep = 0; rep < 2; ++rep) {
+                cl->idx_valid = false;
+                cl->kd_valid = false;
+                GF_HIP(hipStreamSynchronize(c->stream));
+                const auto t0 = std::chrono::steady_clock::now();
+                if (gf_run(ctx, cloud, stages, nstages, cp.data(), sp.data(), GF_WHERE_DEVICE) !=
+                    GF_OK)
+                    throw Error(GF_ERR_STATE, c->last_error);
+                const double ms = std::chrono::duration<double, std::milli>(
+                                      std::chrono::steady_clock::now() - t0)
+                                      .count();
+                t_min = std::min(t_min, ms);
+            }
+            return t_min;
+        };
         double best = 1e300, bestf = factors[0];
+        TuneEntry win{};
         try {
+            // Coordinate descent: the column-size factor, then the search's
+            // z grouping, queries per wave and waves per workgroup.
             for (int fi = 0; fi < nfactors; ++fi) {
                 if (!(factors[fi] > 0)) throw Error(GF_ERR_ARG, "factors must be positive");
                 c->tune.forced_factor = factors[fi];
-                double t_min = 1e300;
-                for (int rep = 0; rep < 2; ++rep) {
-                    cl->idx_valid = false;
-                    cl->kd_valid = false;
-                    GF_HIP(hipStreamSynchronize(c->stream));
-                    const auto t0 = std::chrono::steady_clock::now();
-                    if (gf_run(ctx, cloud, stages, nstages, cp.data(), sp.data(),
-                               GF_WHERE_DEVICE) != GF_OK)
-                        throw Error(GF_ERR_STATE, c->last_error);
-                    const double ms = std::chrono::duration<double, std::milli>(
-                                          std::chrono::steady_clock::now() - t0)
-                                          .count();
-                    t_min = std::min(t_min, ms);
-                }
-                if (t_min < best) {
-                    best = t_min;
+                const double ms = measure();
+                if (ms < best) {
+                    best = ms;
                     bestf = factors[fi];
                 }
             }
+            c->tune.forced_factor = bestf;
+            win.cell_factor = bestf;
+            int bestgz = saved.group_z;
+            for (int gz : {0, 1, 2, 3, 4}) {
+                if (gz == saved.group_z) continue;  // measured above
+                c->tune.group_z = gz;
+                const double ms = measure();
+                if (ms < best) {
+                    best = ms;
+                    bestgz = gz;
+                }
+            }
+            c->tune.group_z = bestgz;
+            win.group_z = bestgz + 1;
+            const int q0 = saved.q_per_wave;
+            int bestq = q0;
+            for (int q : {16, 32, 64}) {
+                if (q == q0) continue;
+                c->tune.q_per_wave = q;
+                const double ms = measure();
+                if (ms < best) {
+                    best = ms;
+                    bestq = q;
+                }
+            }
+            c->tune.q_per_wave = bestq;
+            win.q_per_wave = bestq;
+            int bestw = saved.knn_waves;
+            for (int w : {1, 2, 4}) {
+                if (w == saved.knn_waves || (saved.knn_waves == 0 && w == 4)) continue;
+                c->tune.knn_waves = w;
+                const double ms = measure();
+                if (ms < best) {
+                    best = ms;
+                    bestw = w;
+                }
+            }
+            win.knn_waves = bestw;
         } catch (...) {
-            c->tune.forced_factor = 0.0;
+            c->tune = saved;
             c->timing = timing;
             throw;
         }
-        c->tune.forced_factor = 0.0;
+        c->tune = saved;
         c->timing = timing;
         // Remember the winner for this cloud's bucket.
-        const double* b = cl->bounds;
-        const double sx = b[3] - b[0], sy = b[4] - b[1];
-        const double big = std::max(std::max(sx, sy), std::max(b[5] - b[2], 0.0));
-        const double tiny = std::max(big * 1e-9, 1e-9);
-        const double rho = static_cast<double>(n) / (std::max(sx, tiny) * std::max(sy, tiny));
         const int stride = stages[0].stage == GF_STAGE_NORMAL ? 1 : stages[0].stride;
         const uint32_t fetch = static_cast<uint32_t>(stride) *
                                static_cast<uint32_t>(std::min<uint64_t>(n, stage_neighbors(stages[0])));
-        c->tune.remember(rho, fetch, bestf);
+        c->tune.remember(cloud_rho(cl->bounds, n), fetch, win);
         cl->idx_valid = false;
         cl->kd_valid = false;
         if (best_factor) *best_factor = bestf;
@@ -1709,9 +1773,11 @@ int gf_tuning_save(gf_ctx* ctx, const char* path)
     return guarded(ctx, [&] {
         FILE* f = std::fopen(path, "w");
         if (!f) throw Error(GF_ERR_ARG, std::string("cannot write ") + path);
-        std::fprintf(f, "# gpupdal_amd autotune: rho_bucket k_bucket cell_factor\n");
+        std::fprintf(f, "# gpupdal_amd autotune: rho_bucket k_bucket cell_factor group_z+1 "
+                        "q_per_wave knn_waves (0: the context's setting)\n");
         for (const TuneEntry& e : C(ctx)->tune.table)
-            std::fprintf(f, "%d %d %.17g\n", e.rho_bucket, e.k_bucket, e.cell_factor);
+            std::fprintf(f, "%d %d %.17g %d %d %d\n", e.rho_bucket, e.k_bucket, e.cell_factor,
+                         e.group_z, e.q_per_wave, e.knn_waves);
         std::fclose(f);
     });
 }
@@ -1727,16 +1793,22 @@ int gf_tuning_load(gf_ctx* ctx, const char* path)
         while (std::fgets(line, sizeof(line), f)) {
             if (line[0] == '#') continue;
             TuneEntry e{};
-            if (std::sscanf(line, "%d %d %lf", &e.rho_bucket, &e.k_bucket, &e.cell_factor) == 3 &&
-                e.cell_factor > 0)
-                t.push_back(e);
+            const int got = std::sscanf(line, "%d %d %lf %d %d %d", &e.rho_bucket, &e.k_bucket,
+                                        &e.cell_factor, &e.group_z, &e.q_per_wave,
+                                        &e.knn_waves);
+            const bool shape_ok = e.group_z >= 0 && e.group_z <= 65 &&
+                                  (e.q_per_wave == 0 || e.q_per_wave == 16 ||
+                                   e.q_per_wave == 32 || e.q_per_wave == 64) &&
+                                  (e.knn_waves == 0 || e.knn_waves == 1 || e.knn_waves == 2 ||
+                                   e.knn_waves == 4);
+            if ((got == 3 || got == 6) && e.cell_factor > 0 && shape_ok) t.push_back(e);
         }
         std::fclose(f);
         for (const TuneEntry& e : t) {
             bool found = false;
             for (TuneEntry& x : C(ctx)->tune.table)
                 if (x.rho_bucket == e.rho_bucket && x.k_bucket == e.k_bucket) {
-                    x.cell_factor = e.cell_factor;
+                    x = e;
                     found = true;
                 }
             if (!found) C(ctx)->tune.table.push_back(e);

@@ -85,6 +85,8 @@ struct KnnTuning {
     int32_t ring_max;    // budget before a row is marked incomplete
     int32_t small_col;   // columns up to this size are taken whole
     int32_t group_z;     // z cells beyond the first a query group may span
+    int32_t q_per_wave;  // queries per wave (0: by k)
+    int32_t waves;       // waves per workgroup of the single-cloud search (0: 4)
 };
 
 struct CloudView {

@@ -48,10 +48,15 @@ struct DevBuf {
 
 // Autotuned column-size factors, per (2-D density, result capacity) bucket:
 // rho_bucket = floor(log2(points per m^2)), k_bucket = ceil(log2(fetch)).
+// One autotuned (density, capacity) bucket: the column-size factor and the
+// search's launch shape (0: the context's setting).
 struct TuneEntry {
     int rho_bucket;
     int k_bucket;
     double cell_factor;
+    int group_z = 0;      // 0: default; else z cells + 1
+    int q_per_wave = 0;
+    int knn_waves = 0;
 };
 
 inline int tune_rho_bucket(double rho) { return static_cast<int>(std::floor(std::log2(rho > 1e-300 ? rho : 1e-300))); }
@@ -66,6 +71,8 @@ struct Tuning {
     int ring_max = 8;
     int small_col = 0;
     int group_z = 3;             // measured: flat for knn 16, 4% faster search for knn 32
+    int q_per_wave = 0;          // queries per search wave (0: 64 up to k 32, else 32)
+    int knn_waves = 0;           // waves per search workgroup (0: 4)
     double radius_cell = 0.5;    // radius grids: column edge = radius_cell * radius
     // Exact tree: subtree-per-workgroup threshold (points; 2048-point
     // workgroups, two per CU, measured fastest alone and beside the search).
@@ -89,6 +96,10 @@ struct Tuning {
         else if (!std::strcmp(key, "ring_max") && v >= 1) ring_max = static_cast<int>(v);
         else if (!std::strcmp(key, "small_col") && v >= 0) small_col = static_cast<int>(v);
         else if (!std::strcmp(key, "group_z") && v >= 0) group_z = static_cast<int>(v);
+        else if (!std::strcmp(key, "q_per_wave") && (v == 0 || v == 16 || v == 32 || v == 64))
+            q_per_wave = static_cast<int>(v);
+        else if (!std::strcmp(key, "knn_waves") && (v == 0 || v == 1 || v == 2 || v == 4))
+            knn_waves = static_cast<int>(v);
         else if (!std::strcmp(key, "radius_cell") && v >= 0.2 && v <= 4.0) radius_cell = v;
         else if (!std::strcmp(key, "kd_subtree") && v >= 101 && v <= 4096)
             kd_subtree = static_cast<int>(v);
@@ -112,15 +123,26 @@ struct Tuning {
             if (e.rho_bucket == rb && e.k_bucket == kb) return e.cell_factor;
         return cell_factor;
     }
-    void remember(double rho, uint32_t fetch, double factor)
+    // The autotuned entry of a bucket, or null.
+    const TuneEntry* entry_for(double rho, uint32_t fetch) const
     {
         const int rb = tune_rho_bucket(rho), kb = tune_k_bucket(fetch);
-        for (TuneEntry& e : table)
-            if (e.rho_bucket == rb && e.k_bucket == kb) {
-                e.cell_factor = factor;
+        for (const TuneEntry& e : table)
+            if (e.rho_bucket == rb && e.k_bucket == kb) return &e;
+        return nullptr;
+    }
+    void remember(double rho, uint32_t fetch, const TuneEntry& v)
+    {
+        const int rb = tune_rho_bucket(rho), kb = tune_k_bucket(fetch);
+        TuneEntry e = v;
+        e.rho_bucket = rb;
+        e.k_bucket = kb;
+        for (TuneEntry& x : table)
+            if (x.rho_bucket == rb && x.k_bucket == kb) {
+                x = e;
                 return;
             }
-        table.push_back(TuneEntry{rb, kb, factor});
+        table.push_back(e);
     }
     // Grid-shaping knobs only (the ring knobs do not change the index).
     uint64_t signature() const
@@ -202,6 +224,9 @@ struct Batch {
     void reset(uint32_t ntiles, const uint64_t* counts, hipStream_t s);
 };
 
+// Points per unit of xy extent of bounds b (min xyz, max xyz) holding n
+// points: the density the grid rule and the autotune buckets use.
+double cloud_rho(const double b[6], uint64_t n);
 // cell > 0: that column edge instead of the density rule (radius grids).
 GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch, const Tuning& t,
                      double cell = 0.0);
@@ -285,8 +310,13 @@ struct Context {
         k.ring_max = tune.ring_max;
         k.small_col = tune.small_col;
         k.group_z = tune.group_z;
+        k.q_per_wave = tune.q_per_wave;
+        k.waves = tune.knn_waves;
         return k;
     }
+    // The search shape for a cloud and result capacity: the context's
+    // settings, with an autotuned bucket's overriding them.
+    KnnTuning knn_tuning(const Cloud& cl, uint32_t fetch) const;
 
     void build_index(Cloud& cl, uint32_t fetch, double cell = 0.0);
     // The grid of the cloud's current index (host copy).

@@ -45,7 +45,9 @@ namespace {
 // finds the slots a workgroup's early waves leave idle until its last wave
 // ends.  Tiles of a batch: one; a wave's slot and LDS free as soon as it ends
 // (measured: the batch's search 6% faster, the step 7% faster).
-template <bool kTiled> constexpr int knn_waves() { return kTiled ? 1 : 4; }
+// Waves per workgroup: one for the tiled (batch) search, 1, 2 or 4 (tuning
+// "knn_waves", default 4) for a single cloud.
+constexpr int kKnnWavesDefault = 4;
 constexpr int kRegBatches = 4;
 constexpr uint32_t kRegCand = kRegBatches * 64;
 // Per-wave LDS scratch: 64 compacted entries (16 B), 64 ranked entries, and a
@@ -520,13 +522,12 @@ __device__ __forceinline__ void finish_query(const KnnArgs& a, const WaveGrid& w
     }
 }
 
-template <bool kTiled>
+template <bool kTiled, int kWaves>
 // Four waves per SIMD: the selection is a chain of ballots and LDS
 // round trips, so more resident waves hide more of its latency (measured
 // faster than three waves even with the register spills four waves cost).
-__global__ __launch_bounds__(knn_waves<kTiled>() * kWave) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_knn(KnnArgs a_in)
+__global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_knn(KnnArgs a_in)
 {
-    constexpr int kWaves = knn_waves<kTiled>();
     extern __shared__ uint32_t s_lds[];
     const int lane = __lane_id();
     // Wave-uniform by construction; said so, so that per-wave tables load
@@ -1115,12 +1116,14 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
     a.ntasks = 0;
     a.ntiles = 0;
     const int max_k = tt ? tt->max_k : prog.k;
-    a.q_per_wave = tt ? tt->q_per_wave : knn_queries_per_wave(prog.k);
+    a.q_per_wave = tt ? tt->q_per_wave
+                      : (tune.q_per_wave > 0 ? tune.q_per_wave : knn_queries_per_wave(prog.k));
     a.lds_stride = max_k | 1;  // odd stride: lanes hit distinct LDS banks
     const GridParams& g = cv.g;
     a.margin = 1e-6 * fmin(g.c, g.cz) +
                1e-15 * (fabs(g.ox) + fabs(g.oy) + fabs(g.oz)) + 1e-300;
-    const size_t waves = static_cast<size_t>(tt ? knn_waves<true>() : knn_waves<false>());
+    const int wv = tune.waves == 1 || tune.waves == 2 ? tune.waves : kKnnWavesDefault;
+    const size_t waves = static_cast<size_t>(tt ? 1 : wv);
     const size_t list_words = mode != 0 ? 0 : waves * a.q_per_wave * a.lds_stride;
     const size_t lds = (list_words + waves * kScratchWords) * sizeof(uint32_t);
     if (tt) {
@@ -1130,12 +1133,17 @@ void launch_knn(const CloudView& cv, const FusedProgram& prog, int mode,
         a.ntasks = tt->ntasks;
         a.ntiles = tt->ntiles;
         const uint32_t blocks = div_up(tt->ntasks, waves);
-        hipLaunchKernelGGL(k_knn<true>, dim3(blocks), dim3(waves * kWave), lds, s, a);
+        hipLaunchKernelGGL((k_knn<true, 1>), dim3(blocks), dim3(waves * kWave), lds, s, a);
         return;
     }
     const uint64_t per_block = static_cast<uint64_t>(waves) * a.q_per_wave;
     const uint32_t blocks = div_up(cv.n, per_block);
-    hipLaunchKernelGGL(k_knn<false>, dim3(blocks), dim3(waves * kWave), lds, s, a);
+    if (wv == 1)
+        hipLaunchKernelGGL((k_knn<false, 1>), dim3(blocks), dim3(waves * kWave), lds, s, a);
+    else if (wv == 2)
+        hipLaunchKernelGGL((k_knn<false, 2>), dim3(blocks), dim3(waves * kWave), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_knn<false, 4>), dim3(blocks), dim3(waves * kWave), lds, s, a);
 }
 
 void launch_radius(const CloudView& cv, double radius, int mode, double* cov, uint32_t* counts,

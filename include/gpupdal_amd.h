@@ -141,17 +141,26 @@ int gf_ctx_destroy(gf_ctx *ctx);
 const char *gf_last_error(const gf_ctx *ctx);
 int gf_sync(gf_ctx *ctx);
 /* Tuning knobs: "cell_factor", "cell_z_factor", "ring0", "ring_max",
- * "small_col", "collect_stats".  None of them changes any row's result. */
+ * "small_col", "group_z", "q_per_wave", "knn_waves", "collect_stats" (and
+ * the exact tree's).  None of them changes any row's result. */
 int gf_set_tuning(gf_ctx *ctx, const char *key, double value);
 
 /* ---- autotuning: replaces the reference's grid-configuration heuristics
  *      (makeKnnGridConfig / makeAdaptiveKnnConfig, src/index/SpatialIndex.cpp
  *      :434-631).  gf_autotune times the whole run over a cloud for each
  *      candidate column-size factor (column edge = factor * sqrt(k / (pi *
- *      2-D density))) and keeps the fastest for the cloud's (density, k)
- *      bucket; later grids in that bucket, for clouds and batch tiles, use
- *      it.  No row's result depends on the factor.  The table is saved and
- *      loaded as text. */
+ *      2-D density))), keeps the fastest, then tries the search's launch
+ *      shape one knob at a time -- z cells a query group spans (0..4),
+ *      queries per wave (16, 32, 64), waves per workgroup (1, 2, 4) --
+ *      keeping each change that is faster, and stores the result for the
+ *      cloud's (density, k) bucket.  Later grids in that bucket (clouds and
+ *      batch tiles) use its factor; clouds use its shape, and a batch's one
+ *      search launch the shape of its largest tile's bucket.  No row's result
+ *      depends on any of them.  The table is saved and loaded as text (one
+ *      line per bucket: rho_bucket k_bucket factor group_z+1 q_per_wave
+ *      knn_waves, 0 = the context's setting; three-field lines are read as
+ *      factor only).  The same knobs can be set for a context with
+ *      gf_set_tuning ("group_z", "q_per_wave", "knn_waves"). */
 int gf_autotune(gf_ctx *ctx, gf_cloud *cloud, const gf_stage *stages, int nstages,
                 const double *factors, int nfactors, double *best_factor, double *best_ms);
 int gf_tuning_save(gf_ctx *ctx, const char *path);

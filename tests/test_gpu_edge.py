@@ -211,8 +211,29 @@ def test_autotune_changes_speed_not_results(ctx, tmp_path):
     path = tmp_path / "tune.txt"
     ctx.save_tuning(path)
     text = path.read_text()
-    assert len([l for l in text.splitlines() if not l.startswith("#")]) >= 1
+    rows = [l.split() for l in text.splitlines() if not l.startswith("#")]
+    assert len(rows) >= 1 and all(len(r) == 6 for r in rows)
     ctx.load_tuning(path)
+
+
+@pytest.mark.parametrize("shape", [dict(q_per_wave=16, knn_waves=1),
+                                   dict(q_per_wave=32, knn_waves=2, group_z=0),
+                                   dict(q_per_wave=64, knn_waves=4, group_z=4)])
+def test_search_shapes_change_no_result(ctx, shape):
+    # The autotuner's search-shape knobs: every row's status and columns
+    # are the default shape's.
+    x, y, z = small_cloud(60000, seed=71, dup_frac=0.02)
+    st = [g.stage(g.STAGE_NORMAL, 16), g.stage(g.STAGE_COVFEAT, 24, features=g.F_ALL10)]
+    ref = ctx.cloud(x, y, z).run(st, init=np.nan)
+    c2 = g.Context(0)
+    for k, v in shape.items():
+        c2.set_tuning(k, v)
+    got = c2.cloud(x, y, z).run(st, init=np.nan)
+    c2.close()
+    for sa, sb in zip(ref[1], got[1]):
+        assert np.array_equal(sa & ~np.uint8(INFO), sb & ~np.uint8(INFO))
+    for ca, cb in zip(ref[0][0] + ref[0][1], got[0][0] + got[0][1]):
+        np.testing.assert_allclose(cb, ca, rtol=1e-12, atol=1e-15, equal_nan=True)
 
 
 def test_tree_beside_the_search_repeats_exactly(ctx, oracle):
