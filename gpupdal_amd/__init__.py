@@ -66,7 +66,8 @@ class GfStats(ctypes.Structure):
                 ("grid_z", ctypes.c_uint32), ("key_bits", ctypes.c_uint32),
                 ("search_passes", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
                 ("merges", ctypes.c_uint64), ("ms_tree", ctypes.c_double),
-                ("ms_total", ctypes.c_double), ("deep_rows", ctypes.c_uint64)]
+                ("ms_total", ctypes.c_double), ("deep_rows", ctypes.c_uint64),
+                ("graph", ctypes.c_uint64)]
 
 
 class GfSynth(ctypes.Structure):
@@ -396,6 +397,11 @@ class Context:
         """Plain device copy of count doubles (HBM counter calibration)."""
         self._check(self.L.gf_debug_copy(self.h, dst, src, count), "gf_debug_copy")
 
+    def copy_to_host(self, arr, ptr):
+        """Fills the host array from device memory at ptr (arr.nbytes bytes)."""
+        self._check(self.L.gf_memcpy(self.h, arr.ctypes.data, ptr, arr.nbytes, 2), "gf_memcpy")
+        return arr
+
     def to_device(self, arr):
         """A new device copy of a host array (free it with free())."""
         arr = np.ascontiguousarray(arr)
@@ -633,6 +639,13 @@ class Batch:
 
     def invalidate(self):
         self.ctx.L.gf_batch_invalidate(self.h)
+
+    def set_tile(self, t, x, y, z):
+        """New coordinates for tile t (its point count is fixed)."""
+        x, y, z = _c64(x), _c64(y), _c64(z)
+        self.ctx._check(self.ctx.L.gf_batch_set_tile(self.h, t, x.ctypes.data, y.ctypes.data,
+                                                     z.ctypes.data, WHERE_HOST),
+                        "gf_batch_set_tile")
 
     def _knn_arg(self, tile_knn):
         if tile_knn is None:

@@ -28,10 +28,22 @@ double dec_order(unsigned long long u)
 
 } // namespace
 
+Batch::~Batch() { drop_graph(); }
+
+void Batch::drop_graph()
+{
+    if (gexec_a) (void)hipGraphExecDestroy(gexec_a);
+    if (gexec_b) (void)hipGraphExecDestroy(gexec_b);
+    gexec_a = gexec_b = nullptr;
+    gkey.clear();
+    seen_key.clear();
+}
+
 // New tile sizes: tile starts, segment blocks (each tile cut into blocks of
 // at most 4096 points) and grow-only point buffers.  The index is dropped.
 void Batch::reset(uint32_t ntiles, const uint64_t* counts, hipStream_t s)
 {
+    drop_graph();
     T = ntiles;
     starts.assign(static_cast<size_t>(ntiles) + 1, 0);
     for (uint32_t t = 0; t < ntiles; ++t) starts[t + 1] = starts[t] + counts[t];
@@ -203,6 +215,10 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
         any_exact_tile = any_exact_tile || ti.all_exact;
         max_k = std::max<int>(max_k, static_cast<int>(k));
     }
+    if (tune.batch_graph && tune.kd_overlap_min && N >= tune.kd_overlap_min && T > 0 &&
+        run_batch_graph(b, ns, tiles, fetch, base, ncol_total, max_k, any_exact_tile, cols,
+                        status, where))
+        return;
     points_mark = false;
     build_batch_index(b, fetch);
     // One search launch serves every tile: it takes the shape autotuned for
@@ -348,6 +364,293 @@ void Context::run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* til
     stats.search_passes = ctr[0];
     stats.candidates = ctr[1];
     stats.merges = ctr[2];
+}
+
+// The batch step with no host synchronisation before the exact tail: the
+// bounds, every tile's grid (grid_params on the device, the same rule and
+// factor table as the host's), keys, segmented sort, column starts, search
+// and the unflagged rows' features are captured once per shape into two
+// graphs and replayed.  Column tables are laid out by each tile's column
+// capacity instead of its column count, the sort always takes four 8-bit
+// passes, and the flagged rows' re-solve and features read their count on
+// the device, so no launch depends on a value the step computes.  The exact
+// forest is built on the side stream (its build reads back its own
+// counters) while the second graph runs.  Every row's result equals the
+// launch-by-launch path's.
+bool Context::run_batch_graph(Batch& b, int ns, std::vector<TileInfo>& tiles,
+                              const std::vector<uint32_t>& fetch, const FusedProgram& base,
+                              int ncol_total, int max_k, bool any_exact_tile,
+                              double* const* cols, uint8_t* const* status, int where)
+{
+    const uint32_t T = b.T;
+    const uint64_t N = b.n;
+    const uint32_t nb = static_cast<uint32_t>(b.blocks.size());
+    // Search shape: the context's, or the autotuned one of the bucket of the
+    // largest tile as its last bounds put it.
+    KnnTuning kt = knn_tuning();
+    uint32_t big = 0;
+    for (uint32_t t = 1; t < T; ++t)
+        if (b.starts[t + 1] - b.starts[t] > b.starts[big + 1] - b.starts[big]) big = t;
+    if (b.bounds.size() == static_cast<size_t>(T) * 6) {
+        const uint64_t nbig = b.starts[big + 1] - b.starts[big];
+        if (const TuneEntry* e = tune.entry_for(cloud_rho(&b.bounds[6 * big], nbig), fetch[big])) {
+            if (e->group_z > 0) kt.group_z = e->group_z - 1;
+            if (e->q_per_wave > 0) kt.q_per_wave = e->q_per_wave;
+        }
+    }
+    const int Q = kt.q_per_wave > 0 ? kt.q_per_wave : knn_queries_per_wave(max_k);
+    // Host-known tables: column bases by capacity, wave tasks.
+    std::vector<uint64_t> colbase(T);
+    uint64_t centries = 0;
+    for (uint32_t t = 0; t < T; ++t) {
+        colbase[t] = centries;
+        centries += grid_col_cap(b.starts[t + 1] - b.starts[t]);
+    }
+    if (centries >= (1ull << 32))
+        throw Error(GF_ERR_UNSUPPORTED, "batch column table above 2^32 entries");
+    std::vector<uint32_t> task0(static_cast<size_t>(T) + 1, 0u);
+    uint64_t ntasks = 0;
+    for (uint32_t t = 0; t < T; ++t) {
+        tiles[t].colbase = colbase[t];
+        task0[t] = static_cast<uint32_t>(ntasks);
+        if (!tiles[t].all_exact) ntasks += (tiles[t].n + Q - 1) / Q;
+    }
+    task0[T] = static_cast<uint32_t>(ntasks);
+    if (ntasks > 0xFFFFFF00ull)
+        throw Error(GF_ERR_UNSUPPORTED, "batch above the search's task range");
+    std::vector<TuneFactor> tf;
+    if (!(tune.forced_factor > 0))
+        for (const TuneEntry& e : tune.table) tf.push_back(TuneFactor{e.rho_bucket, e.k_bucket, e.cell_factor});
+    const double deff = tune.forced_factor > 0 ? tune.forced_factor : tune.cell_factor;
+
+    // Every buffer the graphs touch, sized before capture.
+    b.keys_a.ensure(N * 4 + 8);
+    b.vals_a.ensure(N * 4 + 8);
+    b.keys_b.ensure(N * 4 + 8);
+    b.vals_b.ensure(N * 4 + 8);
+    b.colstart.ensure(centries * 4 + 8);
+    b.sx.ensure(N * 8 + 8);
+    b.sy.ensure(N * 8 + 8);
+    b.sz.ensure(N * 8 + 8);
+    b.inv.ensure(N * 4 + 8);
+    b.hist.ensure(static_cast<uint64_t>(nb) * 256 * 4 + 8);
+    scan_tmp.ensure(scan_scratch_words(static_cast<uint64_t>(nb) * 256 + 1) * 4);
+    xyz_stage.ensure(N * 24 + 8);
+    b.benc.ensure(static_cast<uint64_t>(T) * 6 * 8 + 8);
+    b.dgrids.ensure(static_cast<uint64_t>(T) * sizeof(TileGrid) + 8);
+    b.dbounds.ensure(static_cast<uint64_t>(T) * 6 * 8 + 8);
+    b.dfetch.ensure(static_cast<uint64_t>(T) * 4 + 8);
+    b.dcolbase.ensure(static_cast<uint64_t>(T) * 8 + 8);
+    b.dtune.ensure(std::max<size_t>(tf.size(), 1) * sizeof(TuneFactor) + 8);
+    b.dtiles.ensure(static_cast<uint64_t>(T) * sizeof(TileInfo) + 8);
+    b.dtasks.ensure(task0.size() * sizeof(uint32_t) + 16);
+    KnnOutputs out{};
+    if (where == GF_WHERE_DEVICE) {
+        for (int c = 0; c < ncol_total; ++c) out.col[c] = cols[c];
+        for (int i = 0; i < ns; ++i) out.status[i] = status[i];
+    } else {
+        outbuf.ensure(static_cast<uint64_t>(ncol_total) * N * 8 + 8);
+        statbuf.ensure(static_cast<uint64_t>(ns) * N + 8);
+        for (int c = 0; c < ncol_total; ++c)
+            out.col[c] = outbuf.as<double>() + static_cast<uint64_t>(c) * N;
+        for (int i = 0; i < ns; ++i) out.status[i] = statbuf.as<uint8_t>() + i * N;
+    }
+    flagged.ensure(N * 4 + 4);
+    counters.ensure(128);
+    covbuf.ensure(N * 8 * kCovRecord + 8);
+    deep_rows.ensure(N * 4 + 4);
+    out.flagged = flagged.as<uint32_t>();
+    out.nflagged = counters.as<uint32_t>();
+    out.cov = covbuf.as<double>();
+    if (tune.collect_stats)
+        out.counters = reinterpret_cast<unsigned long long*>(counters.as<uint32_t>() + 8);
+
+    TileTables tt{};
+    tt.tiles = b.dtiles.as<TileInfo>();
+    tt.starts = b.dstarts.as<uint64_t>();
+    tt.ntiles = T;
+    tt.task0 = b.dtasks.as<uint32_t>();
+    tt.ntasks = static_cast<uint32_t>(ntasks);
+    tt.max_k = max_k;
+    tt.q_per_wave = Q;
+    FusedProgram prog0 = tiles[0].prog;
+
+    // What the capture depends on.
+    std::vector<uint64_t> key = {T, N, static_cast<uint64_t>(ns),
+                                 static_cast<uint64_t>(ncol_total),
+                                 static_cast<uint64_t>(where), static_cast<uint64_t>(Q),
+                                 static_cast<uint64_t>(kt.group_z), static_cast<uint64_t>(kt.ring0),
+                                 static_cast<uint64_t>(kt.ring_max),
+                                 static_cast<uint64_t>(kt.small_col), any_exact_tile ? 1u : 0u,
+                                 tune.collect_stats ? 1u : 0u, tf.size()};
+    auto put = [&](const void* p, size_t bytes) {
+        for (size_t o = 0; o < bytes; o += 8) {
+            uint64_t v = 0;
+            std::memcpy(&v, static_cast<const char*>(p) + o, std::min<size_t>(8, bytes - o));
+            key.push_back(v);
+        }
+    };
+    put(&deff, 8);
+    put(&tune.cell_z_factor, 8);
+    put(&base, sizeof(base));
+    put(fetch.data(), fetch.size() * 4);
+    put(tf.data(), tf.size() * sizeof(TuneFactor));
+    const void* ptrs[] = {b.x.ptr, b.y.ptr, b.z.ptr, b.dblocks.ptr, b.dstarts.ptr, b.benc.ptr,
+                          b.dgrids.ptr, b.dbounds.ptr, b.dfetch.ptr, b.dcolbase.ptr,
+                          b.dtune.ptr, b.dtiles.ptr, b.dtasks.ptr, b.keys_a.ptr, b.vals_a.ptr,
+                          b.keys_b.ptr, b.vals_b.ptr, b.colstart.ptr, b.sx.ptr, b.sy.ptr,
+                          b.sz.ptr, b.inv.ptr, b.hist.ptr, scan_tmp.ptr, xyz_stage.ptr,
+                          flagged.ptr, counters.ptr, covbuf.ptr};
+    put(ptrs, sizeof(ptrs));
+    put(out.col, sizeof(out.col));
+    put(out.status, sizeof(out.status));
+
+    stats.graph = 1;
+    if (key != b.gkey || !b.gexec_a || !b.gexec_b) {
+        // A shape is captured the second time it runs: one-off shapes (a
+        // serving loop's batches) run launch by launch, capture-free.
+        if (key != b.seen_key) {
+            b.seen_key = key;
+            stats.graph = 0;
+            return false;
+        }
+        b.drop_graph();
+        GF_HIP(hipMemcpyAsync(b.dfetch.ptr, fetch.data(), fetch.size() * 4,
+                              hipMemcpyHostToDevice, stream));
+        GF_HIP(hipMemcpyAsync(b.dcolbase.ptr, colbase.data(), colbase.size() * 8,
+                              hipMemcpyHostToDevice, stream));
+        if (!tf.empty())
+            GF_HIP(hipMemcpyAsync(b.dtune.ptr, tf.data(), tf.size() * sizeof(TuneFactor),
+                                  hipMemcpyHostToDevice, stream));
+        GF_HIP(hipMemcpyAsync(b.dtiles.ptr, tiles.data(),
+                              static_cast<uint64_t>(T) * sizeof(TileInfo), hipMemcpyHostToDevice,
+                              stream));
+        GF_HIP(hipMemcpyAsync(b.dtasks.ptr, task0.data(), task0.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, stream));
+        GF_HIP(hipStreamSynchronize(stream));
+        auto capture = [&](hipGraphExec_t* exec, auto&& body) {
+            hipGraph_t g = nullptr;
+            GF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            GF_HIP(hipStreamEndCapture(stream, &g));
+            const hipError_t e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            GF_HIP(e);
+        };
+        capture(&b.gexec_a, [&] {
+            launch_bounds_init(b.benc.as<unsigned long long>(), T, stream);
+            launch_tile_bounds(b.x.as<double>(), b.y.as<double>(), b.z.as<double>(),
+                               b.dblocks.as<SegBlock>(), nb, b.benc.as<unsigned long long>(),
+                               stream);
+            launch_tile_grids(b.benc.as<unsigned long long>(), b.dstarts.as<uint64_t>(),
+                              b.dfetch.as<uint32_t>(), b.dcolbase.as<uint64_t>(), T,
+                              b.dtune.as<TuneFactor>(), static_cast<int>(tf.size()), deff,
+                              tune.cell_z_factor, b.dgrids.as<TileGrid>(), b.dbounds.as<double>(),
+                              b.dtiles.as<TileInfo>(), stream);
+        });
+        capture(&b.gexec_b, [&] {
+            launch_fill_u32(out.nflagged, 4, 0u, stream);
+            if (out.counters) launch_fill_u32(counters.as<uint32_t>() + 8, 8, 0u, stream);
+            launch_tile_keys(b.x.as<double>(), b.y.as<double>(), b.z.as<double>(),
+                             b.dblocks.as<SegBlock>(), nb, b.dgrids.as<TileGrid>(),
+                             b.keys_a.as<uint32_t>(), b.vals_a.as<uint32_t>(),
+                             xyz_stage.as<double>(), stream);
+            uint32_t *kres, *vres;
+            launch_seg_radix_sort(b.keys_a.as<uint32_t>(), b.vals_a.as<uint32_t>(),
+                                  b.keys_b.as<uint32_t>(), b.vals_b.as<uint32_t>(),
+                                  b.dblocks.as<SegBlock>(), nb, 32, b.hist.as<uint32_t>(),
+                                  scan_tmp.as<uint32_t>(), &kres, &vres, stream);
+            launch_tile_colstart(kres, b.dgrids.as<TileGrid>(), T, centries,
+                                 b.colstart.as<uint32_t>(), stream);
+            launch_gather_sorted(vres, xyz_stage.as<double>(), N, b.sx.as<double>(),
+                                 b.sy.as<double>(), b.sz.as<double>(), b.inv.as<uint32_t>(),
+                                 stream);
+            b.keys = kres;
+            b.perm = vres;
+            CloudView cv{};
+            cv.sx = b.sx.as<double>();
+            cv.sy = b.sy.as<double>();
+            cv.sz = b.sz.as<double>();
+            cv.perm = b.perm;
+            cv.keys = b.keys;
+            cv.colstart = b.colstart.as<uint32_t>();
+            cv.n = N;
+            launch_knn(cv, prog0, 0, out, kt, stream, &tt);
+            if (any_exact_tile) launch_flag_tiles(N, cv.perm, 0, out, tt, stream);
+            launch_features(cv, prog0, out, b.inv.as<uint32_t>(), stream, &tt,
+                            kFeaturesUnflagged);
+        });
+        b.gkey = key;
+        stats.graph = 2;
+    }
+    b.idx_valid = false;  // the eager path's host grids do not describe this index
+
+    CloudView cv{};
+    cv.sx = b.sx.as<double>();
+    cv.sy = b.sy.as<double>();
+    cv.sz = b.sz.as<double>();
+    cv.perm = b.perm;
+    cv.keys = b.keys;
+    cv.colstart = b.colstart.as<uint32_t>();
+    cv.n = N;
+    reset_spans();
+    hipEvent_t t_all = mark();
+    if (where != GF_WHERE_DEVICE)  // rows a stage skips keep their prior values
+        for (int c = 0; c < ncol_total; ++c)
+            GF_HIP(hipMemcpyAsync(out.col[c], cols[c], N * 8, hipMemcpyHostToDevice, stream));
+    GF_HIP(hipGraphLaunch(b.gexec_a, stream));
+    const bool need_forest = !b.kd_valid;
+    if (need_forest) GF_HIP(hipEventRecord(join, stream));
+    GF_HIP(hipGraphLaunch(b.gexec_b, stream));
+    if (need_forest) {
+        // The forest reads the points and the tiles' bounds the first graph
+        // wrote; its build returns when it is done.
+        GF_HIP(hipStreamWaitEvent(side, join, 0));
+        build_forest(b.kd, b.x.as<double>(), b.y.as<double>(), b.z.as<double>(), N,
+                     b.dstarts.as<uint64_t>(), T, b.dbounds.as<double>(), side, true);
+        b.kd_valid = true;
+    }
+    uint32_t* failed = counters.as<uint32_t>() + 5;
+    launch_fill_u32(failed, 1, 0u, stream);
+    launch_kd_resolve(b.kd.view, cv, prog0, 0, out, b.x.as<double>(), b.y.as<double>(),
+                      b.z.as<double>(), static_cast<uint32_t>(std::min<uint64_t>(N, 0xFFFFFFFFu)),
+                      failed, deep_rows.as<uint32_t>(), stream, &tt);
+    launch_features_flagged(cv, prog0, out, b.inv.as<uint32_t>(), N, stream, &tt);
+    span(t_all, mark(), &stats.ms_total);
+    uint32_t h[16];
+    GF_HIP(hipMemcpyAsync(h, counters.ptr, sizeof(h), hipMemcpyDeviceToHost, stream));
+    unsigned long long ctr[4] = {0, 0, 0, 0};
+    if (out.counters)
+        GF_HIP(hipMemcpyAsync(ctr, out.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
+    GF_HIP(hipStreamSynchronize(stream));
+    if (h[5]) {
+        // Trees deeper than the register frames: those rows again with deeper
+        // frames, then every flagged row's features again.
+        resolve_deep(b.kd, T, cv, prog0, 0, out, b.x.as<double>(), b.y.as<double>(),
+                     b.z.as<double>(), h[5], failed, &tt);
+        launch_features_flagged(cv, prog0, out, b.inv.as<uint32_t>(), N, stream, &tt);
+    }
+    if (where == GF_WHERE_HOST) {
+        for (int c = 0; c < ncol_total; ++c)
+            GF_HIP(hipMemcpyAsync(cols[c], out.col[c], N * 8, hipMemcpyDeviceToHost, stream));
+        for (int i = 0; i < ns; ++i)
+            GF_HIP(hipMemcpyAsync(status[i], out.status[i], N, hipMemcpyDeviceToHost, stream));
+    }
+    GF_HIP(hipStreamSynchronize(stream));
+    collect_spans();
+    stats.tie_rows = h[1];
+    stats.incomplete_rows = h[2];
+    stats.points = N;
+    stats.search_passes = ctr[0];
+    stats.candidates = ctr[1];
+    stats.merges = ctr[2];
+    return true;
 }
 
 } // namespace gf

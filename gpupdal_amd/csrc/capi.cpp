@@ -20,14 +20,7 @@ using namespace gf;
 namespace gf {
 
 // Points per unit of the cloud's xy extent (the density rule's rho).
-double cloud_rho(const double b[6], uint64_t n)
-{
-    const double sx = b[3] - b[0], sy = b[4] - b[1];
-    const double big = std::max(std::max(sx, sy), std::max(b[5] - b[2], 0.0));
-    const double tiny = std::max(big * 1e-9, 1e-9);
-    return static_cast<double>(std::max<uint64_t>(n, 1)) /
-           (std::max(sx, tiny) * std::max(sy, tiny));
-}
+double cloud_rho(const double b[6], uint64_t n) { return grid_rho(b, n); }
 
 KnnTuning Context::knn_tuning(const Cloud& cl, uint32_t fetch) const
 {
@@ -44,64 +37,7 @@ KnnTuning Context::knn_tuning(const Cloud& cl, uint32_t fetch) const
 GridParams make_grid(const double b[6], uint64_t n, uint32_t fetch,
                      const Tuning& t, double cell)
 {
-    GridParams g{};
-    g.ox = b[0];
-    g.oy = b[1];
-    g.oz = b[2];
-    const double sx = b[3] - b[0], sy = b[4] - b[1], sz = b[5] - b[2];
-    const double big = std::max(std::max(sx, sy), std::max(sz, 0.0));
-    const double tiny = std::max(big * 1e-9, 1e-9);
-    const double ax = std::max(sx, tiny), ay = std::max(sy, tiny);
-    const double rho = static_cast<double>(std::max<uint64_t>(n, 1)) / (ax * ay);
-    double c = cell > 0.0 ? cell
-                          : t.factor_for(rho, fetch) *
-                                std::sqrt(static_cast<double>(std::max<uint32_t>(fetch, 1)) /
-                                          (3.141592653589793 * rho));
-    if (!(c > tiny)) c = tiny;
-    const uint64_t col_cap = std::max<uint64_t>(4 * n, 4096);
-    for (;;) {
-        g.c = c;
-        g.inv_c = 1.0 / c;
-        // Column counts beyond 2^30 per axis are refused before the cast (the
-        // column cap below refuses them anyway).
-        const double fx = std::floor(sx * g.inv_c) + 1.0, fy = std::floor(sy * g.inv_c) + 1.0;
-        if (!(fx <= 1073741824.0 && fy <= 1073741824.0)) {
-            c *= 2.0;
-            continue;
-        }
-        g.gx = static_cast<uint32_t>(fx);
-        g.gy = static_cast<uint32_t>(fy);
-        g.tiles_x = (g.gx + 7) / 8;
-        const uint64_t tiles_y = (g.gy + 7) / 8;
-        const uint64_t ncols = static_cast<uint64_t>(g.tiles_x) * tiles_y * 64;
-        if (ncols <= col_cap && ncols < (1ull << 30)) {
-            g.ncols = static_cast<uint32_t>(ncols);
-            break;
-        }
-        c *= 1.25;
-    }
-    const uint32_t bcol = bits_for(g.ncols - 1);
-    double cz = c * t.cell_z_factor;
-    g.cz = cz;
-    g.inv_cz = 1.0 / cz;
-    uint64_t gz = static_cast<uint64_t>(std::floor(sz * g.inv_cz)) + 1;
-    uint32_t bz = bits_for(gz - 1);
-    if (bcol + bz > 32) {
-        bz = 32 - bcol;
-        const uint64_t gzmax = 1ull << bz;
-        cz = sz / static_cast<double>(gzmax - 1) * (1.0 + 1e-9);
-        for (;;) {
-            g.cz = cz;
-            g.inv_cz = 1.0 / cz;
-            gz = static_cast<uint64_t>(std::floor(sz * g.inv_cz)) + 1;
-            if (gz <= gzmax) break;
-            cz *= 1.0 + 1e-6;
-        }
-    }
-    g.gz = static_cast<uint32_t>(gz);
-    g.bz = bz;
-    g.key_bits = bcol + bz;
-    return g;
+    return grid_params(b, n, fetch, t.factor_for(grid_rho(b, n), fetch), t.cell_z_factor, cell);
 }
 
 void Context::build_index(Cloud& cl, uint32_t fetch, double cell)

@@ -105,10 +105,103 @@ inline uint32_t div_up(uint64_t a, uint64_t b) {
 }
 
 // Count of bits needed to represent values in [0, v].
-inline uint32_t bits_for(uint64_t v) {
+__host__ __device__ inline uint32_t bits_for(uint64_t v) {
     uint32_t b = 0;
     while (v) { ++b; v >>= 1; }
     return b;
+}
+
+// Points per unit of xy extent of bounds b (min xyz, max xyz) holding n
+// points, each extent held at least at 1e-9 of the largest (and 1e-9).
+__host__ __device__ inline double grid_rho(const double b[6], uint64_t n) {
+    const double sx = b[3] - b[0], sy = b[4] - b[1], sz = b[5] - b[2];
+    const double big = fmax(fmax(sx, sy), fmax(sz, 0.0));
+    const double tiny = fmax(big * 1e-9, 1e-9);
+    return static_cast<double>(n > 1 ? n : 1) / (fmax(sx, tiny) * fmax(sy, tiny));
+}
+
+// Autotune buckets: floor(log2 rho) and ceil(log2 capacity), exactly (frexp
+// and integer bits, the same on host and device).
+__host__ __device__ inline int tune_rho_bucket(double rho) {
+    int e = 0;
+    frexp(rho > 1e-300 ? rho : 1e-300, &e);
+    return e - 1;
+}
+__host__ __device__ inline int tune_k_bucket(uint32_t fetch) {
+    return fetch <= 1 ? 0 : static_cast<int>(bits_for(static_cast<uint64_t>(fetch) - 1));
+}
+
+// The grid of a cloud (or tile) with bounds b and n points for result
+// capacity `fetch`: column edge factor * sqrt(fetch / (pi * rho)) (or
+// `cell` when > 0), grown until the padded column count fits 4n (and 2^30),
+// z cells of cell_z_factor * edge, the z cells' bits cut so that a key fits
+// 32 bits.  The same function on host and device, so both build the same
+// grid.
+__host__ __device__ inline GridParams grid_params(const double b[6], uint64_t n, uint32_t fetch,
+                                                  double factor, double cell_z_factor,
+                                                  double cell) {
+    GridParams g{};
+    g.ox = b[0];
+    g.oy = b[1];
+    g.oz = b[2];
+    const double sx = b[3] - b[0], sy = b[4] - b[1], sz = b[5] - b[2];
+    const double big = fmax(fmax(sx, sy), fmax(sz, 0.0));
+    const double tiny = fmax(big * 1e-9, 1e-9);
+    const double rho = grid_rho(b, n);
+    double c = cell > 0.0 ? cell
+                          : factor * sqrt(static_cast<double>(fetch > 1 ? fetch : 1) /
+                                          (3.141592653589793 * rho));
+    if (!(c > tiny)) c = tiny;
+    const uint64_t col_cap = 4 * n > 4096 ? 4 * n : 4096;
+    for (;;) {
+        g.c = c;
+        g.inv_c = 1.0 / c;
+        // Column counts beyond 2^30 per axis are refused before the cast (the
+        // column cap below refuses them anyway).
+        const double fx = floor(sx * g.inv_c) + 1.0, fy = floor(sy * g.inv_c) + 1.0;
+        if (!(fx <= 1073741824.0 && fy <= 1073741824.0)) {
+            c *= 2.0;
+            continue;
+        }
+        g.gx = static_cast<uint32_t>(fx);
+        g.gy = static_cast<uint32_t>(fy);
+        g.tiles_x = (g.gx + 7) / 8;
+        const uint64_t tiles_y = (g.gy + 7) / 8;
+        const uint64_t ncols = static_cast<uint64_t>(g.tiles_x) * tiles_y * 64;
+        if (ncols <= col_cap && ncols < (1ull << 30)) {
+            g.ncols = static_cast<uint32_t>(ncols);
+            break;
+        }
+        c *= 1.25;
+    }
+    const uint32_t bcol = bits_for(g.ncols - 1);
+    double cz = c * cell_z_factor;
+    g.cz = cz;
+    g.inv_cz = 1.0 / cz;
+    uint64_t gz = static_cast<uint64_t>(floor(sz * g.inv_cz)) + 1;
+    uint32_t bz = bits_for(gz - 1);
+    if (bcol + bz > 32) {
+        bz = 32 - bcol;
+        const uint64_t gzmax = 1ull << bz;
+        cz = sz / static_cast<double>(gzmax - 1) * (1.0 + 1e-9);
+        for (;;) {
+            g.cz = cz;
+            g.inv_cz = 1.0 / cz;
+            gz = static_cast<uint64_t>(floor(sz * g.inv_cz)) + 1;
+            if (gz <= gzmax) break;
+            cz *= 1.0 + 1e-6;
+        }
+    }
+    g.gz = static_cast<uint32_t>(gz);
+    g.bz = bz;
+    g.key_bits = bcol + bz;
+    return g;
+}
+
+// Column entries a grid of a cloud (or tile) of n points may hold: the
+// padded columns are at most max(4n, 4096); plus the end entry.
+__host__ __device__ inline uint64_t grid_col_cap(uint64_t n) {
+    return (4 * n > 4096 ? 4 * n : 4096) + 1;
 }
 
 // ---------------------------------------------------------------- device --

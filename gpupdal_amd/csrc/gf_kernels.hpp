@@ -53,6 +53,23 @@ struct TileGrid {
     uint64_t start, n;
     uint64_t colbase;
 };
+struct TileInfo;
+// Autotuned column factor of one (density, capacity) bucket.
+struct TuneFactor {
+    int32_t rb, kb;
+    double factor;
+};
+// Every tile's bounds encoding reset (min slots all ones, max slots zero).
+void launch_bounds_init(unsigned long long* benc, uint32_t T, hipStream_t s);
+// Every tile's grid from its encoded bounds on the device (grid_params with
+// the factor looked up in `table`, else default_factor): grids[t] (colbase
+// from `colbase`), bounds (6 per tile, decoded) and, if given, the grid,
+// column base and certification margin of infos[t].
+void launch_tile_grids(const unsigned long long* benc, const uint64_t* starts,
+                       const uint32_t* fetch, const uint64_t* colbase, uint32_t T,
+                       const TuneFactor* table, int ntable, double default_factor,
+                       double cell_z_factor, TileGrid* grids, double* bounds, TileInfo* infos,
+                       hipStream_t s);
 void launch_tile_bounds(const double* x, const double* y, const double* z,
                         const SegBlock* blocks, uint32_t nblocks, unsigned long long* benc,
                         hipStream_t s);
@@ -180,6 +197,11 @@ void launch_radius(const CloudView& cv, double radius, int mode, double* cov, ui
 // not flag (launched right after the search, beside the exact tree);
 // kFeaturesFlagged the nflagged rows of out.flagged, after their re-solve.
 constexpr int kFeaturesAll = 0, kFeaturesUnflagged = 1, kFeaturesFlagged = 2;
+// The flagged rows' features with their count read on the device (at most
+// max_rows; the launch shape depends only on max_rows).
+void launch_features_flagged(const CloudView& cv, const FusedProgram& prog,
+                             const KnnOutputs& out, const uint32_t* inv, uint64_t max_rows,
+                             hipStream_t s, const TileTables* tt = nullptr);
 void launch_features(const CloudView& cv, const FusedProgram& prog,
                      const KnnOutputs& out, const uint32_t* inv, hipStream_t s,
                      const TileTables* tt = nullptr, int rows = kFeaturesAll,

@@ -59,8 +59,6 @@ struct TuneEntry {
     int knn_waves = 0;
 };
 
-inline int tune_rho_bucket(double rho) { return static_cast<int>(std::floor(std::log2(rho > 1e-300 ? rho : 1e-300))); }
-inline int tune_k_bucket(uint32_t fetch) { return static_cast<int>(std::ceil(std::log2(static_cast<double>(fetch < 1 ? 1 : fetch)))); }
 
 struct Tuning {
     double cell_factor = 0.6;    // column edge = factor * sqrt(k / (pi * rho2d))
@@ -86,6 +84,9 @@ struct Tuning {
     // a side stream during the search, before knowing whether a row needs
     // it; smaller ones build it only when a row is flagged.  0: never early.
     uint64_t kd_overlap_min = 131072;
+    // Batches of at least kd_overlap_min points run their step as a
+    // captured, replayed graph (0: launch by launch).
+    int batch_graph = 1;
     bool collect_stats = false;
 
     void set(const char* key, double v)
@@ -111,6 +112,7 @@ struct Tuning {
         else if (!std::strcmp(key, "kd_overlap_min") && v >= 0)
             kd_overlap_min = static_cast<uint64_t>(v);
         else if (!std::strcmp(key, "collect_stats")) collect_stats = v != 0.0;
+        else if (!std::strcmp(key, "batch_graph")) batch_graph = v != 0.0 ? 1 : 0;
         else throw Error(GF_ERR_ARG, std::string("unknown or invalid tuning key: ") + key);
         if (ring_max < ring0) ring_max = ring0;
     }
@@ -220,7 +222,16 @@ struct Batch {
     DevBuf dtiles, dtasks;
     bool kd_valid = false;
     KdBuffers kd;
+    // Graph mode: the step from the bounds through the unflagged rows'
+    // features captured once per shape (two parts: bounds and grids, then
+    // the rest, so that the exact forest can start beside the second), and
+    // replayed; gkey holds what the capture depends on.
+    hipGraphExec_t gexec_a = nullptr, gexec_b = nullptr;
+    std::vector<uint64_t> gkey, seen_key;
+    DevBuf dfetch, dcolbase, dtune;
     explicit Batch(Context* c) : ctx(c) {}
+    ~Batch();
+    void drop_graph();
     void reset(uint32_t ntiles, const uint64_t* counts, hipStream_t s);
 };
 
@@ -374,6 +385,13 @@ struct Context {
     void build_batch_index(Batch& b, const std::vector<uint32_t>& fetch);
     void run_batch(Batch& b, const gf_stage* st, int ns, const int32_t* tile_knn,
                    double* const* cols, uint8_t* const* status, int where);
+    // The batch step as a replayed graph (large batches; tuning
+    // "batch_graph").  tiles/fetch: the host tables run_batch prepared.
+    // Returns false, having run nothing, the first time a shape is seen.
+    bool run_batch_graph(Batch& b, int ns, std::vector<TileInfo>& tiles,
+                         const std::vector<uint32_t>& fetch, const FusedProgram& base,
+                         int ncol_total, int max_k, bool any_exact_tile, double* const* cols,
+                         uint8_t* const* status, int where);
     // Timing: mark() records a pooled event on a stream (nullptr when timing
     // is off); span() registers an interval; collect_spans(), after the
     // call's final synchronisation, fills every registered interval.

@@ -643,6 +643,82 @@ void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s)
 }
 
 
+namespace {
+
+__global__ __launch_bounds__(256) void k_bounds_init(unsigned long long* __restrict__ benc,
+                                                     uint32_t T)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 6 * T) return;
+    benc[i] = (i % 6) < 3 ? ~0ull : 0ull;
+}
+
+// The inverse of ord_enc: order-encoded bits back to the double.
+__device__ __forceinline__ double ord_dec(unsigned long long u)
+{
+    const unsigned long long b = (u >> 63) ? (u ^ 0x8000000000000000ull) : ~u;
+    return __longlong_as_double(static_cast<long long>(b));
+}
+
+__global__ __launch_bounds__(64) void k_tile_grids(const unsigned long long* __restrict__ benc,
+                                                   const uint64_t* __restrict__ starts,
+                                                   const uint32_t* __restrict__ fetch,
+                                                   const uint64_t* __restrict__ colbase,
+                                                   uint32_t T, const TuneFactor* __restrict__ table,
+                                                   int ntable, double default_factor,
+                                                   double cell_z_factor,
+                                                   TileGrid* __restrict__ grids,
+                                                   double* __restrict__ bounds,
+                                                   TileInfo* __restrict__ infos)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const uint64_t nt = starts[t + 1] - starts[t];
+    double bd[6];
+    for (int i = 0; i < 6; ++i) bd[i] = nt ? ord_dec(benc[6ull * t + i]) : 0.0;
+    const int rb = tune_rho_bucket(grid_rho(bd, nt)), kb = tune_k_bucket(fetch[t]);
+    double f = default_factor;
+    for (int e = 0; e < ntable; ++e)
+        if (table[e].rb == rb && table[e].kb == kb) {
+            f = table[e].factor;
+            break;
+        }
+    const GridParams g = grid_params(bd, nt, fetch[t], f, cell_z_factor, 0.0);
+    TileGrid& tg = grids[t];
+    tg.g = g;
+    tg.start = starts[t];
+    tg.n = nt;
+    tg.colbase = colbase[t];
+    for (int i = 0; i < 6; ++i) bounds[6ull * t + i] = bd[i];
+    if (infos) {
+        TileInfo& ti = infos[t];
+        ti.g = g;
+        ti.colbase = colbase[t];
+        ti.margin = 1e-6 * fmin(g.c, g.cz) + 1e-15 * (fabs(g.ox) + fabs(g.oy) + fabs(g.oz)) +
+                    1e-300;
+    }
+}
+
+} // namespace
+
+void launch_bounds_init(unsigned long long* benc, uint32_t T, hipStream_t s)
+{
+    if (!T) return;
+    hipLaunchKernelGGL(k_bounds_init, dim3(div_up(6ull * T, 256)), dim3(256), 0, s, benc, T);
+}
+
+void launch_tile_grids(const unsigned long long* benc, const uint64_t* starts,
+                       const uint32_t* fetch, const uint64_t* colbase, uint32_t T,
+                       const TuneFactor* table, int ntable, double default_factor,
+                       double cell_z_factor, TileGrid* grids, double* bounds, TileInfo* infos,
+                       hipStream_t s)
+{
+    if (!T) return;
+    hipLaunchKernelGGL(k_tile_grids, dim3(div_up(T, 64)), dim3(64), 0, s, benc, starts, fetch,
+                       colbase, T, table, ntable, default_factor, cell_z_factor, grids, bounds,
+                       infos);
+}
+
 void launch_tile_bounds(const double* x, const double* y, const double* z,
                         const SegBlock* blocks, uint32_t nblocks, unsigned long long* benc,
                         hipStream_t s)

@@ -1033,16 +1033,14 @@ __device__ __forceinline__ uint32_t tile_of(const uint64_t* __restrict__ starts,
 // row whose record carries a search flag is skipped), or the i-th row of
 // out.flagged by sorted position (kFeaturesFlagged).
 
-__global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov,
-                                                  const uint32_t* __restrict__ inv,
-                                                  const uint32_t* __restrict__ perm,
-                                                  uint64_t n, int rows, FusedProgram prog,
-                                                  KnnOutputs out, const TileInfo* __restrict__ tiles,
-                                                  const uint64_t* __restrict__ starts,
-                                                  uint32_t ntiles)
+__device__ __forceinline__ void features_row(uint64_t i, const double* __restrict__ cov,
+                                             const uint32_t* __restrict__ inv,
+                                             const uint32_t* __restrict__ perm, int rows,
+                                             const FusedProgram& prog, const KnnOutputs& out,
+                                             const TileInfo* __restrict__ tiles,
+                                             const uint64_t* __restrict__ starts,
+                                             uint32_t ntiles)
 {
-    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
     uint64_t qid = i, s;
     if (rows == kFeaturesFlagged) {
         s = out.flagged[i];
@@ -1081,6 +1079,35 @@ __global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov
             for (int cc = 0; cc < sp.ncols; ++cc) out.col[sp.col_offset + cc][qid] = vals[cc];
         out.status[si][qid] = st | (f & kReportedSearchFlags);
     }
+}
+
+__global__ __launch_bounds__(256) void k_features(const double* __restrict__ cov,
+                                                  const uint32_t* __restrict__ inv,
+                                                  const uint32_t* __restrict__ perm,
+                                                  uint64_t n, int rows, FusedProgram prog,
+                                                  KnnOutputs out, const TileInfo* __restrict__ tiles,
+                                                  const uint64_t* __restrict__ starts,
+                                                  uint32_t ntiles)
+{
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    features_row(i, cov, inv, perm, rows, prog, out, tiles, starts, ntiles);
+}
+
+// The flagged rows, their count read on the device (a launch whose shape
+// does not depend on it, as a captured graph needs).
+__global__ __launch_bounds__(256) void k_features_flagged(const double* __restrict__ cov,
+                                                          const uint32_t* __restrict__ inv,
+                                                          const uint32_t* __restrict__ perm,
+                                                          FusedProgram prog, KnnOutputs out,
+                                                          const TileInfo* __restrict__ tiles,
+                                                          const uint64_t* __restrict__ starts,
+                                                          uint32_t ntiles)
+{
+    const uint64_t n = *out.nflagged;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        features_row(i, cov, inv, perm, kFeaturesFlagged, prog, out, tiles, starts, ntiles);
 }
 
 __global__ __launch_bounds__(256) void k_eigen_batch(const double* __restrict__ cov,
@@ -1178,6 +1205,17 @@ void launch_features(const CloudView& cv, const FusedProgram& prog,
     if (n == 0) return;
     hipLaunchKernelGGL(k_features, dim3(div_up(n, 256)), dim3(256), 0, s, out.cov, inv, cv.perm,
                        n, rows, prog, out, tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
+                       tt ? tt->ntiles : 0u);
+}
+
+void launch_features_flagged(const CloudView& cv, const FusedProgram& prog,
+                             const KnnOutputs& out, const uint32_t* inv, uint64_t max_rows,
+                             hipStream_t s, const TileTables* tt)
+{
+    if (!max_rows) return;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(div_up(max_rows, 256), 2048));
+    hipLaunchKernelGGL(k_features_flagged, dim3(blocks), dim3(256), 0, s, out.cov, inv, cv.perm,
+                       prog, out, tt ? tt->tiles : nullptr, tt ? tt->starts : nullptr,
                        tt ? tt->ntiles : 0u);
 }
 

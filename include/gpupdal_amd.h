@@ -132,6 +132,9 @@ typedef struct gf_stats {
     uint64_t deep_rows;       /* flagged rows whose exact search needed more than 64
                                  recursion frames (a tree deeper than 64 levels) and
                                  ran again with frames in device memory */
+    uint64_t graph;           /* batches: 0 launched kernel by kernel, 1 the step was a
+                                 replay of a captured graph, 2 captured in this call
+                                 and replayed */
 } gf_stats;
 
 /* ---- context: replaces pdg::CudaMemoryResource + stream ownership
