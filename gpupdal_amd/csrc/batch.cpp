@@ -441,6 +441,7 @@ bool Context::run_batch_graph(Batch& b, int ns, std::vector<TileInfo>& tiles,
     b.dbounds.ensure(static_cast<uint64_t>(T) * 6 * 8 + 8);
     b.dfetch.ensure(static_cast<uint64_t>(T) * 4 + 8);
     b.dcolbase.ensure(static_cast<uint64_t>(T) * 8 + 8);
+    b.dcolpre.ensure(static_cast<uint64_t>(T) * 8 + 16);
     b.dtune.ensure(std::max<size_t>(tf.size(), 1) * sizeof(TuneFactor) + 8);
     b.dtiles.ensure(static_cast<uint64_t>(T) * sizeof(TileInfo) + 8);
     b.dtasks.ensure(task0.size() * sizeof(uint32_t) + 16);
@@ -495,8 +496,9 @@ bool Context::run_batch_graph(Batch& b, int ns, std::vector<TileInfo>& tiles,
     put(&base, sizeof(base));
     put(fetch.data(), fetch.size() * 4);
     put(tf.data(), tf.size() * sizeof(TuneFactor));
+    const std::vector<uint64_t> shape = key;  // without the buffers' addresses
     const void* ptrs[] = {b.x.ptr, b.y.ptr, b.z.ptr, b.dblocks.ptr, b.dstarts.ptr, b.benc.ptr,
-                          b.dgrids.ptr, b.dbounds.ptr, b.dfetch.ptr, b.dcolbase.ptr,
+                          b.dgrids.ptr, b.dbounds.ptr, b.dfetch.ptr, b.dcolbase.ptr, b.dcolpre.ptr,
                           b.dtune.ptr, b.dtiles.ptr, b.dtasks.ptr, b.keys_a.ptr, b.vals_a.ptr,
                           b.keys_b.ptr, b.vals_b.ptr, b.colstart.ptr, b.sx.ptr, b.sy.ptr,
                           b.sz.ptr, b.inv.ptr, b.hist.ptr, scan_tmp.ptr, xyz_stage.ptr,
@@ -507,14 +509,16 @@ bool Context::run_batch_graph(Batch& b, int ns, std::vector<TileInfo>& tiles,
 
     stats.graph = 1;
     if (key != b.gkey || !b.gexec_a || !b.gexec_b) {
-        // A shape is captured the second time it runs: one-off shapes (a
-        // serving loop's batches) run launch by launch, capture-free.
-        if (key != b.seen_key) {
-            b.seen_key = key;
+        // A shape is captured the second time it runs (and again when a
+        // buffer it uses has moved): one-off shapes (a serving loop's
+        // batches) run launch by launch, capture-free.
+        if (shape != b.seen_key) {
+            b.seen_key = shape;
             stats.graph = 0;
             return false;
         }
         b.drop_graph();
+        b.seen_key = shape;
         GF_HIP(hipMemcpyAsync(b.dfetch.ptr, fetch.data(), fetch.size() * 4,
                               hipMemcpyHostToDevice, stream));
         GF_HIP(hipMemcpyAsync(b.dcolbase.ptr, colbase.data(), colbase.size() * 8,
@@ -566,8 +570,9 @@ bool Context::run_batch_graph(Batch& b, int ns, std::vector<TileInfo>& tiles,
                                   b.keys_b.as<uint32_t>(), b.vals_b.as<uint32_t>(),
                                   b.dblocks.as<SegBlock>(), nb, 32, b.hist.as<uint32_t>(),
                                   scan_tmp.as<uint32_t>(), &kres, &vres, stream);
-            launch_tile_colstart(kres, b.dgrids.as<TileGrid>(), T, centries,
-                                 b.colstart.as<uint32_t>(), stream);
+            launch_tile_colstart_real(kres, b.dgrids.as<TileGrid>(), T,
+                                      b.dcolpre.as<uint64_t>(), b.colstart.as<uint32_t>(),
+                                      centries, stream);
             launch_gather_sorted(vres, xyz_stage.as<double>(), N, b.sx.as<double>(),
                                  b.sy.as<double>(), b.sz.as<double>(), b.inv.as<uint32_t>(),
                                  stream);

@@ -59,6 +59,13 @@ struct TuneFactor {
     int32_t rb, kb;
     double factor;
 };
+// Column starts of the tiles' real columns when the tiles' column tables
+// are laid out by capacity (grids[t].colbase): pre (T + 1 entries) receives
+// the prefix of the real entries; the launch shape depends only on
+// max_entries.
+void launch_tile_colstart_real(const uint32_t* keys, const TileGrid* grids, uint32_t T,
+                               uint64_t* pre, uint32_t* colstart, uint64_t max_entries,
+                               hipStream_t s);
 // Every tile's bounds encoding reset (min slots all ones, max slots zero).
 void launch_bounds_init(unsigned long long* benc, uint32_t T, hipStream_t s);
 // Every tile's grid from its encoded bounds on the device (grid_params with

@@ -228,7 +228,7 @@ struct Batch {
     // replayed; gkey holds what the capture depends on.
     hipGraphExec_t gexec_a = nullptr, gexec_b = nullptr;
     std::vector<uint64_t> gkey, seen_key;
-    DevBuf dfetch, dcolbase, dtune;
+    DevBuf dfetch, dcolbase, dcolpre, dtune;
     explicit Batch(Context* c) : ctx(c) {}
     ~Batch();
     void drop_graph();

@@ -11,6 +11,8 @@
 #include "gf_internal.hpp"
 #include "gf_kernels.hpp"
 
+#include <algorithm>
+
 namespace gf {
 
 // ------------------------------------------------------------- bounds ----
@@ -699,7 +701,88 @@ __global__ __launch_bounds__(64) void k_tile_grids(const unsigned long long* __r
     }
 }
 
+// Exclusive prefix of the tiles' real column entries (ncols + 1 each) from
+// their device grids: one workgroup, T small.
+__global__ __launch_bounds__(256) void k_col_prefix(const TileGrid* __restrict__ grids,
+                                                    uint32_t T, uint64_t* __restrict__ pre)
+{
+    __shared__ uint64_t part[256];
+    const uint32_t per = (T + 255) / 256;
+    const uint32_t lo = threadIdx.x * per, hi = min(T, lo + per);
+    uint64_t sum = 0;
+    for (uint32_t t = lo; t < hi; ++t) sum += static_cast<uint64_t>(grids[t].g.ncols) + 1;
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t run = 0;
+        for (int i = 0; i < 256; ++i) {
+            const uint64_t v = part[i];
+            part[i] = run;
+            run += v;
+        }
+        pre[T] = run;
+    }
+    __syncthreads();
+    uint64_t run = part[threadIdx.x];
+    for (uint32_t t = lo; t < hi; ++t) {
+        pre[t] = run;
+        run += static_cast<uint64_t>(grids[t].g.ncols) + 1;
+    }
+}
+
+// Column starts of every tile's real columns (tiles laid out by capacity
+// at grids[t].colbase), a fixed grid striding over the real entries:
+// entry e of the real-entry order is column e - pre[t] of tile t.
+__global__ __launch_bounds__(256) void k_tile_colstart_real(const uint32_t* __restrict__ keys,
+                                                            const TileGrid* __restrict__ grids,
+                                                            const uint64_t* __restrict__ pre,
+                                                            uint32_t T,
+                                                            uint32_t* __restrict__ colstart)
+{
+    const uint64_t total = pre[T];
+    for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        uint32_t lo_t = 0, hi_t = T;  // largest t with pre[t] <= e
+        while (hi_t - lo_t > 1) {
+            const uint32_t mid = (lo_t + hi_t) >> 1;
+            if (pre[mid] <= e)
+                lo_t = mid;
+            else
+                hi_t = mid;
+        }
+        const TileGrid& tg = grids[lo_t];
+        const uint64_t c = e - pre[lo_t];
+        uint32_t v;
+        if (c >= tg.g.ncols) {
+            v = static_cast<uint32_t>(tg.start + tg.n);
+        } else {
+            const uint64_t key = c << tg.g.bz;
+            uint64_t lo = tg.start, hi = tg.start + tg.n;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (static_cast<uint64_t>(keys[mid]) < key)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            v = static_cast<uint32_t>(lo);
+        }
+        colstart[tg.colbase + c] = v;
+    }
+}
+
 } // namespace
+
+void launch_tile_colstart_real(const uint32_t* keys, const TileGrid* grids, uint32_t T,
+                               uint64_t* pre, uint32_t* colstart, uint64_t max_entries,
+                               hipStream_t s)
+{
+    if (!T) return;
+    hipLaunchKernelGGL(k_col_prefix, dim3(1), dim3(256), 0, s, grids, T, pre);
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(div_up(max_entries, 256), 8192));
+    hipLaunchKernelGGL(k_tile_colstart_real, dim3(blocks), dim3(256), 0, s, keys, grids, pre, T,
+                       colstart);
+}
 
 void launch_bounds_init(unsigned long long* benc, uint32_t T, hipStream_t s)
 {
