@@ -403,23 +403,39 @@ def serving_leg(args):
     th = threading.Thread(target=producer)
     th.start()
     th.join()
-    lat, first, last, batches = [], None, None, set()
-    for tk in tickets:
-        _, _, (ts, tb, td, bid, bt) = srv.result(tk)
-        lat.append(td - ts)
-        first = ts if first is None else min(first, ts)
-        last = td if last is None else max(last, td)
-        batches.add(bid)
+
+    def collect(tks):
+        lat, queue, service, first, last, batches = [], [], [], None, None, set()
+        for tk in tks:
+            _, _, (ts, tb, td, bid, bt) = srv.result(tk)
+            lat.append(td - ts)
+            queue.append(tb - ts)
+            service.append(td - tb)
+            first = ts if first is None else min(first, ts)
+            last = td if last is None else max(last, td)
+            batches.add(bid)
+        return (np.array(lat) * 1e3, np.array(queue) * 1e3, np.array(service) * 1e3,
+                last - first, len(batches))
+
+    lat_ms, q_ms, s_ms, span_s, nbat = collect(tickets)
+    # Capacity: the same requests submitted all at once (the queue never
+    # runs dry), the sustained rate from the first submit to the last result.
+    sat = [srv.submit(*tiles[i], stages, knns[i]) for i in range(len(tiles))]
+    _, _, _, sat_s, sat_bat = collect(sat)
     srv.close()
     pts = sum(len(t[0]) for t in tiles)
-    lat_ms = np.array(lat) * 1e3
+    pct = lambda a, q: float(np.percentile(a, q))
     return {"requests": len(tiles), "rate_per_s": args.rate, "points": int(pts),
             "workload": "Poisson arrivals, sizes log-uniform [5k, 4M], knn in {10, 16}, "
                         "normal + covariancefeatures (Dimensionality), 2 workers, "
                         "200 GB HBM budget, host arrays in and out",
-            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
-            "max_ms": float(lat_ms.max()),
-            "sustained_points_per_s": pts / (last - first), "batches": len(batches)}
+            "p50_ms": pct(lat_ms, 50), "p99_ms": pct(lat_ms, 99), "max_ms": float(lat_ms.max()),
+            "queue_ms": {"p50": pct(q_ms, 50), "p99": pct(q_ms, 99)},
+            "batch_ms": {"p50": pct(s_ms, 50), "p99": pct(s_ms, 99),
+                         "what": "batch start to results on the host: uploads, device, downloads"},
+            "sustained_points_per_s": pts / span_s, "batches": nbat,
+            "offered_points_per_s": pts / float(arrivals[-1]),
+            "capacity_points_per_s": pts / sat_s, "capacity_batches": sat_bat}
 
 
 def precision_leg(args, sample_rows=0):
@@ -564,6 +580,7 @@ def main():
     ap.add_argument("--requests", type=int, default=64, help="requests of the serving leg")
     ap.add_argument("--rate", type=float, default=120.0, help="serving arrivals per second")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone")
     ap.add_argument("--no-las", action="store_true",
                     help="skip the LAS decode and optimal-neighborhood legs")
     ap.add_argument("--las-steps", type=int, default=7)
@@ -593,6 +610,9 @@ def main():
         return
     if args.only_batched:
         print(json.dumps({"batched": batched_leg(args)}))
+        return
+    if args.only_serving:
+        print(json.dumps({"serving": serving_leg(args)}))
         return
     gpu, xyz = gpu_leg(args)
     batched = None if args.no_batched else batched_leg(args)

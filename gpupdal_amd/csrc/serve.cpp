@@ -41,6 +41,39 @@ uint64_t request_bytes(uint64_t n, int ncols)
     return n * (256ull + 8ull * static_cast<uint64_t>(ncols)) + (1ull << 20);
 }
 
+// Page-locked host memory (allocated as such, grow-only), the staging
+// area of a worker's result downloads.
+struct PinnedBuf {
+    void* ptr = nullptr;
+    uint64_t size = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf()
+    {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+    // Page-locking is slow (hundreds of ms for a few GB), so the buffer grows
+    // at least twofold.
+    void ensure(uint64_t bytes)
+    {
+        if (bytes <= size && ptr) return;
+        const uint64_t want = std::max<uint64_t>(bytes, 2 * size);
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        size = 0;
+        GF_HIP(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+        size = want;
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Bytes of one column-piece staging slot (two per worker).
+constexpr uint64_t kStageSlotBytes = 32ull << 20;
+
+constexpr uint8_t kSkipped =
+    GF_ST_ZERO_COV | GF_ST_EIGEN_FAIL | GF_ST_FEATURE_INVALID | GF_ST_FEW_NEIGHBORS;
+
 struct Request {
     uint64_t id = 0;
     const double *x = nullptr, *y = nullptr, *z = nullptr;
@@ -99,9 +132,25 @@ struct Server {
 void Server::worker_main(int)
 {
     std::unique_ptr<Context> ctx;
+    // Page-locked staging: two fixed slots for the column pieces, and the
+    // statuses (3 bytes a point for three stages: grown twofold when needed).
+    PinnedBuf slots, hstat_buf;
+    hipEvent_t slot_ev[2] = {nullptr, nullptr};
+    struct EventGuard {
+        hipEvent_t* ev;
+        ~EventGuard()
+        {
+            for (int i = 0; i < 2; ++i)
+                if (ev[i]) (void)hipEventDestroy(ev[i]);
+        }
+    } event_guard{slot_ev};
     try {
         ctx.reset(new Context(device));
         ctx->timing = false;
+        slots.ensure(2 * kStageSlotBytes);
+        hstat_buf.ensure(16ull << 20);
+        GF_HIP(hipEventCreateWithFlags(&slot_ev[0], hipEventDisableTiming));
+        GF_HIP(hipEventCreateWithFlags(&slot_ev[1], hipEventDisableTiming));
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu);
         create_error = e.what();
@@ -197,28 +246,67 @@ void Server::worker_main(int)
                                       hipMemcpyHostToDevice, ctx->stream));
                 GF_HIP(hipMemcpyAsync(batch.z.as<double>() + o, r->z, r->n * 8,
                                       hipMemcpyHostToDevice, ctx->stream));
-                // Rows a stage skips keep the caller's prior values.
-                for (int c = 0; c < ncols; ++c)
-                    GF_HIP(hipMemcpyAsync(cp[c] + o, r->cols[c], r->n * 8, hipMemcpyHostToDevice,
-                                          ctx->stream));
             }
             if (N) {
                 ctx->stats = gf_stats{};
                 ctx->run_batch(batch, take[0]->st, ns, knn.data(), cp.data(), sp.data(),
                                GF_WHERE_DEVICE);
             }
+            // Results come back through page-locked staging: the statuses
+            // first, then the columns in pieces through two fixed slots, each
+            // piece's DMA running while the host copies the previous piece
+            // out.  Rows a stage skipped keep the caller's prior values (put
+            // back from a copy taken just before the piece lands), so those
+            // never travel to the device.
+            if (N) {
+                hstat_buf.ensure(static_cast<uint64_t>(ns) * N);
+                GF_HIP(hipMemcpyAsync(hstat_buf.ptr, dstat.ptr, static_cast<uint64_t>(ns) * N,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+                GF_HIP(hipStreamSynchronize(ctx->stream));
+            }
+            const uint8_t* hstat = hstat_buf.as<uint8_t>();
+            struct Piece {
+                double* dst;
+                const double* src;
+                uint64_t count;
+                const uint8_t* status;
+            };
+            std::vector<Piece> pieces;
+            const uint64_t slot_elems = kStageSlotBytes / 8;
             for (uint32_t t = 0; t < T; ++t) {
                 Request* r = take[t];
                 const uint64_t o = batch.starts[t];
-                if (!r->n) continue;
-                for (int c = 0; c < ncols; ++c)
-                    GF_HIP(hipMemcpyAsync(r->cols[c], cp[c] + o, r->n * 8, hipMemcpyDeviceToHost,
-                                          ctx->stream));
-                for (int i = 0; i < ns; ++i)
-                    GF_HIP(hipMemcpyAsync(r->status[i], sp[i] + o, r->n, hipMemcpyDeviceToHost,
-                                          ctx->stream));
+                int c = 0;
+                for (int i = 0; i < ns; ++i) {
+                    const uint8_t* srow = hstat + static_cast<uint64_t>(i) * N + o;
+                    const int nc = stage_ncols(r->st[i]);
+                    for (int cc = 0; cc < nc; ++cc, ++c)
+                        for (uint64_t j = 0; j < r->n; j += slot_elems)
+                            pieces.push_back({r->cols[c] + j, cp[c] + o + j,
+                                              std::min<uint64_t>(slot_elems, r->n - j), srow + j});
+                    if (r->n) std::memcpy(r->status[i], srow, r->n);
+                }
             }
-            GF_HIP(hipStreamSynchronize(ctx->stream));
+            std::vector<std::pair<uint64_t, double>> keep;
+            auto copy_out = [&](const Piece& pc, const double* staged) {
+                keep.clear();
+                for (uint64_t j = 0; j < pc.count; ++j)
+                    if (pc.status[j] & kSkipped) keep.emplace_back(j, pc.dst[j]);
+                std::memcpy(pc.dst, staged, pc.count * 8);
+                for (const auto& kv : keep) pc.dst[kv.first] = kv.second;
+            };
+            for (size_t q = 0; q <= pieces.size(); ++q) {
+                if (q < pieces.size()) {
+                    GF_HIP(hipMemcpyAsync(slots.as<double>() + (q & 1) * slot_elems, pieces[q].src,
+                                          pieces[q].count * 8, hipMemcpyDeviceToHost,
+                                          ctx->stream));
+                    GF_HIP(hipEventRecord(slot_ev[q & 1], ctx->stream));
+                }
+                if (q > 0) {
+                    GF_HIP(hipEventSynchronize(slot_ev[(q - 1) & 1]));
+                    copy_out(pieces[q - 1], slots.as<double>() + ((q - 1) & 1) * slot_elems);
+                }
+            }
         } catch (const Error& e) {
             rc = e.code;
             err = e.what();
