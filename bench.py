@@ -39,9 +39,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic bytes per point of the search kernel (k_knn): the query's FP64
-# xyz (24 B) and grid key (4 B) read once, its covariance record written (six
-# FP64 entries and the row-flags word, 56 B).
-SEARCH_BYTES_PER_POINT = 84.0
+# xyz (24 B) and grid key (4 B) read once, its covariance record written (one
+# 64-byte record: six FP64 entries, the row-flags word and one pad word).
+SEARCH_BYTES_PER_POINT = 92.0
 # HBM traffic of one k_knn launch on the bench workload (4,000,000 points,
 # knn=16), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this
 # script (KiB per dispatch), each scaled by the same counter on a 1 GiB copy

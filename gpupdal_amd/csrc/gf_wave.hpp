@@ -37,6 +37,42 @@ __device__ __forceinline__ Ent ent_shfl_xor(const Ent& e, int m)
     return r;
 }
 
+// Lane src's value, as a wave-uniform (scalar) value; src must be
+// wave-uniform.  Unlike a shuffle, the compiler then knows every lane holds
+// the same value, so branches on it stay scalar.
+// (The builtins return int: each word goes through uint32_t, so that the low
+// word is not sign-extended into the high one.)
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, int src)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), src));
+}
+
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int src)
+{
+    const uint32_t lo = lane_u32(static_cast<uint32_t>(v), src);
+    const uint32_t hi = lane_u32(static_cast<uint32_t>(v >> 32), src);
+    return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+}
+
+__device__ __forceinline__ Ent ent_lane(const Ent& e, int src)
+{
+    return Ent{lane_u64(e.d, src), lane_u32(e.id, src), lane_u32(e.j, src)};
+}
+
+// A value every lane already holds, as a wave-uniform (scalar) value.
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ double uniform_f64(double v)
+{
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(b))));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(b >> 32))));
+    return __longlong_as_double(
+        static_cast<long long>(static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32)));
+}
+
 __device__ __forceinline__ Ent ent_shfl(const Ent& e, int src)
 {
     Ent r;

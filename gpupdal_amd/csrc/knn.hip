@@ -212,6 +212,18 @@ __device__ __forceinline__ Ent make_ent(double qx, double qy, double qz, double 
     return Ent{static_cast<uint64_t>(__double_as_longlong(d2)), id, j};
 }
 
+// 1 when the compacted entry e = (d2 bits low, high, id, payload) precedes
+// (d, id) in (d2, id) order.  Evaluated without branches, so that unrolled
+// iterations issue their LDS loads together (short-circuit forms compile to
+// one conditional load, and one wait, per entry).  Non-negative doubles order
+// as their bit patterns.
+__device__ __forceinline__ uint32_t rank_step(const uint4 e, uint64_t d, uint32_t id)
+{
+    const uint64_t od = static_cast<uint64_t>(e.x) | (static_cast<uint64_t>(e.y) << 32);
+    return static_cast<uint32_t>(od < d) |
+           (static_cast<uint32_t>(od == d) & static_cast<uint32_t>(e.z < id));
+}
+
 // Selection over at most 256 candidate entries held in registers (4 per
 // lane).  First a threshold E with need <= #{d2 <= E} <= cap (need = fetch +
 // 1, so the (fetch+1)-th entry that decides ties is kept), each trial counted
@@ -319,14 +331,9 @@ __device__ __forceinline__ void select_entries(const uint64_t (&cd)[kRegBatches]
             v.id = e[2];
             v.j = e[3];
         }
-        const double vd = ent_d2(v);
 #pragma unroll 4
-        for (uint32_t i = 0; i < S; ++i) {
-            const uint4 e = reinterpret_cast<const uint4*>(scratch)[i];
-            const double od = __longlong_as_double(
-                static_cast<long long>(static_cast<uint64_t>(e.x) | (static_cast<uint64_t>(e.y) << 32)));
-            rank += (od < vd || (od == vd && e.z < v.id)) ? 1u : 0u;
-        }
+        for (uint32_t i = 0; i < S; ++i)
+            rank += rank_step(reinterpret_cast<const uint4*>(scratch)[i], v.d, v.id);
         uint32_t* sorted = scratch + 256;
         if (static_cast<uint32_t>(lane) < S) {
             uint32_t* e = sorted + 4 * rank;
@@ -357,6 +364,201 @@ __device__ __forceinline__ void select_entries(const uint64_t (&cd)[kRegBatches]
         merge_sorted(best, c, lane);
         cnt[2] += 1;
     }
+}
+
+// Candidates whose d2 bits are <= t (below t when kStrict); kInfD entries
+// never count.  Non-negative doubles order as their bit patterns.
+template <bool kStrict>
+__device__ __forceinline__ uint32_t count_bits(const uint64_t (&cd)[kRegBatches], uint32_t nbat,
+                                               uint64_t t)
+{
+    uint32_t c = 0;
+#pragma unroll
+    for (int b = 0; b < kRegBatches; ++b)
+        if (static_cast<uint32_t>(b) < nbat)
+            c += __popcll(__ballot(kStrict ? cd[b] < t : cd[b] <= t));
+    return c;
+}
+
+__device__ __forceinline__ uint64_t d2_bits(double d)
+{
+    return static_cast<uint64_t>(__double_as_longlong(d));
+}
+
+// Exact-count selection, for neighborhoods kept as a set (features, stride
+// 1): a threshold E with exactly `fetch` candidates at d2 <= E.  Such an E
+// exists exactly when the fetch-th and (fetch+1)-th smallest distances
+// differ (no boundary tie), and the kept set is then {d2 <= E} whatever the
+// order among equal distances inside it.  Counts are exact (ballots).  First
+// round: kRound thresholds around the hint (the previous query's E) counted
+// together; one with count fetch is E, else the two around fetch bracket it,
+// and when few candidates lie inside the bracket each ranks itself among
+// them and the one at the wanted rank, checked by its count, is E.  Failing
+// that: linear interpolation of the count in d2 inside the bracket, then
+// candidates strictly inside it as pivots.  A bracket with no candidate
+// strictly inside means the fetch-th and (fetch+1)-th distances are equal:
+// false, and the caller takes the sorting selection (as when the trials run
+// out).
+//
+// Thresholds of the first round, as factors of the hint (for points on a
+// surface the count grows about linearly with d2).
+constexpr int kRound = 7;
+
+__device__ __forceinline__ float round_factor(int i)
+{
+    constexpr float f[kRound] = {0.6f, 0.75f, 0.87f, 1.0f, 1.15f, 1.33f, 1.6f};
+    return f[i];
+}
+
+__device__ __forceinline__ bool select_count(const uint64_t (&cd)[kRegBatches], uint32_t nbat,
+                                             uint32_t fetch, double hint, uint64_t& E)
+{
+    const double h = uniform_f64(hint);
+    // First round: counts at kRound thresholds around the hint, independent
+    // of each other (one latency for all of them).
+    uint64_t tb[kRound];
+    uint32_t c[kRound];
+#pragma unroll
+    for (int i = 0; i < kRound; ++i) {
+        tb[i] = d2_bits(h * static_cast<double>(round_factor(i)));
+        c[i] = 0;
+    }
+#pragma unroll
+    for (int b = 0; b < kRegBatches; ++b) {
+        if (static_cast<uint32_t>(b) >= nbat) break;
+#pragma unroll
+        for (int i = 0; i < kRound; ++i) c[i] += __popcll(__ballot(cd[b] <= tb[i]));
+    }
+    // The bracket: the last threshold below fetch, the first above.
+    int ilo = -1, ihi = kRound;
+#pragma unroll
+    for (int i = 0; i < kRound; ++i) {
+        if (c[i] == fetch) {
+            E = tb[i];
+            return true;
+        }
+        if (c[i] < fetch) ilo = i;
+        if (c[i] > fetch && ihi == kRound) ihi = i;
+    }
+    double lo = 0.0, hi = INFINITY;
+    uint32_t clo = 0, chi = 0;
+    uint64_t lob = 0, hib = kInfD;
+#pragma unroll
+    for (int i = 0; i < kRound; ++i) {
+        if (i == ilo) {
+            lob = tb[i];
+            clo = c[i];
+        }
+        if (i == ihi) {
+            hib = tb[i];
+            chi = c[i];
+        }
+    }
+    if (ilo >= 0) lo = bits_d2(lob);
+    if (ihi < kRound) hi = bits_d2(hib);
+    if (ihi < kRound && chi - clo <= 16u) {
+        // Few candidates inside the bracket: each ranks itself among them
+        // (strictly smaller ones), and the one at rank fetch - clo - 1 is the
+        // fetch-th distance when it is alone there; checked by its count.
+        uint64_t mb[kRegBatches];
+        uint32_t rk[kRegBatches];
+#pragma unroll
+        for (int b = 0; b < kRegBatches; ++b) {
+            rk[b] = 0;
+            mb[b] = static_cast<uint32_t>(b) < nbat
+                        ? __ballot((ilo < 0 || cd[b] > lob) && cd[b] <= hib)
+                        : 0ull;
+        }
+#pragma unroll
+        for (int b = 0; b < kRegBatches; ++b) {
+            uint64_t m = mb[b];
+            while (m) {
+                const int l = __builtin_ctzll(m);
+                m &= m - 1;
+                const uint64_t v = lane_u64(cd[b], l);
+#pragma unroll
+                for (int b2 = 0; b2 < kRegBatches; ++b2)
+                    if (static_cast<uint32_t>(b2) < nbat) rk[b2] += v < cd[b2] ? 1u : 0u;
+            }
+        }
+        const uint32_t want = fetch - clo - 1;
+        uint64_t pick = kInfD;
+        uint32_t npick = 0;
+#pragma unroll
+        for (int b = 0; b < kRegBatches; ++b) {
+            if (static_cast<uint32_t>(b) >= nbat) break;
+            const uint64_t m = mb[b] & __ballot(rk[b] == want);
+            npick += static_cast<uint32_t>(__popcll(m));
+            if (m && pick == kInfD) pick = lane_u64(cd[b], __builtin_ctzll(m));
+        }
+        if (npick == 1) {
+                    if (count_bits<false>(cd, nbat, pick) == fetch) {
+                E = pick;
+                return true;
+            }
+        }
+    }
+    // Otherwise: linear interpolation of the count in d2 inside the bracket,
+    // then candidates strictly inside the bracket as pivots.  When the
+    // bracket holds no candidate strictly inside, the fetch-th and
+    // (fetch+1)-th distances are equal: false.
+    double t;
+    if (hi == INFINITY) {
+        t = (ilo >= 0 ? lo : h) * static_cast<double>((static_cast<float>(fetch) + 0.5f) /
+                                                    static_cast<float>(clo > 0 ? clo : 1));
+    } else {
+        const float f = (static_cast<float>(fetch - clo) + 0.5f) / static_cast<float>(chi - clo);
+        t = lo + (hi - lo) * static_cast<double>(f);
+    }
+#pragma unroll 1
+    for (int trial = 0; trial < 16; ++trial) {
+        if (trial >= 3 || !(t > lo && t < hi)) {
+            // Pivot: the first candidate strictly inside (lo, hi).
+            const uint64_t lb = d2_bits(lo), hb = d2_bits(hi);
+            uint64_t piv = kInfD;
+#pragma unroll
+            for (int b = 0; b < kRegBatches; ++b) {
+                if (static_cast<uint32_t>(b) >= nbat || piv != kInfD) continue;
+                const uint64_t m = __ballot(cd[b] > lb && cd[b] < hb);
+                if (m) piv = lane_u64(cd[b], __builtin_ctzll(m));
+            }
+            if (piv == kInfD) return false;
+            t = bits_d2(piv);
+        }
+        const uint64_t tbt = d2_bits(t);
+            const uint32_t ct = count_bits<false>(cd, nbat, tbt);
+        if (ct == fetch) {
+            E = tbt;
+            return true;
+        }
+        if (ct < fetch) {
+            lo = t;
+            clo = ct;
+        } else {
+            hi = t;
+            chi = ct;
+        }
+        if (hi == INFINITY) {
+            // Count ~ d2 through the origin.
+            t = lo * static_cast<double>((static_cast<float>(fetch) + 0.5f) /
+                                         static_cast<float>(clo > 0 ? clo : 1));
+        } else {
+            const float f = (static_cast<float>(fetch - clo) + 0.5f) /
+                            static_cast<float>(chi - clo);
+            t = lo + (hi - lo) * static_cast<double>(f);
+        }
+    }
+    return false;
+}
+
+// Lane l's value of v, for every lane (l wave-uniform).
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(b), l);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(b >> 32), l);
+    return __longlong_as_double(
+        static_cast<long long>(static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32)));
 }
 
 __device__ __forceinline__ uint32_t lanes_below64(uint64_t m)
@@ -405,7 +607,7 @@ __device__ __forceinline__ void search_single(const KnnArgs& a, const WaveGrid& 
             uint32_t lo, len;
             box_ranges(a, wg, b, cb, lane, lo, len);
             const uint32_t incl = wave_incl_scan_u32(len, lane);
-            const uint32_t tot = __shfl(incl, 63);
+            const uint32_t tot = lane_u32(incl, 63);
             const uint32_t excl = incl - len;
             cnt[1] += tot;
             if (ncb <= 64 && tot <= kRegCand) {
@@ -438,7 +640,7 @@ __device__ __forceinline__ void search_single(const KnnArgs& a, const WaveGrid& 
                 Ent c = ent_inf();
                 if (t < tot)
                     c = make_ent(qx, qy, qz, cv.sx[j], cv.sy[j], cv.sz[j], cv.perm[j], j);
-                const Ent worst = ent_shfl(best, fetch);
+                const Ent worst = ent_lane(best, fetch);
                 const bool surv = t < tot && ent_less(c, worst);
                 const uint64_t m = __ballot(surv);
                 if (!m) continue;
@@ -463,7 +665,7 @@ __device__ __forceinline__ void search_single(const KnnArgs& a, const WaveGrid& 
             cnt[2] += 1;
             nbuf = 0;
         }
-        const Ent kth = ent_shfl(best, fetch - 1);
+        const Ent kth = ent_lane(best, fetch - 1);
         if (certified(wg, b, kth, qx, qy, qz)) return;
         if (r >= a.tune.ring_max) {
             incomplete = true;
@@ -498,7 +700,7 @@ __device__ __forceinline__ void finish_query(const KnnArgs& a, const WaveGrid& w
     const bool eq = (lane < fetch) && (lane + 1 < 64) && best.d == nxt.d && best.d != kInfD;
     const bool tie = (stride > 1 || a.mode == 2)
                          ? (__ballot(eq) != 0)
-                         : (__shfl(static_cast<int>(eq), fetch - 1) != 0);
+                         : (lane_u32(eq ? 1u : 0u, fetch - 1) != 0);
     const uint8_t flags = (tie ? ST_TIE : 0) | (incomplete ? ST_INCOMPLETE : 0);
     // Kept list in canonical (d2, id) order: entries 0, stride, 2*stride...
     const Ent kept = ent_shfl(best, (lane * stride) & 63);
@@ -592,6 +794,19 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     uint64_t cnt[3] = {0, 0, 0};  // search passes, candidates, selections
     int qi = 0;
     int rg_prev = 1;  // box half-width of the previous group (density guess)
+    // Features with stride 1 keep each neighborhood as a set: the exact-count
+    // selection serves them; lists, and strided neighborhoods, need the
+    // (d2, id) order.
+    const bool as_set = a.mode == 0 && wg.stride == 1;
+    // The previous query's threshold, the first one tried (queries in sorted
+    // order are close, with similar neighborhood radii).
+    double hint = INFINITY;
+    // The wave's queries, one per lane, loaded once: coordinates and keys.
+    const bool lane_q = lane < nq;
+    const double wx = lane_q ? cv.sx[q0 + lane] : 0.0;
+    const double wy = lane_q ? cv.sy[q0 + lane] : 0.0;
+    const double wz = lane_q ? cv.sz[q0 + lane] : 0.0;
+    const uint32_t wkey = lane_q ? cv.keys[q0 + lane] : 0u;
     while (qi < nq) {
         // The group: the run of queries from qi that lie in one 2x2 block of
         // columns (one column if the block is too dense) with z cells within
@@ -600,12 +815,16 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
         // grown by rg.
         const uint64_t s0 = q0 + qi;
         const bool in_range = qi + lane < nq;
-        const uint32_t key_l = in_range ? cv.keys[s0 + lane] : 0u;
-        const uint32_t key0 = __shfl(key_l, 0);
+        // (shuffled by every lane: a lane outside the wave's queries is still
+        // the source of another's key)
+        const uint32_t key_sh =
+            static_cast<uint32_t>(__shfl(static_cast<int>(wkey), (qi + lane) & 63));
+        const uint32_t key_l = in_range ? key_sh : 0u;
+        const uint32_t key0 = lane_u32(key_l, 0);
         const int cz0 = static_cast<int>(key0 & zmask);
         const int cz_l = static_cast<int>(key_l & zmask);
         const bool zok = in_range && abs(cz_l - cz0) <= a.tune.group_z;
-        const double fx = cv.sx[s0], fy = cv.sy[s0];
+        const double fx = lane_value(wx, qi), fy = lane_value(wy, qi);
         const int qcx = static_cast<int>(cell_coord(fx, g.ox, g.inv_c, g.gx));
         const int qcy = static_cast<int>(cell_coord(fy, g.oy, g.inv_c, g.gy));
         const uint32_t need = static_cast<uint32_t>(fetch + 1);
@@ -624,15 +843,15 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                 zl = min(zl, __shfl_xor(zl, m));
                 zh = max(zh, __shfl_xor(zh, m));
             }
-            zmin = zl;
-            zmax = zh;
+            zmin = uniform_i32(zl);
+            zmax = uniform_i32(zh);
             const int bx0 = level ? (qcx & ~1) : qcx, by0 = level ? (qcy & ~1) : qcy;
             const int bw = level ? 2 : 1;
             box = Box{bx0 - rg, bx0 + bw - 1 + rg, by0 - rg, by0 + bw - 1 + rg, zmin - rg,
                       zmax + rg};
             box_ranges(a, wg, box, 0, lane, lo, len);
             const uint32_t incl = wave_incl_scan_u32(len, lane);
-            tot = __shfl(incl, 63);
+            tot = lane_u32(incl, 63);
             excl = incl - len;
             // Half-width the local density asks for: the need-th neighbor
             // radius of a surface of this 2-D density, with 10% slack.
@@ -687,13 +906,13 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             }
         }
 
-        double hint = INFINITY;
         for (int gi = 0; gi < gm; ++gi) {
             const uint64_t s = s0 + gi;
             const int qslot = qi + gi;
             bool done = false;
             if (cached) {
-                const double qx = cv.sx[s], qy = cv.sy[s], qz = cv.sz[s];
+                const double qx = lane_value(wx, qslot), qy = lane_value(wy, qslot),
+                             qz = lane_value(wz, qslot);
                 cnt[0] += 1;
                 cnt[1] += tot;
                 // The candidates' ids and positions are the cache's own.
@@ -703,14 +922,72 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                     cd[b] = (static_cast<uint32_t>(b) * 64u < tot && pj[b] != kInfU)
                                 ? dist_bits(qx, qy, qz, px[b], py[b], pz[b])
                                 : kInfD;
-                // Neighboring queries of a group have similar need-th
-                // distances: the previous one is the first threshold tried.
-                Ent best;
-                select_entries(cd, pid, pj, tot, fetch, lane, scratch, hint, best, cnt);
-                const Ent nth = ent_shfl(best, static_cast<int>(min(need, tot)) - 1);
-                hint = nth.d == kInfD ? INFINITY : ent_d2(nth);
-                done = certified(wg, box, ent_shfl(best, fetch - 1), qx, qy, qz);
-                if (done) finish_query(a, wg, best, false, s, qslot, lane, nbr, my_flags);
+                const uint32_t nbat = (tot + 63u) >> 6;
+                uint64_t E = 0;
+                if (as_set && hint < INFINITY && tot >= static_cast<uint32_t>(fetch) &&
+                    select_count(cd, nbat, static_cast<uint32_t>(fetch), hint, E)) {
+                    // The kept set is {d2 <= E}, with no boundary tie.  It is
+                    // certified when its fetch-th distance (<= E) is below
+                    // the box clearance less the margin, i.e. when E is, or
+                    // when fetch candidates are.
+                    cnt[2] += 1;
+                    hint = bits_d2(E);
+                    const double R = box_clearance(g, box, qx, qy, qz);
+                    done = R == INFINITY;
+                    if (!done) {
+                        const double Rm = R - wg.margin;
+                        if (Rm > 0.0) {
+                            const double R2 = Rm * Rm;
+                            done = bits_d2(E) < R2 ||
+                                   count_bits<true>(cd, nbat, d2_bits(R2)) >=
+                                       static_cast<uint32_t>(fetch);
+                        }
+                    }
+                    if (done) {
+                        // The list in (d2, id) order, as the sorting path
+                        // keeps it (so the covariance sums in one order
+                        // whatever the grid and launch shape): the kept
+                        // entries are compacted into the scratch, and each
+                        // of the first fetch lanes counts the entries before
+                        // its own and stores its sorted position there.
+                        uint32_t base = 0;
+#pragma unroll
+                        for (int b = 0; b < kRegBatches; ++b) {
+                            if (static_cast<uint32_t>(b) >= nbat) break;
+                            const bool kept = cd[b] <= E;
+                            const uint64_t m = __ballot(kept);
+                            if (kept)
+                                reinterpret_cast<uint4*>(scratch)[base + lanes_below64(m)] =
+                                    make_uint4(static_cast<uint32_t>(cd[b]),
+                                               static_cast<uint32_t>(cd[b] >> 32), pid[b], pj[b]);
+                            base += static_cast<uint32_t>(__popcll(m));
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        if (lane < fetch) {
+                            const uint4 me = reinterpret_cast<const uint4*>(scratch)[lane];
+                            const uint64_t vd = static_cast<uint64_t>(me.x) |
+                                                (static_cast<uint64_t>(me.y) << 32);
+                            uint32_t rank = 0;
+#pragma unroll 4
+                            for (int i = 0; i < fetch; ++i)
+                                rank += rank_step(reinterpret_cast<const uint4*>(scratch)[i],
+                                                  vd, me.z);
+                            nbr[qslot * a.lds_stride + rank] = me.w;
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                } else {
+                    // Neighboring queries of a group have similar need-th
+                    // distances: the previous one is the first threshold tried.
+                    Ent best;
+                    select_entries(cd, pid, pj, tot, fetch, lane, scratch, hint, best, cnt);
+                    const Ent nth = ent_lane(best, static_cast<int>(min(need, tot)) - 1);
+                    hint = nth.d == kInfD ? INFINITY : ent_d2(nth);
+                    done = certified(wg, box, ent_lane(best, fetch - 1), qx, qy, qz);
+                    if (done) finish_query(a, wg, best, false, s, qslot, lane, nbr, my_flags);
+                }
             }
             if (!done) {
                 // Searched alone after the groups, so the group's register
@@ -729,7 +1006,8 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
         const uint32_t word = pending[p];
         const int qslot = static_cast<int>(word & 0xFFu);
         const uint64_t s = q0 + qslot;
-        const double qx = cv.sx[s], qy = cv.sy[s], qz = cv.sz[s];
+        const double qx = lane_value(wx, qslot), qy = lane_value(wy, qslot),
+                     qz = lane_value(wz, qslot);
         const int qcx = static_cast<int>(cell_coord(qx, g.ox, g.inv_c, g.gx));
         const int qcy = static_cast<int>(cell_coord(qy, g.oy, g.inv_c, g.gy));
         const int qcz = static_cast<int>(cell_coord(qz, g.oz, g.inv_cz, g.gz));
@@ -890,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_radius(RadiusArgs a)
         const uint64_t s0 = q0 + qi;
         const bool in_range = qi + lane < nq;
         const uint32_t key_l = in_range ? cv.keys[s0 + lane] : 0u;
-        const uint32_t key0 = __shfl(key_l, 0);
+        const uint32_t key0 = lane_u32(key_l, 0);
         const int cz0 = static_cast<int>(key0 & zmask);
         const int cz_l = static_cast<int>(key_l & zmask);
         const bool zok = in_range && abs(cz_l - cz0) <= a.group_z;
@@ -913,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_radius(RadiusArgs a)
         for (int cb = 0; cb < ncb; cb += kWave) {
             uint32_t lo, len;
             box_ranges(ka, wg, box, cb, lane, lo, len);
-            tot += __shfl(wave_incl_scan_u32(len, lane), 63);
+            tot += lane_u32(wave_incl_scan_u32(len, lane), 63);
         }
         const bool cached = ncb <= kWave && tot <= kRegCand;
         double px[kRegBatches], py[kRegBatches], pz[kRegBatches];
@@ -965,7 +1243,7 @@ __global__ __launch_bounds__(256) void k_radius(RadiusArgs a)
                     uint32_t lo, len;
                     box_ranges(ka, wg, box, cb, lane, lo, len);
                     const uint32_t incl = wave_incl_scan_u32(len, lane);
-                    const uint32_t ctot = __shfl(incl, 63);
+                    const uint32_t ctot = lane_u32(incl, 63);
                     const uint32_t excl = incl - len;
                     for (uint32_t t0 = 0; t0 < ctot; t0 += kRegCand) {
                         double cx[kRegBatches], cy[kRegBatches], czv[kRegBatches];
