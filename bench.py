@@ -46,10 +46,12 @@ SEARCH_BYTES_PER_POINT = 92.0
 # knn=16), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this
 # script (KiB per dispatch), each scaled by the same counter on a 1 GiB copy
 # kernel in the same runs: FETCH_SIZE reads 0.5 of the copied bytes there
-# (factor 2.0), WRITE_SIZE 1.0 (factor 1.0).  The runs, commands and raw
+# (factor 2.0), WRITE_SIZE 1.0 (factor 1.0).  kernel_commit names the commit
+# whose k_knn source the counters were taken on; the runs, commands and raw
 # counter values are in profiles/k_knn_plain_4M_knn16_traffic.md.
-MEASURED_TRAFFIC = {"points": 4_000_000, "knn": 16, "fetch_kib": 92925.3,
-                    "write_kib": 344280.5, "fetch_factor": 2.0, "write_factor": 1.0}
+MEASURED_TRAFFIC = {"points": 4_000_000, "knn": 16, "fetch_kib": 83510.1,
+                    "write_kib": 312652.9, "fetch_factor": 2.0, "write_factor": 1.0,
+                    "kernel_commit": "6472501"}
 # FP64 flops per candidate distance: 3 sub, 3 mul, 2 add.
 FLOPS_PER_CANDIDATE = 8.0
 HBM_PEAK_GBS = 8000.0
@@ -136,6 +138,9 @@ def gpu_leg(args):
         value=n / (ms / 1e3), ms_per_step=ms,
         roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                  "traffic_source": (None if traffic is None else
+                                     "rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated; k_knn as of "
+                                     "commit " + mt["kernel_commit"]),
                   "kernel": "k_knn (kNN search + FP64 covariance)",
                   "bytes_per_point": SEARCH_BYTES_PER_POINT,
                   "kernel_ms": ms_search},
