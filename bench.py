@@ -191,6 +191,40 @@ def gen_tiles(specs, workers=16):
         return list(ex.map(one, specs))
 
 
+def compare_results(pairs, info_mask, acc=None):
+    """Result check of a batched or served run against the same inputs run
+    alone: pairs of (columns, statuses) host arrays, tallied into acc.
+    Statuses must be equal outside the search's informational bits (a row
+    re-solved by the exact search in one run and not the other); columns
+    equal up to 1e-12 relative (the two summation orders of such a row)."""
+    import numpy as np
+    if acc is None:
+        acc = {"rows": 0, "status_mismatch_rows": 0, "nan_mismatch": 0,
+               "values_above_1e-12_rel": 0, "max_rel_diff": 0.0, "values": 0, "identical": 0}
+    for (ca, sa), (cb, sb) in pairs:
+        if sa:
+            acc["rows"] += len(sa[0])
+        for a, b in zip(sa, sb):
+            acc["status_mismatch_rows"] += int(np.count_nonzero((a & ~info_mask) !=
+                                                                (b & ~info_mask)))
+        for a, b in zip(ca, cb):
+            same = a.view(np.uint64) == b.view(np.uint64)
+            acc["values"] += len(a)
+            acc["identical"] += int(np.count_nonzero(same))
+            na, nb = np.isnan(a), np.isnan(b)
+            acc["nan_mismatch"] += int(np.count_nonzero(na != nb))
+            d = ~same & ~na & ~nb
+            if d.any():
+                rel = np.abs(a[d] - b[d]) / np.maximum(np.maximum(np.abs(a[d]), np.abs(b[d])),
+                                                       1e-300)
+                acc["max_rel_diff"] = max(acc["max_rel_diff"], float(rel.max()))
+                acc["values_above_1e-12_rel"] += int(np.count_nonzero(rel > 1e-12))
+    acc["bit_identical_frac"] = acc["identical"] / max(acc["values"], 1)
+    acc["ok"] = (acc["status_mismatch_rows"] == 0 and acc["nan_mismatch"] == 0 and
+                 acc["values_above_1e-12_rel"] == 0)
+    return acc
+
+
 def c4_specs(ntiles, seed=0xC4):
     """The batched configuration: tile sizes log-uniform in [10k, 2M], knn in
     {8, 16, 32}, density uniform in [2, 50] points/m^2 (side = sqrt(n/density))."""
@@ -242,9 +276,13 @@ def batched_leg(args):
             cl.close()
         return len(tuned)
 
-    def measure(ctx):
+    def measure(ctx, check=False):
         cols = [ctx.alloc(n_total * 8) for _ in range(ncols)]
         stat = [ctx.alloc(n_total) for _ in range(2)]
+        # The plain path writes into its own columns when its results are
+        # checked against the batch's.
+        pcols = [ctx.alloc(n_total * 8) for _ in range(ncols)] if check else cols
+        pstat = [ctx.alloc(n_total) for _ in range(2)] if check else stat
         b = ctx.batch(tiles)
         # Two untimed runs: the first runs launch by launch, the second
         # captures the step's graph; the timed steps replay it.
@@ -270,7 +308,7 @@ def batched_leg(args):
                 cl.invalidate()
                 o = int(offs[i])
                 st = [g.stage(g.STAGE_NORMAL, knns[i]), g.stage(g.STAGE_COVFEAT, knns[i])]
-                cl.run_device(st, [c + 8 * o for c in cols], [s + o for s in stat])
+                cl.run_device(st, [c + 8 * o for c in pcols], [s + o for s in pstat])
         plain_step()
         ctx.sync()
         t0 = time.perf_counter()
@@ -280,13 +318,31 @@ def batched_leg(args):
         ms_p = (time.perf_counter() - t0) * 1e3 / args.batch_steps
         for cl in clouds:
             cl.close()
+        checked = None
+        if check:
+            # Every tile of the batch against the same tile run alone, one
+            # column at a time through host memory.
+            info = np.uint8(g.ST_TIE | g.ST_INCOMPLETE)
+            sb = [ctx.copy_to_host(np.empty(n_total, np.uint8), p) for p in stat]
+            sp = [ctx.copy_to_host(np.empty(n_total, np.uint8), p) for p in pstat]
+            acc = compare_results([(([], sb), ([], sp))], info)
+            del sb, sp
+            for c in range(ncols):
+                a = ctx.copy_to_host(np.empty(n_total, np.float64), cols[c])
+                b = ctx.copy_to_host(np.empty(n_total, np.float64), pcols[c])
+                acc = compare_results([(([a], []), ([b], []))], info, acc)
+                del a, b
+            acc["what"] = "every tile's columns and statuses, batch vs the tile run alone"
+            checked = acc
+            for p in pcols + pstat:
+                ctx.free(p)
         for p in cols + stat:
             ctx.free(p)
-        return ms_b, ms_p, graph_mode
+        return ms_b, ms_p, graph_mode, checked
 
     ctx = new_context(args)
     ctx.set_tuning("timing", 0)
-    ms_b_def, ms_p_def, graph_def = measure(ctx)
+    ms_b_def, ms_p_def, graph_def, check = measure(ctx, check=True)
     ctx.close()
     tuned = 0
     ms_b_tun = ms_p_tun = float("inf")
@@ -295,7 +351,7 @@ def batched_leg(args):
         ctx = new_context(args)
         ctx.set_tuning("timing", 0)
         tuned = autotune(ctx)
-        ms_b_tun, ms_p_tun, graph_tun = measure(ctx)
+        ms_b_tun, ms_p_tun, graph_tun, _ = measure(ctx)
         ctx.close()
     ms_batched = min(ms_b_def, ms_b_tun)
     ms_plain = min(ms_p_def, ms_p_tun)
@@ -309,7 +365,8 @@ def batched_leg(args):
             "batched_over_plain": ms_plain / ms_batched,
             "batched_ms": {"default": ms_b_def, "autotuned": ms_b_tun if tuned else None},
             "plain_ms": {"default": ms_p_def, "autotuned": ms_p_tun if tuned else None},
-            "autotuned_buckets": tuned, "graph_replayed": graph_mode == 1}
+            "autotuned_buckets": tuned, "graph_replayed": graph_mode == 1,
+            "check": check}
 
 
 def las_leg(args, xyz):
@@ -409,10 +466,14 @@ def serving_leg(args):
     th.start()
     th.join()
 
-    def collect(tks):
+    kept = []
+
+    def collect(tks, keep=False):
         lat, queue, service, first, last, batches = [], [], [], None, None, set()
         for tk in tks:
-            _, _, (ts, tb, td, bid, bt) = srv.result(tk)
+            cols_r, stat_r, (ts, tb, td, bid, bt) = srv.result(tk)
+            if keep:
+                kept.append(([c for cs in cols_r for c in cs], stat_r))
             lat.append(td - ts)
             queue.append(tb - ts)
             service.append(td - tb)
@@ -422,12 +483,25 @@ def serving_leg(args):
         return (np.array(lat) * 1e3, np.array(queue) * 1e3, np.array(service) * 1e3,
                 last - first, len(batches))
 
-    lat_ms, q_ms, s_ms, span_s, nbat = collect(tickets)
+    lat_ms, q_ms, s_ms, span_s, nbat = collect(tickets, keep=True)
     # Capacity: the same requests submitted all at once (the queue never
     # runs dry), the sustained rate from the first submit to the last result.
     sat = [srv.submit(*tiles[i], stages, knns[i]) for i in range(len(tiles))]
     _, _, _, sat_s, sat_bat = collect(sat)
     srv.close()
+    # Every served request against the same tile run alone (host arrays).
+    ctx = g.Context(0)
+    pairs = []
+    for i, tile in enumerate(tiles):
+        cl = ctx.cloud(*tile)
+        (cn, cf), (sn, sf) = cl.run([g.stage(g.STAGE_NORMAL, knns[i]),
+                                     g.stage(g.STAGE_COVFEAT, knns[i])])
+        cl.close()
+        pairs.append((kept[i], (cn + cf, [sn, sf])))
+    ctx.close()
+    check = compare_results(pairs, np.uint8(g.ST_TIE | g.ST_INCOMPLETE))
+    check["what"] = "every request's columns and statuses, served vs the tile run alone"
+    del pairs, kept[:]
     pts = sum(len(t[0]) for t in tiles)
     pct = lambda a, q: float(np.percentile(a, q))
     return {"requests": len(tiles), "rate_per_s": args.rate, "points": int(pts),
@@ -440,7 +514,8 @@ def serving_leg(args):
                          "what": "batch start to results on the host: uploads, device, downloads"},
             "sustained_points_per_s": pts / span_s, "batches": nbat,
             "offered_points_per_s": pts / float(arrivals[-1]),
-            "capacity_points_per_s": pts / sat_s, "capacity_batches": sat_bat}
+            "capacity_points_per_s": pts / sat_s, "capacity_batches": sat_bat,
+            "check": check}
 
 
 def precision_leg(args, sample_rows=0):
