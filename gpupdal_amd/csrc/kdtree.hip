@@ -88,6 +88,9 @@ constexpr int kLvlThreads = 64;
 constexpr int kPredThreads = kLvlThreads;
 constexpr int kPredItems = 16;
 constexpr uint32_t kPredTile = kPredThreads * kPredItems;
+// Levels of up to this many prediction blocks (4M active points) compute
+// each block's prefix in k_kd_pred_down from the raw block totals.
+constexpr uint32_t kRawSumBlocks = 4096;
 constexpr int kChildThreads = 256;       // one block; small enough to run beside the search
 constexpr uint32_t kMinMaxLds = 128;      // big children staged per block in LDS (6 KB)
 
@@ -327,34 +330,6 @@ __device__ __forceinline__ uint32_t act_find(const uint32_t* __restrict__ off, u
     return lo;
 }
 
-// The parent's div from node nd's extent, then nd's split (big nodes always
-// split).
-__device__ __forceinline__ void kd_split_node(const KdView& t, uint32_t nd)
-{
-    double mn[3], mx[3];
-    for (int d = 0; d < 3; ++d) {
-        mn[d] = dec_order(t.mn[3 * nd + d]);
-        mx[d] = dec_order(t.mx[3 * nd + d]);
-    }
-    kd_parent_div(t, nd, mn, mx);
-    int f;
-    double cv;
-    kd_choose_split(t.box + 6 * static_cast<size_t>(nd), mn, mx, f, cv);
-    t.feat[nd] = f;
-    t.cutval[nd] = cv;
-}
-
-// One thread per big node of the level (A read from the level's header).
-__global__ __launch_bounds__(kLvlThreads) void k_kd_big_split(KdView t,
-                                                             const uint32_t* __restrict__ act,
-                                                             const uint32_t* __restrict__ lvl)
-{
-    const uint32_t A = lvl[0];
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < A;
-         k += gridDim.x * blockDim.x)
-        kd_split_node(t, act[k]);
-}
-
 // planeSplit's two Hoare passes as predicates -- pass 1 "coord < cut" over
 // the node, pass 2 "coord <= cut" over [L, right) where L = lo + #pass 1 --
 // over increasing active indices of one thread: the node's data (range, axis,
@@ -365,20 +340,38 @@ struct PredCursor {
     const double* ax;
     double cv;
 
+    uint32_t nd;
+    int f;
+
+    // Pass 1 of the split-fused reduction (kSplit) chooses the node's split
+    // from its extents and box here; the thread at the node's first active
+    // index stores it (see pred).  Later kernels of the level read it.
+    template <bool kSplit = false>
     __device__ __forceinline__ void load(const KdView& t, const uint32_t* act,
                                          const uint32_t* off, const uint32_t* s1, int pass,
                                          uint32_t kk)
     {
         k = kk;
-        const uint32_t nd = act[k];
+        nd = act[k];
         lo = t.nl[nd];
         t0 = off[k];
         t1 = off[k + 1];
-        ax = axis(t, t.feat[nd]);
-        cv = t.cutval[nd];
+        if (kSplit) {
+            double mn[3], mx[3];
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = dec_order(t.mn[3 * nd + d]);
+                mx[d] = dec_order(t.mx[3 * nd + d]);
+            }
+            kd_choose_split(t.box + 6 * static_cast<size_t>(nd), mn, mx, f, cv);
+        } else {
+            f = t.feat[nd];
+            cv = t.cutval[nd];
+        }
+        ax = axis(t, f);
         L = pass == 1 ? 0u : lo + (s1[t1] - s1[t0]);
     }
 
+    template <bool kSplit = false>
     __device__ __forceinline__ bool pred(const KdView& t, const uint32_t* act,
                                          const uint32_t* off, const uint32_t* s1, int pass,
                                          uint32_t A, uint32_t ti)
@@ -386,7 +379,19 @@ struct PredCursor {
         if (ti >= t1) {
             uint32_t kk = k;
             while (kk + 1 < A && off[kk + 1] <= ti) ++kk;
-            load(t, act, off, s1, pass, kk);
+            load<kSplit>(t, act, off, s1, pass, kk);
+        }
+        if (kSplit && ti == t0) {
+            // The node's one writer: its split, and the parent's div from
+            // the node's extents (kd_split_node's stores).
+            double mn[3], mx[3];
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = dec_order(t.mn[3 * nd + d]);
+                mx[d] = dec_order(t.mx[3 * nd + d]);
+            }
+            kd_parent_div(t, nd, mn, mx);
+            t.feat[nd] = f;
+            t.cutval[nd] = cv;
         }
         const uint32_t p = lo + (ti - t0);
         const double v = ax[p];
@@ -406,11 +411,21 @@ __global__ __launch_bounds__(kPredThreads) void k_kd_pred_reduce(
         return;
     }
     PredCursor c;
-    c.load(t, act, off, s1, pass, act_find(off, A, base));  // one search per block
     uint32_t acc = 0;
-    for (int i = 0; i < kPredItems; ++i) {
-        const uint32_t ti = base + i * kPredThreads + threadIdx.x;
-        if (ti < P && c.pred(t, act, off, s1, pass, A, ti)) ++acc;
+    if (pass == 1) {
+        // The level's splits are chosen here, by the blocks that cover each
+        // node's points.
+        c.load<true>(t, act, off, s1, pass, act_find(off, A, base));  // one search per block
+        for (int i = 0; i < kPredItems; ++i) {
+            const uint32_t ti = base + i * kPredThreads + threadIdx.x;
+            if (ti < P && c.pred<true>(t, act, off, s1, pass, A, ti)) ++acc;
+        }
+    } else {
+        c.load(t, act, off, s1, pass, act_find(off, A, base));
+        for (int i = 0; i < kPredItems; ++i) {
+            const uint32_t ti = base + i * kPredThreads + threadIdx.x;
+            if (ti < P && c.pred(t, act, off, s1, pass, A, ti)) ++acc;
+        }
     }
     uint32_t tot;
     kd_block_scan<kPredThreads>(acc, &tot);
@@ -454,10 +469,13 @@ void launch_kd_scan_top(uint32_t* sums, uint32_t nb, hipStream_t s)
 
 // Exclusive prefix of the predicate over active indices [0, P], written to
 // out (P + 1 entries; the predicate is 0 at P).
+// With raw_sums the block adds up the block totals before it itself (levels
+// of few blocks: no separate scan launch); otherwise sums holds their
+// exclusive scan.
 __global__ __launch_bounds__(kPredThreads) void k_kd_pred_down(
     KdView t, int pass, const uint32_t* __restrict__ act, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ s1,
-    const uint32_t* __restrict__ sums, uint32_t* __restrict__ out)
+    const uint32_t* __restrict__ sums, uint32_t* __restrict__ out, int raw_sums)
 {
     const uint32_t A = lvl[0], P = lvl[1];
     __shared__ uint32_t s[kPredTile];
@@ -481,8 +499,16 @@ __global__ __launch_bounds__(kPredThreads) void k_kd_pred_down(
         v[i] = s[threadIdx.x * kPredItems + i];
         acc += v[i];
     }
+    uint32_t before = 0;
+    if (raw_sums) {
+        uint32_t part = 0;
+        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kPredThreads) part += sums[b];
+        kd_block_scan<kPredThreads>(part, &before);
+    } else {
+        before = sums[blockIdx.x];
+    }
     uint32_t tot;
-    uint32_t ex = kd_block_scan<kPredThreads>(acc, &tot) + sums[blockIdx.x];
+    uint32_t ex = kd_block_scan<kPredThreads>(acc, &tot) + before;
     for (int i = 0; i < kPredItems; ++i) {
         s[threadIdx.x * kPredItems + i] = ex;
         ex += v[i];
@@ -2067,18 +2093,18 @@ uint32_t launch_kd_build(const double* x, const double* y, const double* z, uint
         const uint32_t Pb = P;
         for (int l = 0; l < kLevelsPerRead; ++l) {
             const uint32_t* lvl = w.hdr + kHdrLvl + 2 * cur;
-            const uint32_t gsplit = static_cast<uint32_t>(
-                std::min<uint64_t>(div_up(std::min<uint64_t>(Ab, w.list_cap), kLvlThreads), 256));
-            hipLaunchKernelGGL(k_kd_big_split, dim3(gsplit), dim3(kLvlThreads), 0, s, t,
-                               w.act[cur], lvl);
+            // Pass 1's reduction chooses the level's splits (no split
+            // launch); levels of at most kRawSumBlocks blocks skip the scan
+            // launch (each block adds up the block totals before it).
             for (int pass = 1; pass <= 2; ++pass) {
                 uint32_t* sc = pass == 1 ? w.s1 : w.s2;
                 const uint32_t nb = div_up(static_cast<uint64_t>(Pb) + 1, kPredTile);
                 hipLaunchKernelGGL(k_kd_pred_reduce, dim3(nb), dim3(kPredThreads), 0, s, t,
                                    pass, w.act[cur], w.off[cur], lvl, w.s1, w.sums);
-                launch_kd_scan_top(w.sums, nb, s);
+                const int raw = nb <= kRawSumBlocks ? 1 : 0;
+                if (!raw) launch_kd_scan_top(w.sums, nb, s);
                 hipLaunchKernelGGL(k_kd_pred_down, dim3(nb), dim3(kPredThreads), 0, s, t,
-                                   pass, w.act[cur], w.off[cur], lvl, w.s1, w.sums, sc);
+                                   pass, w.act[cur], w.off[cur], lvl, w.s1, w.sums, sc, raw);
                 hipLaunchKernelGGL(k_kd_publish, dim3(div_up(Pb, kLvlTile)), dim3(kLvlThreads),
                                    0, s, t, pass, w.act[cur], w.off[cur], lvl, w.s1, sc, w.tmp);
                 hipLaunchKernelGGL(k_kd_swap, dim3(div_up(Pb, kLvlTile)), dim3(kLvlThreads), 0,
