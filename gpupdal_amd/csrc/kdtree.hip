@@ -383,7 +383,7 @@ struct PredCursor {
         }
         if (kSplit && ti == t0) {
             // The node's one writer: its split, and the parent's div from
-            // the node's extents (kd_split_node's stores).
+            // the node's extents.
             double mn[3], mx[3];
             for (int d = 0; d < 3; ++d) {
                 mn[d] = dec_order(t.mn[3 * nd + d]);
