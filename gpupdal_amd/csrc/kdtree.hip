@@ -88,9 +88,11 @@ constexpr int kLvlThreads = 64;
 constexpr int kPredThreads = kLvlThreads;
 constexpr int kPredItems = 16;
 constexpr uint32_t kPredTile = kPredThreads * kPredItems;
-// Levels of up to this many prediction blocks (4M active points) compute
-// each block's prefix in k_kd_pred_down from the raw block totals.
-constexpr uint32_t kRawSumBlocks = 4096;
+// Levels of up to this many prediction blocks (512K active points) compute
+// each block's prefix in k_kd_pred_down from the raw block totals (at most
+// eight loads per thread); larger levels pay one scan launch instead, which
+// costs less than thousands of blocks each adding up thousands of totals.
+constexpr uint32_t kRawSumBlocks = 512;
 constexpr int kChildThreads = 256;       // one block; small enough to run beside the search
 constexpr uint32_t kMinMaxLds = 128;      // big children staged per block in LDS (6 KB)
 
