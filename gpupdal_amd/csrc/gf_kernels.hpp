@@ -111,6 +111,7 @@ struct KnnTuning {
     int32_t group_z;     // z cells beyond the first a query group may span
     int32_t q_per_wave;  // queries per wave (0: by k)
     int32_t waves;       // waves per workgroup of the single-cloud search (0: 4)
+    int32_t set_select;  // exact-count selection for set neighborhoods (1; 0: sorting)
 };
 
 struct CloudView {

@@ -71,6 +71,7 @@ struct Tuning {
     int group_z = 3;             // measured: flat for knn 16, 4% faster search for knn 32
     int q_per_wave = 0;          // queries per search wave (0: 64 up to k 32, else 32)
     int knn_waves = 0;           // waves per search workgroup (0: 4)
+    int set_select = 1;          // exact-count selection for set neighborhoods (0: sorting)
     double radius_cell = 0.5;    // radius grids: column edge = radius_cell * radius
     // Exact tree: subtree-per-workgroup threshold (points; 2048-point
     // workgroups, two per CU, measured fastest alone and beside the search).
@@ -101,6 +102,8 @@ struct Tuning {
             q_per_wave = static_cast<int>(v);
         else if (!std::strcmp(key, "knn_waves") && (v == 0 || v == 1 || v == 2 || v == 4))
             knn_waves = static_cast<int>(v);
+        else if (!std::strcmp(key, "set_select") && (v == 0 || v == 1))
+            set_select = static_cast<int>(v);
         else if (!std::strcmp(key, "radius_cell") && v >= 0.2 && v <= 4.0) radius_cell = v;
         else if (!std::strcmp(key, "kd_subtree") && v >= 101 && v <= 4096)
             kd_subtree = static_cast<int>(v);
@@ -323,6 +326,7 @@ struct Context {
         k.group_z = tune.group_z;
         k.q_per_wave = tune.q_per_wave;
         k.waves = tune.knn_waves;
+        k.set_select = tune.set_select;
         return k;
     }
     // The search shape for a cloud and result capacity: the context's

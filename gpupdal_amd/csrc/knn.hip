@@ -797,7 +797,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     // Features with stride 1 keep each neighborhood as a set: the exact-count
     // selection serves them; lists, and strided neighborhoods, need the
     // (d2, id) order.
-    const bool as_set = a.mode == 0 && wg.stride == 1;
+    const bool as_set = a.mode == 0 && wg.stride == 1 && a.tune.set_select != 0;
     // The previous query's threshold, the first one tried (queries in sorted
     // order are close, with similar neighborhood radii).
     double hint = INFINITY;

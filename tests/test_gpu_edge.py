@@ -236,6 +236,39 @@ def test_search_shapes_change_no_result(ctx, shape):
         np.testing.assert_allclose(cb, ca, rtol=1e-12, atol=1e-15, equal_nan=True)
 
 
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+@pytest.mark.parametrize("cloud", ["dups", "lattice", "dense"])
+def test_set_selection_equals_sorting(ctx, cloud):
+    # Feature neighborhoods with stride 1 are kept as sets, found by the
+    # exact-count threshold selection; switching it off (set_select=0) runs
+    # the sorting selection that lists use.  Every status bit and every
+    # column bit must agree: the same sets, summed in the same (d2, id)
+    # order, flagged (ties, uncertified rows) the same way.
+    if cloud == "dups":
+        x, y, z = small_cloud(60000, seed=73, dup_frac=0.03)
+    elif cloud == "lattice":
+        g1 = np.arange(40, dtype=np.float64)
+        X, Y, Z = np.meshgrid(g1, g1, np.arange(5, dtype=np.float64), indexing="ij")
+        x, y, z = X.ravel() * 0.5 + 500000.0, Y.ravel() * 0.5 + 4000000.0, Z.ravel() * 0.5
+    else:
+        x, y, z = small_cloud(80000, seed=74, side=40.0)
+    for st in ([g.stage(g.STAGE_NORMAL, 8), g.stage(g.STAGE_COVFEAT, 8, features=g.F_ALL10)],
+               [g.stage(g.STAGE_EIGENVALUES, 16), g.stage(g.STAGE_COVFEAT, 16)],
+               [g.stage(g.STAGE_NORMAL, 32)]):
+        fast = ctx.cloud(x, y, z).run(st, init=np.nan)
+        c2 = g.Context(0)
+        c2.set_tuning("set_select", 0)
+        slow = c2.cloud(x, y, z).run(st, init=np.nan)
+        c2.close()
+        for sa, sb in zip(fast[1], slow[1]):
+            assert np.array_equal(sa, sb), (cloud, st[0].knn)
+        for ca, cb in zip([c for cs in fast[0] for c in cs], [c for cs in slow[0] for c in cs]):
+            assert _bits_equal(ca, cb), (cloud, st[0].knn)
+
+
 def test_tree_beside_the_search_repeats_exactly(ctx, oracle):
     """Large clouds build the exact tree on a second stream from the moment
     their bounds are known, while the index build and the search run on the
