@@ -54,18 +54,34 @@ constexpr uint32_t kRegCand = kRegBatches * 64;
 // 64-entry list of queries left for the growing search.
 constexpr int kScratchWords = 256 + 256 + 64;
 
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ a,
-                                                    uint32_t lo, uint32_t hi,
-                                                    uint32_t key)
+// First positions in a[lo, hi) (sorted ascending) of keys >= k0 and of keys
+// >= k1 (k0 <= k1): two binary searches advanced together, so that their
+// loads are issued side by side (half the dependent load chain of running
+// them one after the other).
+__device__ __forceinline__ void lower_bound2_u32(const uint32_t* __restrict__ a, uint32_t lo,
+                                                 uint32_t hi, uint32_t k0, uint32_t k1,
+                                                 uint32_t& r0, uint32_t& r1)
 {
-    while (lo < hi) {
-        uint32_t mid = lo + ((hi - lo) >> 1);
-        if (a[mid] < key)
-            lo = mid + 1;
-        else
-            hi = mid;
+    uint32_t lo0 = lo, hi0 = hi, lo1 = lo, hi1 = hi;
+    while (lo0 < hi0 || lo1 < hi1) {
+        const uint32_t m0 = lo0 + ((hi0 - lo0) >> 1), m1 = lo1 + ((hi1 - lo1) >> 1);
+        const uint32_t v0 = lo0 < hi0 ? a[m0] : 0u;
+        const uint32_t v1 = lo1 < hi1 ? a[m1] : 0u;
+        if (lo0 < hi0) {
+            if (v0 < k0)
+                lo0 = m0 + 1;
+            else
+                hi0 = m0;
+        }
+        if (lo1 < hi1) {
+            if (v1 < k1)
+                lo1 = m1 + 1;
+            else
+                hi1 = m1;
+        }
     }
-    return lo;
+    r0 = lo0;
+    r1 = lo1;
 }
 
 // Block index -> logical block, so that each XCD (blocks are dealt to the 8
@@ -159,8 +175,7 @@ __device__ __forceinline__ void box_ranges(const KnnArgs& a, const WaveGrid& wg,
     if (!zfull && b1 - b0 > static_cast<uint32_t>(a.tune.small_col)) {
         const uint32_t klo = (col << g.bz) + static_cast<uint32_t>(max(b.z0, 0));
         const uint32_t khi = (col << g.bz) + static_cast<uint32_t>(min(b.z1, static_cast<int>(g.gz) - 1) + 1);
-        b0 = lower_bound_u32(cv.keys, b0, b1, klo);
-        b1 = lower_bound_u32(cv.keys, b0, b1, khi);
+        lower_bound2_u32(cv.keys, b0, b1, klo, khi, b0, b1);
     }
     lo = b0;
     len = b1 - b0;
